@@ -1,0 +1,980 @@
+// cgocn_api.hip -- host side of libcgocn.so: the C ABI declared in
+// include/cgocn.h.  Handles own a HIP stream, the per-grid constant tables and
+// one device allocation per member; kernels are launched batched over member
+// lists (one launch per kernel per step covers every listed member).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cgocn.h"
+#include "cgocn_dev.h"
+
+namespace cgo {
+void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int lb, hipStream_t);
+void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int lb, hipStream_t);
+void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
+void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
+void launch_bgc(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
+void launch_inventory(const MemberDev*, int32_t member, double* out, const double* w, int prec, hipStream_t);
+}  // namespace cgo
+
+using namespace cgo;
+
+namespace {
+
+constexpr int TABLE_CAP = 4096;
+constexpr double CONST_PI = 3.141592653589793;  // gem_cmn.f90:688
+constexpr double CONST_REARTH = 6.37E+06;       // gem_cmn.f90:696
+constexpr double CONV_M3_KG = 1027.649;         // gem_cmn.f90:510
+
+struct GridH {
+    cgo_grid_desc d{};
+    GridDev dev{};
+    std::vector<int32_t> k1;      // halo'd
+    std::vector<double> mj, mk;   // tables
+    std::vector<int32_t> wet;     // wet column ids, vocn order
+    std::vector<double> V;        // [ncell]
+    std::vector<double> invw;     // [ncell] inventory weights dphi*ds(j)*dz(k)
+    double Vtot = 0;
+    void* dbuf = nullptr;
+    double* d_invw = nullptr;
+};
+
+struct MemberH {
+    bool alive = false;
+    int grid = -1, L = 0, prec = 0, bgc = 0;
+    MemberDev dev{};
+    cgo_member_params p{};
+    cgo_bgc_params bp{};
+    void* dbuf = nullptr;
+    size_t bytes = 0;
+};
+
+struct Graph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int32_t* d_lists = nullptr;
+};
+
+// host copy of the device carry weight: 2^-16 ulp(x)
+double carry_unit_h(float x) {
+    int e = (x == 0.0f) ? -126 : ilogbf(x);
+    if (e < -126) e = -126;
+    return ldexp(1.0, e - 23 - 16);
+}
+
+}  // namespace
+
+struct cgo_engine {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::string err;
+    std::vector<GridH*> grids;
+    std::vector<MemberH*> members;
+    MemberDev* d_table = nullptr;
+    bool table_dirty = false;
+    int lb[2] = {4, 4};
+    std::vector<Graph> graphs;
+    int32_t* d_list = nullptr;  // scratch lists for eager steps
+    size_t d_list_cap = 0;
+    std::vector<int32_t> last_lists;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static int fail(cgo_engine* e, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e) e->err = buf;
+    return code;
+}
+
+#define HIPCHK(e, x)                                                                     \
+    do {                                                                                 \
+        hipError_t _r = (x);                                                             \
+        if (_r != hipSuccess) return fail(e, CGO_E_HIP, "%s: %s", #x, hipGetErrorString(_r)); \
+    } while (0)
+
+static MemberH* member(cgo_engine* e, int32_t m) {
+    if (!e || m < 0 || m >= (int32_t)e->members.size() || !e->members[m] || !e->members[m]->alive) return nullptr;
+    return e->members[m];
+}
+
+static int sync_table(cgo_engine* e) {
+    if (!e->table_dirty) return CGO_OK;
+    std::vector<MemberDev> t(e->members.size());
+    for (size_t i = 0; i < e->members.size(); i++)
+        if (e->members[i] && e->members[i]->alive) t[i] = e->members[i]->dev;
+    if (!t.empty()) HIPCHK(e, hipMemcpy(e->d_table, t.data(), t.size() * sizeof(MemberDev), hipMemcpyHostToDevice));
+    e->table_dirty = false;
+    return CGO_OK;
+}
+
+extern "C" {
+
+int cgo_create(int32_t device, cgo_engine** out) {
+    if (!out) return CGO_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return CGO_E_HIP;
+    cgo_engine* e = new cgo_engine();
+    e->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&e->d_table, TABLE_CAP * sizeof(MemberDev)) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        delete e;
+        return CGO_E_HIP;
+    }
+    *out = e;
+    return CGO_OK;
+}
+
+int cgo_destroy(cgo_engine* e) {
+    if (!e) return CGO_E_ARG;
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->s);
+    for (auto& g : e->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        if (g.d_lists) hipFree(g.d_lists);
+    }
+    for (auto* m : e->members)
+        if (m) {
+            if (m->dbuf) hipFree(m->dbuf);
+            delete m;
+        }
+    for (auto* g : e->grids)
+        if (g) {
+            if (g->dbuf) hipFree(g->dbuf);
+            delete g;
+        }
+    if (e->d_list) hipFree(e->d_list);
+    if (e->d_table) hipFree(e->d_table);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->s) hipStreamDestroy(e->s);
+    delete e;
+    return CGO_OK;
+}
+
+int cgo_last_error(const cgo_engine* e, char* buf, size_t n) {
+    if (!e || !buf || n == 0) return CGO_E_ARG;
+    snprintf(buf, n, "%s", e->err.c_str());
+    return CGO_OK;
+}
+
+int cgo_sync(cgo_engine* e) {
+    if (!e) return CGO_E_ARG;
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    return CGO_OK;
+}
+
+int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
+    if (!e || !d || !grid_id || !d->k1) return fail(e, CGO_E_ARG, "grid_create: null argument");
+    const int mi = d->maxi, mj = d->maxj, mk = d->maxk;
+    if (mi < 3 || mj < 3 || mk < 2 || mk > MAXK_DEV) return fail(e, CGO_E_ARG, "grid_create: bad dims %d %d %d", mi, mj, mk);
+    const double* jarr[] = {d->rc, d->rc2, d->cv, d->cv2, d->rds, d->dsv, d->rdsv, d->ds, d->sv};
+    const double* karr[] = {d->dt, d->rdz, d->rdza, d->dza, d->ssmax, d->dz};
+    for (auto* p : jarr)
+        if (!p) return fail(e, CGO_E_ARG, "grid_create: null per-j metric");
+    for (auto* p : karr)
+        if (!p) return fail(e, CGO_E_ARG, "grid_create: null per-k metric");
+    GridH* g = new GridH();
+    g->d = *d;
+    const int JS = mj + 2, KS = mk + 2, ncol = mi * mj, ncell = ncol * mk;
+    g->k1.assign(d->k1, d->k1 + (size_t)JS * (mi + 2));
+    g->mj.assign((size_t)NMJ * JS, 0.0);
+    g->mk.assign((size_t)NMK * KS, 0.0);
+    for (int j = 0; j < JS; j++) {
+        g->mj[MJ_RC * JS + j] = d->rc[j];
+        g->mj[MJ_RC2 * JS + j] = d->rc2[j];
+        g->mj[MJ_CV * JS + j] = d->cv[j];
+        g->mj[MJ_CV2 * JS + j] = d->cv2[j];
+        g->mj[MJ_RDS * JS + j] = d->rds[j];
+        g->mj[MJ_DSVM * JS + j] = (j >= 1) ? d->dsv[std::min(j, mj - 1)] : 0.0;
+        g->mj[MJ_RDSV * JS + j] = d->rdsv[j];
+        g->mj[MJ_DS * JS + j] = d->ds[j];
+        g->mj[MJ_SV * JS + j] = d->sv[j];
+    }
+    // BIOGEM geometry, sub_init_phys_ocn (biogem_data.f90:1098-1140)
+    std::vector<double> dzl(KS + 1, 0.0), dzal(KS + 1, 0.0);
+    for (int k = 1; k <= mk; k++) {
+        dzl[k] = d->dz[k];
+        dzal[k] = d->dza[k];
+    }
+    dzal[mk] = dzl[mk] / 2.0;
+    for (int k = 0; k < KS; k++) {
+        g->mk[MK_DT * KS + k] = d->dt[k];
+        g->mk[MK_RDZ * KS + k] = d->rdz[k];
+        g->mk[MK_RDZA * KS + k] = d->rdza[k];
+        g->mk[MK_DZA * KS + k] = d->dza[k];
+        g->mk[MK_SSMAX * KS + k] = d->ssmax[k];
+        g->mk[MK_DZ * KS + k] = d->dz[k];
+    }
+    for (int k = 1; k <= mk; k++) {
+        double sm = 0.0, sb = 0.0;
+        for (int q = k; q <= mk; q++) sm += d->dsc * dzal[q];
+        for (int q = k; q <= mk; q++) sb += d->dsc * dzl[q];
+        g->mk[MK_DMID * KS + k] = sm;
+        g->mk[MK_DBOT * KS + k] = sb;
+        g->mk[MK_DD * KS + k] = d->dsc * dzl[k];
+    }
+    // wet columns in the reference vector order (i outer, j inner; biogem_lib.f90:1069-1080)
+    auto K1 = [&](int I, int J) { return g->k1[(size_t)J * (mi + 2) + I]; };
+    for (int I = 1; I <= mi; I++)
+        for (int J = 1; J <= mj; J++)
+            if (K1(I, J) <= mk) g->wet.push_back((I - 1) + mi * (J - 1));
+    g->V.assign(ncell, 0.0);
+    g->invw.assign(ncell, 0.0);
+    for (int I = 1; I <= mi; I++)
+        for (int J = 1; J <= mj; J++)
+            for (int k = K1(I, J); k <= mk; k++) {
+                const double A = 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / mi) * (d->sv[J] - d->sv[J - 1]);
+                const size_t c = (size_t)(k - 1) * ncol + (I - 1) + (size_t)mi * (J - 1);
+                g->V[c] = g->mk[MK_DD * KS + k] * A;
+                g->invw[c] = d->dphi * d->ds[J] * d->dz[k];
+            }
+    g->Vtot = 0.0;
+    for (int32_t c : g->wet) {
+        const int I = c % mi + 1, J = c / mi + 1;
+        double part = 0.0;
+        for (int k = K1(I, J); k <= mk; k++) part += g->V[(size_t)(k - 1) * ncol + c];
+        g->Vtot += part;
+    }
+    // one device buffer: k1 | mj | mk | mjf | mkf | wet | V | invw
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_k1 = take(g->k1.size() * 4), o_mj = take(g->mj.size() * 8), o_mk = take(g->mk.size() * 8);
+    const size_t o_mjf = take(g->mj.size() * 4), o_mkf = take(g->mk.size() * 4), o_wet = take(g->wet.size() * 4 + 4);
+    const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8);
+    hipSetDevice(e->device);
+    if (hipMalloc(&g->dbuf, off) != hipSuccess) {
+        delete g;
+        return fail(e, CGO_E_NOMEM, "grid_create: hipMalloc(%zu)", off);
+    }
+    char* b = static_cast<char*>(g->dbuf);
+    std::vector<float> mjf(g->mj.begin(), g->mj.end()), mkf(g->mk.begin(), g->mk.end());
+    HIPCHK(e, hipMemcpy(b + o_k1, g->k1.data(), g->k1.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_mj, g->mj.data(), g->mj.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_mk, g->mk.data(), g->mk.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_mjf, mjf.data(), mjf.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_mkf, mkf.data(), mkf.size() * 4, hipMemcpyHostToDevice));
+    if (!g->wet.empty()) HIPCHK(e, hipMemcpy(b + o_wet, g->wet.data(), g->wet.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_V, g->V.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b + o_w, g->invw.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
+    GridDev& gd = g->dev;
+    gd.maxi = mi;
+    gd.maxj = mj;
+    gd.maxk = mk;
+    gd.ncol = ncol;
+    gd.ncell = ncell;
+    gd.nwet = (int)g->wet.size();
+    gd.k1 = reinterpret_cast<const int32_t*>(b + o_k1);
+    gd.mj = reinterpret_cast<const double*>(b + o_mj);
+    gd.mk = reinterpret_cast<const double*>(b + o_mk);
+    gd.mjf = reinterpret_cast<const float*>(b + o_mjf);
+    gd.mkf = reinterpret_cast<const float*>(b + o_mkf);
+    gd.wetcols = reinterpret_cast<const int32_t*>(b + o_wet);
+    gd.V = reinterpret_cast<const double*>(b + o_V);
+    g->d_invw = reinterpret_cast<double*>(b + o_w);
+    gd.dphi = d->dphi;
+    gd.rdphi = d->rdphi;
+    gd.rVtot = g->Vtot > 0 ? 1.0 / g->Vtot : 0.0;
+    for (int q = 0; q < 5; q++) gd.ec[q] = d->ec[q];
+    e->grids.push_back(g);
+    *grid_id = (int32_t)e->grids.size() - 1;
+    return CGO_OK;
+}
+
+// byte layout of one member allocation
+struct MemLayout {
+    size_t st[2], lo[2], u, sflux, rho, cost, ctrl, ocn, part, settle, carb, M, rM, vd, light, mld, colsum, colold,
+        scal, total;
+};
+
+static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
+    MemLayout m{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t ncell = g->dev.ncell, ncol = g->dev.ncol, nwet = g->dev.nwet;
+    const size_t es = prec == CGO_FP64 ? 8 : 4;
+    m.st[0] = take(L * ncell * es);
+    m.st[1] = take(L * ncell * es);
+    const bool lo = prec == CGO_FP32C && carry;
+    m.lo[0] = lo ? take(L * ncell * 2) : 0;
+    m.lo[1] = lo ? take(L * ncell * 2) : 0;
+    m.u = take(3 * ncell * es);
+    m.sflux = take(L * ncol * 8);
+    m.rho = take(ncell * 8);
+    m.cost = take(ncol * 8);
+    m.ctrl = take(256);  // parity, steps, diag[4], dmax
+    if (bgc) {
+        m.ocn = take((2 + NBGC) * ncell * 8);
+        m.part = take(NSED * ncell * 8);
+        m.settle = take(NSED * ncol * 8);
+        m.carb = take(NCARB * ncol * 8);
+        m.M = take(ncell * 8);
+        m.rM = take(ncell * 8);
+        m.vd = take(NBGC * ncell * 8);
+        m.light = take(ncol * 8);
+        m.mld = take(ncol * 8);
+        m.colsum = take((L + 1) * nwet * 8);
+        m.colold = take((L + 1) * nwet * 8);
+        m.scal = take((SC_FACT0 + L + 1) * 8);
+    }
+    m.total = off;
+    return m;
+}
+
+int64_t cgo_member_bytes(cgo_engine* e, int32_t grid_id, int32_t maxl, int32_t precision, int32_t bgc) {
+    if (!e || grid_id < 0 || grid_id >= (int)e->grids.size()) return CGO_E_ARG;
+    return (int64_t)layout(e->grids[grid_id], maxl, precision, 1, bgc).total;
+}
+
+static int alloc_member(cgo_engine* e, MemberH* m) {
+    const GridH* g = e->grids[m->grid];
+    MemLayout ly = layout(g, m->L, m->prec, m->p.carry, m->bgc);
+    void* old = m->dbuf;
+    const bool had_sflux = m->dev.sflux != nullptr;
+    std::vector<char> keep;
+    MemLayout oldly = layout(g, m->L, m->prec, m->p.carry, 0);
+    if (old) {  // growing to the BIOGEM layout keeps the GOLDSTEIN part
+        keep.resize(oldly.total);
+        HIPCHK(e, hipMemcpy(keep.data(), old, oldly.total, hipMemcpyDeviceToHost));
+    }
+    void* buf = nullptr;
+    if (hipMalloc(&buf, ly.total) != hipSuccess) return fail(e, CGO_E_NOMEM, "member: hipMalloc(%zu) failed", ly.total);
+    HIPCHK(e, hipMemset(buf, 0, ly.total));
+    if (old) {
+        HIPCHK(e, hipMemcpy(buf, keep.data(), oldly.total, hipMemcpyHostToDevice));
+        hipFree(old);
+    }
+    m->dbuf = buf;
+    m->bytes = ly.total;
+    char* b = static_cast<char*>(buf);
+    MemberDev& d = m->dev;
+    d.g = g->dev;
+    d.L = m->L;
+    d.prec = m->prec;
+    d.diso = m->p.diso;
+    d.bgc = m->bgc;
+    d.diff1 = m->p.diff[0];
+    d.diff2 = m->p.diff[1];
+    d.diff1f = (float)m->p.diff[0];
+    d.diff2f = (float)m->p.diff[1];
+    for (int q = 0; q < 2; q++) {
+        d.st[q] = b + ly.st[q];
+        d.lo[q] = (m->prec == CGO_FP32C && m->p.carry) ? reinterpret_cast<int16_t*>(b + ly.lo[q]) : nullptr;
+    }
+    d.u = b + ly.u;
+    d.sflux = nullptr;
+    d.rho = reinterpret_cast<double*>(b + ly.rho);
+    d.cost = reinterpret_cast<double*>(b + ly.cost);
+    d.parity = reinterpret_cast<int32_t*>(b + ly.ctrl);
+    d.steps = reinterpret_cast<int64_t*>(b + ly.ctrl + 8);
+    d.diag = reinterpret_cast<int32_t*>(b + ly.ctrl + 16);
+    d.dmax = reinterpret_cast<double*>(b + ly.ctrl + 32);
+    if (m->bgc) {
+        d.ocn = reinterpret_cast<double*>(b + ly.ocn);
+        d.part = reinterpret_cast<double*>(b + ly.part);
+        d.settle = reinterpret_cast<double*>(b + ly.settle);
+        d.carb = reinterpret_cast<double*>(b + ly.carb);
+        d.M = reinterpret_cast<double*>(b + ly.M);
+        d.rM = reinterpret_cast<double*>(b + ly.rM);
+        d.vd = reinterpret_cast<double*>(b + ly.vd);
+        d.light = reinterpret_cast<double*>(b + ly.light);
+        d.mld = reinterpret_cast<double*>(b + ly.mld);
+        d.colsum = reinterpret_cast<double*>(b + ly.colsum);
+        d.colold = reinterpret_cast<double*>(b + ly.colold);
+        d.scal = reinterpret_cast<double*>(b + ly.scal);
+    }
+    m->dev.sflux = had_sflux ? reinterpret_cast<double*>(b + ly.sflux) : nullptr;
+    e->table_dirty = true;
+    return CGO_OK;
+}
+
+int cgo_member_create(cgo_engine* e, int32_t grid_id, int32_t maxl, int32_t precision, const cgo_member_params* p,
+                      int32_t* member_id) {
+    if (!e || !p || !member_id) return fail(e, CGO_E_ARG, "member_create: null argument");
+    if (grid_id < 0 || grid_id >= (int)e->grids.size()) return fail(e, CGO_E_ARG, "member_create: bad grid %d", grid_id);
+    if (maxl < 2 || maxl > 256) return fail(e, CGO_E_ARG, "member_create: maxl %d outside 2..256", maxl);
+    if (precision != CGO_FP64 && precision != CGO_FP32C) return fail(e, CGO_E_ARG, "member_create: bad precision");
+    int slot = -1;
+    for (size_t i = 0; i < e->members.size(); i++)
+        if (!e->members[i] || !e->members[i]->alive) {
+            slot = (int)i;
+            break;
+        }
+    if (slot < 0) {
+        if ((int)e->members.size() >= TABLE_CAP) return fail(e, CGO_E_NOMEM, "member_create: table full");
+        e->members.push_back(nullptr);
+        slot = (int)e->members.size() - 1;
+    }
+    MemberH* m = e->members[slot] ? e->members[slot] : new MemberH();
+    *m = MemberH();
+    m->grid = grid_id;
+    m->L = maxl;
+    m->prec = precision;
+    m->p = *p;
+    e->members[slot] = m;
+    hipSetDevice(e->device);
+    int rc = alloc_member(e, m);
+    if (rc) return rc;
+    m->alive = true;
+    e->table_dirty = true;
+    *member_id = slot;
+    return CGO_OK;
+}
+
+int cgo_member_destroy(cgo_engine* e, int32_t id) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "member_destroy: bad member %d", id);
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (m->dbuf) hipFree(m->dbuf);
+    m->dbuf = nullptr;
+    m->alive = false;
+    e->table_dirty = true;
+    return CGO_OK;
+}
+
+int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
+    MemberH* m = member(e, id);
+    if (!m || !u) return fail(e, CGO_E_ARG, "set_velocity: bad argument");
+    const GridDev& g = m->dev.g;
+    const int mi = g.maxi, mj = g.maxj, mk = g.maxk;
+    const size_t ncell = g.ncell;
+    std::vector<double> h(3 * ncell);
+    // u(n, 0:maxi, 0:maxj, maxk): element (n,i,j,k) at n + 3*(i + (maxi+1)*(j + (maxj+1)*(k-1)))
+    for (int k = 1; k <= mk; k++)
+        for (int J = 1; J <= mj; J++)
+            for (int I = 1; I <= mi; I++)
+                for (int n = 0; n < 3; n++)
+                    h[(size_t)n * ncell + (size_t)(k - 1) * g.ncol + (I - 1) + (size_t)mi * (J - 1)] =
+                        u[n + 3 * ((size_t)I + (size_t)(mi + 1) * ((size_t)J + (size_t)(mj + 1) * (k - 1)))];
+    if (m->prec == CGO_FP64) {
+        HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(h.begin(), h.end());
+        HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return CGO_OK;
+}
+
+int cgo_set_sflux(cgo_engine* e, int32_t id, const double* sflux) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "set_sflux: bad member");
+    const GridH* g = e->grids[m->grid];
+    MemLayout ly = layout(g, m->L, m->prec, m->p.carry, m->bgc);
+    if (!sflux) {
+        m->dev.sflux = nullptr;
+    } else {
+        const int mi = g->dev.maxi, mj = g->dev.maxj, L = m->L;
+        std::vector<double> h((size_t)L * g->dev.ncol);
+        for (int J = 0; J < mj; J++)
+            for (int I = 0; I < mi; I++)
+                for (int l = 0; l < L; l++)
+                    h[(size_t)l * g->dev.ncol + I + (size_t)mi * J] = sflux[l + (size_t)L * (I + (size_t)mi * J)];
+        double* dst = reinterpret_cast<double*>(static_cast<char*>(m->dbuf) + ly.sflux);
+        HIPCHK(e, hipMemcpy(dst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        m->dev.sflux = dst;
+    }
+    e->table_dirty = true;
+    return CGO_OK;
+}
+
+// go_ts(maxl,maxi,maxj,maxk) <-> device [l][k][j][i]
+static void to_device_order(const GridDev& g, int L, const double* go, std::vector<double>& h) {
+    const size_t ncell = g.ncell;
+    h.resize((size_t)L * ncell);
+    for (size_t c = 0; c < ncell; c++)
+        for (int l = 0; l < L; l++) h[(size_t)l * ncell + c] = go[(size_t)l + (size_t)L * c];
+}
+
+int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
+    MemberH* m = member(e, id);
+    if (!m || !go_ts) return fail(e, CGO_E_ARG, "upload_state: bad argument");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    const GridDev& g = m->dev.g;
+    std::vector<double> h;
+    to_device_order(g, m->L, go_ts, h);
+    const size_t n = h.size();
+    for (int q = 0; q < 2; q++) {
+        if (m->prec == CGO_FP64) {
+            HIPCHK(e, hipMemcpy(m->dev.st[q], h.data(), n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(n);
+            std::vector<int16_t> lo(n);
+            for (size_t i = 0; i < n; i++) {
+                f[i] = (float)h[i];
+                double r = rint((h[i] - (double)f[i]) / carry_unit_h(f[i]));
+                lo[i] = (int16_t)std::min(std::max(r, -32767.0), 32767.0);
+            }
+            HIPCHK(e, hipMemcpy(m->dev.st[q], f.data(), n * 4, hipMemcpyHostToDevice));
+            if (m->dev.lo[q]) HIPCHK(e, hipMemcpy(m->dev.lo[q], lo.data(), n * 2, hipMemcpyHostToDevice));
+        }
+    }
+    const int32_t zero = 0;
+    HIPCHK(e, hipMemcpy(m->dev.parity, &zero, 4, hipMemcpyHostToDevice));
+    return CGO_OK;
+}
+
+static int download_dev(cgo_engine* e, MemberH* m, std::vector<double>& h) {
+    const GridDev& g = m->dev.g;
+    const size_t n = (size_t)m->L * g.ncell;
+    int32_t par = 0;
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(&par, m->dev.parity, 4, hipMemcpyDeviceToHost));
+    h.resize(n);
+    if (m->prec == CGO_FP64) {
+        HIPCHK(e, hipMemcpy(h.data(), m->dev.st[par], n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(n);
+        std::vector<int16_t> lo(n, 0);
+        HIPCHK(e, hipMemcpy(f.data(), m->dev.st[par], n * 4, hipMemcpyDeviceToHost));
+        if (m->dev.lo[par]) HIPCHK(e, hipMemcpy(lo.data(), m->dev.lo[par], n * 2, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) h[i] = (double)f[i] + (double)lo[i] * carry_unit_h(f[i]);
+    }
+    return CGO_OK;
+}
+
+int cgo_download_state(cgo_engine* e, int32_t id, double* go_ts) {
+    MemberH* m = member(e, id);
+    if (!m || !go_ts) return fail(e, CGO_E_ARG, "download_state: bad argument");
+    std::vector<double> h;
+    int rc = download_dev(e, m, h);
+    if (rc) return rc;
+    const size_t ncell = m->dev.g.ncell;
+    for (size_t c = 0; c < ncell; c++)
+        for (int l = 0; l < m->L; l++) go_ts[(size_t)l + (size_t)m->L * c] = h[(size_t)l * ncell + c];
+    return CGO_OK;
+}
+
+int cgo_download_rho(cgo_engine* e, int32_t id, double* rho) {
+    MemberH* m = member(e, id);
+    if (!m || !rho) return fail(e, CGO_E_ARG, "download_rho: bad argument");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(rho, m->dev.rho, (size_t)m->dev.g.ncell * 8, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
+int cgo_inventory(cgo_engine* e, int32_t id, double* inv) {
+    MemberH* m = member(e, id);
+    if (!m || !inv) return fail(e, CGO_E_ARG, "inventory: bad argument");
+    int rc = sync_table(e);
+    if (rc) return rc;
+    double* d_out = nullptr;
+    HIPCHK(e, hipMalloc(&d_out, (size_t)m->L * 8));
+    launch_inventory(e->d_table, id, d_out, e->grids[m->grid]->d_invw, m->prec, e->s);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(inv, d_out, (size_t)m->L * 8, hipMemcpyDeviceToHost));
+    hipFree(d_out);
+    return CGO_OK;
+}
+
+int cgo_get_diag(cgo_engine* e, int32_t id, cgo_diag* d) {
+    MemberH* m = member(e, id);
+    if (!m || !d) return fail(e, CGO_E_ARG, "get_diag: bad argument");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    int64_t steps = 0;
+    int32_t diag[4] = {0, 0, 0, 0};
+    double dmax = 0.0;
+    HIPCHK(e, hipMemcpy(&steps, m->dev.steps, 8, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(diag, m->dev.diag, 16, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(&dmax, m->dev.dmax, 8, hipMemcpyDeviceToHost));
+    std::vector<double> cost(m->dev.g.ncol);
+    HIPCHK(e, hipMemcpy(cost.data(), m->dev.cost, cost.size() * 8, hipMemcpyDeviceToHost));
+    d->steps = steps;
+    d->limps = diag[0];
+    d->carb_fail = diag[1];
+    d->carb_iters = (uint32_t)diag[2];
+    d->dmax = dmax;
+    d->cost = 0.0;
+    for (double c : cost) d->cost += c;
+    d->smean = 0.0;
+    if (m->bgc) HIPCHK(e, hipMemcpy(&d->smean, m->dev.scal + SC_SMEAN, 8, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
+void cgo_bgc_default_params(cgo_bgc_params* p, double dtyr) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->slot_DIC = p->slot_ALK = p->slot_PO4 = p->slot_O2 = p->slot_DOM_C = p->slot_DOM_P = -1;
+    p->caco3 = 1;
+    p->dtyr = dtyr;
+    p->k0_PO4 = 2.0E-06;
+    p->c0_PO4 = 0.050E-06;
+    p->DOMfrac = 0.66;
+    p->DOMlifetime = 0.5;
+    p->red_POP_POC = 106.0;
+    p->red_POP_PO2 = -138.0;
+    p->red_PON_ALK = -1.00;
+    p->red_POP_PON = 16.0;
+    p->red_POC_CaCO3 = 0.2;
+    p->red_POC_CaCO3_pP = 0.0;
+    p->POC_frac2 = 0.05;
+    p->POC_dfrac2 = 0.0;
+    p->POC_c0frac2 = 0.1E-6;
+    p->POC_eL1 = 500.0;
+    p->POC_eL2 = 1000000.0;
+    p->CaCO3_frac2 = 0.5;
+    p->CaCO3_eL1 = 1000.0;
+    p->CaCO3_eL2 = 1000000.0;
+    p->sinkrate = 125.0 / (1.0 / 365.25);
+    p->remin_k_O2 = 1.0;
+    p->remin_c0_O2 = 8.0E-6;
+    p->carb_tol = 0.001;
+    p->carb_maxit = 100;
+    p->dsc = 5.0E3;
+}
+
+int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const double* light, const double* mld) {
+    MemberH* m = member(e, id);
+    if (!m || !bp || !light || !mld) return fail(e, CGO_E_ARG, "bgc_init: bad argument");
+    const int32_t slots[NBGC] = {bp->slot_DIC, bp->slot_PO4, bp->slot_O2, bp->slot_ALK, bp->slot_DOM_C, bp->slot_DOM_P};
+    for (int b = 0; b < NBGC; b++)
+        if (slots[b] >= m->L || slots[b] == 0 || slots[b] == 1 || slots[b] < -1)
+            return fail(e, CGO_E_ARG, "bgc_init: species slot %d invalid for maxl %d", slots[b], m->L);
+    if (bp->caco3 && (bp->slot_DIC < 0 || bp->slot_ALK < 0)) return fail(e, CGO_E_ARG, "bgc_init: CaCO3 needs DIC and ALK");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    std::vector<double> ts;
+    int rc = download_dev(e, m, ts);  // current state, device order
+    if (rc) return rc;
+    if (!m->bgc) {
+        m->bgc = 1;
+        rc = alloc_member(e, m);
+        if (rc) return rc;
+    }
+    m->bp = *bp;
+    BgcParams& P = m->dev.bp;
+    int nplane = 2;
+    for (int b = 0; b < NBGC; b++) {
+        P.slot[b] = slots[b];
+        P.ocn[b] = slots[b] >= 0 ? nplane++ : -1;
+    }
+    P.caco3 = bp->caco3;
+    P.dtyr = bp->dtyr;
+    P.k0_PO4 = bp->k0_PO4;
+    P.c0_PO4 = bp->c0_PO4;
+    P.DOMfrac = bp->DOMfrac;
+    P.DOMlifetime = bp->DOMlifetime;
+    P.red_POP_POC = bp->red_POP_POC;
+    P.red_POP_PO2 = bp->red_POP_PO2;
+    P.red_PON_ALK = bp->red_PON_ALK;
+    P.red_POP_PON = bp->red_POP_PON;
+    P.red_POC_CaCO3 = bp->red_POC_CaCO3;
+    P.red_POC_CaCO3_pP = bp->red_POC_CaCO3_pP;
+    P.POC_frac2 = bp->POC_frac2;
+    P.POC_dfrac2 = bp->POC_dfrac2;
+    P.POC_c0frac2 = bp->POC_c0frac2;
+    P.POC_eL1 = bp->POC_eL1;
+    P.POC_eL2 = bp->POC_eL2;
+    P.CaCO3_frac2 = bp->CaCO3_frac2;
+    P.CaCO3_eL1 = bp->CaCO3_eL1;
+    P.CaCO3_eL2 = bp->CaCO3_eL2;
+    P.sinkrate = bp->sinkrate;
+    P.remin_k_O2 = bp->remin_k_O2;
+    P.remin_c0_O2 = bp->remin_c0_O2;
+    P.carb_tol = bp->carb_tol;
+    P.carb_maxit = bp->carb_maxit;
+    P.dsc = bp->dsc;
+    // host-side initial state, as sub_init_phys_ocn / sub_init_carb and the
+    // inverse of the ts normalisation (biogem.f90:2046)
+    const GridH* g = e->grids[m->grid];
+    const GridDev& gd = g->dev;
+    const int mi = gd.maxi, mk = gd.maxk, L = m->L;
+    const size_t ncell = gd.ncell, ncol = gd.ncol, nwet = gd.nwet;
+    auto K1 = [&](int I, int J) { return g->k1[(size_t)J * (mi + 2) + I]; };
+    std::vector<double> Mh(ncell, 0.0), rMh(ncell, 0.0), Sabs(ncell, 0.0);
+    for (int32_t c : g->wet) {
+        const int I = c % mi + 1, J = c / mi + 1;
+        for (int k = K1(I, J); k <= mk; k++) {
+            const size_t cc = (size_t)(k - 1) * ncol + c;
+            Mh[cc] = CONV_M3_KG * g->V[cc];
+            rMh[cc] = 1.0 / Mh[cc];
+            Sabs[cc] = ts[ncell + cc] + SALN0;
+        }
+    }
+    double Smean = 0.0;
+    const double rV = gd.rVtot;
+    for (int32_t c : g->wet) {
+        const int I = c % mi + 1, J = c / mi + 1;
+        double part = 0.0;
+        for (int k = K1(I, J); k <= mk; k++) part += Sabs[(size_t)(k - 1) * ncol + c] * g->V[(size_t)(k - 1) * ncol + c];
+        Smean += part * rV;
+    }
+    // ocn of every tracer slot (needed for the OLD inventories); planes kept for T, S, species
+    std::vector<double> ocn_all((size_t)L * ncell, 0.0);
+    for (int32_t c : g->wet) {
+        const int I = c % mi + 1, J = c / mi + 1;
+        for (int k = K1(I, J); k <= mk; k++) {
+            const size_t cc = (size_t)(k - 1) * ncol + c;
+            ocn_all[cc] = ts[cc] + ZEROC;
+            ocn_all[ncell + cc] = Sabs[cc];
+            for (int l = 2; l < L; l++) ocn_all[(size_t)l * ncell + cc] = ts[(size_t)l * ncell + cc] * Sabs[cc] / Smean;
+        }
+    }
+    std::vector<double> planes((size_t)(2 + NBGC) * ncell, 0.0);
+    std::copy(ocn_all.begin(), ocn_all.begin() + 2 * ncell, planes.begin());
+    for (int b = 0; b < NBGC; b++)
+        if (P.slot[b] >= 0)
+            std::copy(ocn_all.begin() + (size_t)P.slot[b] * ncell, ocn_all.begin() + (size_t)(P.slot[b] + 1) * ncell,
+                      planes.begin() + (size_t)P.ocn[b] * ncell);
+    std::vector<double> colold((size_t)(L + 1) * nwet, 0.0);
+    for (size_t w = 0; w < nwet; w++) {
+        const int c = g->wet[w];
+        const int I = c % mi + 1, J = c / mi + 1;
+        for (int l = 2; l < L; l++) {
+            double s = 0.0;
+            for (int k = K1(I, J); k <= mk; k++) {
+                const size_t cc = (size_t)(k - 1) * ncol + c;
+                s += ocn_all[(size_t)l * ncell + cc] * Mh[cc];
+            }
+            colold[(size_t)l * nwet + w] = s;
+        }
+    }
+    std::vector<double> carb((size_t)NCARB * ncol, 0.0);
+    for (size_t c = 0; c < ncol; c++) carb[IC_H * ncol + c] = pow(10.0, -7.8);
+    std::vector<double> lt(ncol), ml(ncol), scal(SC_FACT0 + L + 1, 0.0);
+    for (size_t c = 0; c < ncol; c++) {
+        lt[c] = light[c];
+        ml[c] = mld[c];
+    }
+    scal[SC_SMEAN] = Smean;
+    MemberDev& d = m->dev;
+    HIPCHK(e, hipMemcpy(d.ocn, planes.data(), planes.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemset(d.part, 0, (size_t)NSED * ncell * 8));
+    HIPCHK(e, hipMemset(d.settle, 0, (size_t)NSED * ncol * 8));
+    HIPCHK(e, hipMemset(d.vd, 0, (size_t)NBGC * ncell * 8));
+    HIPCHK(e, hipMemcpy(d.carb, carb.data(), carb.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.M, Mh.data(), ncell * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.rM, rMh.data(), ncell * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.light, lt.data(), ncol * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.mld, ml.data(), ncol * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.colold, colold.data(), colold.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.scal, scal.data(), scal.size() * 8, hipMemcpyHostToDevice));
+    e->table_dirty = true;
+    return CGO_OK;
+}
+
+int cgo_bgc_download(cgo_engine* e, int32_t id, int32_t what, double* out) {
+    MemberH* m = member(e, id);
+    if (!m || !out || !m->bgc) return fail(e, CGO_E_ARG, "bgc_download: bad argument or no BIOGEM state");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    const GridDev& g = m->dev.g;
+    const size_t ncell = g.ncell, ncol = g.ncol;
+    const int L = m->L;
+    const BgcParams& P = m->dev.bp;
+    if (what == 0 || what == 3) {
+        const bool anom = what == 3;
+        const int np = anom ? NBGC : 2 + NBGC;
+        std::vector<double> h((size_t)np * ncell);
+        HIPCHK(e, hipMemcpy(h.data(), anom ? m->dev.vd : m->dev.ocn, h.size() * 8, hipMemcpyDeviceToHost));
+        std::fill(out, out + (size_t)L * ncell, 0.0);
+        for (size_t c = 0; c < ncell; c++) {
+            if (!anom) {
+                out[(size_t)L * c] = h[c];
+                out[1 + (size_t)L * c] = h[ncell + c];
+            }
+            for (int b = 0; b < NBGC; b++)
+                if (P.slot[b] >= 0) out[(size_t)P.slot[b] + (size_t)L * c] = h[(size_t)(anom ? b : P.ocn[b]) * ncell + c];
+        }
+    } else if (what == 1) {
+        std::vector<double> h((size_t)NSED * ncell);
+        HIPCHK(e, hipMemcpy(h.data(), m->dev.part, h.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < ncell; c++)
+            for (int s = 0; s < NSED; s++) out[s + (size_t)NSED * c] = h[(size_t)s * ncell + c];
+    } else if (what == 2) {
+        std::vector<double> h((size_t)NCARB * ncol);
+        HIPCHK(e, hipMemcpy(h.data(), m->dev.carb, h.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < ncol; c++)
+            for (int s = 0; s < NCARB; s++) out[s + (size_t)NCARB * c] = h[(size_t)s * ncol + c];
+    } else {
+        return fail(e, CGO_E_ARG, "bgc_download: what %d", what);
+    }
+    return CGO_OK;
+}
+
+// ---------------------------------------------------------------- stepping
+struct Lists {
+    // [p64 list][p32 list][bgc p64][bgc p32][all]
+    std::vector<int32_t> v;
+    int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
+    int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0};
+};
+
+static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
+    std::vector<int32_t> g[5];
+    for (int i = 0; i < n; i++) {
+        MemberH* m = member(e, ids[i]);
+        if (!m) return fail(e, CGO_E_ARG, "step: bad member %d", ids[i]);
+        const int p = m->prec == CGO_FP64 ? 0 : 1;
+        g[p].push_back(ids[i]);
+        if (m->bgc) g[2 + p].push_back(ids[i]);
+        g[4].push_back(ids[i]);
+        ls.maxcol[p] = std::max(ls.maxcol[p], m->dev.g.ncol);
+        ls.maxL[p] = std::max(ls.maxL[p], m->L);
+        ls.maxwet[p] = std::max(ls.maxwet[p], m->dev.g.nwet);
+        if (m->bgc) ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
+    }
+    for (int q = 0; q < 5; q++) {
+        ls.off[q] = (int)ls.v.size();
+        ls.n[q] = (int)g[q].size();
+        ls.v.insert(ls.v.end(), g[q].begin(), g[q].end());
+    }
+    return CGO_OK;
+}
+
+static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int nsteps) {
+    for (int it = 0; it < nsteps; it++) {
+        for (int p = 0; p < 2; p++) {
+            if (!ls.n[p]) continue;
+            const int32_t* l = d_lists + ls.off[p];
+            if (p == 0)
+                launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[0], e->s);
+            else
+                launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[1], e->s);
+            launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+            if (ls.n[2 + p]) launch_bgc(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], p, e->s);
+        }
+        launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
+    }
+}
+
+static int upload_lists(cgo_engine* e, const Lists& ls) {
+    if (ls.v == e->last_lists && e->d_list) return CGO_OK;
+    if (ls.v.size() > e->d_list_cap) {
+        if (e->d_list) hipFree(e->d_list);
+        e->d_list_cap = std::max<size_t>(ls.v.size(), 1024);
+        HIPCHK(e, hipMalloc(&e->d_list, e->d_list_cap * 4));
+    }
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(e->d_list, ls.v.data(), ls.v.size() * 4, hipMemcpyHostToDevice));
+    e->last_lists = ls.v;
+    return CGO_OK;
+}
+
+int cgo_step(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
+    if (!e || !ids || n <= 0 || nsteps < 0) return fail(e, CGO_E_ARG, "step: bad argument");
+    hipSetDevice(e->device);
+    Lists ls;
+    int rc = make_lists(e, ids, n, ls);
+    if (rc) return rc;
+    if ((rc = sync_table(e))) return rc;
+    if ((rc = upload_lists(e, ls))) return rc;
+    enqueue_steps(e, ls, e->d_list, nsteps);
+    HIPCHK(e, hipGetLastError());
+    return CGO_OK;
+}
+
+int cgo_graph_capture(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps, int32_t* graph_id) {
+    if (!e || !ids || n <= 0 || nsteps <= 0 || !graph_id) return fail(e, CGO_E_ARG, "graph_capture: bad argument");
+    hipSetDevice(e->device);
+    Lists ls;
+    int rc = make_lists(e, ids, n, ls);
+    if (rc) return rc;
+    if ((rc = sync_table(e))) return rc;
+    Graph gr;
+    HIPCHK(e, hipMalloc(&gr.d_lists, ls.v.size() * 4));
+    HIPCHK(e, hipMemcpy(gr.d_lists, ls.v.data(), ls.v.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+    enqueue_steps(e, ls, gr.d_lists, nsteps);
+    hipError_t r = hipStreamEndCapture(e->s, &gr.graph);
+    if (r != hipSuccess) {
+        hipFree(gr.d_lists);
+        return fail(e, CGO_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(r));
+    }
+    r = hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0);
+    if (r != hipSuccess) {
+        hipGraphDestroy(gr.graph);
+        hipFree(gr.d_lists);
+        return fail(e, CGO_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(r));
+    }
+    int slot = -1;
+    for (size_t i = 0; i < e->graphs.size(); i++)
+        if (!e->graphs[i].exec) slot = (int)i;
+    if (slot < 0) {
+        e->graphs.push_back(gr);
+        slot = (int)e->graphs.size() - 1;
+    } else {
+        e->graphs[slot] = gr;
+    }
+    *graph_id = slot;
+    return CGO_OK;
+}
+
+int cgo_graph_launch(cgo_engine* e, int32_t gid, int32_t nreps) {
+    if (!e || gid < 0 || gid >= (int)e->graphs.size() || !e->graphs[gid].exec) return fail(e, CGO_E_ARG, "graph_launch: bad graph");
+    if (e->table_dirty) return fail(e, CGO_E_STATE, "graph_launch: members changed since capture; capture again");
+    for (int r = 0; r < nreps; r++) HIPCHK(e, hipGraphLaunch(e->graphs[gid].exec, e->s));
+    return CGO_OK;
+}
+
+int cgo_graph_destroy(cgo_engine* e, int32_t gid) {
+    if (!e || gid < 0 || gid >= (int)e->graphs.size() || !e->graphs[gid].exec) return fail(e, CGO_E_ARG, "graph_destroy: bad graph");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    Graph& g = e->graphs[gid];
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+    hipFree(g.d_lists);
+    g = Graph();
+    return CGO_OK;
+}
+
+int cgo_set_tile(cgo_engine* e, int32_t precision, int32_t tb) {
+    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (tb != 2 && tb != 4 && tb != 8))
+        return fail(e, CGO_E_ARG, "set_tile: precision %d tracer block %d", precision, tb);
+    e->lb[precision == CGO_FP64 ? 0 : 1] = tb;
+    return CGO_OK;
+}
+
+int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
+    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 2) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
+    Lists ls;
+    int rc = make_lists(e, ids, n, ls);
+    if (rc) return rc;
+    if ((rc = sync_table(e))) return rc;
+    if ((rc = upload_lists(e, ls))) return rc;
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipEventRecord(e->ev0, e->s));
+    for (int r = 0; r < reps; r++)
+        for (int p = 0; p < 2; p++) {
+            if (!ls.n[p]) continue;
+            const int32_t* l = e->d_list + ls.off[p];
+            if (which == 0) {
+                if (p == 0)
+                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[0], e->s);
+                else
+                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[1], e->s);
+            } else if (which == 1) {
+                launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+            } else if (ls.n[2 + p]) {
+                launch_bgc(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], p, e->s);
+            }
+        }
+    HIPCHK(e, hipEventRecord(e->ev1, e->s));
+    HIPCHK(e, hipEventSynchronize(e->ev1));
+    float ms = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    *avg_ms = (double)ms / reps;
+    return CGO_OK;
+}
+
+}  // extern "C"

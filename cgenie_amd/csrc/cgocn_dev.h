@@ -1,0 +1,105 @@
+// cgocn_dev.h -- device-side data model of the MI355X ocean tracer engine.
+//
+// Layout in HBM (per member, i fastest, no halos):
+//   state  [L][maxk][maxj][maxi]   (double, or float + int16 carry plane)
+//   u      [3][maxk][maxj][maxi]   u(n,i,j,k) of goldstein.f90:684 (interior)
+//   sflux  [L][maxj][maxi]         ts(l,i,j,maxk+1), goldstein.f90:167-170
+//   rho    [maxk][maxj][maxi]
+// Per grid: k1 with its reference halo [(maxj+2)][(maxi+2)] (land rows at
+// j = 0 and maxj+1, periodic copies at i = 0 and maxi+1, goldstein.f90:1114-1123)
+// and per-j / per-k metric tables.  Fortran indices are used in the kernels
+// where they index the metric tables (I = i0 + 1, J = j0 + 1, k = kz + 1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cgo {
+
+// per-j metric table rows, each of length maxj+2 (index = Fortran j)
+enum { MJ_RC = 0, MJ_RC2, MJ_CV, MJ_CV2, MJ_RDS, MJ_DSVM, MJ_RDSV, MJ_DS, MJ_SV, NMJ };
+// per-k metric table rows, each of length maxk+2 (index = Fortran k)
+enum { MK_DT = 0, MK_RDZ, MK_RDZA, MK_DZA, MK_SSMAX, MK_DZ, MK_DBOT, MK_DMID, MK_DD, NMK };
+
+// BIOGEM sediment (particulate) slots in the reference's is order
+enum { S_POC = 0, S_POP, S_CACO3, S_POCFR2, S_CACO3FR2, NSED };
+// carb descriptors (gem_cmn.f90:268-277, 0-based)
+enum { IC_H = 0, IC_FUG_CO2, IC_CONC_CO2, IC_CONC_CO3, IC_CONC_HCO3, IC_OHM_CAL, IC_OHM_ARG,
+       IC_DCO3_CAL, IC_DCO3_ARG, IC_RF0, NCARB };
+// the BGC species the engine carries, in the reference's io order
+// (io_DIC=3, io_PO4=8, io_O2=10, io_ALK=12, io_DOM_C=15, io_DOM_P=20)
+enum { B_DIC = 0, B_PO4, B_O2, B_ALK, B_DOMC, B_DOMP, NBGC };
+
+constexpr int MAXK_DEV = 48;      // deepest grid the column kernels accept
+constexpr double NULLSMALL = +0.999999E-19;  // gem_cmn.f90:719
+constexpr double ZEROC = 273.15;
+constexpr double SALN0 = 34.9;
+
+struct GridDev {
+    int maxi, maxj, maxk, ncol, ncell, nwet;
+    const int32_t* k1;      // [(maxj+2)*(maxi+2)]
+    const double* mj;       // [NMJ*(maxj+2)]
+    const double* mk;       // [NMK*(maxk+2)]
+    const float* mjf;       // float copies for the FP32 form
+    const float* mkf;
+    const int32_t* wetcols; // [nwet] column ids i0 + maxi*j0, reference vector order (i outer, j inner)
+    const double* V;        // [ncell] cell volumes (biogem_data.f90:1128)
+    double dphi, rdphi, rVtot;
+    double ec[5];
+};
+
+struct BgcParams {
+    int32_t slot[NBGC];     // compact tracer slot of each species, -1 if absent
+    int32_t ocn[NBGC];      // plane of each species in the ocn block, -1 if absent
+    int32_t caco3;
+    double dtyr, k0_PO4, c0_PO4, DOMfrac, DOMlifetime;
+    double red_POP_POC, red_POP_PO2, red_PON_ALK, red_POP_PON, red_POC_CaCO3, red_POC_CaCO3_pP;
+    double POC_frac2, POC_dfrac2, POC_c0frac2, POC_eL1, POC_eL2;
+    double CaCO3_frac2, CaCO3_eL1, CaCO3_eL2;
+    double sinkrate, remin_k_O2, remin_c0_O2, carb_tol, dsc;
+    int32_t carb_maxit;
+};
+
+struct MemberDev {
+    GridDev g;
+    int32_t L, prec, diso, bgc;
+    double diff1, diff2;
+    float diff1f, diff2f;
+    void* st[2];            // ping-pong state buffers
+    int16_t* lo[2];         // FP32 form: carry planes (fraction of an ulp / 2^16), or null
+    const void* u;          // velocity (double, or float in the FP32 form)
+    const double* sflux;    // [L][ncol] or null
+    double* rho;            // [ncell]
+    double* cost;           // [ncol]
+    int32_t* parity;        // device scalar: which st[] is current
+    int64_t* steps;         // device scalar: completed steps
+    // BIOGEM
+    BgcParams bp;
+    double* ocn;            // [2 + nbgc_present][ncell]: T(K), S, BGC species (absolute)
+    double* part;           // [NSED][ncell]
+    double* settle;         // [NSED][ncol] bio_settle at k1
+    double* carb;           // [NCARB][ncol]
+    double* M;              // [ncell]
+    double* rM;             // [ncell]
+    double* vd;             // [NBGC][ncell] anomalies of the present species
+    double* light;          // [ncol]
+    double* mld;            // [ncol]
+    double* colsum;         // [(L+1)][nwet] column partial sums (NEW_l, S_new)
+    double* colold;         // [L][nwet] column partial sums of OLD_l for the next coupling
+    double* scal;           // member scalars, see SC_*
+    int32_t* diag;          // [4] limps, carb_fail, carb_iters, spare
+    double* dmax;           // device scalar
+};
+
+// member scalar slots (scal[])
+enum { SC_SMEAN = 0, SC_SNEW, SC_SRATIO, SC_RSRATIO, SC_FACT0 };  // SC_FACT0 + l : OLD_l * rNEW_l
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int k1at(const GridDev& g, int I, int J) { return g.k1[J * (g.maxi + 2) + I]; }
+
+// eos, goldstein.f90:3048-3061 (ieos = 0)
+template <typename R>
+__device__ __forceinline__ R eos(R e0, R e1, R e2, R e3, R t, R s) {
+    return e0 * t + e1 * s + e2 * (t * t) + e3 * (t * t * t);
+}
+
+}  // namespace cgo

@@ -1,0 +1,17 @@
+// k_tstep_f32.hip -- FP32 instantiation of the flux kernel (the FP32 form:
+// FP32 face fluxes, the update accumulated with its rounding residual carried).
+#include "tstep_impl.h"
+
+namespace cgo {
+
+void launch_flux32(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int lb, hipStream_t s) {
+    const dim3 grid((maxcol + FLUX_THREADS - 1) / FLUX_THREADS, (maxL + lb - 1) / lb, n);
+    if (lb == 2)
+        hipLaunchKernelGGL((k_flux<float, 2>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+    else if (lb == 8)
+        hipLaunchKernelGGL((k_flux<float, 8>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+    else
+        hipLaunchKernelGGL((k_flux<float, 4>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+}
+
+}  // namespace cgo

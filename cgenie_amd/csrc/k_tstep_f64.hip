@@ -1,0 +1,65 @@
+// k_tstep_f64.hip -- FP64 instantiations of the GOLDSTEIN tracer-step
+// kernels, the end-of-step flip and the inventory reduction.  Built with
+// -ffp-contract=off (see tstep_impl.h).
+#define CGO_DEFINE_FLIP
+#include "tstep_impl.h"
+
+namespace cgo {
+
+void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int lb, hipStream_t s) {
+    const dim3 grid((maxcol + FLUX_THREADS - 1) / FLUX_THREADS, (maxL + lb - 1) / lb, n);
+    if (lb == 2)
+        hipLaunchKernelGGL((k_flux<double, 2>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+    else if (lb == 8)
+        hipLaunchKernelGGL((k_flux<double, 8>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+    else
+        hipLaunchKernelGGL((k_flux<double, 4>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+}
+
+void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int prec, hipStream_t s) {
+    const dim3 grid((nwetmax + 63) / 64, n);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_co<double>, grid, dim3(64), 0, s, t, l);
+    else
+        hipLaunchKernelGGL(k_co<float>, grid, dim3(64), 0, s, t, l);
+}
+
+void launch_flip(const MemberDev* t, const int32_t* l, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_flip, dim3((n + 63) / 64), dim3(64), 0, s, t, l, n);
+}
+
+// per-tracer volume-weighted sum over the member's current state
+template <typename R>
+__global__ __launch_bounds__(256) void k_inventory(const MemberDev* __restrict__ mems, int32_t m, double* out,
+                                                   const double* __restrict__ w) {
+    const MemberDev& M = mems[m];
+    const int l = blockIdx.x;
+    const int par = *M.parity;
+    const size_t ncell = M.g.ncell;
+    const size_t base = (size_t)l * ncell;
+    double s = 0.0;
+    for (size_t c = threadIdx.x; c < ncell; c += 256) {
+        const double wc = w[c];
+        if (wc != 0.0) s += StateIO<R>::ld(M.st[par], M.lo[par], base + c) * wc;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[l] = red[0];
+}
+
+void launch_inventory(const MemberDev* t, int32_t m, double* out, const double* w, int prec, hipStream_t s) {
+    // one block per tracer; the member's L bounds the grid via the caller's buffer
+    MemberDev h;
+    hipMemcpy(&h, t + m, sizeof(MemberDev), hipMemcpyDeviceToHost);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_inventory<double>, dim3(h.L), dim3(256), 0, s, t, m, out, w);
+    else
+        hipLaunchKernelGGL(k_inventory<float>, dim3(h.L), dim3(256), 0, s, t, m, out, w);
+}
+
+}  // namespace cgo

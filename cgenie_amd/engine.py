@@ -1,0 +1,264 @@
+"""Python binding of libcgocn.so (the C ABI in include/cgocn.h).
+
+A thin ctypes layer: arrays cross the boundary in the reference's Fortran
+layouts (see cgenie_amd/grid.py), exactly as a Fortran caller would pass
+them.  There is no fallback: if the HIP library is missing or no GPU is
+visible the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import grid as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcgocn.so")
+
+FP64, FP32C = 0, 1
+
+_d, _i32, _i64 = C.c_double, C.c_int32, C.c_int64
+_dp = C.POINTER(C.c_double)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("maxi", _i32), ("maxj", _i32), ("maxk", _i32), ("k1", C.POINTER(_i32)), ("dphi", _d), ("rdphi", _d)] + \
+        [(n, _dp) for n in ("rc", "rc2", "cv", "cv2", "rds", "dsv", "rdsv", "ds", "sv",
+                            "dt", "rdz", "rdza", "dza", "ssmax", "dz")] + \
+        [("ec", _d * 5), ("dsc", _d)]
+
+
+class MemberParams(C.Structure):
+    _fields_ = [("diff", _d * 2), ("diso", _i32), ("carry", _i32)]
+
+
+class BgcParams(C.Structure):
+    _fields_ = [("slot_DIC", _i32), ("slot_ALK", _i32), ("slot_PO4", _i32), ("slot_O2", _i32),
+                ("slot_DOM_C", _i32), ("slot_DOM_P", _i32), ("caco3", _i32)] + \
+        [(n, _d) for n in ("dtyr", "k0_PO4", "c0_PO4", "DOMfrac", "DOMlifetime", "red_POP_POC", "red_POP_PO2",
+                           "red_PON_ALK", "red_POP_PON", "red_POC_CaCO3", "red_POC_CaCO3_pP", "POC_frac2",
+                           "POC_dfrac2", "POC_c0frac2", "POC_eL1", "POC_eL2", "CaCO3_frac2", "CaCO3_eL1",
+                           "CaCO3_eL2", "sinkrate", "remin_k_O2", "remin_c0_O2", "carb_tol", "dsc")] + \
+        [("carb_maxit", _i32)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("steps", _i64), ("limps", _i32), ("carb_fail", _i32), ("carb_iters", _i64), ("dmax", _d),
+                ("cost", _d), ("smean", _d)]
+
+
+class CgoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP engine; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CgoError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    ip = C.POINTER(_i32)
+    sig = {
+        "cgo_create": ([_i32, C.POINTER(P)], C.c_int),
+        "cgo_destroy": ([P], C.c_int),
+        "cgo_last_error": ([P, C.c_char_p, C.c_size_t], C.c_int),
+        "cgo_sync": ([P], C.c_int),
+        "cgo_grid_create": ([P, C.POINTER(GridDesc), ip], C.c_int),
+        "cgo_member_create": ([P, _i32, _i32, _i32, C.POINTER(MemberParams), ip], C.c_int),
+        "cgo_member_destroy": ([P, _i32], C.c_int),
+        "cgo_member_bytes": ([P, _i32, _i32, _i32, _i32], _i64),
+        "cgo_set_velocity": ([P, _i32, _dp], C.c_int),
+        "cgo_set_sflux": ([P, _i32, _dp], C.c_int),
+        "cgo_upload_state": ([P, _i32, _dp], C.c_int),
+        "cgo_download_state": ([P, _i32, _dp], C.c_int),
+        "cgo_download_rho": ([P, _i32, _dp], C.c_int),
+        "cgo_inventory": ([P, _i32, _dp], C.c_int),
+        "cgo_get_diag": ([P, _i32, C.POINTER(Diag)], C.c_int),
+        "cgo_bgc_default_params": ([C.POINTER(BgcParams), _d], None),
+        "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
+        "cgo_bgc_download": ([P, _i32, _i32, _dp], C.c_int),
+        "cgo_step": ([P, ip, _i32, _i32], C.c_int),
+        "cgo_graph_capture": ([P, ip, _i32, _i32, ip], C.c_int),
+        "cgo_graph_launch": ([P, _i32, _i32], C.c_int),
+        "cgo_graph_destroy": ([P, _i32], C.c_int),
+        "cgo_set_tile": ([P, _i32, _i32], C.c_int),
+        "cgo_time_kernel": ([P, _i32, ip, _i32, _i32, _dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _ids(members) -> tuple:
+    arr = (_i32 * len(members))(*members)
+    return arr, len(members)
+
+
+def default_bgc_params(dtyr: float, ts: G.TracerSet | None = None, **kw) -> BgcParams:
+    p = BgcParams()
+    load().cgo_bgc_default_params(C.byref(p), dtyr)
+    if ts is not None:
+        p.slot_DIC, p.slot_ALK, p.slot_PO4, p.slot_O2 = ts.DIC, ts.ALK, ts.PO4, ts.O2
+        p.slot_DOM_C, p.slot_DOM_P = ts.DOM_C, ts.DOM_P
+        p.caco3 = 1 if (ts.DIC >= 0 and ts.ALK >= 0) else 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Engine:
+    """One device context (stream, grids, members)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        self.h = C.c_void_p()
+        rc = self.L.cgo_create(device, C.byref(self.h))
+        if rc != 0:
+            raise CgoError(f"cgo_create({device}) failed ({rc}): no usable HIP device")
+        self._keep = []
+        self.members = {}
+
+    def close(self):
+        if self.h:
+            self.L.cgo_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int, what: str):
+        if rc != 0:
+            buf = C.create_string_buffer(512)
+            self.L.cgo_last_error(self.h, buf, 512)
+            raise CgoError(f"{what}: rc={rc}: {buf.value.decode()}")
+
+    # -- grids and members
+    def add_grid(self, g: G.Grid, k1: np.ndarray) -> int:
+        k1 = np.ascontiguousarray(k1, dtype=np.int32)
+        keep = [k1]
+        d = GridDesc()
+        d.maxi, d.maxj, d.maxk = g.maxi, g.maxj, g.maxk
+        d.k1 = k1.ctypes.data_as(C.POINTER(_i32))
+        d.dphi, d.rdphi = g.dphi, g.rdphi
+        for n in ("rc", "rc2", "cv", "cv2", "rds", "dsv", "rdsv", "ds", "sv", "dt", "rdz", "rdza", "dza", "ssmax", "dz"):
+            a = np.ascontiguousarray(getattr(g, n), dtype=np.float64)
+            keep.append(a)
+            setattr(d, n, _ptr(a))
+        for q in range(5):
+            d.ec[q] = float(g.ec[q])
+        d.dsc = G.DSC
+        gid = _i32()
+        self._chk(self.L.cgo_grid_create(self.h, C.byref(d), C.byref(gid)), "grid_create")
+        self._keep.append(keep)
+        return gid.value
+
+    def add_member(self, grid_id: int, L: int, precision: int = FP64, diff=(None, None), diso: bool = True,
+                   carry: bool = True) -> int:
+        p = MemberParams()
+        p.diff[0], p.diff[1] = diff
+        p.diso = 1 if diso else 0
+        p.carry = 1 if carry else 0
+        mid = _i32()
+        self._chk(self.L.cgo_member_create(self.h, grid_id, L, precision, C.byref(p), C.byref(mid)), "member_create")
+        self.members[mid.value] = L
+        return mid.value
+
+    def remove_member(self, m: int):
+        self._chk(self.L.cgo_member_destroy(self.h, m), "member_destroy")
+        self.members.pop(m, None)
+
+    def member_bytes(self, grid_id: int, L: int, precision: int, bgc: bool) -> int:
+        return int(self.L.cgo_member_bytes(self.h, grid_id, L, precision, 1 if bgc else 0))
+
+    # -- data
+    def set_velocity(self, m: int, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(self.L.cgo_set_velocity(self.h, m, _ptr(u)), "set_velocity")
+
+    def set_sflux(self, m: int, sflux: np.ndarray | None):
+        if sflux is None:
+            self._chk(self.L.cgo_set_sflux(self.h, m, None), "set_sflux")
+        else:
+            s = np.ascontiguousarray(sflux, dtype=np.float64)
+            self._chk(self.L.cgo_set_sflux(self.h, m, _ptr(s)), "set_sflux")
+
+    def upload(self, m: int, go_ts: np.ndarray):
+        a = np.ascontiguousarray(go_ts, dtype=np.float64)
+        self._chk(self.L.cgo_upload_state(self.h, m, _ptr(a)), "upload_state")
+
+    def download(self, m: int, shape) -> np.ndarray:
+        out = np.zeros(shape)
+        self._chk(self.L.cgo_download_state(self.h, m, _ptr(out)), "download_state")
+        return out
+
+    def rho(self, m: int, g: G.Grid) -> np.ndarray:
+        out = np.zeros((g.maxk, g.maxj, g.maxi))
+        self._chk(self.L.cgo_download_rho(self.h, m, _ptr(out)), "download_rho")
+        return out
+
+    def inventory(self, m: int) -> np.ndarray:
+        out = np.zeros(self.members[m])
+        self._chk(self.L.cgo_inventory(self.h, m, _ptr(out)), "inventory")
+        return out
+
+    def diag(self, m: int) -> Diag:
+        d = Diag()
+        self._chk(self.L.cgo_get_diag(self.h, m, C.byref(d)), "get_diag")
+        return d
+
+    def bgc_init(self, m: int, p: BgcParams, light: np.ndarray, mld: np.ndarray):
+        lt = np.ascontiguousarray(light, dtype=np.float64)
+        ml = np.ascontiguousarray(mld, dtype=np.float64)
+        self._chk(self.L.cgo_bgc_init(self.h, m, C.byref(p), _ptr(lt), _ptr(ml)), "bgc_init")
+
+    def bgc_download(self, m: int, what: int, shape) -> np.ndarray:
+        out = np.zeros(shape)
+        self._chk(self.L.cgo_bgc_download(self.h, m, what, _ptr(out)), "bgc_download")
+        return out
+
+    # -- stepping
+    def step(self, members, nsteps: int = 1):
+        arr, n = _ids(members)
+        self._chk(self.L.cgo_step(self.h, arr, n, nsteps), "step")
+
+    def capture(self, members, nsteps: int) -> int:
+        arr, n = _ids(members)
+        gid = _i32()
+        self._chk(self.L.cgo_graph_capture(self.h, arr, n, nsteps, C.byref(gid)), "graph_capture")
+        return gid.value
+
+    def launch(self, graph: int, nreps: int = 1):
+        self._chk(self.L.cgo_graph_launch(self.h, graph, nreps), "graph_launch")
+
+    def drop_graph(self, graph: int):
+        self._chk(self.L.cgo_graph_destroy(self.h, graph), "graph_destroy")
+
+    def sync(self):
+        self._chk(self.L.cgo_sync(self.h), "sync")
+
+    def set_tile(self, precision: int, tracer_block: int):
+        self._chk(self.L.cgo_set_tile(self.h, precision, tracer_block), "set_tile")
+
+    def time_kernel(self, which: int, members, reps: int) -> float:
+        arr, n = _ids(members)
+        ms = _d()
+        self._chk(self.L.cgo_time_kernel(self.h, which, arr, n, reps, C.byref(ms)), "time_kernel")
+        return ms.value

@@ -1,0 +1,138 @@
+/*
+ * cgocn.h -- C ABI of the MI355X ocean tracer engine (libcgocn.so).
+ *
+ * The engine replaces the tracer half of one cGENIE ocean step, in the
+ * coupling order of genie.f90:306-313 (step_goldstein) and 360-381
+ * (step_biogem, biogem_tracercoupling):
+ *
+ *   tstepo            goldstein.f90:2280-2432  -> cgo_step (GOLDSTEIN part)
+ *   tstepo_flux, eos  goldstein.f90:2436-2642, 3048-3082
+ *   co                goldstein.f90:2657-2777
+ *   step_biogem       biogem.f90:528-1877      -> cgo_step (BIOGEM part), the
+ *                     1N1T_PO4MM biology, DOM/particulate remineralisation and
+ *                     surface carbonate chemistry of the species carried
+ *   biogem_tracercoupling  biogem.f90:1885-2077 -> cgo_step (coupling part)
+ *
+ * The reference exposes these as Fortran module subroutines on module state
+ * (goldstein_lib.f90:1-180, biogem_lib.f90); there is no C interface
+ * upstream.  Host arrays cross this boundary only in the reference's own
+ * Fortran layouts (column-major, the lower bounds of its ALLOCATE
+ * statements), so a Fortran caller passes its arrays unchanged through
+ * iso_c_binding.  Every call returns 0 on success or a negative CGO_E_*
+ * code; cgo_last_error() holds the message.  A handle is not re-entrant:
+ * use one per host thread.  Device memory is owned by the handle.
+ */
+#ifndef CGOCN_H
+#define CGOCN_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGO_OK 0
+#define CGO_E_ARG (-1)
+#define CGO_E_HIP (-2)
+#define CGO_E_STATE (-3)
+#define CGO_E_NOMEM (-4)
+
+#define CGO_FP64 0      /* plain FP64 form */
+#define CGO_FP32C 1     /* FP32 advection-diffusion, compensated; FP64 co, chemistry, sums */
+
+typedef struct cgo_engine cgo_engine;
+
+/* Grid metrics as the reference computes them in initialise_goldstein
+ * (goldstein.f90:906-1075).  Per-j arrays have maxj+2 entries indexed by the
+ * Fortran j (0..maxj+1), per-k arrays maxk+2 entries indexed by the Fortran
+ * k (0..maxk+1); entries the reference leaves undefined may hold anything.
+ * k1 is k1(0:maxi+1, 0:maxj+1) of goldstein_lib.f90:9 (land where k1 > maxk). */
+typedef struct cgo_grid_desc {
+    int32_t maxi, maxj, maxk;
+    const int32_t *k1;
+    double dphi, rdphi;
+    const double *rc, *rc2, *cv, *cv2, *rds, *dsv, *rdsv, *ds, *sv;
+    const double *dt, *rdz, *rdza, *dza, *ssmax, *dz;
+    double ec[5];
+    double dsc;     /* depth scale in m (goldstein_lib.f90:50), BIOGEM geometry */
+} cgo_grid_desc;
+
+typedef struct cgo_member_params {
+    double diff[2];   /* non-dimensional diff(1), diff(2) (goldstein.f90:1406-1407) */
+    int32_t diso;     /* isoneutral flux on (goldstein-defaults.nml:56) */
+    int32_t carry;    /* CGO_FP32C only: carry the rounding residual (int16 plane) */
+} cgo_member_params;
+
+/* BIOGEM parameters (biogem-defaults.nml:233-303, gem-defaults.nml:196-199)
+ * and the compact tracer slot (0-based, -1 = absent) of each species. */
+typedef struct cgo_bgc_params {
+    int32_t slot_DIC, slot_ALK, slot_PO4, slot_O2, slot_DOM_C, slot_DOM_P;
+    int32_t caco3;
+    double dtyr, k0_PO4, c0_PO4, DOMfrac, DOMlifetime;
+    double red_POP_POC, red_POP_PO2, red_PON_ALK, red_POP_PON, red_POC_CaCO3, red_POC_CaCO3_pP;
+    double POC_frac2, POC_dfrac2, POC_c0frac2, POC_eL1, POC_eL2;
+    double CaCO3_frac2, CaCO3_eL1, CaCO3_eL2;
+    double sinkrate, remin_k_O2, remin_c0_O2, carb_tol, dsc;
+    int32_t carb_maxit;
+} cgo_bgc_params;
+
+typedef struct cgo_diag {
+    int64_t steps;       /* completed ocean steps */
+    int32_t limps;       /* slope-limited points, summed (tstepo_flux 2596-2602) */
+    int32_t carb_fail;   /* failed carbonate solves (gem_carbchem.f90:431-478) */
+    int64_t carb_iters;  /* summed fixed-point iterations of successful solves */
+    double dmax;         /* max isoneutral vertical diffusion number (2605-2608) */
+    double cost;         /* summed convection count (co, 2759) */
+    double smean;        /* current mean salinity of the BIOGEM state */
+} cgo_diag;
+
+/* lifecycle */
+int cgo_create(int32_t device, cgo_engine **out);
+int cgo_destroy(cgo_engine *e);
+int cgo_last_error(const cgo_engine *e, char *buf, size_t n);
+int cgo_sync(cgo_engine *e);
+
+/* grids and members */
+int cgo_grid_create(cgo_engine *e, const cgo_grid_desc *g, int32_t *grid_id);
+int cgo_member_create(cgo_engine *e, int32_t grid_id, int32_t maxl, int32_t precision,
+                      const cgo_member_params *p, int32_t *member_id);
+int cgo_member_destroy(cgo_engine *e, int32_t member_id);
+int64_t cgo_member_bytes(cgo_engine *e, int32_t grid_id, int32_t maxl, int32_t precision, int32_t bgc);
+
+/* uploads / downloads in reference layouts */
+int cgo_set_velocity(cgo_engine *e, int32_t m, const double *u);        /* u(3,0:maxi,0:maxj,maxk) */
+int cgo_set_sflux(cgo_engine *e, int32_t m, const double *sflux);       /* ts(:,1:maxi,1:maxj,maxk+1) or NULL */
+int cgo_upload_state(cgo_engine *e, int32_t m, const double *go_ts);    /* go_ts(maxl,maxi,maxj,maxk) */
+int cgo_download_state(cgo_engine *e, int32_t m, double *go_ts);
+int cgo_download_rho(cgo_engine *e, int32_t m, double *rho);            /* rho(1:maxi,1:maxj,1:maxk) */
+int cgo_inventory(cgo_engine *e, int32_t m, double *inv);               /* maxl, diag2 weights */
+int cgo_get_diag(cgo_engine *e, int32_t m, cgo_diag *d);
+
+/* BIOGEM: state from the current go_ts (ocn = ts*S/Smean, T+273.15,
+ * S+34.9), sub_init_phys_ocn geometry, [H] start 10**(-7.8).
+ * light = solfor/solar_constant (maxi,maxj); mld in m (maxi,maxj). */
+void cgo_bgc_default_params(cgo_bgc_params *p, double dtyr);
+int cgo_bgc_init(cgo_engine *e, int32_t m, const cgo_bgc_params *p, const double *light, const double *mld);
+/* what: 0 = ocn (maxl,maxi,maxj,maxk: T, S and the carried species, other
+ * slots zero), 1 = bio_part (5,maxi,maxj,maxk), 2 = carb (10,maxi,maxj),
+ * 3 = last anomaly vdocn (maxl,maxi,maxj,maxk) */
+int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
+
+/* stepping: nsteps ocean steps of every listed member, batched into one
+ * launch per kernel (members with BIOGEM run the full coupling order) */
+int cgo_step(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
+/* the same work captured once as a HIP graph and replayed */
+int cgo_graph_capture(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps, int32_t *graph_id);
+int cgo_graph_launch(cgo_engine *e, int32_t graph_id, int32_t nreps);
+int cgo_graph_destroy(cgo_engine *e, int32_t graph_id);
+
+/* tuning and measurement */
+int cgo_set_tile(cgo_engine *e, int32_t precision, int32_t tracer_block);   /* 2, 4 or 8 */
+/* average device time (ms) of one launch of kernel `which` (0 = flux,
+ * 1 = co, 2 = bgc column + coupling) over the listed members, from HIP events
+ * on the engine stream; the state is advanced by the timed launches */
+int cgo_time_kernel(cgo_engine *e, int32_t which, const int32_t *members, int32_t n, int32_t reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+/*
+ * cgo_oracle.h -- CPU restatement of the cGENIE ocean tracer timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine
+ * in cgenie_amd/.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product path never calls it.
+ *
+ * Every routine restates one reference routine with the reference's loop
+ * order, carried fluxes and operand order (file:line cited at each function
+ * in cgo_oracle.c).  All arrays use the reference's own Fortran layouts
+ * (column-major, lower bounds as in the reference ALLOCATE statements):
+ *
+ *   go_ts  (maxl, maxi, maxj, maxk)            goldstein.f90:31, 443
+ *   k1     (0:maxi+1, 0:maxj+1)  int32         goldstein_lib.f90:9, goldstein.f90:655
+ *   u      (3, 0:maxi, 0:maxj, maxk)           goldstein.f90:684
+ *   sflux  (maxl, maxi, maxj)  = ts(:,1:maxi,1:maxj,maxk+1)   goldstein.f90:167-170
+ *   rho    (0:maxi+1, 0:maxj+1, 0:maxk)        goldstein.f90:704
+ */
+#ifndef CGO_ORACLE_H
+#define CGO_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OG_MAXJ 256
+#define OG_MAXK 64
+
+typedef struct og_grid {
+    int maxi, maxj, maxk;
+    double dphi, rdphi;
+    /* horizontal metrics, index 0..maxj+1 (Fortran index == C index) */
+    double s[OG_MAXJ + 2], sv[OG_MAXJ + 2], c[OG_MAXJ + 2], cv[OG_MAXJ + 2];
+    double ds[OG_MAXJ + 2], rds[OG_MAXJ + 2], rc[OG_MAXJ + 2], rc2[OG_MAXJ + 2];
+    double dsv[OG_MAXJ + 2], rdsv[OG_MAXJ + 2], rcv[OG_MAXJ + 2], cv2[OG_MAXJ + 2];
+    double rds2[OG_MAXJ + 2];
+    /* vertical metrics, index 0..maxk+1 */
+    double dz[OG_MAXK + 2], dza[OG_MAXK + 2], rdz[OG_MAXK + 2], rdza[OG_MAXK + 2];
+    double zw[OG_MAXK + 2], zro[OG_MAXK + 2], dt[OG_MAXK + 2], ssmax[OG_MAXK + 2];
+    double ec[5];
+    double diff[2];   /* non-dimensional horizontal / vertical diffusivity */
+    double tsc, dtyr; /* time scale (s); ocean step in years */
+} og_grid;
+
+/* goldstein.f90:906-1075, 1406-1407, 2057-2070 */
+void og_grid_init(og_grid *g, int maxi, int maxj, int maxk, int nyear,
+                  double yearlen, double kappa_h, double kappa_v,
+                  double ssmaxsurf, double ssmaxdeep);
+
+/* One step_goldstein tracer step (goldstein.f90:149-187, 235, 443-444):
+ * builds halo'd ts/ts1 from go_ts, runs tstepo (flux, co, halos, ts1=ts) and
+ * writes the interior back.  rho is in/out (its wet entries are rewritten),
+ * cost is accumulated, limps/dmax are the tstepo_flux diagnostics. */
+void og_goldstein_step(const og_grid *g, int maxl, const int32_t *k1,
+                       double *go_ts, const double *u, const double *sflux,
+                       double *rho, double *cost, int32_t *mldpk, int diso,
+                       int32_t *limps, double *dmax);
+
+/* Per-tracer volume-weighted inventory over wet cells, the diag2 weight
+ * dphi*ds(j)*dz(k) (goldstein_diag.f90:39-41).  inv has maxl entries. */
+void og_inventory(const og_grid *g, int maxl, const int32_t *k1,
+                  const double *go_ts, double *inv);
+
+/* eos / eosd, goldstein.f90:3048-3082 (ieos = 0 form) */
+double og_eos(const double *ec, double t, double s);
+void og_eosd(const double *ec, double t1, double t2, double s1, double s2,
+             double rdz, double *dzrho, double *tec);
+
+/* ---------------- carbonate chemistry (gem_carbchem.f90) ---------------- */
+enum { OG_ICC_K = 0, OG_ICC_K1, OG_ICC_K2, OG_ICC_KB, OG_ICC_KW, OG_ICC_KSI,
+       OG_ICC_KHF, OG_ICC_KHSO4, OG_ICC_KP1, OG_ICC_KP2, OG_ICC_KP3,
+       OG_ICC_KCAL, OG_ICC_KARG, OG_ICC_QCO2, OG_ICC_QO2, OG_ICC_KH2S,
+       OG_ICC_KNH4, OG_N_CARBCONST };
+enum { OG_IC_H = 0, OG_IC_FUG_CO2, OG_IC_CONC_CO2, OG_IC_CONC_CO3,
+       OG_IC_CONC_HCO3, OG_IC_OHM_CAL, OG_IC_OHM_ARG, OG_IC_DCO3_CAL,
+       OG_IC_DCO3_ARG, OG_IC_RF0, OG_N_CARB };
+#define OG_N_CARBALK 13
+
+/* sub_calc_carbconst, Mehrbach default set (gem_carbchem.f90:59-268) */
+void og_calc_carbconst(double D, double T, double S, double *cc);
+/* salinity-derived totals (gem_carbchem.f90:834-890) */
+double og_calc_Ca(double S);
+double og_calc_Btot(double S);
+double og_calc_SO4tot(double S);
+double og_calc_Ftot(double S);
+/* sub_calc_carb fixed point (gem_carbchem.f90:326-546).  carb[OG_IC_H] is the
+ * warm start on entry.  Returns the iteration count (>0) or a negative code:
+ * -1 negative [H] estimate, -2 iteration limit.  On failure carb is left as on
+ * entry (the reference's random re-seed path, 468-476/527-536, is not taken). */
+int og_calc_carb(double DIC, double ALK, double Ca, double PO4, double SiO2,
+                 double Btot, double SO4tot, double Ftot, double H2S,
+                 double NH4, const double *cc, double *carb, double *carbalk,
+                 double tol, int maxit);
+/* sub_calc_carb_RF0 (gem_carbchem.f90:553-669); writes carb[OG_IC_RF0] */
+int og_calc_carb_RF0(double DIC, double ALK, double PO4, double SiO2,
+                     double Btot, double SO4tot, double Ftot, double H2S,
+                     double NH4, const double *cc, double *carb, double tol,
+                     int maxit);
+
+/* ---------------- BIOGEM (biogem.f90, biogem_box.f90) ---------------- */
+/* compact sediment-tracer slots, in the reference's is order
+ * (is_POC=3, is_POP=8, is_CaCO3=14, is_POC_frac2=33, is_CaCO3_frac2=34) */
+enum { OG_S_POC = 0, OG_S_POP, OG_S_CACO3, OG_S_POCFR2, OG_S_CACO3FR2, OG_NSED };
+
+typedef struct og_bgc_params {
+    /* compact ocean-tracer slot of each selected species, -1 if absent;
+     * slot 0 = T, slot 1 = S as in the GOLDSTEIN ts array */
+    int l_DIC, l_ALK, l_PO4, l_O2, l_DOMC, l_DOMP;
+    int caco3;            /* CaCO3 particulate selected (needs DIC and ALK) */
+    double dtyr;          /* BIOGEM step (yr) */
+    double k0_PO4, c0_PO4;                /* biogem-defaults.nml:235-237 */
+    double DOMfrac, DOMlifetime;          /* 0.66, 0.5 yr */
+    double red_POP_POC, red_POP_PO2, red_PON_ALK, red_POP_PON;
+    double red_POC_CaCO3, red_POC_CaCO3_pP;
+    double POC_frac2, POC_dfrac2, POC_c0frac2, POC_eL1, POC_eL2;
+    double CaCO3_frac2, CaCO3_eL1, CaCO3_eL2;
+    double sinkrate;      /* m/yr */
+    double remin_k_O2, remin_c0_O2;
+    double carb_tol;      /* par_carbchem_pH_tolerance */
+    int carb_maxit;
+    double dsc;           /* goldstein_dsc (m) */
+} og_bgc_params;
+
+/* Reference defaults (biogem-defaults.nml:233-303, gem-defaults.nml:196-199). */
+void og_bgc_default_params(og_bgc_params *p, double dtyr);
+
+typedef struct og_bgc_state {
+    int maxi, maxj, maxk, maxl;
+    /* ocn (maxl, maxi, maxj, maxk): T in K, S absolute, BGC tracers absolute */
+    double *ocn;
+    double *part;     /* (OG_NSED, maxi, maxj, maxk) bio_part */
+    double *settle;   /* (OG_NSED, maxi, maxj, maxk) bio_settle */
+    double *carb;     /* (OG_N_CARB, maxi, maxj) surface carb */
+    double *carbconst;/* (OG_N_CARBCONST, maxi, maxj) surface carbconst */
+    double *M, *rM, *V; /* (maxi, maxj, maxk) phys_ocn ipo_M/rM/V */
+    double Dbot[OG_MAXK + 2], Dmid[OG_MAXK + 2], dD[OG_MAXK + 2]; /* by k */
+    double *light;    /* (maxi, maxj) solfor/solar_constant */
+    double *mld;      /* (maxi, maxj) mixed-layer depth, m */
+    int32_t carb_fail; /* count of cells whose carbonate solve failed */
+    int32_t carb_iter; /* iterations of the last surface solve sweep (sum) */
+} og_bgc_state;
+
+/* sub_init_phys_ocn geometry (biogem_data.f90:1098-1140) and ocn from go_ts
+ * (ocn = ts*S/Smean for BGC and dye slots, T+273.15, S+34.9);
+ * carb H start 10**(-7.8) and a surface solve (biogem_data.f90:2336-2432). */
+void og_bgc_init(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
+                 int maxl, const int32_t *k1, const double *go_ts,
+                 const double *light, const double *mld);
+void og_bgc_free(og_bgc_state *st);
+
+/* step_biogem restricted to the selected species (biogem.f90:596-611,
+ * 873-969, 985-1105, 1693, 1811-1844).  Writes vdocn (maxl,maxi,maxj,maxk). */
+void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
+                    const int32_t *k1, double *vdocn);
+
+/* biogem_tracercoupling (biogem.f90:1885-2077); go_ts in/out */
+void og_tracercoupling(og_bgc_state *st, const og_grid *g, const int32_t *k1,
+                       double *go_ts, const double *vdocn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

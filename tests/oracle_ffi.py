@@ -1,0 +1,201 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py only.  It is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_carbchem.so")
+
+OG_MAXJ, OG_MAXK = 256, 64
+NSED, NCARB, NCC = 5, 10, 17
+
+_d = C.c_double
+_i = C.c_int
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class OgGrid(C.Structure):
+    J, K = OG_MAXJ + 2, OG_MAXK + 2
+    _fields_ = [("maxi", _i), ("maxj", _i), ("maxk", _i), ("dphi", _d), ("rdphi", _d)] + \
+        [(n, _d * (OG_MAXJ + 2)) for n in ("s", "sv", "c", "cv", "ds", "rds", "rc", "rc2", "dsv", "rdsv",
+                                           "rcv", "cv2", "rds2")] + \
+        [(n, _d * (OG_MAXK + 2)) for n in ("dz", "dza", "rdz", "rdza", "zw", "zro", "dt", "ssmax")] + \
+        [("ec", _d * 5), ("diff", _d * 2), ("tsc", _d), ("dtyr", _d)]
+
+
+class OgBgcParams(C.Structure):
+    _fields_ = [("l_DIC", _i), ("l_ALK", _i), ("l_PO4", _i), ("l_O2", _i), ("l_DOMC", _i), ("l_DOMP", _i),
+                ("caco3", _i), ("dtyr", _d), ("k0_PO4", _d), ("c0_PO4", _d), ("DOMfrac", _d),
+                ("DOMlifetime", _d), ("red_POP_POC", _d), ("red_POP_PO2", _d), ("red_PON_ALK", _d),
+                ("red_POP_PON", _d), ("red_POC_CaCO3", _d), ("red_POC_CaCO3_pP", _d), ("POC_frac2", _d),
+                ("POC_dfrac2", _d), ("POC_c0frac2", _d), ("POC_eL1", _d), ("POC_eL2", _d),
+                ("CaCO3_frac2", _d), ("CaCO3_eL1", _d), ("CaCO3_eL2", _d), ("sinkrate", _d),
+                ("remin_k_O2", _d), ("remin_c0_O2", _d), ("carb_tol", _d), ("carb_maxit", _i), ("dsc", _d)]
+
+
+class OgBgcState(C.Structure):
+    _fields_ = [("maxi", _i), ("maxj", _i), ("maxk", _i), ("maxl", _i)] + \
+        [(n, C.POINTER(_d)) for n in ("ocn", "part", "settle", "carb", "carbconst", "M", "rM", "V")] + \
+        [(n, _d * (OG_MAXK + 2)) for n in ("Dbot", "Dmid", "dD")] + \
+        [("light", C.POINTER(_d)), ("mld", C.POINTER(_d)), ("carb_fail", C.c_int32), ("carb_iter", C.c_int32)]
+
+
+def build(force: bool = False) -> None:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True, capture_output=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        L.og_grid_init.argtypes = [C.POINTER(OgGrid), _i, _i, _i, _i, _d, _d, _d, _d, _d]
+        L.og_goldstein_step.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp, C.c_void_p, _dp, _dp,
+                                        C.c_void_p, _i, C.POINTER(C.c_int32), C.POINTER(_d)]
+        L.og_inventory.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp]
+        L.og_calc_carbconst.argtypes = [_d, _d, _d, _dp]
+        L.og_calc_carb.argtypes = [_d] * 10 + [_dp, _dp, C.c_void_p, _d, _i]
+        L.og_calc_carb.restype = _i
+        L.og_calc_carb_RF0.argtypes = [_d] * 9 + [_dp, _dp, _d, _i]
+        L.og_calc_carb_RF0.restype = _i
+        for f in ("og_calc_Ca", "og_calc_Btot", "og_calc_SO4tot", "og_calc_Ftot"):
+            getattr(L, f).argtypes = [_d]
+            getattr(L, f).restype = _d
+        L.og_eos.argtypes = [_dp, _d, _d]
+        L.og_eos.restype = _d
+        L.og_bgc_default_params.argtypes = [C.POINTER(OgBgcParams), _d]
+        L.og_bgc_init.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i, _i32p,
+                                  _dp, _dp, _dp]
+        L.og_bgc_free.argtypes = [C.POINTER(OgBgcState)]
+        L.og_biogem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i32p, _dp]
+        L.og_tracercoupling.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), _i32p, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def grid(maxi=36, maxj=36, maxk=16, nyear=96, yearlen=365.25, kappa_h=2000.0, kappa_v=2.7e-5,
+         ssmaxsurf=10.0, ssmaxdeep=10.0) -> OgGrid:
+    g = OgGrid()
+    lib().og_grid_init(C.byref(g), maxi, maxj, maxk, nyear, yearlen, kappa_h, kappa_v, ssmaxsurf, ssmaxdeep)
+    return g
+
+
+def bgc_params(dtyr: float, ts=None, carb_tol=1e-12, carb_maxit=1000, **kw) -> OgBgcParams:
+    p = OgBgcParams()
+    lib().og_bgc_default_params(C.byref(p), dtyr)
+    if ts is not None:
+        p.l_DIC, p.l_ALK, p.l_PO4, p.l_O2 = ts.DIC, ts.ALK, ts.PO4, ts.O2
+        p.l_DOMC, p.l_DOMP = ts.DOM_C, ts.DOM_P
+        p.caco3 = 1 if (ts.DIC >= 0 and ts.ALK >= 0) else 0
+    p.carb_tol, p.carb_maxit = carb_tol, carb_maxit
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class GoldsteinOracle:
+    """Holds the arrays of one member and steps it with the C restatement."""
+
+    def __init__(self, g: OgGrid, k1: np.ndarray, go_ts: np.ndarray, u: np.ndarray,
+                 sflux: np.ndarray | None = None, diso: bool = True):
+        self.g, self.k1 = g, np.ascontiguousarray(k1, dtype=np.int32)
+        self.ts = np.ascontiguousarray(go_ts, dtype=np.float64).copy()
+        self.u = np.ascontiguousarray(u, dtype=np.float64)
+        self.L = self.ts.shape[-1]
+        self.sflux = None if sflux is None else np.ascontiguousarray(sflux, dtype=np.float64)
+        self.rho = np.zeros((g.maxk + 1, g.maxj + 2, g.maxi + 2))
+        self.cost = np.zeros((g.maxj, g.maxi))
+        self.diso = 1 if diso else 0
+        self.limps = C.c_int32(0)
+        self.dmax = _d(0.0)
+
+    def step(self, n: int = 1):
+        L = lib()
+        sf = None if self.sflux is None else self.sflux.ctypes.data_as(C.c_void_p)
+        for _ in range(n):
+            L.og_goldstein_step(C.byref(self.g), self.L, self.k1, self.ts, self.u, sf, self.rho, self.cost,
+                                None, self.diso, C.byref(self.limps), C.byref(self.dmax))
+
+    def inventory(self) -> np.ndarray:
+        inv = np.zeros(self.L)
+        lib().og_inventory(C.byref(self.g), self.L, self.k1, self.ts, inv)
+        return inv
+
+
+class BiogemOracle:
+    """BIOGEM state of one member plus the GOLDSTEIN oracle it couples to."""
+
+    def __init__(self, gold: GoldsteinOracle, p: OgBgcParams, light: np.ndarray, mld: np.ndarray):
+        self.gold, self.p = gold, p
+        self.st = OgBgcState()
+        self.light = np.ascontiguousarray(light, dtype=np.float64)
+        self.mld = np.ascontiguousarray(mld, dtype=np.float64)
+        lib().og_bgc_init(C.byref(self.st), C.byref(gold.g), C.byref(p), gold.L, gold.k1, gold.ts, self.light,
+                          self.mld)
+        g = gold.g
+        self.ncell = g.maxi * g.maxj * g.maxk
+        self.vdocn = np.zeros_like(gold.ts)
+
+    def __del__(self):
+        try:
+            lib().og_bgc_free(C.byref(self.st))
+        except Exception:
+            pass
+
+    def arr(self, name: str, per_cell: int = 1, cells: int | None = None) -> np.ndarray:
+        n = (self.ncell if cells is None else cells) * per_cell
+        return np.ctypeslib.as_array(getattr(self.st, name), shape=(n,))
+
+    def step(self, n: int = 1, goldstein: bool = True):
+        """genie.f90 order: step_goldstein, step_biogem, tracercoupling."""
+        L = lib()
+        for _ in range(n):
+            if goldstein:
+                self.gold.step(1)
+            L.og_biogem_step(C.byref(self.st), C.byref(self.gold.g), C.byref(self.p), self.gold.k1, self.vdocn)
+            L.og_tracercoupling(C.byref(self.st), C.byref(self.gold.g), self.gold.k1, self.gold.ts, self.vdocn)
+
+    def ocn(self) -> np.ndarray:
+        g = self.gold.g
+        return self.arr("ocn", self.gold.L).reshape(g.maxk, g.maxj, g.maxi, self.gold.L).copy()
+
+    def part(self) -> np.ndarray:
+        g = self.gold.g
+        return self.arr("part", NSED).reshape(g.maxk, g.maxj, g.maxi, NSED).copy()
+
+    def carb(self) -> np.ndarray:
+        g = self.gold.g
+        return self.arr("carb", NCARB, g.maxi * g.maxj).reshape(g.maxj, g.maxi, NCARB).copy()
+
+
+_ref = None
+
+
+def ref_carbchem():
+    """The reference's gem_carbchem compiled in place (oracle/_ref), or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        R = C.CDLL(REF_LIB)
+        R.ref_set_params.argtypes = [_d, _i]
+        R.ref_calc_carbconst.argtypes = [_d, _d, _d, _dp]
+        R.ref_salt_totals.argtypes = [_d, _dp]
+        R.ref_calc_carb.argtypes = [_d] * 10 + [_dp, _dp, _dp]
+        R.ref_calc_carb.restype = _i
+        R.ref_calc_carb_rf0.argtypes = [_d] * 9 + [_dp, _dp]
+        _ref = R
+    return _ref

@@ -1,0 +1,223 @@
+"""HIP engine vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bar: the FP64 path within 1e-12 per cell, relative, with a floor of 1e-3 of
+the tracer's largest magnitude for cells near zero (rel_err in cases.py);
+land masks and indexing exact (land cells are never written); inventories
+conserved to round-off.  The FP64 GOLDSTEIN step is also checked for bit
+identity with the oracle (both round every operation as written)."""
+import sys
+
+import numpy as np
+import pytest
+
+from cgenie_amd import engine as E
+from cgenie_amd import grid as G
+import oracle_ffi as O
+from cases import make_case, rel_err, wet
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def new_member(eng, c, prec=E.FP64, carry=True, diso=True):
+    gid = eng.add_grid(c.g, c.k1)
+    m = eng.add_member(gid, c.tset.L, prec, c.diff, diso=diso, carry=carry)
+    eng.set_velocity(m, c.u)
+    eng.set_sflux(m, c.sflux)
+    eng.upload(m, c.ts)
+    return m
+
+
+@pytest.mark.parametrize("dims,kind,L,random_u,sflux,diso,nsteps", [
+    ((36, 36, 16), "ts", 2, False, False, True, 96),      # config 1 shape, one model year
+    ((36, 36, 16), "ts", 5, True, True, True, 12),        # w, surface fluxes, odd tracer count
+    ((18, 18, 8), "ts", 7, True, False, False, 12),       # isoneutral off
+    ((72, 72, 32), "ts", 3, False, False, True, 3),       # the largest serving grid
+])
+def test_tstep_fp64_matches_oracle(eng, dims, kind, L, random_u, sflux, diso, nsteps):
+    say("case", dims, kind, L, random_u, sflux, diso, nsteps)
+    c = make_case(*dims, kind=kind, L=L, random_u=random_u, sflux=sflux)
+    m = new_member(eng, c, diso=diso)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u, c.sflux, diso=diso)
+    eng.step([m], nsteps)
+    gold.step(nsteps)
+    got = eng.download(m, c.ts.shape)
+    w = wet(c)
+    err = rel_err(got, gold.ts, w)
+    nexact = np.count_nonzero(got[w] == gold.ts[w])
+    say("  max rel err per tracer", err, "bit-identical cells", nexact, "of", got[w].size)
+    assert np.all(err <= 1e-12)
+    assert np.array_equal(got[~w], c.ts[~w])          # land untouched
+    rho = eng.rho(m, c.g)
+    rref = gold.rho[1:, 1:-1, 1:-1]
+    assert np.abs(rho[w] - rref[w]).max() <= 1e-12 * np.abs(rref[w]).max()
+    d = eng.diag(m)
+    assert d.steps == nsteps
+    assert abs(d.cost - gold.cost.sum()) < 0.5
+    eng.remove_member(m)
+
+
+def test_tstep_fp64_bit_identical_to_oracle(eng):
+    c = make_case(kind="ts", L=4, seed=5)
+    m = new_member(eng, c)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    eng.step([m], 8)
+    gold.step(8)
+    got = eng.download(m, c.ts.shape)
+    w = wet(c)
+    diff = np.count_nonzero(got[w] != gold.ts[w])
+    say("cells differing from the oracle:", diff)
+    assert diff == 0
+    eng.remove_member(m)
+
+
+def test_inventory_conserved_96_steps(eng):
+    c = make_case(kind="ts", L=6, seed=9)
+    m = new_member(eng, c)
+    i0 = eng.inventory(m)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    assert np.allclose(i0, gold.inventory(), rtol=1e-13, atol=0)
+    eng.step([m], 96)
+    i1 = eng.inventory(m)
+    say("inventory rel change", (i1 - i0) / np.abs(i0))
+    big = np.abs(i0) > 1e-3 * np.abs(i0).max()
+    assert np.all(np.abs(i1 - i0)[big] <= 1e-12 * np.abs(i0)[big])
+    eng.remove_member(m)
+
+
+@pytest.mark.parametrize("tb", [2, 4, 8])
+def test_tracer_block_shapes_agree(eng, tb):
+    c = make_case(kind="ts", L=11, seed=21)
+    eng.set_tile(E.FP64, tb)
+    m = new_member(eng, c)
+    eng.step([m], 5)
+    got = eng.download(m, c.ts.shape)
+    eng.set_tile(E.FP64, 4)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    gold.step(5)
+    w = wet(c)
+    assert np.count_nonzero(got[w] != gold.ts[w]) == 0
+    eng.remove_member(m)
+
+
+@pytest.mark.parametrize("kind,mld", [("config2", 0.0), ("domc", 150.0)])
+def test_biogem_fp64_matches_oracle(eng, kind, mld):
+    """GOLDSTEIN + BIOGEM + tracercoupling in the genie.f90 order, 24 tracers."""
+    say("bgc case", kind, mld)
+    c = make_case(kind=kind, seed=31, mld_max=mld)
+    m = new_member(eng, c)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    eng.bgc_init(m, p, c.light, c.mld)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset), c.light, c.mld)
+    n = 12
+    eng.step([m], n)
+    bio.step(n)
+    got = eng.download(m, c.ts.shape)
+    w = wet(c)
+    err = rel_err(got, gold.ts, w)
+    say("  ts max rel err", err.max(), "per tracer", err[:8])
+    assert np.all(err <= 1e-12)
+    ocn = eng.bgc_download(m, 0, c.ts.shape)
+    oref = bio.ocn()
+    sl = [0, 1] + [q for q in (c.tset.DIC, c.tset.ALK, c.tset.PO4, c.tset.O2, c.tset.DOM_C, c.tset.DOM_P) if q >= 0]
+    eo = rel_err(ocn[..., sl], oref[..., sl], w)
+    say("  ocn max rel err", eo)
+    assert np.all(eo <= 1e-12)
+    part = eng.bgc_download(m, 1, (c.g.maxk, c.g.maxj, c.g.maxi, 5))
+    pref = bio.part()
+    scale = np.abs(pref).max(axis=(0, 1, 2)) + 1e-300
+    ep = (np.abs(part - pref).max(axis=(0, 1, 2)) / scale)
+    say("  part err", ep)
+    assert np.all(ep <= 1e-12)
+    carb = eng.bgc_download(m, 2, (c.g.maxj, c.g.maxi, 10))
+    cref = bio.carb()
+    wc = c.k1[1:-1, 1:-1] <= c.g.maxk
+    ec = np.abs(carb[wc] - cref[wc]) / np.maximum(np.abs(cref[wc]), 1e-300)
+    say("  carb max rel err", ec.max(axis=0))
+    assert np.all(ec[:, :9] <= 1e-10)
+    d = eng.diag(m)
+    assert d.carb_fail == 0 and bio.st.carb_fail == 0
+    eng.remove_member(m)
+
+
+def test_batched_members_equal_alone(eng):
+    """Members of different grids, tracer counts and parameters stepped in one
+    launch per kernel give the bits each gives alone."""
+    cases = [make_case(kind="config2", seed=41), make_case(18, 18, 8, kind="ts", L=9, seed=42, kappa_h=800.0),
+             make_case(kind="domc", seed=43, kappa_v=8e-5, vscale=1.4)]
+    ms, alone = [], []
+    for q, c in enumerate(cases):
+        m = new_member(eng, c)
+        if q != 1:
+            eng.bgc_init(m, E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000), c.light, c.mld)
+        ms.append(m)
+    eng.step(ms, 6)
+    batched = [eng.download(m, c.ts.shape) for m, c in zip(ms, cases)]
+    for m in ms:
+        eng.remove_member(m)
+    for q, c in enumerate(cases):
+        m = new_member(eng, c)
+        if q != 1:
+            eng.bgc_init(m, E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000), c.light, c.mld)
+        eng.step([m], 6)
+        alone.append(eng.download(m, c.ts.shape))
+        eng.remove_member(m)
+    for a, b in zip(batched, alone):
+        assert np.array_equal(a, b)
+
+
+def test_graph_replay_equals_eager(eng):
+    """A captured year replays to the eager result; replays are separated by
+    eager work and synchronisation and every replay is checked."""
+    c = make_case(kind="config2", seed=51)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    m1 = new_member(eng, c)
+    eng.bgc_init(m1, p, c.light, c.mld)
+    m2 = new_member(eng, c)
+    eng.bgc_init(m2, p, c.light, c.mld)
+    gid = eng.capture([m1], 4)
+    for rep in range(3):
+        eng.launch(gid, 1)
+        eng.sync()
+        eng.step([m2], 4)
+        eng.sync()
+        a = eng.download(m1, c.ts.shape)
+        b = eng.download(m2, c.ts.shape)
+        say("replay", rep, "identical:", np.array_equal(a, b))
+        assert np.array_equal(a, b)
+    eng.drop_graph(gid)
+    eng.remove_member(m1)
+    eng.remove_member(m2)
+
+
+@pytest.mark.parametrize("carry", [True, False])
+def test_fp32_form_tracks_fp64(eng, carry):
+    c = make_case(kind="config2", seed=61)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    m64 = new_member(eng, c)
+    eng.bgc_init(m64, p, c.light, c.mld)
+    m32 = new_member(eng, c, prec=E.FP32C, carry=carry)
+    eng.bgc_init(m32, p, c.light, c.mld)
+    eng.step([m64, m32], 96)
+    a = eng.download(m32, c.ts.shape)
+    b = eng.download(m64, c.ts.shape)
+    err = rel_err(a, b, wet(c))
+    say("FP32 (carry=%s) vs FP64 after one year: max rel err" % carry, err.max(), err[:7])
+    assert np.all(err <= (1e-5 if carry else 1e-4))
+    i32, i64 = eng.inventory(m32), eng.inventory(m64)
+    big = np.abs(i64) > 1e-3 * np.abs(i64).max()
+    say("  inventory rel diff", np.abs(i32 - i64)[big].max() / np.abs(i64)[big].max())
+    eng.remove_member(m32)
+    eng.remove_member(m64)

@@ -1,0 +1,251 @@
+"""The oracle pinned: against the reference compiled in place (carbonate
+chemistry), against the reference's grid formulas, and against closed forms
+and invariants for the parts no compile of the reference can reach
+(goldstein.f90 needs netCDF, biogem.f90 its data files)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from cgenie_amd import grid as G
+import oracle_ffi as O
+from cases import make_case, wet
+
+
+def test_grid_matches_oracle_bitwise():
+    # cgenie_amd/grid.py and oracle og_grid_init both restate goldstein.f90:906-1075
+    for dims in [(36, 36, 16), (18, 18, 8), (72, 72, 32)]:
+        g, og = G.make_grid(*dims), O.grid(*dims)
+        J, K = dims[1] + 2, dims[2] + 2
+        for n in ("s", "sv", "c", "cv", "ds", "rds", "rc", "rc2", "dsv", "rdsv", "rcv", "cv2"):
+            assert np.array_equal(getattr(g, n), np.array(getattr(og, n)[:J])), n
+        for n in ("dz", "dza", "rdz", "rdza", "zw", "zro", "dt", "ssmax"):
+            assert np.array_equal(getattr(g, n), np.array(getattr(og, n)[:K])), n
+        assert g.dphi == og.dphi and list(g.ec[:4]) == list(og.ec)[:4]
+
+
+def test_grid_closed_forms():
+    # equal-sin(lat): ds(j) = 2/maxj; vertical levels sum to the full depth (dsc)
+    g = G.make_grid()
+    assert np.allclose(g.ds[1:37], 2.0 / 36, rtol=1e-13)
+    assert abs(g.dz[1:17].sum() - 1.0) < 1e-13          # -zw(0) = 1 (5000 m)
+    assert abs(g.zw[0] + 1.0) < 1e-13
+    # log-spaced: thickness ratio constant, dz(k)/dz(k+1) = z1/ez0 + 1
+    r = g.dz[1:16] / g.dz[2:17]
+    assert np.allclose(r, r[0], rtol=1e-12)
+    # ssmax constant when surf == deep (goldstein.f90:2058-2060)
+    assert np.all(g.ssmax[1:16] == 10.0)
+
+
+def _ref():
+    R = O.ref_carbchem()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree: make -C oracle ref)")
+    return R
+
+
+@pytest.mark.parametrize("tol,maxit", [(1e-3, 100), (1e-12, 1000)])
+def test_carbchem_matches_compiled_reference(tol, maxit):
+    """og_calc_carbconst/og_calc_carb/og_calc_carb_RF0 vs the reference's
+    gem_carbchem.f90 compiled with amdflang: bit-identical."""
+    R, L = _ref(), O.lib()
+    R.ref_set_params(tol, maxit)
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        D, T, S = rng.uniform(0, 5500), 273.15 + rng.uniform(-3, 38), rng.uniform(24, 45)
+        a, b = np.zeros(17), np.zeros(17)
+        L.og_calc_carbconst(D, T, S, a)
+        R.ref_calc_carbconst(D, T, S, b)
+        assert np.array_equal(a, b)
+        tot = np.zeros(4)
+        R.ref_salt_totals(S, tot)
+        assert [L.og_calc_Ca(S), L.og_calc_Btot(S), L.og_calc_SO4tot(S), L.og_calc_Ftot(S)] == list(tot)
+        DIC, ALK = rng.uniform(1.8e-3, 2.2e-3), rng.uniform(2.2e-3, 2.45e-3)
+        PO4 = rng.uniform(0, 3e-6)
+        c1 = np.zeros(10); c1[0] = 10 ** -7.8 * rng.uniform(0.5, 2)
+        c2 = c1.copy()
+        ca1, ca2 = np.zeros(13), np.zeros(13)
+        it = L.og_calc_carb(DIC, ALK, tot[0], PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, a, c1, ca1.ctypes.data,
+                            tol, maxit)
+        st = R.ref_calc_carb(DIC, ALK, tot[0], PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, b, c2, ca2)
+        assert (it > 0) == (st == 0)
+        assert np.array_equal(c1, c2) and np.array_equal(ca1, ca2)
+        L.og_calc_carb_RF0(DIC, ALK, PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, a, c1, tol, maxit)
+        R.ref_calc_carb_rf0(DIC, ALK, PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, b, c2)
+        assert c1[9] == c2[9]
+
+
+def test_carbonate_root_satisfies_alkalinity():
+    """At the fixed point H1 = H2, so the carbonate alkalinity quadratic gives
+    [HCO3]+2[CO3] = CA and HCO3^2/(CO2 CO3) = k1/k2 (independent check)."""
+    L = O.lib()
+    cc = np.zeros(17)
+    L.og_calc_carbconst(25.0, 288.15, 35.0, cc)
+    carb = np.zeros(10); carb[0] = 10 ** -7.8
+    it = L.og_calc_carb(2.0e-3, 2.3e-3, L.og_calc_Ca(35.0), 1e-6, 0, L.og_calc_Btot(35.0), L.og_calc_SO4tot(35.0),
+                        L.og_calc_Ftot(35.0), 0, 0, cc, carb, None, 1e-12, 1000)
+    assert it > 0
+    H, CO2, CO3, HCO3 = carb[0], carb[2], carb[3], carb[4]
+    assert abs(CO2 + CO3 + HCO3 - 2.0e-3) / 2.0e-3 < 1e-12               # DIC closure
+    assert abs(HCO3 * HCO3 / (CO2 * CO3) - cc[1] / cc[2]) / (cc[1] / cc[2]) < 1e-9
+    assert abs(cc[1] * CO2 / HCO3 - H) / H < 1e-9                          # H1 = H
+    assert 7.5 < -np.log10(H) < 8.5
+
+
+def test_carbconst_closed_form_mehrbach():
+    """k1 at the surface is the Mehrbach et al. (1973, refit by Dickson and
+    Millero 1987) pK1 = 3670.7/T - 62.008 + 9.7944 lnT - 0.0118 S + 0.000116 S^2."""
+    L = O.lib()
+    cc = np.zeros(17)
+    T, S = 298.15, 35.0
+    L.og_calc_carbconst(0.0, T, S, cc)
+    pk1 = 3670.7 / T - 62.008 + 9.7944 * np.log(T) - 0.0118 * S + 0.000116 * S * S
+    pk2 = 1394.7 / T + 4.777 - 0.0184 * S + 0.000118 * S * S
+    assert abs(-np.log10(cc[1]) - pk1) < 1e-12
+    assert abs(-np.log10(cc[2]) - pk2) < 1e-12
+    assert abs(cc[0] - cc[1] / cc[2]) / cc[0] < 1e-15
+
+
+def test_oracle_conserves_inventory():
+    c = make_case(kind="ts", L=4)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    i0 = gold.inventory()
+    gold.step(96)
+    i1 = gold.inventory()
+    big = np.abs(i0) > 1e-3 * np.abs(i0).max()
+    assert np.all(np.abs(i1 - i0)[big] / np.abs(i0)[big] < 1e-12)
+    assert gold.cost.sum() > 0   # the seeded noise is statically unstable somewhere: co ran
+
+
+def test_oracle_random_velocity_conserves():
+    # flux form: any velocity field (here with w and no solenoidal structure) conserves
+    c = make_case(kind="ts", L=3, random_u=True, maxi=18, maxj=18, maxk=8)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    i0 = gold.inventory()
+    gold.step(20)
+    i1 = gold.inventory()
+    assert np.all(np.abs(i1 - i0) <= 1e-12 * np.abs(i0).max())
+
+
+def test_uniform_tracer_stays_uniform():
+    c = make_case(kind="ts", L=3)
+    c.ts[..., 2][wet(c)] = 0.7
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    gold.step(10)
+    v = gold.ts[..., 2][wet(c)]
+    assert np.abs(v - 0.7).max() < 1e-13
+
+
+def test_vertical_diffusion_matches_textbook_column():
+    """One wet column walled by land: the step reduces to explicit vertical
+    diffusion F(k) = -kv (q(k+1)-q(k))/dza(k), q += -dt (F(k)-F(k-1))/dz(k)."""
+    g = G.make_grid(8, 8, 16)
+    og = O.grid(8, 8, 16, kappa_h=2000.0, kappa_v=1e-4)
+    k1 = np.full((10, 10), G.LAND_K1, dtype=np.int32)
+    k1[4, 4] = 3
+    u = np.zeros((16, 9, 9, 3))
+    ts = np.zeros((16, 8, 8, 3))
+    rng = np.random.default_rng(3)
+    col = np.zeros((16, 3))
+    col[2:, 0] = 2 + 18 * np.exp(-np.arange(14)[::-1] / 5)   # stable T profile (warm on top)
+    col[2:, 1] = 0.0
+    col[2:, 2] = rng.uniform(0, 1, 14)
+    ts[:, 3, 3, :] = col
+    gold = O.GoldsteinOracle(og, k1, ts, u)
+    gold.step(1)
+    kv = og.diff[1]
+    q = col[:, 2].copy()
+    F = np.zeros(17)                      # F[k] = flux through the top of level k (1-based)
+    for k in range(3, 16):
+        F[k] = -(q[k] - q[k - 1]) * g.rdza[k] * kv
+    new = q.copy()
+    for k in range(3, 17):
+        new[k - 1] = q[k - 1] - g.dt[k] * (F[k] - F[k - 1]) * g.rdz[k]
+    assert np.abs(gold.ts[2:, 3, 3, 2] - new[2:]).max() < 1e-14
+
+
+def test_convection_mixes_inversion_to_thickness_weighted_mean():
+    """Two-level column with dense water on top: co (goldstein.f90:2657-2777)
+    leaves both levels at the thickness-weighted mean of every tracer."""
+    g = G.make_grid(8, 8, 16)
+    og = O.grid(8, 8, 16, kappa_h=1e-20, kappa_v=1e-20)
+    k1 = np.full((10, 10), G.LAND_K1, dtype=np.int32)
+    k1[4, 4] = 15
+    u = np.zeros((16, 9, 9, 3))
+    ts = np.zeros((16, 8, 8, 3))
+    ts[14, 3, 3] = [10.0, 0.0, 0.25]   # level 15 (bottom): warm
+    ts[15, 3, 3] = [2.0, 0.5, 0.75]    # level 16 (top): cold and salty -> unstable
+    gold = O.GoldsteinOracle(og, k1, ts, u, diso=False)
+    gold.step(1)
+    w = np.array([g.dz[15], g.dz[16]])
+    for l in range(3):
+        m = (ts[14, 3, 3, l] * w[0] + ts[15, 3, 3, l] * w[1]) / w.sum()
+        assert abs(gold.ts[14, 3, 3, l] - m) < 1e-12 and abs(gold.ts[15, 3, 3, l] - m) < 1e-12
+    assert gold.cost.sum() == 1.0
+
+
+def test_eos_polynomial():
+    og = O.grid()
+    L = O.lib()
+    ec = np.array(list(og.ec))
+    for t, s in [(0.0, 0.0), (10.0, 0.3), (-1.5, -0.2)]:
+        want = ec[0] * t + ec[1] * s + ec[2] * t * t + ec[3] * t ** 3
+        assert abs(L.og_eos(ec, t, s) - want) <= 1e-15 * max(1.0, abs(want))
+
+
+def _bgc_oracle(kind="config2", steps=0, mld=0.0, seed=11, maxi=18, maxj=18, maxk=8):
+    c = make_case(maxi, maxj, maxk, kind=kind, seed=seed, mld_max=mld)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    p = O.bgc_params(1.0 / 96, c.tset)
+    bio = O.BiogemOracle(gold, p, c.light, c.mld)
+    return c, gold, bio
+
+
+def _P_total(c, gold, bio):
+    g = c.g
+    ocn = bio.ocn()
+    M = bio.arr("M").reshape(g.maxk, g.maxj, g.maxi)
+    part = bio.part()
+    settle = bio.arr("settle", O.NSED).reshape(g.maxk, g.maxj, g.maxi, O.NSED)
+    w = wet(c)
+    P = ((ocn[..., c.tset.PO4] + (ocn[..., c.tset.DOM_P] if c.tset.DOM_P >= 0 else 0)) * M)[w].sum()
+    P += (part[..., 1] * M)[w].sum()
+    k1 = c.k1[1:-1, 1:-1]
+    for j in range(g.maxj):
+        for i in range(g.maxi):
+            if k1[j, i] <= g.maxk:
+                P += settle[k1[j, i] - 1, j, i, 1]
+    return P
+
+
+@pytest.mark.parametrize("kind", ["config2", "domc"])
+def test_biogem_oracle_conserves_phosphorus(kind):
+    """PO4 + DOM_P in water, POP in flight and POP in the bottom rain are one
+    closed P budget (uptake, DOM split, remineralisation and the closed-system
+    sediment return only move P), also through the salinity-normalised coupling."""
+    c, gold, bio = _bgc_oracle(kind)
+    P0 = _P_total(c, gold, bio)
+    bio.step(12)
+    P1 = _P_total(c, gold, bio)
+    assert abs(P1 - P0) / P0 < 1e-12
+    assert bio.st.carb_fail == 0
+
+
+def test_biogem_oracle_uptake_closed_form():
+    """One surface cell: dPO4 = dtyr * kI * PO4/(PO4+c0) * k0 (biogem_box.f90:606-616);
+    the surface anomaly of PO4 (no remineralisation on the first step) is -dPO4."""
+    c, gold, bio = _bgc_oracle("config2")
+    bio.step(1, goldstein=False)
+    ocn0 = None
+    g = c.g
+    vd = bio.vdocn
+    k1 = c.k1[1:-1, 1:-1]
+    j, i = np.argwhere(k1 <= g.maxk)[0]
+    # ocn PO4 before the step is ts*S/Smean from the initial state
+    c2, gold2, bio2 = _bgc_oracle("config2")
+    po4 = bio2.ocn()[g.maxk - 1, j, i, c.tset.PO4]
+    kI = c.light[j, i]
+    d = (1.0 / 96) * 1.0 * kI * (po4 / (po4 + 0.05e-6)) * 2.0e-6
+    assert abs(vd[g.maxk - 1, j, i, c.tset.PO4] + d) <= 1e-18
+    assert abs(vd[g.maxk - 1, j, i, c.tset.DOM_P] - 0.66 * d) <= 1e-18 + 1e-12 * d
