@@ -1,0 +1,221 @@
+"""Benchmark of the MI355X ocean tracer step (GOLDSTEIN tstepo + BIOGEM +
+tracer coupling) in member-years per second.
+
+Workload (one line of JSON on stdout): an ensemble of 256 members on the
+36x36x16 grid with 24 tracers (T, S, DIC, ALK, PO4, O2, DOM_P, 17 dyes),
+seeded per-member parameters (kappa_h U[500,5000] m2/s, kappa_v
+log-U[1e-5,1e-4] m2/s, velocity scale U[0.5,1.5], POC e-folding depth
+U[300,900] m), BIOGEM every ocean step, FP32-compensated advection and
+diffusion with FP64 convection, chemistry and sums.  One "step" is one ocean
+step of the whole ensemble; a model year is 96 steps.  The run is captured
+as a HIP graph of one model year and replayed.
+
+Also reported: the same ensemble in FP64, the plain FP64 single-member path,
+p50/p99 model-year latency, the roofline of the dominant kernel (flux) and
+the CPU oracle timed on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+STEPS_PER_YEAR = 96
+HBM_PEAK_GBS = 8000.0
+
+
+def ensemble_inputs(n_members: int, seed: int = 2024, kind: str = "config2"):
+    from cgenie_amd import grid as G
+
+    g = G.make_grid(36, 36, 16, nyear=STEPS_PER_YEAR)
+    k1 = G.make_k1(g, seed)
+    tset = G.tracer_set(kind)
+    base_u = G.make_velocity(g, k1, seed + 1)
+    ts0 = G.make_tracers(g, k1, tset, seed + 2)
+    light = G.make_light(g, seed + 3)
+    mld = G.make_mld(g, seed + 4, 0.0)
+    rng = np.random.default_rng(seed + 5)
+    members = []
+    for q in range(n_members):
+        kh = rng.uniform(500.0, 5000.0)
+        kv = math.exp(rng.uniform(math.log(1e-5), math.log(1e-4)))
+        vs = rng.uniform(0.5, 1.5)
+        eL1 = rng.uniform(300.0, 900.0)
+        members.append(dict(diff=G.diffusivities(kh, kv), u=base_u * vs, eL1=eL1))
+    return g, k1, tset, ts0, light, mld, members
+
+
+def algorithmic_bytes_per_member_step(n_wet: int, L: int, s_state: int, s_vel: int) -> int:
+    """Flux kernel: read and write every tracer once, read u, v, w, write rho."""
+    return n_wet * (2 * L * s_state + 3 * s_vel + 8)
+
+
+def build_ensemble(eng, E, n, precision, carry=True, seed=2024):
+    g, k1, tset, ts0, light, mld, mem = ensemble_inputs(n, seed)
+    gid = eng.add_grid(g, k1)
+    ids = []
+    for p in mem:
+        m = eng.add_member(gid, tset.L, precision, p["diff"], diso=True, carry=carry)
+        eng.set_velocity(m, p["u"])
+        eng.upload(m, ts0)
+        bp = E.default_bgc_params(1.0 / STEPS_PER_YEAR, tset, carb_tol=1e-12, carb_maxit=1000, POC_eL1=p["eL1"])
+        eng.bgc_init(m, bp, light, mld)
+        ids.append(m)
+    n_wet = int(((np.arange(1, 17)[:, None, None]) >= k1[None, 1:37, 1:37]).sum())
+    return ids, tset, n_wet, g
+
+
+def timed_years(eng, graph, years):
+    """Replay the one-year graph; return per-year wall seconds."""
+    import ctypes as C
+
+    out = []
+    for _ in range(years):
+        t0 = time.perf_counter()
+        eng.launch(graph, 1)
+        eng.sync()
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def run_gpu(args):
+    from cgenie_amd import engine as E
+
+    eng = E.Engine(0)
+    res = {}
+    n = args.members
+    years = max(1, args.steps // STEPS_PER_YEAR)
+    warm_years = max(1, args.warmup // STEPS_PER_YEAR) if args.warmup > 0 else 0
+    from cgenie_amd import autotune
+
+    tuned = {}
+    for prec, tag in ((E.FP32C, "fp32c"), (E.FP64, "fp64")):
+        ids, tset, n_wet, g = build_ensemble(eng, E, n, prec)
+        tuned[tag] = autotune.tune(eng, ids, (36, 36, 16), tset.L, prec, use_cache=False, write_cache=False)
+        print(f"# autotune {tag}: {tuned[tag]}", file=sys.stderr, flush=True)
+        graph = eng.capture(ids, STEPS_PER_YEAR)
+        if warm_years:
+            eng.launch(graph, warm_years)
+        eng.sync()
+        per_year = timed_years(eng, graph, years)
+        tot = sum(per_year)
+        flux_ms = eng.time_kernel(0, ids, 20)
+        co_ms = eng.time_kernel(1, ids, 20)
+        bgc_ms = eng.time_kernel(2, ids, 5)
+        d = eng.diag(ids[0])
+        s_state = 8 if prec == E.FP64 else 6
+        s_vel = 8 if prec == E.FP64 else 4
+        B = algorithmic_bytes_per_member_step(n_wet, tset.L, s_state, s_vel) * n
+        res[tag] = dict(member_years_per_s=n * years / tot, ms_per_step=1e3 * tot / (years * STEPS_PER_YEAR),
+                        p50_year_s=float(np.percentile(per_year, 50)), p99_year_s=float(np.percentile(per_year, 99)),
+                        flux_ms=flux_ms, co_ms=co_ms, bgc_coupling_ms=bgc_ms, flux_bytes=B,
+                        flux_gbs=B / (flux_ms * 1e-3) / 1e9, carb_fail=int(d.carb_fail), n_wet=n_wet, L=tset.L,
+                        tracer_block=tuned[tag]["tracer_block"])
+        eng.drop_graph(graph)
+        for m in ids:
+            eng.remove_member(m)
+    # plain FP64 single member, eager launches (the reference-shaped path)
+    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"])
+    ids, tset, n_wet, g = build_ensemble(eng, E, 1, E.FP64)
+    eng.step(ids, STEPS_PER_YEAR)
+    eng.sync()
+    t0 = time.perf_counter()
+    ny = max(1, min(years, 3))
+    eng.step(ids, STEPS_PER_YEAR * ny)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    res["fp64_single"] = dict(member_years_per_s=ny / dt, ms_per_step=1e3 * dt / (ny * STEPS_PER_YEAR))
+    for m in ids:
+        eng.remove_member(m)
+    eng.close()
+    return res
+
+
+def _cpu_worker(args):
+    """One serial member-year of the CPU oracle (BIOGEM every step)."""
+    q, nsteps = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    from cgenie_amd import grid as G
+
+    g, k1, tset, ts0, light, mld, mem = ensemble_inputs(1, 2024 + q)
+    og = O.grid(36, 36, 16, nyear=STEPS_PER_YEAR)
+    og.diff[0], og.diff[1] = mem[0]["diff"]
+    gold = O.GoldsteinOracle(og, k1, ts0, mem[0]["u"])
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / STEPS_PER_YEAR, tset, POC_eL1=mem[0]["eL1"]), light, mld)
+    t0 = time.perf_counter()
+    bio.step(nsteps)
+    return time.perf_counter() - t0
+
+
+def run_cpu_baseline(cores: int, nsteps: int):
+    """The oracle (a port of the reference arithmetic), one serial member per
+    core, each running nsteps ocean steps.  Runs before the GPU is touched."""
+    import multiprocessing as mp
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    O.build()
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        times = pool.map(_cpu_worker, [(q, nsteps) for q in range(cores)])
+    wall = time.perf_counter() - t0
+    member_years = cores * nsteps / STEPS_PER_YEAR
+    return dict(value=member_years / max(times), unit="member-years/s", cores=cores, kind="port",
+                sample=f"{cores} members x {nsteps} steps (36x36x16, 24 tracers, BIOGEM every step), "
+                       f"one serial member per core; per-member {np.mean(times):.2f} s, wall {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=960)
+    ap.add_argument("--warmup", type=int, default=96)
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("CGO_CPU_CORES", "16")))
+    ap.add_argument("--cpu-steps", type=int, default=96)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(ensemble members shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+    cpu = None if args.no_cpu else run_cpu_baseline(cores, args.cpu_steps)
+    print(f"# cpu baseline done: {cpu}", file=sys.stderr, flush=True)
+    r = run_gpu(args)
+    main_ = r["fp32c"]
+    roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4), traffic=None)
+    line = dict(
+        metric="member-years/s at 36x36x16x24 tracers (FP32-comp batched ensemble); p50/p99 year latency",
+        value=round(main_["member_years_per_s"], 2), unit="member-years/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(main_["ms_per_step"], 4), higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32+carry/f64", data="synthetic",
+        config=dict(workload=f"ensemble {args.members} members x 36x36x16 x 24 tracers, BIOGEM every step, "
+                             "FP32-compensated advection-diffusion, FP64 co/chemistry/sums, 1-year HIP graph",
+                    members=args.members, grid="36x36x16", tracers=24, steps_per_year=STEPS_PER_YEAR),
+        p50_year_s=main_["p50_year_s"], p99_year_s=main_["p99_year_s"],
+        fp64_batched_member_years_per_s=round(r["fp64"]["member_years_per_s"], 2),
+        fp64_single_member_years_per_s=round(r["fp64_single"]["member_years_per_s"], 3),
+        speedup_vs_fp64_single=round(main_["member_years_per_s"] / r["fp64_single"]["member_years_per_s"], 1),
+        kernels_ms=dict(fp32c=dict(flux=main_["flux_ms"], co=main_["co_ms"], bgc_coupling=main_["bgc_coupling_ms"],
+                                   tracer_block=main_["tracer_block"]),
+                        fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
+                                  bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"])),
+        roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,8 +16,8 @@
 #include "cgocn_dev.h"
 
 namespace cgo {
-void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int lb, hipStream_t);
-void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int lb, hipStream_t);
+void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, hipStream_t);
+void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bgc(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
@@ -39,6 +39,7 @@ struct GridH {
     std::vector<int32_t> k1;      // halo'd
     std::vector<double> mj, mk;   // tables
     std::vector<int32_t> wet;     // wet column ids, vocn order
+    std::vector<int32_t> wets, vpos; // ascending ids and their vocn positions
     std::vector<double> V;        // [ncell]
     std::vector<double> invw;     // [ncell] inventory weights dphi*ds(j)*dz(k)
     double Vtot = 0;
@@ -232,6 +233,15 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     for (int I = 1; I <= mi; I++)
         for (int J = 1; J <= mj; J++)
             if (K1(I, J) <= mk) g->wet.push_back((I - 1) + mi * (J - 1));
+    {
+        std::vector<std::pair<int32_t, int32_t>> o;
+        for (size_t q = 0; q < g->wet.size(); q++) o.push_back({g->wet[q], (int32_t)q});
+        std::sort(o.begin(), o.end());
+        for (auto& pr : o) {
+            g->wets.push_back(pr.first);
+            g->vpos.push_back(pr.second);
+        }
+    }
     g->V.assign(ncell, 0.0);
     g->invw.assign(ncell, 0.0);
     for (int I = 1; I <= mi; I++)
@@ -258,6 +268,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     };
     const size_t o_k1 = take(g->k1.size() * 4), o_mj = take(g->mj.size() * 8), o_mk = take(g->mk.size() * 8);
     const size_t o_mjf = take(g->mj.size() * 4), o_mkf = take(g->mk.size() * 4), o_wet = take(g->wet.size() * 4 + 4);
+    const size_t o_wets = take(g->wet.size() * 4 + 4), o_vpos = take(g->wet.size() * 4 + 4);
     const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8);
     hipSetDevice(e->device);
     if (hipMalloc(&g->dbuf, off) != hipSuccess) {
@@ -271,7 +282,11 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     HIPCHK(e, hipMemcpy(b + o_mk, g->mk.data(), g->mk.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b + o_mjf, mjf.data(), mjf.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b + o_mkf, mkf.data(), mkf.size() * 4, hipMemcpyHostToDevice));
-    if (!g->wet.empty()) HIPCHK(e, hipMemcpy(b + o_wet, g->wet.data(), g->wet.size() * 4, hipMemcpyHostToDevice));
+    if (!g->wet.empty()) {
+        HIPCHK(e, hipMemcpy(b + o_wet, g->wet.data(), g->wet.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(b + o_wets, g->wets.data(), g->wet.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(b + o_vpos, g->vpos.data(), g->wet.size() * 4, hipMemcpyHostToDevice));
+    }
     HIPCHK(e, hipMemcpy(b + o_V, g->V.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b + o_w, g->invw.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
     GridDev& gd = g->dev;
@@ -287,6 +302,8 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     gd.mjf = reinterpret_cast<const float*>(b + o_mjf);
     gd.mkf = reinterpret_cast<const float*>(b + o_mkf);
     gd.wetcols = reinterpret_cast<const int32_t*>(b + o_wet);
+    gd.wetsorted = reinterpret_cast<const int32_t*>(b + o_wets);
+    gd.vpos = reinterpret_cast<const int32_t*>(b + o_vpos);
     gd.V = reinterpret_cast<const double*>(b + o_V);
     g->d_invw = reinterpret_cast<double*>(b + o_w);
     gd.dphi = d->dphi;
@@ -301,7 +318,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
 // byte layout of one member allocation
 struct MemLayout {
     size_t st[2], lo[2], u, sflux, rho, cost, ctrl, ocn, part, settle, carb, M, rM, vd, light, mld, colsum, colold,
-        scal, total;
+        scal, rtab, mink, total;
 };
 
 static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
@@ -337,6 +354,8 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
         m.colsum = take((L + 1) * nwet * 8);
         m.colold = take((L + 1) * nwet * 8);
         m.scal = take((SC_FACT0 + L + 1) * 8);
+        m.rtab = take((size_t)RT_N * (g->dev.maxk + 2) * 8);
+        m.mink = take((size_t)(g->dev.maxk + 2) * 4);
     }
     m.total = off;
     return m;
@@ -403,6 +422,8 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
         d.colsum = reinterpret_cast<double*>(b + ly.colsum);
         d.colold = reinterpret_cast<double*>(b + ly.colold);
         d.scal = reinterpret_cast<double*>(b + ly.scal);
+        d.rtab = reinterpret_cast<const double*>(b + ly.rtab);
+        d.mink = reinterpret_cast<const int32_t*>(b + ly.mink);
     }
     m->dev.sflux = had_sflux ? reinterpret_cast<double*>(b + ly.sflux) : nullptr;
     e->table_dirty = true;
@@ -757,7 +778,30 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
         ml[c] = mld[c];
     }
     scal[SC_SMEAN] = Smean;
+    // remineralisation tables, evaluated as sub_box_remin_part does per level
+    const int KS = mk + 2;
+    const double* Dbot = &g->mk[MK_DBOT * KS];
+    const double* dD = &g->mk[MK_DD * KS];
+    std::vector<double> rt((size_t)RT_N * KS, 0.0);
+    std::vector<int32_t> mink(KS, 0);
+    for (int kk = 1; kk <= mk; kk++) {
+        if (kk < mk) rt[RT_LAYER * KS + kk] = dD[kk + 1] / dD[kk];
+        rt[RT_POC1 * KS + kk] = (1.0 - exp(-dD[kk] / bp->POC_eL1));
+        rt[RT_POC2 * KS + kk] = (1.0 - exp(-dD[kk] / bp->POC_eL2));
+        rt[RT_CAL1 * KS + kk] = (1.0 - exp(-dD[kk] / bp->CaCO3_eL1));
+        rt[RT_CAL2 * KS + kk] = (1.0 - exp(-dD[kk] / bp->CaCO3_eL2));
+    }
+    for (int k = 1; k <= mk; k++) {
+        const double maxD = Dbot[k] + bp->dtyr * bp->sinkrate;
+        for (int kk = k - 1; kk >= 1; kk--)
+            if (Dbot[kk] > maxD) {
+                mink[k] = kk;
+                break;
+            }
+    }
     MemberDev& d = m->dev;
+    HIPCHK(e, hipMemcpy(const_cast<double*>(d.rtab), rt.data(), rt.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(const_cast<int32_t*>(d.mink), mink.data(), mink.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.ocn, planes.data(), planes.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemset(d.part, 0, (size_t)NSED * ncell * 8));
     HIPCHK(e, hipMemset(d.settle, 0, (size_t)NSED * ncol * 8));
@@ -816,7 +860,7 @@ struct Lists {
     // [p64 list][p32 list][bgc p64][bgc p32][all]
     std::vector<int32_t> v;
     int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
-    int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0};
+    int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
@@ -830,6 +874,7 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         g[4].push_back(ids[i]);
         ls.maxcol[p] = std::max(ls.maxcol[p], m->dev.g.ncol);
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
+        ls.maxi[p] = std::max(ls.maxi[p], m->dev.g.maxi);
         ls.maxwet[p] = std::max(ls.maxwet[p], m->dev.g.nwet);
         if (m->bgc) ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
     }
@@ -847,9 +892,9 @@ static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists
             if (!ls.n[p]) continue;
             const int32_t* l = d_lists + ls.off[p];
             if (p == 0)
-                launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[0], e->s);
+                launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
             else
-                launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[1], e->s);
+                launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
             launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
             if (ls.n[2 + p]) launch_bgc(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], p, e->s);
         }
@@ -939,7 +984,7 @@ int cgo_graph_destroy(cgo_engine* e, int32_t gid) {
 }
 
 int cgo_set_tile(cgo_engine* e, int32_t precision, int32_t tb) {
-    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (tb != 2 && tb != 4 && tb != 8))
+    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (tb != 2 && tb != 4 && tb != 6 && tb != 8))
         return fail(e, CGO_E_ARG, "set_tile: precision %d tracer block %d", precision, tb);
     e->lb[precision == CGO_FP64 ? 0 : 1] = tb;
     return CGO_OK;
@@ -960,9 +1005,9 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             const int32_t* l = e->d_list + ls.off[p];
             if (which == 0) {
                 if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[0], e->s);
+                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
                 else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], e->lb[1], e->s);
+                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
             } else if (ls.n[2 + p]) {

@@ -42,6 +42,8 @@ struct GridDev {
     const float* mjf;       // float copies for the FP32 form
     const float* mkf;
     const int32_t* wetcols; // [nwet] column ids i0 + maxi*j0, reference vector order (i outer, j inner)
+    const int32_t* wetsorted; // [nwet] the same ids ascending (i fastest): the thread order of column kernels
+    const int32_t* vpos;    // [nwet] reference vector position of wetsorted[w] (partial-sum slot)
     const double* V;        // [ncell] cell volumes (biogem_data.f90:1128)
     double dphi, rdphi, rVtot;
     double ec[5];
@@ -86,15 +88,30 @@ struct MemberDev {
     double* colsum;         // [(L+1)][nwet] column partial sums (NEW_l, S_new)
     double* colold;         // [L][nwet] column partial sums of OLD_l for the next coupling
     double* scal;           // member scalars, see SC_*
+    const double* rtab;     // [RT_N][maxk+2] remineralisation factors per level (host-computed)
+    const int32_t* mink;    // [maxk+2] deepest-reach level of particles leaving level k (0 = bottom)
     int32_t* diag;          // [4] limps, carb_fail, carb_iters, spare
     double* dmax;           // device scalar
 };
+
+// remineralisation table rows: layer-thickness ratio dD(k+1)/dD(k) and the
+// e-folding fractions 1 - exp(-dD/eL) (biogem_box.f90:2555-2561, 2583-2584, 2664-2689)
+enum { RT_LAYER = 0, RT_POC1, RT_POC2, RT_CAL1, RT_CAL2, RT_N };
 
 // member scalar slots (scal[])
 enum { SC_SMEAN = 0, SC_SNEW, SC_SRATIO, SC_RSRATIO, SC_FACT0 };  // SC_FACT0 + l : OLD_l * rNEW_l
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ int k1at(const GridDev& g, int I, int J) { return g.k1[J * (g.maxi + 2) + I]; }
+// Pointers read from the member table are generic (flat) to the compiler;
+// casting them to the global address space turns flat loads/stores into
+// global_load/global_store, which do not tie up the LDS counter.
+#define AS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ AS_GLOBAL T* gp(T* p) { return (AS_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ const AS_GLOBAL T* gp(const T* p) { return (const AS_GLOBAL T*)p; }
+
+__device__ __forceinline__ int k1at(const GridDev& g, int I, int J) { return ((const AS_GLOBAL int32_t*)g.k1)[J * (g.maxi + 2) + I]; }
 
 // eos, goldstein.f90:3048-3061 (ieos = 0)
 template <typename R>

@@ -58,7 +58,8 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     const int w = blockIdx.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
     const BgcParams& p = M.bp;
-    const int col = g.wetcols[w];
+    const int col = g.wetsorted[w];
+    const int slot = g.vpos[w];
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
     const int I = col % maxi + 1, J = col / maxi + 1;
     const int kb = k1at(g, I, J);
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     const int KS = maxk + 2;
     const double* Dbot = g.mk + MK_DBOT * KS;
     const double* Dmid = g.mk + MK_DMID * KS;
-    const double* dD = g.mk + MK_DD * KS;
+    const double* rt = M.rtab;
     double* ocn = M.ocn;  // plane 0 = T, 1 = S, then species planes
     double* vd = M.vd;
     Conv cv;
@@ -120,14 +121,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
             if (p.caco3) tot = tot + pp(S_CACO3, k);
             if (!(tot > NULLSMALL)) continue;
             int min_k = kb - 1;
-            if (k != kb) {
-                const double maxD = Dbot[k] + p.dtyr * p.sinkrate;
-                for (int kk = k - 1; kk >= kb; kk--)
-                    if (Dbot[kk] > maxD) {
-                        min_k = kk;
-                        break;
-                    }
-            }
+            if (k != kb && M.mink[k] >= kb) min_k = M.mink[k];
             double t[NSED];
             for (int s = 0; s < NSED; s++) t[s] = pp(s, k);
             // settling through level k itself (kk = k of the settle loop)
@@ -139,19 +133,18 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
             }
             for (int kk = k - 1; kk >= min_k; kk--) {
                 if (kk < kb) continue;
-                const double layerratio = dD[kk + 1] / dD[kk];
-                const double dDk = dD[kk];
+                const double layerratio = rt[RT_LAYER * KS + kk];
                 double tn[NSED];
                 for (int s = 0; s < NSED; s++) tn[s] = 0.0;
                 if (p.caco3) {
-                    const double f1 = (1.0 - exp(-dDk / p.CaCO3_eL1));
-                    const double f2 = (1.0 - exp(-dDk / p.CaCO3_eL2));
+                    const double f1 = rt[RT_CAL1 * KS + kk];
+                    const double f2 = rt[RT_CAL2 * KS + kk];
                     CaCO3_ratio = 1.0 - ((1.0 - t[S_CACO3FR2]) * f1 + t[S_CACO3FR2] * f2);
                     tn[S_CACO3FR2] = t[S_CACO3FR2] > NULLSMALL ? (1.0 - f2) * t[S_CACO3FR2] / CaCO3_ratio : 0.0;
                 }
                 {
-                    const double f1 = (1.0 - exp(-dDk / p.POC_eL1));
-                    const double f2 = (1.0 - exp(-dDk / p.POC_eL2));
+                    const double f1 = rt[RT_POC1 * KS + kk];
+                    const double f2 = rt[RT_POC2 * KS + kk];
                     POC_ratio = 1.0 - ((1.0 - t[S_POCFR2]) * f1 + t[S_POCFR2] * f2);
                     tn[S_POCFR2] = t[S_POCFR2] > NULLSMALL ? (1.0 - f2) * t[S_POCFR2] / POC_ratio : 0.0;
                 }
@@ -281,7 +274,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
             const double Sn = StateIO<R>::ld(st, lo, tstride + c) + SALN0 + 0.0;
             sS += Sn * g.V[c];
         }
-        M.colsum[w] = sS;
+        M.colsum[slot] = sS;
         for (int l = 2; l < L; l++) {
             double s = 0.0;
             for (int k = kb; k <= maxk; k++) {
@@ -289,35 +282,55 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
                 const double lv = StateIO<R>::ld(st, lo, (size_t)l * tstride + c) * ocn[ncell + c] * rSold;
                 s += lv * M.M[c];
             }
-            M.colsum[(size_t)l * g.nwet + w] = s;
+            M.colsum[(size_t)l * g.nwet + slot] = s;
         }
     }
 }
 
-// global sums in the reference's column order, then the coupling scalars
-__global__ __launch_bounds__(64) void k_couple_reduce(const MemberDev* __restrict__ mems,
-                                                      const int32_t* __restrict__ list) {
+// sequential sum in the given order, loads issued ahead of the dependent adds
+__device__ __forceinline__ double seq_sum(const double* __restrict__ a, int n, double scale, bool scaled) {
+    double tot = 0.0;
+    int i = 0;
+    for (; i + 16 <= n; i += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = a[i + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++) tot += scaled ? v[u] * scale : v[u];
+    }
+    for (; i < n; i++) tot += scaled ? a[i] * scale : a[i];
+    return tot;
+}
+
+// global sums in the reference's column order (partialtot(n) summed over
+// n = 1..n_vocn, biogem.f90:1927-2031), then the coupling scalars
+__global__ __launch_bounds__(128) void k_couple_reduce(const MemberDev* __restrict__ mems,
+                                                       const int32_t* __restrict__ list) {
     const MemberDev& M = mems[list[blockIdx.x]];
     const int nwet = M.g.nwet, L = M.L;
-    for (int q = threadIdx.x; q < L; q += blockDim.x) {
-        if (q == 1) continue;
-        if (q == 0) {
-            double tot = 0.0;
-            const double rV = M.g.rVtot;
-            for (int n = 0; n < nwet; n++) tot += M.colsum[n] * rV;
-            M.scal[SC_SNEW] = tot;
-        } else {
-            double nw = 0.0, od = 0.0;
-            for (int n = 0; n < nwet; n++) nw += M.colsum[(size_t)q * nwet + n];
-            for (int n = 0; n < nwet; n++) od += M.colold[(size_t)q * nwet + n];
-            const double rnew = fabs(nw) < NULLSMALL ? 0.0 : 1.0 / nw;
-            M.scal[SC_FACT0 + q] = od * rnew;
+    __shared__ double s_new[256], s_old[256];
+    const int t = threadIdx.x;
+    for (int q = t; q < 2 * L; q += blockDim.x) {
+        const int l = q < L ? q : q - L;
+        if (l == 1) continue;
+        if (q < L) {
+            const double v = seq_sum(M.colsum + (size_t)l * nwet, nwet, M.g.rVtot, l == 0);
+            if (l < 256) s_new[l] = v;
+        } else if (l >= 2) {
+            const double v = seq_sum(M.colold + (size_t)l * nwet, nwet, 0.0, false);
+            if (l < 256) s_old[l] = v;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double Snew = M.scal[SC_SNEW];
+    for (int l = 2 + t; l < L; l += blockDim.x) {
+        const double nw = s_new[l];
+        const double rnew = fabs(nw) < NULLSMALL ? 0.0 : 1.0 / nw;
+        M.scal[SC_FACT0 + l] = s_old[l] * rnew;
+    }
+    if (t == 0) {
+        const double Snew = s_new[0];
         const double Sratio = Snew / M.scal[SC_SMEAN];
+        M.scal[SC_SNEW] = Snew;
         M.scal[SC_SRATIO] = Sratio;
         M.scal[SC_RSRATIO] = 1.0 / Sratio;
     }
@@ -331,7 +344,8 @@ __global__ __launch_bounds__(64) void k_couple_apply(const MemberDev* __restrict
     const int w = blockIdx.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
     const BgcParams& p = M.bp;
-    const int col = g.wetcols[w];
+    const int col = g.wetsorted[w];
+    const int slot = g.vpos[w];
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
     const int kb = k1at(g, col % maxi + 1, col / maxi + 1);
     const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell, tstride = (size_t)maxk * plane;
@@ -373,7 +387,7 @@ __global__ __launch_bounds__(64) void k_couple_apply(const MemberDev* __restrict
             StateIO<R>::st(st, lo, ci, (Snew / snew[k]) * v);
             part += v * mnew[k];
         }
-        M.colold[(size_t)l * g.nwet + w] = part;
+        M.colold[(size_t)l * g.nwet + slot] = part;
     }
     for (int k = kb; k <= maxk; k++) {
         const size_t c = (size_t)(k - 1) * plane + col;
@@ -395,7 +409,7 @@ void launch_bgc(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
         hipLaunchKernelGGL(k_bgc_col<double>, GRID_COLS(nwetmax), dim3(64), 0, s, mems, list);
     else
         hipLaunchKernelGGL(k_bgc_col<float>, GRID_COLS(nwetmax), dim3(64), 0, s, mems, list);
-    hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(64), 0, s, mems, list);
+    hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(128), 0, s, mems, list);
     if (prec == 0)
         hipLaunchKernelGGL(k_couple_apply<double>, GRID_COLS(nwetmax), dim3(64), 0, s, mems, list);
     else

@@ -6,14 +6,18 @@
 
 namespace cgo {
 
-void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int lb, hipStream_t s) {
+void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int maxi, int lb, hipStream_t s) {
     const dim3 grid((maxcol + FLUX_THREADS - 1) / FLUX_THREADS, (maxL + lb - 1) / lb, n);
+    const int tw = FLUX_THREADS + 2 * maxi;
+    const size_t lds = flux_lds_bytes(tw, lb, sizeof(double));
     if (lb == 2)
-        hipLaunchKernelGGL((k_flux<double, 2>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+        hipLaunchKernelGGL((k_flux<double, 2>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
     else if (lb == 8)
-        hipLaunchKernelGGL((k_flux<double, 8>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+        hipLaunchKernelGGL((k_flux<double, 8>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
+    else if (lb == 6)
+        hipLaunchKernelGGL((k_flux<double, 6>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
     else
-        hipLaunchKernelGGL((k_flux<double, 4>), grid, dim3(FLUX_THREADS), 0, s, t, l);
+        hipLaunchKernelGGL((k_flux<double, 4>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
 }
 
 void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int prec, hipStream_t s) {

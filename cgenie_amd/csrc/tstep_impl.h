@@ -69,34 +69,75 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
 }
 
+// LDS layout of k_flux: the k1 tile, a ring of three level planes (k, k+1
+// and the prefetched k+2) of LB tracer slots plus T and S, each covering the
+// flattened column range [c0 - maxi, c0 + FLUX_THREADS + maxi) -- every
+// east/west (periodic wrap included, offsets below maxi) and north/south
+// neighbour of the block's columns -- and the carried bottom-face flux.
+constexpr int FLUX_NBUF = 3;
+constexpr int FLUX_PF = 3;  // tile positions per thread (TW <= 3 * FLUX_THREADS)
+__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es) {
+    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * 128 * es;
+}
+
 template <typename R, int LB>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
-                                                       const int32_t* __restrict__ list) {
+                                                       const int32_t* __restrict__ list, int tw) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const MemberDev& M = mems[list[blockIdx.z]];
     const GridDev& g = M.g;
-    const int maxi = g.maxi, maxj = g.maxj, maxk = g.maxk, L = M.L;
-    const int col = blockIdx.x * FLUX_THREADS + threadIdx.x;
+    const int maxi = g.maxi, maxj = g.maxj, maxk = g.maxk, L = M.L, ncol = g.ncol;
+    const int c0 = blockIdx.x * FLUX_THREADS;
     const int l0 = blockIdx.y * LB;
-    if (col >= g.ncol || l0 >= L) return;
-    const int i0 = col % maxi, j0 = col / maxi;
+    if (c0 >= ncol || l0 >= L) return;  // uniform for the block
+    const int nl = (L - l0) < LB ? (L - l0) : LB;
+    const int t0 = c0 - maxi > 0 ? c0 - maxi : 0;
+    const int t1 = c0 + FLUX_THREADS + maxi < ncol ? c0 + FLUX_THREADS + maxi : ncol;
+    const int TW = t1 - t0;
+    int* sk1 = reinterpret_cast<int*>(smem);
+    R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
+    R* sfb = sp + FLUX_NBUF * (LB + 2) * tw;
+    // slot q < LB: tracer l0+q; T and S sit in slots 0/1 when l0 == 0, else LB/LB+1
+    const int qT = l0 == 0 ? 0 : LB, qS = l0 == 0 ? 1 : LB + 1;
+    const int nslot = l0 == 0 ? nl : LB + 2;  // slots staged per level
+    __shared__ int s_kmin;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_kmin = maxk + 1;
+    for (int pp = tid; pp < TW; pp += FLUX_THREADS) {
+        const int c = t0 + pp;
+        sk1[pp] = k1at(g, c % maxi + 1, c / maxi + 1);
+    }
+    __syncthreads();
+    const int col = c0 + tid;
+    const bool active = col < ncol;
+    const int i0 = active ? col % maxi : 0, j0 = active ? col / maxi : 0;
     const int I = i0 + 1, J = j0 + 1;
-    const int k1c = k1at(g, I, J);
-    if (k1c > maxk) return;  // land column: nothing is updated
+    const int k1c = active ? k1at(g, I, J) : maxk + 1;
     const int k1e = k1at(g, I + 1, J), k1w = k1at(g, I - 1, J);
     const int k1n = k1at(g, I, J + 1), k1s = k1at(g, I, J - 1);
+    if (k1c <= maxk) atomicMin(&s_kmin, k1c);
+    __syncthreads();
+    const int kmin = s_kmin;
+    if (kmin > maxk) return;  // every column of the block is land
+
     const int par = *M.parity;
-    const R* __restrict__ src = static_cast<const R*>(M.st[par]);
-    R* __restrict__ dst = static_cast<R*>(M.st[par ^ 1]);
+    const AS_GLOBAL R* __restrict__ src = gp(static_cast<const R*>(M.st[par]));
+    AS_GLOBAL R* __restrict__ dst = gp(static_cast<R*>(M.st[par ^ 1]));
     const int16_t* __restrict__ lo_src = M.lo[par];
     int16_t* __restrict__ lo_dst = M.lo[par ^ 1];
-    const R* __restrict__ u = static_cast<const R*>(M.u);
-    const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
-    const int ce = j0 * maxi + (i0 + 1 == maxi ? 0 : i0 + 1);
+    const AS_GLOBAL R* __restrict__ u = gp(static_cast<const R*>(M.u));
+    AS_GLOBAL double* __restrict__ rhop = gp(M.rho);
+    const int plane = ncol, tstride = maxk * ncol, ncell = g.ncell;
+    // neighbour positions in the tile
+    const int p = col - t0;
+    const int pe = p + (i0 + 1 == maxi ? 1 - maxi : 1);
+    const int pw = p + (i0 == 0 ? maxi - 1 : -1);
+    const int pn = p + (j0 + 1 < maxj ? maxi : 0);
+    const int ps = p - (j0 > 0 ? maxi : 0);
     const int cw = j0 * maxi + (i0 == 0 ? maxi - 1 : i0 - 1);
-    const int cn = (j0 + 1 < maxj ? j0 + 1 : maxj - 1) * maxi + i0;
     const int cs = (j0 > 0 ? j0 - 1 : 0) * maxi + i0;
-    const R* mj = Met<R>::j(g);
-    const R* mk = Met<R>::k(g);
+    const AS_GLOBAL R* mj = gp(Met<R>::j(g));
+    const AS_GLOBAL R* mk = gp(Met<R>::k(g));
     const int JS = maxj + 2, KS = maxk + 2;
     const R rc = mj[MJ_RC * JS + J], rc2 = mj[MJ_RC2 * JS + J];
     const R cvn = mj[MJ_CV * JS + J], cv2n = mj[MJ_CV2 * JS + J];
@@ -109,185 +150,203 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R diffv = sizeof(R) == 8 ? (R)M.diff2 : (R)M.diff2f;
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
-    const int nl = (L - l0) < LB ? (L - l0) : LB;
+    constexpr bool ANOM = sizeof(R) == 4;  // FP32 form: fluxes of q - q_cell plus q_cell * div(u)
 
-    // carried values: [0]=c, [1]=e, [2]=w, [3]=n, [4]=s at level k (cur) and k+1 (up)
-    R cur[LB][5], up[LB][5], fb[LB];
-    R tcur[2][5], tup[2][5];
+    auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * (LB + 2) + q) * tw + pp]; };
+    auto FB = [&](int q) -> R& { return sfb[q * FLUX_THREADS + tid]; };
+    auto slot_tracer = [&](int q) { return q < LB ? l0 + q : q - LB; };
+    // prefetch of one plane into registers, then into the LDS ring
+    R pf[LB + 2][FLUX_PF];
+    auto fetch = [&](int kz) {
+        const int o = kz * plane + t0;
 #pragma unroll
-    for (int q = 0; q < LB; q++) fb[q] = (R)0;
+        for (int r = 0; r < FLUX_PF; r++) {
+            const int pp = tid + r * FLUX_THREADS;
+            const bool ok = pp < TW && kz + 1 >= sk1[pp < TW ? pp : 0];
+#pragma unroll
+            for (int q = 0; q < LB + 2; q++)
+                if (q < nslot || q >= LB) pf[q][r] = (ok && (q < nslot)) ? src[slot_tracer(q) * tstride + o + pp] : (R)0;
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < FLUX_PF; r++) {
+            const int pp = tid + r * FLUX_THREADS;
+            if (pp < TW) {
+#pragma unroll
+                for (int q = 0; q < LB + 2; q++)
+                    if (q < nslot) SP(buf, q, pp) = pf[q][r];
+            }
+        }
+    };
+
+    for (int q = 0; q < LB; q++) FB(q) = (R)0;
     int32_t limps = 0;
-    R dmaxv = (R)0;
-    bool have_cur = false;
-
-    for (int kz = k1c - 1; kz < maxk; kz++) {
-        const int k = kz + 1;
-        const size_t o = (size_t)kz * plane;
-        if (!have_cur) {
-#pragma unroll
-            for (int q = 0; q < LB; q++) {
-                if (q < nl) {
-                    const R* s = src + (size_t)(l0 + q) * tstride + o;
-                    cur[q][0] = s[col]; cur[q][1] = s[ce]; cur[q][2] = s[cw]; cur[q][3] = s[cn]; cur[q][4] = s[cs];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const R* s = src + (size_t)t * tstride + o;
-                tcur[t][0] = s[col]; tcur[t][1] = s[ce]; tcur[t][2] = s[cw]; tcur[t][3] = s[cn]; tcur[t][4] = s[cs];
-            }
-            have_cur = true;
-        }
-        const bool top = (k == maxk);
-        if (!top) {
-            const size_t ou = o + plane;
-#pragma unroll
-            for (int q = 0; q < LB; q++) {
-                if (q < nl) {
-                    const R* s = src + (size_t)(l0 + q) * tstride + ou;
-                    up[q][0] = s[col]; up[q][1] = s[ce]; up[q][2] = s[cw]; up[q][3] = s[cn]; up[q][4] = s[cs];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const R* s = src + (size_t)t * tstride + ou;
-                tup[t][0] = s[col]; tup[t][1] = s[ce]; tup[t][2] = s[cw]; tup[t][3] = s[cn]; tup[t][4] = s[cs];
-            }
-        }
-        // face velocities and Peclet-weighted upwinding (2489-2523)
-        const R ue_ = u[o + col], uw_ = u[o + cw];
-        const R un_ = u[plane * maxk + o + col], us_ = u[plane * maxk + o + cs];
-        const R ua_ = u[2 * plane * maxk + o + col];
-        const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k];
-        const R dt = mk[MK_DT * KS + k], rdz = mk[MK_RDZ * KS + k];
-        R pec;
-        pec = ue_ * dphi / diff1; const R upe = pec / ((R)2.0 + fabs(pec));
-        pec = uw_ * dphi / diff1; const R upw = pec / ((R)2.0 + fabs(pec));
-        pec = un_ * dsvn / diff1; const R upn = pec / ((R)2.0 + fabs(pec));
-        pec = us_ * dsvs / diff1; const R ups = pec / ((R)2.0 + fabs(pec));
-        pec = ua_ * dza / diffv; const R upa = pec / ((R)2.0 + fabs(pec));
-        const bool oe = k >= max(k1c, k1e), ow = k >= max(k1w, k1c);
-        const bool on = k >= max(k1c, k1n), os = k >= max(k1s, k1c);
-
-        // isoneutral coefficients from T and S (2489-2501, 2561-2605)
-        bool iso = false;
-        R dzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
-        if (!top) {
-            const R tatw = (R)0.5 * (tcur[0][0] + tup[0][0]);
-            const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
-            dzrho = (e1 * (tup[1][0] - tcur[1][0]) - tec * (tup[0][0] - tcur[0][0])) * rdza;
-            if (M.diso && dzrho < (R)-1.0E-12) {
-                iso = true;
-                const R rdzrho = (R)1.0 / dzrho;
-                R tv1 = 0;
-                R dxT[4], dxS[4], dyT[4], dyS[4];
-                // ina order: (knp,nnp) = (0,0), (0,1), (1,0), (1,1)
-                dxT[0] = k >= k1w ? (tcur[0][0] - tcur[0][2]) * rc * rdphi : (R)0;
-                dxS[0] = k >= k1w ? (tcur[1][0] - tcur[1][2]) * rc * rdphi : (R)0;
-                dyT[0] = k >= k1s ? (tcur[0][0] - tcur[0][4]) * cvs * rdsvs : (R)0;
-                dyS[0] = k >= k1s ? (tcur[1][0] - tcur[1][4]) * cvs * rdsvs : (R)0;
-                dxT[1] = k >= k1e ? (tcur[0][1] - tcur[0][0]) * rc * rdphi : (R)0;
-                dxS[1] = k >= k1e ? (tcur[1][1] - tcur[1][0]) * rc * rdphi : (R)0;
-                dyT[1] = k >= k1n ? (tcur[0][3] - tcur[0][0]) * cvn * rdsvn : (R)0;
-                dyS[1] = k >= k1n ? (tcur[1][3] - tcur[1][0]) * cvn * rdsvn : (R)0;
-                dxT[2] = k + 1 >= k1w ? (tup[0][0] - tup[0][2]) * rc * rdphi : (R)0;
-                dxS[2] = k + 1 >= k1w ? (tup[1][0] - tup[1][2]) * rc * rdphi : (R)0;
-                dyT[2] = k + 1 >= k1s ? (tup[0][0] - tup[0][4]) * cvs * rdsvs : (R)0;
-                dyS[2] = k + 1 >= k1s ? (tup[1][0] - tup[1][4]) * cvs * rdsvs : (R)0;
-                dxT[3] = k + 1 >= k1e ? (tup[0][1] - tup[0][0]) * rc * rdphi : (R)0;
-                dxS[3] = k + 1 >= k1e ? (tup[1][1] - tup[1][0]) * rc * rdphi : (R)0;
-                dyT[3] = k + 1 >= k1n ? (tup[0][3] - tup[0][0]) * cvn * rdsvn : (R)0;
-                dyS[3] = k + 1 >= k1n ? (tup[1][3] - tup[1][0]) * cvn * rdsvn : (R)0;
-#pragma unroll
-                for (int n = 0; n < 4; n++) {
-                    dxr[n] = scc * dxS[n] - tec * dxT[n];
-                    dyr[n] = scc * dyS[n] - tec * dyT[n];
-                    tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
-                }
-                tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
-                const R ssmax = mk[MK_SSMAX * KS + k];
-                if (tv1 > ssmax) {
-                    slim = ssmax * ssmax / (tv1 * tv1);
-                    limps++;
-                } else {
-                    slim = 1;
-                }
-                tv1 = tv1 * slim * diff1 * rdza;
-                const R tvd = tv1 * dt * rdza;
-                if (tvd > dmaxv) dmaxv = tvd;
-            }
-        }
-        double newT = 0, newS = 0;
-#pragma unroll
-        for (int q = 0; q < LB; q++) {
-            if (q >= nl) continue;
-            const R Tc = cur[q][0], Te = cur[q][1], Tw = cur[q][2], Tn = cur[q][3], Ts = cur[q][4];
-            const R fe = oe ? ue_ * rc * (((R)1.0 - upe) * Te + ((R)1.0 + upe) * Tc) * (R)0.5 - (Te - Tc) * rc2 * diff1
-                            : (R)0;
-            const R fw = ow ? uw_ * rc * (((R)1.0 - upw) * Tc + ((R)1.0 + upw) * Tw) * (R)0.5 - (Tc - Tw) * rc2 * diff1
-                            : (R)0;
-            const R fn = on ? cvn * un_ * (((R)1.0 - upn) * Tn + ((R)1.0 + upn) * Tc) * (R)0.5 - cv2n * (Tn - Tc) * diff1
-                            : (R)0;
-            const R fs = os ? cvs * us_ * (((R)1.0 - ups) * Tc + ((R)1.0 + ups) * Ts) * (R)0.5 - cv2s * (Tc - Ts) * diff1
-                            : (R)0;
-            R fa;
-            if (top) {
-                fa = M.sflux ? (R)M.sflux[(size_t)(l0 + q) * plane + col] : (R)0;
-            } else {
-                const R Tu = up[q][0];
-                fa = ua_ * (((R)1.0 - upa) * Tu + ((R)1.0 + upa) * Tc) * (R)0.5;
-                fa = fa - (Tu - Tc) * rdza * diffv;
-                if (iso) {
-                    const R dzts = (Tu - Tc) * rdza;
-                    R dx[4], dy[4];
-                    dx[0] = k >= k1w ? (Tc - Tw) * rc * rdphi : (R)0;
-                    dy[0] = k >= k1s ? (Tc - Ts) * cvs * rdsvs : (R)0;
-                    dx[1] = k >= k1e ? (Te - Tc) * rc * rdphi : (R)0;
-                    dy[1] = k >= k1n ? (Tn - Tc) * cvn * rdsvn : (R)0;
-                    dx[2] = k + 1 >= k1w ? (up[q][0] - up[q][2]) * rc * rdphi : (R)0;
-                    dy[2] = k + 1 >= k1s ? (up[q][0] - up[q][4]) * cvs * rdsvs : (R)0;
-                    dx[3] = k + 1 >= k1e ? (up[q][1] - up[q][0]) * rc * rdphi : (R)0;
-                    dy[3] = k + 1 >= k1n ? (up[q][3] - up[q][0]) * cvn * rdsvn : (R)0;
-                    R tv = 0;
-#pragma unroll
-                    for (int n = 0; n < 4; n++)
-                        tv = tv + ((R)2 * dzrho * dx[n] - dxr[n] * dzts) * dxr[n] +
-                             ((R)2 * dzrho * dy[n] - dyr[n] * dzts) * dyr[n];
-                    tv = (R)0.25 * slim * diff1 * tv / (dzrho * dzrho);
-                    fa = fa + tv;
-                }
-            }
-            const R div = ((fe - fw) * rdphi + (fn - fs) * rds) + (fa - fb[q]) * rdz;
-            const size_t idx = (size_t)(l0 + q) * tstride + o + col;
-            double nv;
-            if (sizeof(R) == 4 && lo_src) {
-                // compensated accumulation: the rounding residual of every
-                // update is carried to the next step in the int16 plane
-                nv = StateIO<R>::ld(src, lo_src, idx) + (double)(-(dt * div));
-                StateIO<R>::st(dst, lo_dst, idx, nv);
-            } else {
-                const R v = Tc - dt * div;
-                dst[idx] = v;
-                nv = (double)v;
-            }
-            fb[q] = fa;
-            if (l0 + q == 0) newT = nv;
-            if (l0 + q == 1) newS = nv;
-        }
-        if (l0 == 0) M.rho[o + col] = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
-        // carry level k+1 down
-        if (!top) {
-#pragma unroll
-            for (int q = 0; q < LB; q++)
-#pragma unroll
-                for (int n = 0; n < 5; n++) cur[q][n] = up[q][n];
-#pragma unroll
-            for (int t = 0; t < 2; t++)
-#pragma unroll
-                for (int n = 0; n < 5; n++) tcur[t][n] = tup[t][n];
-        }
+    R dmaxv = (R)0, wb = (R)0;
+    fetch(kmin - 1);
+    commit(kmin % FLUX_NBUF);
+    if (kmin < maxk) {
+        fetch(kmin);
+        commit((kmin + 1) % FLUX_NBUF);
     }
-    if (l0 == 0 && M.diso) {
+    __syncthreads();
+    for (int k = kmin; k <= maxk; k++) {
+        const int kz = k - 1;
+        const int cb = k % FLUX_NBUF, ub = (k + 1) % FLUX_NBUF;
+        const bool top = (k == maxk);
+        const bool more = k + 2 <= maxk;
+        if (more) fetch(kz + 2);
+        if (active && k >= k1c) {
+            const int o = kz * plane;
+            // face velocities and Peclet-weighted upwinding (goldstein.f90:2489-2523)
+            const R ue_ = u[o + col], uw_ = u[o + cw];
+            const R un_ = u[ncell + o + col], us_ = u[ncell + o + cs];
+            const R ua_ = u[2 * ncell + o + col];
+            const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k];
+            const R dt = mk[MK_DT * KS + k], rdz = mk[MK_RDZ * KS + k];
+            R pec;
+            pec = ue_ * dphi / diff1; const R upe = pec / ((R)2.0 + fabs(pec));
+            pec = uw_ * dphi / diff1; const R upw = pec / ((R)2.0 + fabs(pec));
+            pec = un_ * dsvn / diff1; const R upn = pec / ((R)2.0 + fabs(pec));
+            pec = us_ * dsvs / diff1; const R ups = pec / ((R)2.0 + fabs(pec));
+            pec = ua_ * dza / diffv; const R upa = pec / ((R)2.0 + fabs(pec));
+            const bool oe = k >= max(k1c, k1e), ow = k >= max(k1w, k1c);
+            const bool on = k >= max(k1c, k1n), os = k >= max(k1s, k1c);
+            // isoneutral coefficients from T and S (2489-2501, 2561-2605)
+            bool iso = false;
+            R dzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            if (!top) {
+                const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p), Sc = SP(cb, qS, p), Su = SP(ub, qS, p);
+                const R tatw = (R)0.5 * (Tc + Tu);
+                const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
+                dzrho = (e1 * (Su - Sc) - tec * (Tu - Tc)) * rdza;
+                if (M.diso && dzrho < (R)-1.0E-12) {
+                    iso = true;
+                    const R rdzrho = (R)1.0 / dzrho;
+                    R tv1 = 0;
+                    // ina order (knp, nnp) = (0,0), (0,1), (1,0), (1,1)
+#pragma unroll
+                    for (int n = 0; n < 4; n++) {
+                        const int b = n < 2 ? cb : ub;
+                        const int kn = n < 2 ? k : k + 1;
+                        const bool east = n & 1;
+                        const bool mx = kn >= (east ? k1e : k1w), my = kn >= (east ? k1n : k1s);
+                        const R dxT = mx ? (east ? SP(b, qT, pe) - SP(b, qT, p) : SP(b, qT, p) - SP(b, qT, pw)) * rc * rdphi : (R)0;
+                        const R dxS = mx ? (east ? SP(b, qS, pe) - SP(b, qS, p) : SP(b, qS, p) - SP(b, qS, pw)) * rc * rdphi : (R)0;
+                        const R dyT = my ? (east ? (SP(b, qT, pn) - SP(b, qT, p)) * cvn * rdsvn
+                                                 : (SP(b, qT, p) - SP(b, qT, ps)) * cvs * rdsvs) : (R)0;
+                        const R dyS = my ? (east ? (SP(b, qS, pn) - SP(b, qS, p)) * cvn * rdsvn
+                                                 : (SP(b, qS, p) - SP(b, qS, ps)) * cvs * rdsvs) : (R)0;
+                        dxr[n] = scc * dxS - tec * dxT;
+                        dyr[n] = scc * dyS - tec * dyT;
+                        tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
+                    }
+                    tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
+                    const R ssmax = mk[MK_SSMAX * KS + k];
+                    if (tv1 > ssmax) {
+                        slim = ssmax * ssmax / (tv1 * tv1);
+                        limps++;
+                    } else {
+                        slim = 1;
+                    }
+                    tv1 = tv1 * slim * diff1 * rdza;
+                    const R tvd = tv1 * dt * rdza;
+                    if (tvd > dmaxv) dmaxv = tvd;
+                }
+            }
+            // discrete divergence of the open faces (FP32 form only)
+            R divu = 0;
+            if (ANOM) {
+                divu = ((oe ? ue_ * rc : (R)0) - (ow ? uw_ * rc : (R)0)) * rdphi +
+                       ((on ? cvn * un_ : (R)0) - (os ? cvs * us_ : (R)0)) * rds +
+                       ((top ? (R)0 : ua_) - (k > k1c ? wb : (R)0)) * rdz;
+            }
+            const R isoc = (R)0.25 * slim * diff1;
+            double newT = 0, newS = 0;
+#pragma unroll 1
+            for (int q = 0; q < nl; q++) {
+                const R Tc = SP(cb, q, p), Te = SP(cb, q, pe), Tw = SP(cb, q, pw), Tn = SP(cb, q, pn), Ts = SP(cb, q, ps);
+                R fe, fw, fn, fs, fa, fbn = 0;
+                if (!ANOM) {
+                    fe = oe ? ue_ * rc * (((R)1.0 - upe) * Te + ((R)1.0 + upe) * Tc) * (R)0.5 - (Te - Tc) * rc2 * diff1 : (R)0;
+                    fw = ow ? uw_ * rc * (((R)1.0 - upw) * Tc + ((R)1.0 + upw) * Tw) * (R)0.5 - (Tc - Tw) * rc2 * diff1 : (R)0;
+                    fn = on ? cvn * un_ * (((R)1.0 - upn) * Tn + ((R)1.0 + upn) * Tc) * (R)0.5 - cv2n * (Tn - Tc) * diff1 : (R)0;
+                    fs = os ? cvs * us_ * (((R)1.0 - ups) * Tc + ((R)1.0 + ups) * Ts) * (R)0.5 - cv2s * (Tc - Ts) * diff1 : (R)0;
+                } else {
+                    const R dE = Te - Tc, dW = Tw - Tc, dN = Tn - Tc, dS = Ts - Tc;
+                    fe = oe ? ue_ * rc * ((R)1.0 - upe) * dE * (R)0.5 - dE * rc2 * diff1 : (R)0;
+                    fw = ow ? uw_ * rc * ((R)1.0 + upw) * dW * (R)0.5 + dW * rc2 * diff1 : (R)0;
+                    fn = on ? cvn * un_ * ((R)1.0 - upn) * dN * (R)0.5 - cv2n * dN * diff1 : (R)0;
+                    fs = os ? cvs * us_ * ((R)1.0 + ups) * dS * (R)0.5 + cv2s * dS * diff1 : (R)0;
+                }
+                if (top) {
+                    fa = M.sflux ? (R)gp(M.sflux)[(l0 + q) * plane + col] : (R)0;
+                } else {
+                    const R Tu = SP(ub, q, p);
+                    R nadv = -(Tu - Tc) * rdza * diffv;
+                    if (!ANOM) {
+                        fa = ua_ * (((R)1.0 - upa) * Tu + ((R)1.0 + upa) * Tc) * (R)0.5;
+                        fa = fa - (Tu - Tc) * rdza * diffv;
+                    }
+                    if (iso) {
+                        const R dzts = (Tu - Tc) * rdza;
+                        R tv = 0;
+#pragma unroll
+                        for (int n = 0; n < 4; n++) {
+                            const int b = n < 2 ? cb : ub;
+                            const bool east = n & 1;
+                            const int kn = n < 2 ? k : k + 1;
+                            const R dx = kn >= (east ? k1e : k1w)
+                                             ? (east ? SP(b, q, pe) - SP(b, q, p) : SP(b, q, p) - SP(b, q, pw)) * rc * rdphi
+                                             : (R)0;
+                            const R dy = kn >= (east ? k1n : k1s)
+                                             ? (east ? (SP(b, q, pn) - SP(b, q, p)) * cvn * rdsvn
+                                                     : (SP(b, q, p) - SP(b, q, ps)) * cvs * rdsvs)
+                                             : (R)0;
+                            tv = tv + ((R)2 * dzrho * dx - dxr[n] * dzts) * dxr[n] + ((R)2 * dzrho * dy - dyr[n] * dzts) * dyr[n];
+                        }
+                        tv = isoc * tv / (dzrho * dzrho);
+                        if (!ANOM) fa = fa + tv;
+                        nadv = nadv + tv;
+                    }
+                    if (ANOM) {
+                        const R dU = Tu - Tc;
+                        fa = ua_ * ((R)1.0 - upa) * dU * (R)0.5 + nadv;       // relative to this cell
+                        fbn = -(ua_ * ((R)1.0 + upa) * dU * (R)0.5) + nadv;   // relative to the cell above
+                    }
+                }
+                const R div = ((fe - fw) * rdphi + (fn - fs) * rds) + (fa - FB(q)) * rdz;
+                const int idx = (l0 + q) * tstride + o + col;
+                double nv;
+                if (ANOM) {
+                    const R inc = -(dt * (div + Tc * divu));
+                    if (lo_src) {
+                        // compensated accumulation: the rounding residual of every
+                        // update is carried to the next step in the int16 plane
+                        nv = StateIO<R>::ld((const R*)src, lo_src, idx) + (double)inc;
+                        StateIO<R>::st((R*)dst, lo_dst, idx, nv);
+                    } else {
+                        const R v = Tc + inc;
+                        dst[idx] = v;
+                        nv = (double)v;
+                    }
+                    FB(q) = fbn;
+                } else {
+                    const R v = Tc - dt * div;
+                    dst[idx] = v;
+                    nv = (double)v;
+                    FB(q) = fa;
+                }
+                if (l0 + q == 0) newT = nv;
+                if (l0 + q == 1) newS = nv;
+            }
+            if (l0 == 0) rhop[o + col] = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
+            wb = ua_;
+        }
+        if (more) commit((k + 2) % FLUX_NBUF);
+        __syncthreads();
+    }
+    if (l0 == 0 && M.diso && active) {
         if (limps) atomicAdd(&M.diag[0], limps);
         if (dmaxv > (R)0) atomic_max_pos(M.dmax, (double)dmaxv);
     }
@@ -301,7 +360,7 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     const GridDev& g = M.g;
     const int w = blockIdx.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
-    const int col = g.wetcols[w];
+    const int col = g.wetsorted[w];
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
     const int I = col % maxi + 1, J = col / maxi + 1;
     const int k1 = k1at(g, I, J);
