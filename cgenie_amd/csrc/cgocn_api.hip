@@ -20,7 +20,8 @@ void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int 
 void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
-void launch_bgc(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
+void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
+void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int prec, hipStream_t);
 void launch_inventory(const MemberDev*, int32_t member, double* out, const double* w, int prec, hipStream_t);
 }  // namespace cgo
 
@@ -50,6 +51,8 @@ struct GridH {
 struct MemberH {
     bool alive = false;
     int grid = -1, L = 0, prec = 0, bgc = 0;
+    int wstream = 4;  // particle-stream window of the BIOGEM march (4, 8 or 16)
+    bool coupling_due = false;  // cgo_bio_step ran; cgo_tracercoupling must follow
     MemberDev dev{};
     cgo_member_params p{};
     cgo_bgc_params bp{};
@@ -673,6 +676,7 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
         if (slots[b] >= m->L || slots[b] == 0 || slots[b] == 1 || slots[b] < -1)
             return fail(e, CGO_E_ARG, "bgc_init: species slot %d invalid for maxl %d", slots[b], m->L);
     if (bp->caco3 && (bp->slot_DIC < 0 || bp->slot_ALK < 0)) return fail(e, CGO_E_ARG, "bgc_init: CaCO3 needs DIC and ALK");
+    if (m->L > 256) return fail(e, CGO_E_ARG, "bgc_init: at most 256 tracers with BIOGEM (maxl %d)", m->L);
     HIPCHK(e, hipStreamSynchronize(e->s));
     std::vector<double> ts;
     int rc = download_dev(e, m, ts);  // current state, device order
@@ -799,6 +803,21 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
                 break;
             }
     }
+    // Window of the register-resident particle streams: at any level the
+    // live streams come from at most W-1 consecutive source levels above it.
+    // With multi-level sources (sinking shorter than the ocean depth) a stream
+    // from k settles at mink[k], or reaches the floor (mink 0): it spans
+    // k - mink[k] levels.  A single surface source needs no window.
+    {
+        int span = 1;
+        if (bp->dtyr * bp->sinkrate <= bp->dsc)
+            for (int k = 2; k <= mk; k++) span = std::max(span, k - mink[k]);
+        int W = 4;
+        while (W < span + 1 && W < 16) W *= 2;
+        if (span + 1 > W)
+            return fail(e, CGO_E_ARG, "bgc_init: particles sink through %d levels in one step (at most 15 supported)", span);
+        m->wstream = W;
+    }
     MemberDev& d = m->dev;
     HIPCHK(e, hipMemcpy(const_cast<double*>(d.rtab), rt.data(), rt.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(const_cast<int32_t*>(d.mink), mink.data(), mink.size() * 4, hipMemcpyHostToDevice));
@@ -861,6 +880,7 @@ struct Lists {
     std::vector<int32_t> v;
     int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
+    int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
@@ -876,7 +896,11 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
         ls.maxi[p] = std::max(ls.maxi[p], m->dev.g.maxi);
         ls.maxwet[p] = std::max(ls.maxwet[p], m->dev.g.nwet);
-        if (m->bgc) ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
+        if (m->bgc) {
+            ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
+            ls.maxk_bgc[p] = std::max(ls.maxk_bgc[p], m->dev.g.maxk);
+            ls.wstream[p] = std::max(ls.wstream[p], m->wstream);
+        }
     }
     for (int q = 0; q < 5; q++) {
         ls.off[q] = (int)ls.v.size();
@@ -886,20 +910,52 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
     return CGO_OK;
 }
 
+// tstepo: flux + co per precision, then the parity flip (goldstein.f90:2280-2432)
+static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    for (int p = 0; p < 2; p++) {
+        if (!ls.n[p]) continue;
+        const int32_t* l = d_lists + ls.off[p];
+        if (p == 0)
+            launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
+        else
+            launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
+        launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+    }
+    launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
+}
+
+// step_biogem for the members with BIOGEM state (biogem.f90:528-1877)
+static void enqueue_bio(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    for (int p = 0; p < 2; p++)
+        if (ls.n[2 + p])
+            launch_bio(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
+}
+
+// biogem_tracercoupling (biogem.f90:1885-2077)
+static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    for (int p = 0; p < 2; p++)
+        if (ls.n[2 + p])
+            launch_coupling(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p], p,
+                            e->s);
+}
+
+// whole ocean steps in the genie.f90 order
 static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int nsteps) {
     for (int it = 0; it < nsteps; it++) {
-        for (int p = 0; p < 2; p++) {
-            if (!ls.n[p]) continue;
-            const int32_t* l = d_lists + ls.off[p];
-            if (p == 0)
-                launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
-            else
-                launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
-            launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
-            if (ls.n[2 + p]) launch_bgc(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], p, e->s);
-        }
-        launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
+        enqueue_tstepo(e, ls, d_lists);
+        enqueue_bio(e, ls, d_lists);
+        enqueue_coupling(e, ls, d_lists);
     }
+}
+
+// a member between cgo_bio_step and cgo_tracercoupling accepts only the coupling
+static int check_not_due(cgo_engine* e, const int32_t* ids, int32_t n, const char* who) {
+    for (int i = 0; i < n; i++) {
+        MemberH* m = member(e, ids[i]);
+        if (m && m->coupling_due)
+            return fail(e, CGO_E_STATE, "%s: member %d ran cgo_bio_step without cgo_tracercoupling", who, ids[i]);
+    }
+    return CGO_OK;
 }
 
 static int upload_lists(cgo_engine* e, const Lists& ls) {
@@ -915,16 +971,64 @@ static int upload_lists(cgo_engine* e, const Lists& ls) {
     return CGO_OK;
 }
 
-int cgo_step(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
-    if (!e || !ids || n <= 0 || nsteps < 0) return fail(e, CGO_E_ARG, "step: bad argument");
+static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
     hipSetDevice(e->device);
-    Lists ls;
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
     if ((rc = sync_table(e))) return rc;
-    if ((rc = upload_lists(e, ls))) return rc;
+    return upload_lists(e, ls);
+}
+
+int cgo_step(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
+    if (!e || !ids || n <= 0 || nsteps < 0) return fail(e, CGO_E_ARG, "step: bad argument");
+    Lists ls;
+    int rc = check_not_due(e, ids, n, "step");
+    if (rc || (rc = prepare(e, ids, n, ls))) return rc;
     enqueue_steps(e, ls, e->d_list, nsteps);
     HIPCHK(e, hipGetLastError());
+    return CGO_OK;
+}
+
+int cgo_tstepo(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
+    if (!e || !ids || n <= 0 || nsteps < 0) return fail(e, CGO_E_ARG, "tstepo: bad argument");
+    Lists ls;
+    int rc = check_not_due(e, ids, n, "tstepo");
+    if (rc || (rc = prepare(e, ids, n, ls))) return rc;
+    for (int it = 0; it < nsteps; it++) enqueue_tstepo(e, ls, e->d_list);
+    HIPCHK(e, hipGetLastError());
+    return CGO_OK;
+}
+
+int cgo_bio_step(cgo_engine* e, const int32_t* ids, int32_t n) {
+    if (!e || !ids || n <= 0) return fail(e, CGO_E_ARG, "bio_step: bad argument");
+    for (int i = 0; i < n; i++) {
+        MemberH* m = member(e, ids[i]);
+        if (!m) return fail(e, CGO_E_ARG, "bio_step: bad member %d", ids[i]);
+        if (!m->bgc) return fail(e, CGO_E_STATE, "bio_step: member %d has no BIOGEM state (cgo_bgc_init)", ids[i]);
+    }
+    Lists ls;
+    int rc = check_not_due(e, ids, n, "bio_step");
+    if (rc || (rc = prepare(e, ids, n, ls))) return rc;
+    enqueue_bio(e, ls, e->d_list);
+    HIPCHK(e, hipGetLastError());
+    for (int i = 0; i < n; i++) member(e, ids[i])->coupling_due = true;
+    return CGO_OK;
+}
+
+int cgo_tracercoupling(cgo_engine* e, const int32_t* ids, int32_t n) {
+    if (!e || !ids || n <= 0) return fail(e, CGO_E_ARG, "tracercoupling: bad argument");
+    for (int i = 0; i < n; i++) {
+        MemberH* m = member(e, ids[i]);
+        if (!m) return fail(e, CGO_E_ARG, "tracercoupling: bad member %d", ids[i]);
+        if (!m->coupling_due)
+            return fail(e, CGO_E_STATE, "tracercoupling: member %d needs cgo_bio_step first", ids[i]);
+    }
+    Lists ls;
+    int rc = prepare(e, ids, n, ls);
+    if (rc) return rc;
+    enqueue_coupling(e, ls, e->d_list);
+    HIPCHK(e, hipGetLastError());
+    for (int i = 0; i < n; i++) member(e, ids[i])->coupling_due = false;
     return CGO_OK;
 }
 
@@ -932,7 +1036,9 @@ int cgo_graph_capture(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nste
     if (!e || !ids || n <= 0 || nsteps <= 0 || !graph_id) return fail(e, CGO_E_ARG, "graph_capture: bad argument");
     hipSetDevice(e->device);
     Lists ls;
-    int rc = make_lists(e, ids, n, ls);
+    int rc = check_not_due(e, ids, n, "graph_capture");
+    if (rc) return rc;
+    rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
     if ((rc = sync_table(e))) return rc;
     Graph gr;
@@ -1011,7 +1117,9 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
             } else if (ls.n[2 + p]) {
-                launch_bgc(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], p, e->s);
+                launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
+                launch_coupling(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],
+                                p, e->s);
             }
         }
     HIPCHK(e, hipEventRecord(e->ev1, e->s));

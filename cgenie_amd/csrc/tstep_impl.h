@@ -444,14 +444,13 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
 }
 
 #ifdef CGO_DEFINE_FLIP
-// end of step: flip the ping-pong parity, count the step, roll the mean salinity
+// end of the GOLDSTEIN step: flip the ping-pong parity, count the step
 __global__ void k_flip(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list, int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const MemberDev& M = mems[list[t]];
     *M.parity ^= 1;
     *M.steps += 1;
-    if (M.bgc) M.scal[SC_SMEAN] = M.scal[SC_SNEW];
 }
 #endif
 

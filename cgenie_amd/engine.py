@@ -86,6 +86,9 @@ def load() -> C.CDLL:
         "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
         "cgo_bgc_download": ([P, _i32, _i32, _dp], C.c_int),
         "cgo_step": ([P, ip, _i32, _i32], C.c_int),
+        "cgo_tstepo": ([P, ip, _i32, _i32], C.c_int),
+        "cgo_bio_step": ([P, ip, _i32], C.c_int),
+        "cgo_tracercoupling": ([P, ip, _i32], C.c_int),
         "cgo_graph_capture": ([P, ip, _i32, _i32, ip], C.c_int),
         "cgo_graph_launch": ([P, _i32, _i32], C.c_int),
         "cgo_graph_destroy": ([P, _i32], C.c_int),
@@ -238,6 +241,21 @@ class Engine:
     def step(self, members, nsteps: int = 1):
         arr, n = _ids(members)
         self._chk(self.L.cgo_step(self.h, arr, n, nsteps), "step")
+
+    def tstepo(self, members, nsteps: int = 1):
+        """tstepo (goldstein.f90:2280-2432): transport only."""
+        arr, n = _ids(members)
+        self._chk(self.L.cgo_tstepo(self.h, arr, n, nsteps), "tstepo")
+
+    def bio_step(self, members):
+        """step_biogem (biogem.f90:528-1877) for the carried species."""
+        arr, n = _ids(members)
+        self._chk(self.L.cgo_bio_step(self.h, arr, n), "bio_step")
+
+    def tracercoupling(self, members):
+        """biogem_tracercoupling (biogem.f90:1885-2077); follows bio_step."""
+        arr, n = _ids(members)
+        self._chk(self.L.cgo_tracercoupling(self.h, arr, n), "tracercoupling")
 
     def capture(self, members, nsteps: int) -> int:
         arr, n = _ids(members)

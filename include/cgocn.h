@@ -5,13 +5,14 @@
  * coupling order of genie.f90:306-313 (step_goldstein) and 360-381
  * (step_biogem, biogem_tracercoupling):
  *
- *   tstepo            goldstein.f90:2280-2432  -> cgo_step (GOLDSTEIN part)
+ *   tstepo            goldstein.f90:2280-2432  -> cgo_tstepo
  *   tstepo_flux, eos  goldstein.f90:2436-2642, 3048-3082
  *   co                goldstein.f90:2657-2777
- *   step_biogem       biogem.f90:528-1877      -> cgo_step (BIOGEM part), the
+ *   step_biogem       biogem.f90:528-1877      -> cgo_bio_step: the
  *                     1N1T_PO4MM biology, DOM/particulate remineralisation and
  *                     surface carbonate chemistry of the species carried
- *   biogem_tracercoupling  biogem.f90:1885-2077 -> cgo_step (coupling part)
+ *   biogem_tracercoupling  biogem.f90:1885-2077 -> cgo_tracercoupling
+ *   all three in order                         -> cgo_step, cgo_graph_capture
  *
  * The reference exposes these as Fortran module subroutines on module state
  * (goldstein_lib.f90:1-180, biogem_lib.f90); there is no C interface
@@ -118,15 +119,33 @@ int cgo_bgc_init(cgo_engine *e, int32_t m, const cgo_bgc_params *p, const double
 int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
 
 /* stepping: nsteps ocean steps of every listed member, batched into one
- * launch per kernel (members with BIOGEM run the full coupling order) */
+ * launch per kernel (members with BIOGEM run the full coupling order:
+ * cgo_tstepo, cgo_bio_step, cgo_tracercoupling) */
 int cgo_step(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
-/* the same work captured once as a HIP graph and replayed */
+
+/* The three calls of one ocean step for a host that drives the genie.f90
+ * order itself (genie.f90:306-313, 360-381).
+ *
+ * cgo_tstepo replaces tstepo (goldstein.f90:2280-2432) as called from
+ * step_goldstein (goldstein.f90:17-38, 235): tstepo_flux + eos, co, and the
+ * ts1 = ts hand-over, nsteps times.
+ * cgo_bio_step replaces step_biogem (biogem.f90:528-547) for the biology,
+ * remineralisation and surface chemistry of the carried species; dtyr is the
+ * one given to cgo_bgc_init.
+ * cgo_tracercoupling replaces biogem_tracercoupling(dum_ts, dum_ts1)
+ * (biogem.f90:1885-1890); go_ts and go_ts1 are the one device state.  It
+ * must follow cgo_bio_step (the column sums it needs are taken there) and
+ * nothing else may step those members in between (CGO_E_STATE). */
+int cgo_tstepo(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
+int cgo_bio_step(cgo_engine *e, const int32_t *members, int32_t n);
+int cgo_tracercoupling(cgo_engine *e, const int32_t *members, int32_t n);
+/* nsteps of cgo_step captured once as a HIP graph and replayed */
 int cgo_graph_capture(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps, int32_t *graph_id);
 int cgo_graph_launch(cgo_engine *e, int32_t graph_id, int32_t nreps);
 int cgo_graph_destroy(cgo_engine *e, int32_t graph_id);
 
 /* tuning and measurement */
-int cgo_set_tile(cgo_engine *e, int32_t precision, int32_t tracer_block);   /* 2, 4 or 8 */
+int cgo_set_tile(cgo_engine *e, int32_t precision, int32_t tracer_block);   /* 2, 4, 6 or 8 */
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
  * 1 = co, 2 = bgc column + coupling) over the listed members, from HIP events
  * on the engine stream; the state is advanced by the timed launches */
