@@ -6,8 +6,9 @@ BIOGEM every step, kappa_h 2000 m2/s, kappa_v 2.7e-5 m2/s, seed 2024.
   python tests/drift_study.py oracle OUTDIR [--years N]
       the CPU oracle (FP64 restatement), states saved at the checkpoints
   python tests/drift_study.py gpu OUTDIR [--years N]
-      FP64, FP32-compensated (carry) and plain FP32 members stepped together
-      on the GPU, states saved at the checkpoints
+      FP64, FP32-compensated (carry) and plain FP32 members, and an FP64
+      member started from a state perturbed by 1e-7 relative, stepped
+      together on the GPU; states saved at the checkpoints
   python tests/drift_study.py compare DIR_ORACLE DIR_GPU
       per-tracer max relative error and inventory drift at each checkpoint
 
@@ -81,10 +82,15 @@ def run_gpu(outdir, years):
     gid = eng.add_grid(g, k1)
     p = E.default_bgc_params(1.0 / 96, tset, carb_tol=1e-12, carb_maxit=1000)
     ms = {}
-    for tag, prec, carry in (("fp64", E.FP64, True), ("fp32c", E.FP32C, True), ("fp32", E.FP32C, False)):
+    # fp64p: the FP64 path from a state perturbed by 1e-7 relative (seeded),
+    # the model's own sensitivity to a perturbation of FP32 rounding size
+    rng = np.random.default_rng(SEED + 9)
+    ts_p = ts * (1.0 + 1e-7 * rng.standard_normal(ts.shape))
+    for tag, prec, carry, t0s in (("fp64", E.FP64, True, ts), ("fp32c", E.FP32C, True, ts),
+                                  ("fp32", E.FP32C, False, ts), ("fp64p", E.FP64, True, ts_p)):
         m = eng.add_member(gid, tset.L, prec, diff, diso=True, carry=carry)
         eng.set_velocity(m, u)
-        eng.upload(m, ts)
+        eng.upload(m, t0s)
         eng.bgc_init(m, p, light, mld)
         ms[tag] = m
     graph = eng.capture(list(ms.values()), 96)
@@ -127,7 +133,7 @@ def compare(dir_ref, dir_gpu, ref_tag="oracle"):
             ref = ref[w]
         iref = json.load(open(fr[:-4] + ".json"))["inventory"]
         row = {}
-        for tag in ("fp64", "fp32c", "fp32"):
+        for tag in ("fp64", "fp32c", "fp32", "fp64p"):
             f = os.path.join(dir_gpu, f"{tag}_y{y:04d}.npy")
             if not os.path.exists(f):
                 continue

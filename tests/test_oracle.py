@@ -249,3 +249,89 @@ def test_biogem_oracle_uptake_closed_form():
     d = (1.0 / 96) * 1.0 * kI * (po4 / (po4 + 0.05e-6)) * 2.0e-6
     assert abs(vd[g.maxk - 1, j, i, c.tset.PO4] + d) <= 1e-18
     assert abs(vd[g.maxk - 1, j, i, c.tset.DOM_P] - 0.66 * d) <= 1e-18 + 1e-12 * d
+
+
+def test_isoneutral_flux_closed_form():
+    """Linear EOS (ec3 = ec4 = 0), S uniform, u = 0 and fields linear in
+    longitude and depth, T = a phi + b z, C = ac phi + bc z.  The isoneutral
+    vertical flux of tstepo_flux (goldstein.f90:2561-2621) at a face whose
+    four corner columns are wet is then diff(1) a rc^2 (2 b ac - a bc) / b^2
+    (the vertical row (2 Sx, 2 Sy, S^2) of the combined isoneutral + skew
+    tensor with slope Sx = a rc / b), the same at every such face, so only
+    the surface cell feels it: diso on minus off there is
+    dt(maxk) rdz(maxk) times that flux."""
+    c = make_case(18, 18, 8, kind="ts", L=3, seed=5)
+    og_on, og_off = O.grid(18, 18, 8), O.grid(18, 18, 8)
+    for og in (og_on, og_off):
+        og.ec[2] = 0.0
+        og.ec[3] = 0.0
+    g, mk = c.g, 8
+    a, b, ac, bc = 1.0, 5.0, 0.3, -2.0
+    phi = (np.arange(1, g.maxi + 1) - 0.5) * g.dphi
+    z = np.array(og_on.zro)[1:mk + 1]
+    ts = np.zeros_like(c.ts)
+    ts[..., 0] = 10.0 + a * phi[None, None, :] + b * z[:, None, None]
+    ts[..., 2] = 1.0 + ac * phi[None, None, :] + bc * z[:, None, None]
+    u0 = np.zeros_like(c.u)
+    on = O.GoldsteinOracle(og_on, c.k1, ts, u0, diso=True)
+    off = O.GoldsteinOracle(og_off, c.k1, ts, u0, diso=False)
+    on.step(1)
+    off.step(1)
+    k1 = c.k1
+    checked = 0
+    for J in range(3, g.maxj - 1):
+        rc = og_on.rc[J]
+        if (a * rc / b) ** 2 >= 10.0:
+            continue
+        for I in range(2, g.maxi):
+            if max(k1[J, I], k1[J, I - 1], k1[J, I + 1], k1[J - 1, I], k1[J + 1, I]) > mk - 1:
+                continue
+            fac = og_on.dt[mk] * (1.0 / og_on.dz[mk]) * og_on.diff[0]
+            want_T = fac * a * a * rc * rc / b
+            want_C = fac * a * rc * rc * (2 * b * ac - a * bc) / (b * b)
+            dT = on.ts[mk - 1, J - 1, I - 1, 0] - off.ts[mk - 1, J - 1, I - 1, 0]
+            dC = on.ts[mk - 1, J - 1, I - 1, 2] - off.ts[mk - 1, J - 1, I - 1, 2]
+            assert abs(dT - want_T) <= 1e-8 * abs(want_T)
+            assert abs(dC - want_C) <= 1e-8 * abs(want_C)
+            checked += 1
+    assert checked >= 10
+
+
+def test_particulate_remin_closed_form():
+    """One column with POC/POP only at the surface (frac2 = 0), no light, no
+    DOM_C (so no DOM remineralisation), no rain from before.  The sinking flux
+    leaving level k, F(k) = t(k) dD(k), is the exponential profile
+    F0 exp(-(Dbot(k) - Dbot(maxk)) / eL1) (biogem_box.f90:2555-2561,
+    2664-2689), so the PO4 released in level k is (F(k+1) - F(k)) / dD(k)
+    (Redfield factor 1 with O2 present, POP -> PO4 = 1) down to the level
+    min_k that one step of sinking reaches, where the rest stays in flight."""
+    c = make_case(kind="config2", seed=3)
+    g = c.g
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    p = O.bgc_params(1.0 / 96, c.tset)
+    bio = O.BiogemOracle(gold, p, np.zeros_like(c.light), c.mld)
+    k1 = c.k1[1:-1, 1:-1]
+    j, i = np.argwhere(k1 == k1.min())[0]
+    part = bio.arr("part", O.NSED).reshape(g.maxk, g.maxj, g.maxi, O.NSED)
+    part[:] = 0.0
+    X = 2.0e-6
+    part[g.maxk - 1, j, i, 0] = X            # POC
+    part[g.maxk - 1, j, i, 1] = X / 106.0    # POP
+    bio.step(1, goldstein=False)
+    Dbot, dD = np.array(bio.st.Dbot), np.array(bio.st.dD)
+    mk, eL = g.maxk, p.POC_eL1
+    reach = Dbot[mk] + p.dtyr * p.sinkrate
+    min_k = next(kk for kk in range(mk - 1, 0, -1) if Dbot[kk] > reach)
+    assert min_k >= k1[j, i]
+    F0 = (X / 106.0) * dD[mk]
+    F = lambda kk: F0 * np.exp(-(Dbot[kk] - Dbot[mk]) / eL)
+    vd = bio.vdocn
+    for kk in range(mk - 1, min_k - 1, -1):
+        Fin = F0 if kk == mk - 1 else F(kk + 1)
+        want = (Fin - F(kk)) / dD[kk]
+        got = vd[kk - 1, j, i, c.tset.PO4]
+        assert abs(got - want) <= 1e-11 * abs(want), (kk, got, want)
+    for kk in range(min_k - 1, k1[j, i] - 1, -1):
+        assert vd[kk - 1, j, i, c.tset.PO4] == 0.0
+    left = bio.part()[min_k - 1, j, i, 1]
+    assert abs(left - F(min_k) / dD[min_k]) <= 1e-11 * left
