@@ -335,3 +335,77 @@ def test_particulate_remin_closed_form():
         assert vd[kk - 1, j, i, c.tset.PO4] == 0.0
     left = bio.part()[min_k - 1, j, i, 1]
     assert abs(left - F(min_k) / dD[min_k]) <= 1e-11 * left
+
+
+def test_dom_remin_closed_form():
+    """No light, no particles, no rain: step_biogem reduces to DOM
+    remineralisation (biogem_box.f90:2287-2406).  A fraction dtyr/tau of DOM_C
+    and DOM_P returns to DIC and PO4 (Redfield factor 1 with O2 present) and
+    consumes O2 by the Redfield ratio: -138 O2 per 106 C + 1 P, of which -2 per
+    P (the POP column of conv_sed_ocn, gem_util.f90:79-98)."""
+    c = make_case(kind="domc", seed=11)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    p = O.bgc_params(1.0 / 96, c.tset)
+    bio = O.BiogemOracle(gold, p, np.zeros_like(c.light), c.mld)
+    ocn0 = bio.ocn()
+    bio.step(1, goldstein=False)
+    vd, t, w = bio.vdocn, c.tset, wet(c)
+    ratio = p.dtyr / p.DOMlifetime
+    domc, domp = ocn0[..., t.DOM_C][w], ocn0[..., t.DOM_P][w]
+    assert np.all(domc > 1e-19)
+    tol = lambda x: 1e-13 * np.abs(x) + 1e-30
+    assert np.all(np.abs(vd[..., t.DOM_C][w] + ratio * domc) <= tol(domc))
+    assert np.all(np.abs(vd[..., t.DOM_P][w] + ratio * domp) <= tol(domp))
+    assert np.all(np.abs(vd[..., t.DIC][w] - ratio * domc) <= tol(domc))
+    assert np.all(np.abs(vd[..., t.PO4][w] - ratio * domp) <= tol(domp))
+    o2 = ratio * (domc * (p.red_POP_PO2 + 2.0) / p.red_POP_POC - 2.0 * domp)
+    assert np.all(np.abs(vd[..., t.O2][w] - o2) <= 1e-12 * np.abs(o2))
+
+
+def _ring_case(kappa_h):
+    """18x18x8 ocean with no land between the polar walls; T, S uniform."""
+    c = make_case(18, 18, 8, kind="ts", L=3, seed=2)
+    og = O.grid(18, 18, 8, kappa_h=kappa_h, kappa_v=1e-20)
+    k1 = np.ones_like(c.k1)
+    k1[0, :] = k1[-1, :] = 94
+    phi = (np.arange(1, 19) - 0.5) * c.g.dphi
+    return c, og, k1, phi
+
+
+def test_zonal_upwind_advection_shifts_one_cell_at_courant_one():
+    """|pec| -> inf gives ups -> 1, first-order upwind (goldstein.f90:2517-2528):
+    with u dt rdphi rc(j) = 1 a zonal profile moves exactly one cell east
+    per step (and one west for u < 0), periodic across the doorway."""
+    c, og, k1, phi = _ring_case(1e-20)
+    q = 1.0 + 0.5 * np.cos(3 * phi) + 0.2 * np.sin(5 * phi)
+    ts = np.zeros_like(c.ts)
+    ts[..., 0] = 5.0
+    ts[..., 2] = q[None, None, :]
+    J0 = 9
+    for sign in (1.0, -1.0):
+        U = sign / (og.dt[8] * og.rdphi * og.rc[J0])
+        u = np.zeros_like(c.u)
+        u[..., 0] = U
+        o = O.GoldsteinOracle(og, k1, ts, u, diso=False)
+        o.step(1)
+        want = np.roll(q, 1 if sign > 0 else -1)
+        for k in range(8):
+            assert np.allclose(o.ts[k, J0 - 1, :, 2], want, rtol=0, atol=1e-13)
+
+
+def test_zonal_diffusion_discrete_eigenmode():
+    """u = 0: the step is explicit zonal diffusion with the rc2 metric,
+    dq = dt rdphi rc2(j) diff(1) (q(i+1) - 2 q(i) + q(i-1)); a cos(m phi) mode
+    is an eigenvector with eigenvalue 2 cos(m dphi) - 2."""
+    c, og, k1, phi = _ring_case(2000.0)
+    m = 4
+    ts = np.zeros_like(c.ts)
+    ts[..., 0] = 5.0
+    ts[..., 2] = (2.0 + np.cos(m * phi))[None, None, :]
+    o = O.GoldsteinOracle(og, k1, ts, np.zeros_like(c.u), diso=False)
+    o.step(1)
+    for J in range(1, 19):
+        lam = og.dt[8] * og.rdphi * og.rc2[J] * og.diff[0] * (2 * np.cos(m * c.g.dphi) - 2)
+        got = o.ts[:, J - 1, :, 2] - ts[:, J - 1, :, 2]
+        want = lam * np.cos(m * phi)[None, :]
+        assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(lam) + 1e-15)
