@@ -5,14 +5,16 @@
 // exp/log/pow differ from glibc, within 1 ulp).
 //
 // k_bgc_col   one thread per (member, wet column), everything it carries
-//             down the column held in registers: the column partial sums of
-//             biogem_tracercoupling (biogem.f90:1984-1997, 2016-2025), the
-//             surface carbonate chemistry (1026-1084), then one march from
+//             down the column held in registers: the surface carbonate
+//             chemistry (biogem.f90:1026-1084), then one march from
 //             the surface to k1 that applies, level by level, DOM
 //             remineralisation (biogem_box.f90:2287-2406), particulate
 //             remineralisation (2412-2877), the sediment return of the
 //             previous step's rain (biogem.f90:873-942) and PO4 uptake
 //             (biogem_box.f90:346-1515).  Every cell is read and written once.
+// k_couple_sums    one thread per (member, wet column, tracer): the column
+//             partial sums of the state after transport (biogem.f90:1984-1997,
+//             2016-2025).
 // k_couple_reduce  one block per member: the global sums, in the reference's
 //             column order (biogem.f90:1927-2031), staged through LDS.
 // k_couple_apply   one block per (member, 64 wet columns): the rescale and
@@ -126,8 +128,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     if (w >= g.nwet) return;
     const BgcParams& p = M.bp;
     const int col = g.wetsorted[w];
-    const int slot = g.vpos[w];
-    const int maxi = g.maxi, maxk = g.maxk, L = M.L;
+    const int maxi = g.maxi, maxk = g.maxk;
     const int kb = k1at(g, col % maxi + 1, col / maxi + 1);
     const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell;
     const int KS = maxk + 2;
@@ -140,44 +141,10 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     const bool hasO2 = p.slot[B_O2] >= 0, hasDOMC = p.slot[B_DOMC] >= 0, hasDOMP = p.slot[B_DOMP] >= 0;
     const bool hasPO4 = p.slot[B_PO4] >= 0;
 
-    // (1) column partial sums of the state after transport (biogem.f90:1984-1997, 2016-2025)
-    {
-        const int par = *M.parity;  // the transport step has flipped: st[par] is the new state
-        const void* st = M.st[par];
-        const int16_t* lo = M.lo[par];
-        const size_t tstride = (size_t)maxk * plane;
-        const double rSold = 1.0 / M.scal[SC_SMEAN];
-        double sS = 0.0;
-#pragma unroll 4
-        for (int k = kb; k <= maxk; k++) {
-            const size_t c = cell(k);
-            const double Sn = StateIO<R>::ld(st, lo, tstride + c) + SALN0 + 0.0;
-            sS += Sn * g.V[c];
-        }
-        M.colsum[slot] = sS;
-        for (int l0 = 2; l0 < L; l0 += 4) {
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-            for (int k = kb; k <= maxk; k++) {
-                const size_t c = cell(k);
-                const double os = ocn[ncell + c], mc = Mm[c];
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    if (l0 + j < L) {
-                        const double lv = StateIO<R>::ld(st, lo, (size_t)(l0 + j) * tstride + c) * os * rSold;
-                        acc[j] += lv * mc;
-                    }
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (l0 + j < L) M.colsum[(size_t)(l0 + j) * g.nwet + slot] = acc[j];
-        }
-    }
-
     Conv cv;
     make_conv(p, cv);
 
-    // (2) surface carbonate chemistry (biogem.f90:1026-1084)
+    // (1) surface carbonate chemistry (biogem.f90:1026-1084)
     double ohm = 0.0;
     if (p.slot[B_DIC] >= 0 && p.slot[B_ALK] >= 0) {
         double* carb = M.carb + col;
@@ -197,7 +164,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         ohm = carb[IC_OHM_CAL * plane];
     }
 
-    // (3) PO4 uptake of this column (biogem_box.f90:346-1515): the same
+    // (2) PO4 uptake of this column (biogem_box.f90:346-1515): the same
     //     particulate and dissolved change at every level from kstart up
     int kstart = maxk + 1;
     double upart[NSED] = {0.0, 0.0, 0.0, 0.0, 0.0}, upt[4] = {0.0, 0.0, 0.0, 0.0}, domC = 0.0, domP = 0.0;
@@ -246,7 +213,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         for (int s = 0; s < NSED; s++) upart[s] = bp[s];
     }
 
-    // (4) the march, surface to k1
+    // (3) the march, surface to k1
     double* __restrict__ part = M.part;
     double* __restrict__ vd = M.vd;
     const double* __restrict__ pO2 = hasO2 ? ocn + (size_t)p.ocn[B_O2] * ncell : nullptr;
@@ -407,6 +374,71 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     }
 #pragma unroll
     for (int s = 0; s < NSED; s++) M.settle[(size_t)s * plane + col] = settle[s];
+}
+
+// column partial sums of the state after transport, one thread per (column,
+// sum): sum 0 is S (biogem.f90:2016-2025), sum q >= 1 is tracer q + 1
+// (1984-1997); levels k1..maxk in order, loads issued eight levels at a time
+constexpr int SUMS_THREADS = 256;
+constexpr int SUMS_COLS = 64;
+constexpr int SUMS_UNROLL = 8;
+
+template <typename R>
+__global__ __launch_bounds__(SUMS_THREADS) void k_couple_sums(const MemberDev* __restrict__ mems,
+                                                              const int32_t* __restrict__ list) {
+    __shared__ int32_t s_col[SUMS_COLS], s_kb[SUMS_COLS];
+    const MemberDev& M = mems[list[blockIdx.y]];
+    const GridDev& g = M.g;
+    const int w0 = blockIdx.x * SUMS_COLS;
+    if (w0 >= g.nwet) return;  // uniform over the block
+    const int t = threadIdx.x, maxi = g.maxi, maxk = g.maxk, L = M.L;
+    if (t < SUMS_COLS) {
+        const int w = w0 + t;
+        const int col = w < g.nwet ? g.wetsorted[w] : 0;
+        s_col[t] = col;
+        s_kb[t] = w < g.nwet ? k1at(g, col % maxi + 1, col / maxi + 1) : maxk + 1;
+    }
+    __syncthreads();
+    const int cc = t % SUMS_COLS, w = w0 + cc;
+    if (w >= g.nwet) return;  // no barrier below
+    const int col = s_col[cc], kb = s_kb[cc], slot = g.vpos[w];
+    const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell, tstride = (size_t)maxk * plane;
+    const int par = *M.parity;  // the transport step has flipped: st[par] is the new state
+    const void* st = M.st[par];
+    const int16_t* lo = M.lo[par];
+    const double rSold = 1.0 / M.scal[SC_SMEAN];
+    const double* __restrict__ os = M.ocn + ncell;
+    const double* __restrict__ V = g.V;
+    const double* __restrict__ Mm = M.M;
+    for (int q = t / SUMS_COLS; q < L - 1; q += SUMS_THREADS / SUMS_COLS) {
+        const int l = q == 0 ? 1 : q + 1;
+        double acc = 0.0;
+        for (int k0 = kb; k0 <= maxk; k0 += SUMS_UNROLL) {
+            double x[SUMS_UNROLL], a[SUMS_UNROLL], b[SUMS_UNROLL];
+#pragma unroll
+            for (int u = 0; u < SUMS_UNROLL; u++) {
+                if (k0 + u <= maxk) {
+                    const size_t c = (size_t)(k0 + u - 1) * plane + col;
+                    x[u] = StateIO<R>::ld(st, lo, (size_t)l * tstride + c);
+                    a[u] = q == 0 ? V[c] : os[c];
+                    b[u] = q == 0 ? 0.0 : Mm[c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SUMS_UNROLL; u++) {
+                if (k0 + u <= maxk) {
+                    if (q == 0) {
+                        const double Sn = x[u] + SALN0 + 0.0;
+                        acc += Sn * a[u];
+                    } else {
+                        const double lv = x[u] * a[u] * rSold;
+                        acc += lv * b[u];
+                    }
+                }
+            }
+        }
+        M.colsum[(size_t)(q == 0 ? 0 : l) * g.nwet + slot] = acc;
+    }
 }
 
 // global sums in the reference's column order (partialtot(n) summed over
@@ -616,6 +648,11 @@ void launch_bio(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
 void launch_coupling(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax, int prec,
                      hipStream_t s) {
     const dim3 gapp((nwetmax + APPLY_COLS - 1) / APPLY_COLS, nlist);
+    const dim3 gsum((nwetmax + SUMS_COLS - 1) / SUMS_COLS, nlist);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_couple_sums<double>, gsum, dim3(SUMS_THREADS), 0, s, mems, list);
+    else
+        hipLaunchKernelGGL(k_couple_sums<float>, gsum, dim3(SUMS_THREADS), 0, s, mems, list);
     hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), RED_LDS_BYTES, s, mems, list);
     if (prec == 0)
         hipLaunchKernelGGL(k_couple_apply<double>, gapp, dim3(APPLY_THREADS), apply_lds_bytes(maxkmax), s, mems, list);

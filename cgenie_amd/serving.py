@@ -1,0 +1,215 @@
+"""Continuous-batching service for ensemble runs of the ocean tracer step.
+
+Requests (a grid, a tracer count, a run length in model years, a precision
+form and a seed) arrive over time.  The server keeps every admitted run
+resident on the GPU and steps all of them together: each round is one
+batched call (one launch per kernel for the whole mixed set of grids, tracer
+counts and precisions).  Between rounds it retires finished runs, frees
+their memory and admits waiting requests while the HBM budget allows, so
+new work joins runs already on the device instead of queueing behind them.
+
+A run's model year is `nyear` ocean steps of its grid (48 at 18x18x8, 96
+otherwise, as the reference's job configuration picks).  The server records
+the wall time at which each model year of each run completes; the per-year
+latency is the time between consecutive year completions of a run.
+
+Runs with at least seven tracers carry BIOGEM (T, S, DIC, ALK, PO4, O2,
+DOM_P, then dyes) and run the full step (tstepo, step_biogem,
+biogem_tracercoupling); smaller runs are T, S and dyes on GOLDSTEIN alone.
+"""
+from __future__ import annotations
+
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine as E
+from . import grid as G
+
+GRIDS = ((18, 18, 8), (36, 36, 16), (72, 72, 32))
+
+
+def steps_per_year(dims) -> int:
+    return 48 if dims == (18, 18, 8) else 96
+
+
+@dataclass
+class Request:
+    rid: int
+    dims: tuple
+    L: int
+    years: int
+    precision: int
+    seed: int
+    arrival: float = 0.0          # seconds after the server starts
+    # filled in by the server
+    member: int = -1
+    nbytes: int = 0
+    steps_done: int = 0
+    admitted: float = -1.0
+    finished: float = -1.0
+    year_done: list = field(default_factory=list)
+    result: np.ndarray | None = None
+    inventory: np.ndarray | None = None
+
+    @property
+    def nyear(self) -> int:
+        return steps_per_year(self.dims)
+
+    @property
+    def total_steps(self) -> int:
+        return self.years * self.nyear
+
+
+class GridCache:
+    """One engine grid (mask, metrics) and base velocity per grid shape."""
+
+    def __init__(self, eng: E.Engine, seed: int = 2024):
+        self.eng, self.seed, self.g = eng, seed, {}
+
+    def get(self, dims):
+        if dims not in self.g:
+            g = G.make_grid(*dims, nyear=steps_per_year(dims))
+            k1 = G.make_k1(g, self.seed)
+            u = G.make_velocity(g, k1, self.seed + 1)
+            gid = self.eng.add_grid(g, k1)
+            self.g[dims] = (gid, g, k1, u)
+        return self.g[dims]
+
+
+def request_inputs(req: Request, g: G.Grid, k1: np.ndarray, base_u: np.ndarray):
+    """Seeded inputs of one run: tracers, velocity scale, diffusivities, light."""
+    rng = np.random.default_rng(req.seed)
+    kind = "config2" if req.L >= 7 else "ts"
+    tset = G.tracer_set(kind, req.L)
+    ts = G.make_tracers(g, k1, tset, req.seed + 1)
+    kh = rng.uniform(500.0, 5000.0)
+    kv = math.exp(rng.uniform(math.log(1e-5), math.log(1e-4)))
+    vs = rng.uniform(0.5, 1.5)
+    light = G.make_light(g, req.seed + 2)
+    mld = G.make_mld(g, req.seed + 3, 0.0)
+    return tset, ts, base_u * vs, G.diffusivities(kh, kv), light, mld
+
+
+class Server:
+    def __init__(self, eng: E.Engine, hbm_budget: float = 200e9, round_steps: int = 48,
+                 keep_results: bool = False):
+        self.eng = eng
+        self.budget = hbm_budget
+        self.round_steps = round_steps
+        self.keep = keep_results
+        self.grids = GridCache(eng)
+        self.waiting: list = []       # heap of (arrival, rid, Request)
+        self.active: dict = {}        # member id -> Request
+        self.done: list = []
+        self.used = 0
+        self.t0 = None
+        self.rounds = 0
+
+    # -- requests
+    def submit(self, req: Request):
+        heapq.heappush(self.waiting, (req.arrival, req.rid, req))
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def _bytes(self, req: Request) -> int:
+        gid = self.grids.get(req.dims)[0]
+        return self.eng.member_bytes(gid, req.L, req.precision, req.L >= 7)
+
+    def _admit(self):
+        now = self.now()
+        while self.waiting and self.waiting[0][0] <= now:
+            req = self.waiting[0][2]
+            nb = self._bytes(req)
+            if self.used + nb > self.budget:
+                break                    # first come, first served: wait for memory
+            heapq.heappop(self.waiting)
+            gid, g, k1, base_u = self.grids.get(req.dims)
+            tset, ts, u, diff, light, mld = request_inputs(req, g, k1, base_u)
+            m = self.eng.add_member(gid, req.L, req.precision, diff, diso=True, carry=True)
+            self.eng.set_velocity(m, u)
+            self.eng.upload(m, ts)
+            if req.L >= 7:
+                p = E.default_bgc_params(1.0 / req.nyear, tset, carb_tol=1e-12, carb_maxit=1000)
+                self.eng.bgc_init(m, p, light, mld)
+            req.member, req.nbytes, req.admitted = m, nb, self.now()
+            req._shape = ts.shape
+            self.used += nb
+            self.active[m] = req
+
+    def _retire(self, req: Request):
+        if self.keep:
+            req.result = self.eng.download(req.member, req._shape)
+            req.inventory = self.eng.inventory(req.member)
+        self.eng.remove_member(req.member)
+        self.used -= req.nbytes
+        req.finished = self.now()
+        del self.active[req.member]
+        self.done.append(req)
+
+    def run(self, until_idle: bool = True, max_seconds: float | None = None, log=None):
+        """Serve until every submitted request is done (or max_seconds pass)."""
+        self.t0 = time.perf_counter() if self.t0 is None else self.t0
+        while True:
+            self._admit()
+            if not self.active:
+                if not self.waiting or (max_seconds is not None and self.now() > max_seconds):
+                    break
+                time.sleep(max(0.0, min(self.waiting[0][0] - self.now(), 0.05)))
+                continue
+            if max_seconds is not None and self.now() > max_seconds:
+                break
+            # one round: every active run advances by up to round_steps
+            n = min(self.round_steps, min(r.total_steps - r.steps_done for r in self.active.values()))
+            ids = list(self.active.keys())
+            self.eng.step(ids, n)
+            self.eng.sync()
+            t = self.now()
+            self.rounds += 1
+            for r in list(self.active.values()):
+                before = r.steps_done // r.nyear
+                r.steps_done += n
+                for _ in range(r.steps_done // r.nyear - before):
+                    r.year_done.append(t)
+                if r.steps_done >= r.total_steps:
+                    self._retire(r)
+            if log and self.rounds % 20 == 0:
+                log(f"t={t:.1f}s active={len(self.active)} waiting={len(self.waiting)} done={len(self.done)} "
+                    f"hbm={self.used / 1e9:.2f} GB")
+        return self.stats()
+
+    def stats(self) -> dict:
+        lat, member_years = [], 0.0
+        for r in self.done + list(self.active.values()):
+            prev = r.admitted
+            for t in r.year_done:
+                lat.append(t - prev)
+                prev = t
+            member_years += r.steps_done / r.nyear
+        wall = self.now()
+        return dict(wall_s=wall, member_years=member_years, member_years_per_s=member_years / max(wall, 1e-9),
+                    requests_done=len(self.done), requests_active=len(self.active),
+                    requests_waiting=len(self.waiting), rounds=self.rounds,
+                    p50_year_s=float(np.percentile(lat, 50)) if lat else None,
+                    p99_year_s=float(np.percentile(lat, 99)) if lat else None)
+
+
+def poisson_requests(n: int, rate: float, seed: int, max_years: int = 1000, min_years: int = 10,
+                     grids=GRIDS, L_range=(2, 40)) -> list:
+    """A seeded request stream: Poisson arrivals, grids uniform over `grids`,
+    tracer counts uniform in L_range, run lengths log-uniform in
+    [min_years, max_years], precision FP32C or FP64 with equal odds."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for q in range(n):
+        t += rng.exponential(1.0 / rate)
+        dims = grids[rng.integers(len(grids))]
+        L = int(rng.integers(L_range[0], L_range[1] + 1))
+        years = int(round(math.exp(rng.uniform(math.log(min_years), math.log(max_years)))))
+        prec = E.FP32C if rng.random() < 0.5 else E.FP64
+        out.append(Request(q, dims, L, years, prec, seed * 1000 + q, arrival=t))
+    return out

@@ -133,9 +133,10 @@ int cgo_step(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
  * remineralisation and surface chemistry of the carried species; dtyr is the
  * one given to cgo_bgc_init.
  * cgo_tracercoupling replaces biogem_tracercoupling(dum_ts, dum_ts1)
- * (biogem.f90:1885-1890); go_ts and go_ts1 are the one device state.  It
- * must follow cgo_bio_step (the column sums it needs are taken there) and
- * nothing else may step those members in between (CGO_E_STATE). */
+ * (biogem.f90:1885-1890); go_ts and go_ts1 are the one device state.  As in
+ * the reference it must follow cgo_bio_step (it applies the anomalies that
+ * call leaves), and nothing else may step those members in between
+ * (CGO_E_STATE otherwise). */
 int cgo_tstepo(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
 int cgo_bio_step(cgo_engine *e, const int32_t *members, int32_t n);
 int cgo_tracercoupling(cgo_engine *e, const int32_t *members, int32_t n);
