@@ -139,6 +139,23 @@ def run_gpu(args):
     return res
 
 
+def run_serving(seconds: float, rate: float = 4.0, seed: int = 5):
+    """The serving form: a seeded Poisson stream (mean `rate` requests/s) of
+    runs on 18x18x8, 36x36x16 and 72x72x32 with 2-40 tracers, 10-1000 model
+    years, FP32C or FP64, admitted against a 200 GB HBM budget and stepped
+    together; served for `seconds` of wall time."""
+    from cgenie_amd import engine as E
+    from cgenie_amd import serving as S
+
+    eng = E.Engine(0)
+    srv = S.Server(eng, hbm_budget=200e9, round_steps=48)
+    for r in S.poisson_requests(int(rate * seconds * 2) + 8, rate, seed):
+        srv.submit(r)
+    st = srv.run(max_seconds=seconds, log=lambda m: print("# serve " + m, file=sys.stderr, flush=True))
+    eng.close()
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+
+
 def _cpu_worker(args):
     """One serial member-year of the CPU oracle (BIOGEM every step)."""
     q, nsteps = args
@@ -185,6 +202,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("CGO_CPU_CORES", "16")))
     ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -194,6 +212,7 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline(cores, args.cpu_steps)
     print(f"# cpu baseline done: {cpu}", file=sys.stderr, flush=True)
     r = run_gpu(args)
+    serving = run_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     main_ = r["fp32c"]
     roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4), traffic=None)
@@ -213,7 +232,7 @@ def main():
                                    tracer_block=main_["tracer_block"]),
                         fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
                                   bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"])),
-        roofline=roof, cpu_baseline=cpu)
+        roofline=roof, cpu_baseline=cpu, serving=serving)
     print(json.dumps(line), flush=True)
 
 

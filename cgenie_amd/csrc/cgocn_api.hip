@@ -339,7 +339,7 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
     const bool lo = prec == CGO_FP32C && carry;
     m.lo[0] = lo ? take(L * ncell * 2) : 0;
     m.lo[1] = lo ? take(L * ncell * 2) : 0;
-    m.u = take(3 * ncell * es);
+    m.u = take((prec == CGO_FP64 ? 3 : 4) * ncell * es);  // FP32C: u, v, w and the FP64 divergence
     m.sflux = take(L * ncol * 8);
     m.rho = take(ncell * 8);
     m.cost = take(ncol * 8);
@@ -494,7 +494,36 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
     if (m->prec == CGO_FP64) {
         HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), h.data(), h.size() * 8, hipMemcpyHostToDevice));
     } else {
-        std::vector<float> f(h.begin(), h.end());
+        // FP32 form: float velocities plus the discrete divergence of the open
+        // faces of each wet cell evaluated in FP64 from the FP64 velocities
+        // (the flux kernel works with q - q_cell and adds q_cell * div u)
+        const GridH* gh = e->grids[m->grid];
+        const int JS = mj + 2, KS = mk + 2;
+        const size_t ncol = g.ncol;
+        auto K1 = [&](int I, int J) { return gh->k1[(size_t)J * (mi + 2) + I]; };
+        std::vector<float> f(4 * ncell, 0.0f);
+        for (size_t x = 0; x < 3 * ncell; x++) f[x] = (float)h[x];
+        for (int k = 1; k <= mk; k++)
+            for (int J = 1; J <= mj; J++)
+                for (int I = 1; I <= mi; I++) {
+                    const int k1c = K1(I, J);
+                    if (k < k1c) continue;
+                    const size_t o = (size_t)(k - 1) * ncol;
+                    const size_t c = o + (I - 1) + (size_t)mi * (J - 1);
+                    const size_t cw = o + (size_t)mi * (J - 1) + (I == 1 ? mi - 1 : I - 2);
+                    const size_t cs = o + (size_t)mi * (J > 1 ? J - 2 : 0) + (I - 1);
+                    const bool oe = k >= std::max(k1c, K1(I + 1, J)), ow = k >= std::max(K1(I - 1, J), k1c);
+                    const bool on = k >= std::max(k1c, K1(I, J + 1)), os = k >= std::max(K1(I, J - 1), k1c);
+                    const double rc = gh->mj[MJ_RC * JS + J], rds = gh->mj[MJ_RDS * JS + J];
+                    const double cvn = gh->mj[MJ_CV * JS + J], cvs = gh->mj[MJ_CV * JS + J - 1];
+                    const double rdz = gh->mk[MK_RDZ * KS + k];
+                    const double ue = h[c], uw = h[cw], un = h[ncell + c], us = h[ncell + cs];
+                    const double ua = k == mk ? 0.0 : h[2 * ncell + c];
+                    const double wb = k > k1c ? h[2 * ncell + c - ncol] : 0.0;
+                    const double div = ((oe ? ue * rc : 0.0) - (ow ? uw * rc : 0.0)) * g.rdphi +
+                                       ((on ? cvn * un : 0.0) - (os ? cvs * us : 0.0)) * rds + (ua - wb) * rdz;
+                    f[3 * ncell + c] = (float)div;
+                }
         HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), f.data(), f.size() * 4, hipMemcpyHostToDevice));
     }
     return CGO_OK;

@@ -57,7 +57,11 @@ template <> struct StateIO<float> {
         float h = (float)x;
         static_cast<float*>(s)[i] = h;
         if (lo) {
-            double r = rint((x - (double)h) / carry_unit(h));
+            // x - h in units of carry_unit(h): scaling by the exact power of two
+            // 2^(39 - e) rounds exactly as dividing by 2^(e - 39) would
+            int e = (h == 0.0f) ? -126 : ilogbf(h);
+            if (e < -126) e = -126;
+            double r = rint(ldexp(x - (double)h, 23 + 16 - e));
             r = fmin(fmax(r, -32767.0), 32767.0);
             lo[i] = (int16_t)r;
         }
@@ -76,8 +80,18 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
 // neighbour of the block's columns -- and the carried bottom-face flux.
 constexpr int FLUX_NBUF = 3;
 constexpr int FLUX_PF = 3;  // tile positions per thread (TW <= 3 * FLUX_THREADS)
+// The FP32 form also stages the carry of every value, as a float residual,
+// so differences of neighbours are taken at the stored (~40-bit) precision.
 __host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es) {
-    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * 128 * es;
+    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * 128 * es +
+           (es == 4 ? (size_t)FLUX_NBUF * (lb + 2) * tw * 4 : 0);
+}
+
+// the carry c of a float h as a float: c * 2^-16 ulp(h) (exact unless |h| < 2^-87)
+__device__ __forceinline__ float carry_residual(float h, int16_t c) {
+    int e = (h == 0.0f) ? -126 : ilogbf(h);
+    if (e < -126) e = -126;
+    return ldexpf((float)c, e - 23 - 16);
 }
 
 template <typename R, int LB>
@@ -97,6 +111,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     int* sk1 = reinterpret_cast<int*>(smem);
     R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
     R* sfb = sp + FLUX_NBUF * (LB + 2) * tw;
+    float* sr = reinterpret_cast<float*>(sfb + LB * FLUX_THREADS);  // FP32 form: residual ring
     // slot q < LB: tracer l0+q; T and S sit in slots 0/1 when l0 == 0, else LB/LB+1
     const int qT = l0 == 0 ? 0 : LB, qS = l0 == 0 ? 1 : LB + 1;
     const int nslot = l0 == 0 ? nl : LB + 2;  // slots staged per level
@@ -120,6 +135,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const int kmin = s_kmin;
     if (kmin > maxk) return;  // every column of the block is land
 
+    constexpr bool ANOM = sizeof(R) == 4;  // FP32 form: fluxes of q - q_cell plus q_cell * div(u)
     const int par = *M.parity;
     const AS_GLOBAL R* __restrict__ src = gp(static_cast<const R*>(M.st[par]));
     AS_GLOBAL R* __restrict__ dst = gp(static_cast<R*>(M.st[par ^ 1]));
@@ -150,13 +166,24 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R diffv = sizeof(R) == 8 ? (R)M.diff2 : (R)M.diff2f;
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
-    constexpr bool ANOM = sizeof(R) == 4;  // FP32 form: fluxes of q - q_cell plus q_cell * div(u)
 
     auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * (LB + 2) + q) * tw + pp]; };
+    auto SR = [&](int buf, int q, int pp) -> float& { return sr[(buf * (LB + 2) + q) * tw + pp]; };
+    // difference of two staged values; the FP32 form adds the residuals
+    auto D = [&](int buf, int q, int pa, int pb) -> R {
+        if (ANOM) return (SP(buf, q, pa) - SP(buf, q, pb)) + (R)(SR(buf, q, pa) - SR(buf, q, pb));
+        return SP(buf, q, pa) - SP(buf, q, pb);
+    };
+    // difference across levels (same column)
+    auto DK = [&](int bu, int bc, int q, int pp) -> R {
+        if (ANOM) return (SP(bu, q, pp) - SP(bc, q, pp)) + (R)(SR(bu, q, pp) - SR(bc, q, pp));
+        return SP(bu, q, pp) - SP(bc, q, pp);
+    };
     auto FB = [&](int q) -> R& { return sfb[q * FLUX_THREADS + tid]; };
     auto slot_tracer = [&](int q) { return q < LB ? l0 + q : q - LB; };
     // prefetch of one plane into registers, then into the LDS ring
     R pf[LB + 2][FLUX_PF];
+    int16_t pc[ANOM ? LB + 2 : 1][FLUX_PF];
     auto fetch = [&](int kz) {
         const int o = kz * plane + t0;
 #pragma unroll
@@ -165,7 +192,11 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const bool ok = pp < TW && kz + 1 >= sk1[pp < TW ? pp : 0];
 #pragma unroll
             for (int q = 0; q < LB + 2; q++)
-                if (q < nslot || q >= LB) pf[q][r] = (ok && (q < nslot)) ? src[slot_tracer(q) * tstride + o + pp] : (R)0;
+                if (q < nslot || q >= LB) {
+                    const bool ld = ok && (q < nslot);
+                    pf[q][r] = ld ? src[slot_tracer(q) * tstride + o + pp] : (R)0;
+                    if (ANOM) pc[ANOM ? q : 0][r] = (ld && lo_src) ? lo_src[slot_tracer(q) * tstride + o + pp] : (int16_t)0;
+                }
         }
     };
     auto commit = [&](int buf) {
@@ -175,14 +206,17 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             if (pp < TW) {
 #pragma unroll
                 for (int q = 0; q < LB + 2; q++)
-                    if (q < nslot) SP(buf, q, pp) = pf[q][r];
+                    if (q < nslot) {
+                        SP(buf, q, pp) = pf[q][r];
+                        if (ANOM) SR(buf, q, pp) = carry_residual((float)pf[q][r], pc[ANOM ? q : 0][r]);
+                    }
             }
         }
     };
 
     for (int q = 0; q < LB; q++) FB(q) = (R)0;
     int32_t limps = 0;
-    R dmaxv = (R)0, wb = (R)0;
+    R dmaxv = (R)0;
     fetch(kmin - 1);
     commit(kmin % FLUX_NBUF);
     if (kmin < maxk) {
@@ -216,10 +250,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             bool iso = false;
             R dzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
             if (!top) {
-                const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p), Sc = SP(cb, qS, p), Su = SP(ub, qS, p);
+                const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p);
                 const R tatw = (R)0.5 * (Tc + Tu);
                 const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
-                dzrho = (e1 * (Su - Sc) - tec * (Tu - Tc)) * rdza;
+                dzrho = (e1 * DK(ub, cb, qS, p) - tec * DK(ub, cb, qT, p)) * rdza;
                 if (M.diso && dzrho < (R)-1.0E-12) {
                     iso = true;
                     const R rdzrho = (R)1.0 / dzrho;
@@ -231,12 +265,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         const int kn = n < 2 ? k : k + 1;
                         const bool east = n & 1;
                         const bool mx = kn >= (east ? k1e : k1w), my = kn >= (east ? k1n : k1s);
-                        const R dxT = mx ? (east ? SP(b, qT, pe) - SP(b, qT, p) : SP(b, qT, p) - SP(b, qT, pw)) * rc * rdphi : (R)0;
-                        const R dxS = mx ? (east ? SP(b, qS, pe) - SP(b, qS, p) : SP(b, qS, p) - SP(b, qS, pw)) * rc * rdphi : (R)0;
-                        const R dyT = my ? (east ? (SP(b, qT, pn) - SP(b, qT, p)) * cvn * rdsvn
-                                                 : (SP(b, qT, p) - SP(b, qT, ps)) * cvs * rdsvs) : (R)0;
-                        const R dyS = my ? (east ? (SP(b, qS, pn) - SP(b, qS, p)) * cvn * rdsvn
-                                                 : (SP(b, qS, p) - SP(b, qS, ps)) * cvs * rdsvs) : (R)0;
+                        const R dxT = mx ? (east ? D(b, qT, pe, p) : D(b, qT, p, pw)) * rc * rdphi : (R)0;
+                        const R dxS = mx ? (east ? D(b, qS, pe, p) : D(b, qS, p, pw)) * rc * rdphi : (R)0;
+                        const R dyT = my ? (east ? D(b, qT, pn, p) * cvn * rdsvn : D(b, qT, p, ps) * cvs * rdsvs) : (R)0;
+                        const R dyS = my ? (east ? D(b, qS, pn, p) * cvn * rdsvn : D(b, qS, p, ps) * cvs * rdsvs) : (R)0;
                         dxr[n] = scc * dxS - tec * dxT;
                         dyr[n] = scc * dyS - tec * dyT;
                         tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
@@ -255,12 +287,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 }
             }
             // discrete divergence of the open faces (FP32 form only)
-            R divu = 0;
-            if (ANOM) {
-                divu = ((oe ? ue_ * rc : (R)0) - (ow ? uw_ * rc : (R)0)) * rdphi +
-                       ((on ? cvn * un_ : (R)0) - (os ? cvs * us_ : (R)0)) * rds +
-                       ((top ? (R)0 : ua_) - (k > k1c ? wb : (R)0)) * rdz;
-            }
+            // (evaluated in FP64 from the FP64 velocities when they are set)
+            const R divu = ANOM ? u[3 * ncell + o + col] : (R)0;
             const R isoc = (R)0.25 * slim * diff1;
             double newT = 0, newS = 0;
 #pragma unroll 1
@@ -273,7 +301,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     fn = on ? cvn * un_ * (((R)1.0 - upn) * Tn + ((R)1.0 + upn) * Tc) * (R)0.5 - cv2n * (Tn - Tc) * diff1 : (R)0;
                     fs = os ? cvs * us_ * (((R)1.0 - ups) * Tc + ((R)1.0 + ups) * Ts) * (R)0.5 - cv2s * (Tc - Ts) * diff1 : (R)0;
                 } else {
-                    const R dE = Te - Tc, dW = Tw - Tc, dN = Tn - Tc, dS = Ts - Tc;
+                    const R dE = D(cb, q, pe, p), dW = D(cb, q, pw, p), dN = D(cb, q, pn, p), dS = D(cb, q, ps, p);
                     fe = oe ? ue_ * rc * ((R)1.0 - upe) * dE * (R)0.5 - dE * rc2 * diff1 : (R)0;
                     fw = ow ? uw_ * rc * ((R)1.0 + upw) * dW * (R)0.5 + dW * rc2 * diff1 : (R)0;
                     fn = on ? cvn * un_ * ((R)1.0 - upn) * dN * (R)0.5 - cv2n * dN * diff1 : (R)0;
@@ -283,13 +311,14 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     fa = M.sflux ? (R)gp(M.sflux)[(l0 + q) * plane + col] : (R)0;
                 } else {
                     const R Tu = SP(ub, q, p);
-                    R nadv = -(Tu - Tc) * rdza * diffv;
+                    const R dUc = DK(ub, cb, q, p);
+                    R nadv = -dUc * rdza * diffv;
                     if (!ANOM) {
                         fa = ua_ * (((R)1.0 - upa) * Tu + ((R)1.0 + upa) * Tc) * (R)0.5;
                         fa = fa - (Tu - Tc) * rdza * diffv;
                     }
                     if (iso) {
-                        const R dzts = (Tu - Tc) * rdza;
+                        const R dzts = dUc * rdza;
                         R tv = 0;
 #pragma unroll
                         for (int n = 0; n < 4; n++) {
@@ -297,11 +326,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                             const bool east = n & 1;
                             const int kn = n < 2 ? k : k + 1;
                             const R dx = kn >= (east ? k1e : k1w)
-                                             ? (east ? SP(b, q, pe) - SP(b, q, p) : SP(b, q, p) - SP(b, q, pw)) * rc * rdphi
+                                             ? (east ? D(b, q, pe, p) : D(b, q, p, pw)) * rc * rdphi
                                              : (R)0;
                             const R dy = kn >= (east ? k1n : k1s)
-                                             ? (east ? (SP(b, q, pn) - SP(b, q, p)) * cvn * rdsvn
-                                                     : (SP(b, q, p) - SP(b, q, ps)) * cvs * rdsvs)
+                                             ? (east ? D(b, q, pn, p) * cvn * rdsvn : D(b, q, p, ps) * cvs * rdsvs)
                                              : (R)0;
                             tv = tv + ((R)2 * dzrho * dx - dxr[n] * dzts) * dxr[n] + ((R)2 * dzrho * dy - dyr[n] * dzts) * dyr[n];
                         }
@@ -310,9 +338,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         nadv = nadv + tv;
                     }
                     if (ANOM) {
-                        const R dU = Tu - Tc;
-                        fa = ua_ * ((R)1.0 - upa) * dU * (R)0.5 + nadv;       // relative to this cell
-                        fbn = -(ua_ * ((R)1.0 + upa) * dU * (R)0.5) + nadv;   // relative to the cell above
+                        fa = ua_ * ((R)1.0 - upa) * dUc * (R)0.5 + nadv;       // relative to this cell
+                        fbn = -(ua_ * ((R)1.0 + upa) * dUc * (R)0.5) + nadv;   // relative to the cell above
                     }
                 }
                 const R div = ((fe - fw) * rdphi + (fn - fs) * rds) + (fa - FB(q)) * rdz;
@@ -323,7 +350,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     if (lo_src) {
                         // compensated accumulation: the rounding residual of every
                         // update is carried to the next step in the int16 plane
-                        nv = StateIO<R>::ld((const R*)src, lo_src, idx) + (double)inc;
+                        nv = ((double)Tc + (double)SR(cb, q, p)) + (double)inc;
                         StateIO<R>::st((R*)dst, lo_dst, idx, nv);
                     } else {
                         const R v = Tc + inc;
@@ -341,7 +368,6 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 if (l0 + q == 1) newS = nv;
             }
             if (l0 == 0) rhop[o + col] = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
-            wb = ua_;
         }
         if (more) commit((k + 2) % FLUX_NBUF);
         __syncthreads();

@@ -22,6 +22,7 @@ from __future__ import annotations
 import heapq
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -108,6 +109,9 @@ class Server:
         self.used = 0
         self.t0 = None
         self.rounds = 0
+        # host-side inputs of arrived requests are prepared off the stepping loop
+        self.pool = ThreadPoolExecutor(max_workers=2)
+        self.prep: dict = {}
 
     # -- requests
     def submit(self, req: Request):
@@ -120,16 +124,28 @@ class Server:
         gid = self.grids.get(req.dims)[0]
         return self.eng.member_bytes(gid, req.L, req.precision, req.L >= 7)
 
+    def _prepare(self, req: Request):
+        if req.rid not in self.prep:
+            gid, g, k1, base_u = self.grids.get(req.dims)
+            self.prep[req.rid] = self.pool.submit(request_inputs, req, g, k1, base_u)
+        return self.prep[req.rid]
+
     def _admit(self):
         now = self.now()
+        for _, _, r in self.waiting:
+            if r.arrival <= now:
+                self._prepare(r)
         while self.waiting and self.waiting[0][0] <= now:
             req = self.waiting[0][2]
+            fut = self._prepare(req)
+            if not fut.done():
+                break                    # inputs still being prepared
             nb = self._bytes(req)
             if self.used + nb > self.budget:
                 break                    # first come, first served: wait for memory
             heapq.heappop(self.waiting)
-            gid, g, k1, base_u = self.grids.get(req.dims)
-            tset, ts, u, diff, light, mld = request_inputs(req, g, k1, base_u)
+            gid = self.grids.get(req.dims)[0]
+            tset, ts, u, diff, light, mld = self.prep.pop(req.rid).result()
             m = self.eng.add_member(gid, req.L, req.precision, diff, diso=True, carry=True)
             self.eng.set_velocity(m, u)
             self.eng.upload(m, ts)
@@ -159,7 +175,7 @@ class Server:
             if not self.active:
                 if not self.waiting or (max_seconds is not None and self.now() > max_seconds):
                     break
-                time.sleep(max(0.0, min(self.waiting[0][0] - self.now(), 0.05)))
+                time.sleep(min(max(self.waiting[0][0] - self.now(), 0.002), 0.05))
                 continue
             if max_seconds is not None and self.now() > max_seconds:
                 break

@@ -49,11 +49,22 @@ class CountingEngine:
         del self.members[m]
 
 
+class SyncPool:
+    """Runs submitted work at once (deterministic admission in the test)."""
+
+    def submit(self, fn, *a):
+        from concurrent.futures import Future
+        f = Future()
+        f.set_result(fn(*a))
+        return f
+
+
 def test_scheduler_admits_by_budget_and_recycles(monkeypatch):
     monkeypatch.setattr(S, "request_inputs", lambda req, g, k1, u: (None, np.zeros(1), None, (0, 0), None, None))
     monkeypatch.setattr(S.GridCache, "get", lambda self, dims: (0, None, None, None))
     eng = CountingEngine(bytes_per_member=100)
     srv = S.Server(eng, hbm_budget=1000, round_steps=48)
+    srv.pool = SyncPool()
     # three runs of 4 tracers (400 B each): only two fit at once
     reqs = [S.Request(q, (36, 36, 16), 4, years=2 + q, precision=E.FP64, seed=q) for q in range(3)]
     for r in reqs:
