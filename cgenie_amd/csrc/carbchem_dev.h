@@ -125,7 +125,7 @@ __device__ __forceinline__ void carb_sweep(double H, double DIC, double ALK, dou
 // sub_calc_carb: >0 iterations, -1 negative [H] estimate, -2 iteration limit
 // (ctrl_carbchem_fail: carb is left unchanged, no random re-seed)
 __device__ inline int calc_carb(double DIC, double ALK, double Ca, double PO4, double Bt, double SO4t, double Ft,
-                                const double* cc, double* carb, size_t cs, double tol, int maxit) {
+                                const double* cc, AS_GLOBAL double* carb, size_t cs, double tol, int maxit) {
     int n = 1;
     double H = carb[IC_H * cs];
     Spec o;
@@ -152,7 +152,7 @@ __device__ inline int calc_carb(double DIC, double ALK, double Ca, double PO4, d
 }
 
 __device__ inline int calc_carb_RF0(double DIC, double ALK, double PO4, double Bt, double SO4t, double Ft,
-                                    const double* cc, double* carb, size_t cs, double tol, int maxit) {
+                                    const double* cc, AS_GLOBAL double* carb, size_t cs, double tol, int maxit) {
     int n = 1;
     double H = carb[IC_H * cs];
     const double DICr = DIC + 1.0e-6;

@@ -12,16 +12,18 @@
 //             remineralisation (2412-2877), the sediment return of the
 //             previous step's rain (biogem.f90:873-942) and PO4 uptake
 //             (biogem_box.f90:346-1515).  Every cell is read and written once.
-// k_couple_sums    one thread per (member, wet column, tracer): the column
-//             partial sums of the state after transport (biogem.f90:1984-1997,
+// k_couple_sums    one block per (member, 64 wet columns): the column partial
+//             sums of the state after transport (biogem.f90:1984-1997,
 //             2016-2025).
 // k_couple_reduce  one block per member: the global sums, in the reference's
 //             column order (biogem.f90:1927-2031), staged through LDS.
 // k_couple_apply   one block per (member, 64 wet columns): the rescale and
-//             re-normalisation of every tracer (2033-2061), one thread per
-//             (column, tracer), then T, S, masses and particulates per cell.
+//             re-normalisation of every tracer, T, S, masses and
+//             particulates (2001-2061).
 #include "carbchem_dev.h"
 #include "tstep_impl.h"
+
+#include <algorithm>
 
 namespace cgo {
 
@@ -72,8 +74,8 @@ __device__ __forceinline__ void add_remin(const BgcParams& p, const Conv& cv, do
 // one level of a sinking particle stream (the kk body of the settling loop of
 // sub_box_remin_part, biogem_box.f90:2555-2693): t holds the flux leaving
 // level kk+1 and becomes the flux leaving kk; prem is what dissolves in kk
-__device__ __forceinline__ void stream_step(const BgcParams& p, const double* rt, int KS, int kk, double* t,
-                                            double* prem) {
+__device__ __forceinline__ void stream_step(const BgcParams& p, const AS_GLOBAL double* rt, int KS, int kk,
+                                            double* t, double* prem) {
     const double layerratio = rt[RT_LAYER * KS + kk];
     double tn[NSED];
 #pragma unroll
@@ -127,16 +129,16 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     const int w = blockIdx.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
     const BgcParams& p = M.bp;
-    const int col = g.wetsorted[w];
+    const int col = gp(g.wetsorted)[w];
     const int maxi = g.maxi, maxk = g.maxk;
     const int kb = k1at(g, col % maxi + 1, col / maxi + 1);
     const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell;
     const int KS = maxk + 2;
-    const double* __restrict__ Dbot = g.mk + MK_DBOT * KS;
-    const double* __restrict__ Dmid = g.mk + MK_DMID * KS;
-    const double* __restrict__ rt = M.rtab;
-    const double* __restrict__ ocn = M.ocn;  // plane 0 = T, 1 = S, then species planes
-    const double* __restrict__ Mm = M.M;
+    const AS_GLOBAL double* __restrict__ Dbot = gp(g.mk) + MK_DBOT * KS;
+    const AS_GLOBAL double* __restrict__ Dmid = gp(g.mk) + MK_DMID * KS;
+    const AS_GLOBAL double* __restrict__ rt = gp(M.rtab);
+    const AS_GLOBAL double* __restrict__ ocn = gp(M.ocn);  // plane 0 = T, 1 = S, then species planes
+    const AS_GLOBAL double* __restrict__ Mm = gp(M.M);
     auto cell = [&](int k) -> size_t { return (size_t)(k - 1) * plane + col; };
     const bool hasO2 = p.slot[B_O2] >= 0, hasDOMC = p.slot[B_DOMC] >= 0, hasDOMP = p.slot[B_DOMP] >= 0;
     const bool hasPO4 = p.slot[B_PO4] >= 0;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     // (1) surface carbonate chemistry (biogem.f90:1026-1084)
     double ohm = 0.0;
     if (p.slot[B_DIC] >= 0 && p.slot[B_ALK] >= 0) {
-        double* carb = M.carb + col;
+        AS_GLOBAL double* carb = gp(M.carb) + col;
         const size_t cs = cell(maxk);
         const double Tsur = ocn[cs], Ssur = ocn[ncell + cs];
         double cc[NCC];
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     double upart[NSED] = {0.0, 0.0, 0.0, 0.0, 0.0}, upt[4] = {0.0, 0.0, 0.0, 0.0}, domC = 0.0, domP = 0.0;
     if (hasPO4) {
         int k_mld = kb;
-        const double mld = M.mld[col];
+        const double mld = gp(M.mld)[col];
         for (int k = maxk; k >= 1; k--)
             if (Dbot[k] >= mld) {
                 k_mld = k;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         const double lPO4 = ocn[(size_t)p.ocn[B_PO4] * ncell + cell(maxk)];
         const double kPO4 = lPO4 / (lPO4 + p.c0_PO4);
         const double ficefree = (1.0 - 0.0);
-        const double kI = M.light[col];
+        const double kI = gp(M.light)[col];
         double dPO4 = 0.0;
         if (lPO4 > NULLSMALL) dPO4 = p.dtyr * ficefree * kI * kPO4 * p.k0_PO4;
         double DOMfrac = p.DOMfrac;
@@ -214,18 +216,18 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
     }
 
     // (3) the march, surface to k1
-    double* __restrict__ part = M.part;
-    double* __restrict__ vd = M.vd;
-    const double* __restrict__ pO2 = hasO2 ? ocn + (size_t)p.ocn[B_O2] * ncell : nullptr;
-    const double* __restrict__ pDC = hasDOMC ? ocn + (size_t)p.ocn[B_DOMC] * ncell : nullptr;
-    const double* __restrict__ pDP = hasDOMP ? ocn + (size_t)p.ocn[B_DOMP] * ncell : nullptr;
+    AS_GLOBAL double* __restrict__ part = gp(M.part);
+    AS_GLOBAL double* __restrict__ vd = gp(M.vd);
+    const AS_GLOBAL double* __restrict__ pO2 = hasO2 ? ocn + (size_t)p.ocn[B_O2] * ncell : nullptr;
+    const AS_GLOBAL double* __restrict__ pDC = hasDOMC ? ocn + (size_t)p.ocn[B_DOMC] * ncell : nullptr;
+    const AS_GLOBAL double* __restrict__ pDP = hasDOMP ? ocn + (size_t)p.ocn[B_DOMP] * ncell : nullptr;
     const int klim = (p.dtyr * p.sinkrate <= p.dsc) ? kb : maxk;
     const double ratio = (p.DOMlifetime > p.dtyr) ? p.dtyr / p.DOMlifetime : 1.0;
     const bool pfirst = hasDOMP && p.slot[B_DOMP] < p.slot[B_DOMC];
-    const double Mkb = Mm[cell(kb)], rMkb = M.rM[cell(kb)];
+    const double Mkb = Mm[cell(kb)], rMkb = gp(M.rM)[cell(kb)];
     double sprev[NSED];
 #pragma unroll
-    for (int s = 0; s < NSED; s++) sprev[s] = M.settle[(size_t)s * plane + col];
+    for (int s = 0; s < NSED; s++) sprev[s] = gp(M.settle)[(size_t)s * plane + col];
     double settle[NSED] = {0.0, 0.0, 0.0, 0.0, 0.0};
     auto load = [&](int k, Lvl& v) {
         const size_t c = cell(k);
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
                 if (kk == kb)
                     settle_add(cur.p);
                 else
-                    push(cur.p, M.mink[kk] >= kb ? M.mink[kk] : kb - 1);
+                    push(cur.p, gp(M.mink)[kk] >= kb ? gp(M.mink)[kk] : kb - 1);
             }
         }
         // sediment return of the previous step's rain (biogem.f90:873-923, 937-942)
@@ -373,78 +375,132 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         }
     }
 #pragma unroll
-    for (int s = 0; s < NSED; s++) M.settle[(size_t)s * plane + col] = settle[s];
+    for (int s = 0; s < NSED; s++) gp(M.settle)[(size_t)s * plane + col] = settle[s];
 }
 
-// column partial sums of the state after transport, one thread per (column,
-// sum): sum 0 is S (biogem.f90:2016-2025), sum q >= 1 is tracer q + 1
-// (1984-1997); levels k1..maxk in order, loads issued eight levels at a time
-constexpr int SUMS_THREADS = 256;
-constexpr int SUMS_COLS = 64;
-constexpr int SUMS_UNROLL = 8;
+// Column kernels of the coupling: one block per (member, 64 wet columns).
+// Thread (c, g) of the block owns column c's cells at levels g+1, g+5, ...
+// (CPL_LG level groups, NJ cells), so each cell's data is loaded once and the
+// tracers are taken in groups of `tg`: every thread forms its cells'
+// products for the group, the products go to LDS, and one thread per
+// (column, tracer) adds its column from k1 up, in the reference's order.
+constexpr int CPL_THREADS = 256;
+constexpr int CPL_COLS = 64;
+constexpr int CPL_LG = CPL_THREADS / CPL_COLS;         // level groups
+constexpr int CPL_MAXTG = 4;                            // tracers per LDS round
+constexpr size_t CPL_LDS_CAP = 64 * 1024;
 
-template <typename R>
-__global__ __launch_bounds__(SUMS_THREADS) void k_couple_sums(const MemberDev* __restrict__ mems,
-                                                              const int32_t* __restrict__ list) {
-    __shared__ int32_t s_col[SUMS_COLS], s_kb[SUMS_COLS];
-    const MemberDev& M = mems[list[blockIdx.y]];
-    const GridDev& g = M.g;
-    const int w0 = blockIdx.x * SUMS_COLS;
-    if (w0 >= g.nwet) return;  // uniform over the block
-    const int t = threadIdx.x, maxi = g.maxi, maxk = g.maxk, L = M.L;
-    if (t < SUMS_COLS) {
+__host__ __device__ inline int cpl_tg(int maxk) {
+    int tg = (int)(CPL_LDS_CAP / ((size_t)maxk * CPL_COLS * 8));
+    return tg > CPL_MAXTG ? CPL_MAXTG : (tg < 1 ? 1 : tg);
+}
+
+struct CplTile {
+    int col[CPL_COLS], kb[CPL_COLS], slot[CPL_COLS];
+};
+
+__device__ __forceinline__ void cpl_tile(const GridDev& g, int w0, CplTile& T) {
+    const int t = threadIdx.x;
+    if (t < CPL_COLS) {
         const int w = w0 + t;
-        const int col = w < g.nwet ? g.wetsorted[w] : 0;
-        s_col[t] = col;
-        s_kb[t] = w < g.nwet ? k1at(g, col % maxi + 1, col / maxi + 1) : maxk + 1;
+        const int col = w < g.nwet ? gp(g.wetsorted)[w] : 0;
+        T.col[t] = col;
+        T.kb[t] = w < g.nwet ? k1at(g, col % g.maxi + 1, col / g.maxi + 1) : g.maxk + 1;
+        T.slot[t] = w < g.nwet ? gp(g.vpos)[w] : 0;
     }
     __syncthreads();
-    const int cc = t % SUMS_COLS, w = w0 + cc;
-    if (w >= g.nwet) return;  // no barrier below
-    const int col = s_col[cc], kb = s_kb[cc], slot = g.vpos[w];
+}
+
+// column partial sums of the state after transport: sum 0 is S
+// (biogem.f90:2016-2025), sum q >= 1 is tracer q + 1 (1984-1997)
+template <typename R, int NJ, int TG>  // NJ cells per thread (NJ * CPL_LG >= maxk), TG >= tg
+__global__ __launch_bounds__(CPL_THREADS) void k_couple_sums(const MemberDev* __restrict__ mems,
+                                                             const int32_t* __restrict__ list) {
+    extern __shared__ double prod[];  // [tg][maxk][CPL_COLS]
+    __shared__ CplTile T;
+    const MemberDev& M = mems[list[blockIdx.y]];
+    const GridDev& g = M.g;
+    const int w0 = blockIdx.x * CPL_COLS;
+    if (w0 >= g.nwet) return;  // uniform over the block
+    cpl_tile(g, w0, T);
+    const int c = threadIdx.x % CPL_COLS, lg = threadIdx.x / CPL_COLS;
+    const int maxk = g.maxk, L = M.L, nsum = L - 1, tg = cpl_tg(maxk) < TG ? cpl_tg(maxk) : TG;
+    const int col = T.col[c], kb = T.kb[c];
     const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell, tstride = (size_t)maxk * plane;
-    const int par = *M.parity;  // the transport step has flipped: st[par] is the new state
+    const int par = *gp(M.parity);  // the transport step has flipped: st[par] is the new state
     const void* st = M.st[par];
     const int16_t* lo = M.lo[par];
-    const double rSold = 1.0 / M.scal[SC_SMEAN];
-    const double* __restrict__ os = M.ocn + ncell;
-    const double* __restrict__ V = g.V;
-    const double* __restrict__ Mm = M.M;
-    for (int q = t / SUMS_COLS; q < L - 1; q += SUMS_THREADS / SUMS_COLS) {
-        const int l = q == 0 ? 1 : q + 1;
-        double acc = 0.0;
-        for (int k0 = kb; k0 <= maxk; k0 += SUMS_UNROLL) {
-            double x[SUMS_UNROLL], a[SUMS_UNROLL], b[SUMS_UNROLL];
+    const double rSold = 1.0 / gp(M.scal)[SC_SMEAN];
+    double os[NJ], mm[NJ];
 #pragma unroll
-            for (int u = 0; u < SUMS_UNROLL; u++) {
-                if (k0 + u <= maxk) {
-                    const size_t c = (size_t)(k0 + u - 1) * plane + col;
-                    x[u] = StateIO<R>::ld(st, lo, (size_t)l * tstride + c);
-                    a[u] = q == 0 ? V[c] : os[c];
-                    b[u] = q == 0 ? 0.0 : Mm[c];
-                }
-            }
+    for (int j = 0; j < NJ; j++) {
+        const int k = 1 + lg + CPL_LG * j;
+        if (k <= maxk && k >= kb) {
+            const size_t cc = (size_t)(k - 1) * plane + col;
+            os[j] = gp(M.ocn)[ncell + cc];
+            mm[j] = gp(M.M)[cc];
+        }
+    }
+    for (int q0 = 0; q0 < nsum; q0 += tg) {
+        // all loads of the group first, then the products
+        typename StateIO<R>::Raw x[TG][NJ];
+        double a[NJ];
 #pragma unroll
-            for (int u = 0; u < SUMS_UNROLL; u++) {
-                if (k0 + u <= maxk) {
-                    if (q == 0) {
-                        const double Sn = x[u] + SALN0 + 0.0;
-                        acc += Sn * a[u];
-                    } else {
-                        const double lv = x[u] * a[u] * rSold;
-                        acc += lv * b[u];
+        for (int qt = 0; qt < TG; qt++) {
+            const int q = q0 + qt, l = q == 0 ? 1 : q + 1;
+            if (qt < tg && q < nsum) {
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int k = 1 + lg + CPL_LG * j;
+                    if (k <= maxk && k >= kb) {
+                        const size_t cc = (size_t)(k - 1) * plane + col;
+                        x[qt][j] = StateIO<R>::ldr(st, lo, (size_t)l * tstride + cc);
+                        if (q == 0) a[j] = gp(g.V)[cc];
                     }
                 }
             }
         }
-        M.colsum[(size_t)(q == 0 ? 0 : l) * g.nwet + slot] = acc;
+#pragma unroll
+        for (int qt = 0; qt < TG; qt++) {
+            const int q = q0 + qt;
+            if (qt < tg && q < nsum) {
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int k = 1 + lg + CPL_LG * j;
+                    if (k <= maxk && k >= kb) {
+                        const double xv = StateIO<R>::val(x[qt][j], lo != nullptr);
+                        double v;
+                        if (q == 0) {
+                            const double Sn = xv + SALN0 + 0.0;
+                            v = Sn * a[j];
+                        } else {
+                            const double lv = xv * os[j] * rSold;
+                            v = lv * mm[j];
+                        }
+                        prod[((size_t)qt * maxk + (k - 1)) * CPL_COLS + c] = v;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        {
+            const int qt = lg;  // one thread per (column, sum of the group)
+            const int q = q0 + qt;
+            if (qt < tg && q < nsum && w0 + c < g.nwet) {
+                double acc = 0.0;
+                for (int k = kb; k <= maxk; k++) acc += prod[((size_t)qt * maxk + (k - 1)) * CPL_COLS + c];
+                gp(M.colsum)[(size_t)(q == 0 ? 0 : q + 1) * g.nwet + T.slot[c]] = acc;
+            }
+        }
+        __syncthreads();
     }
 }
 
 // global sums in the reference's column order (partialtot(n) summed over
-// n = 1..n_vocn, biogem.f90:1927-2031), then the coupling scalars.  Rows of
-// column partials are staged in LDS by the whole block; each sum is then one
-// thread's sequential pass over its row.
+// n = 1..n_vocn, biogem.f90:1927-2031), then the coupling scalars.  The
+// block stages a chunk of columns of every partial-sum row in LDS, then each
+// sum is one thread's sequential pass over its row of the chunk; chunks go in
+// column order, so every sum is added in the reference's order.
 constexpr int RED_THREADS = 256;
 constexpr int RED_LDS_BYTES = 56 * 1024;
 constexpr int RED_MAXL = 256;
@@ -455,178 +511,194 @@ __global__ __launch_bounds__(RED_THREADS) void k_couple_reduce(const MemberDev* 
     __shared__ double s_val[2 * RED_MAXL];
     const MemberDev& M = mems[list[blockIdx.x]];
     const int nwet = M.g.nwet, L = M.L;
-    const int stride = nwet | 1;  // odd row stride in doubles
-    int nrow = RED_LDS_BYTES / (8 * stride);
-    if (nrow > 16) nrow = 16;
-    const bool staged = nrow > 0;  // rows longer than the LDS budget are summed from global memory
-    if (!staged) nrow = RED_THREADS;
     // sum q: 0 -> S (scaled by 1/Vtot), 1 + (l-2) -> NEW_l, L-1 + (l-2) -> OLD_l
     const int nsum = 1 + 2 * (L - 2);
-    auto src = [&](int q) -> const double* {
-        if (q == 0) return M.colsum;
-        if (q < L - 1) return M.colsum + (size_t)(q + 1) * nwet;
-        return M.colold + (size_t)(q - L + 3) * nwet;
+    auto src = [&](int q) -> const AS_GLOBAL double* {
+        if (q == 0) return gp(M.colsum);
+        if (q < L - 1) return gp(M.colsum) + (size_t)(q + 1) * nwet;
+        return gp(M.colold) + (size_t)(q - L + 3) * nwet;
     };
+    int ch = RED_LDS_BYTES / (8 * nsum);   // columns per chunk, odd (LDS banks)
+    if (ch > nwet) ch = nwet;
+    if ((ch & 1) == 0) ch -= 1;
+    if (ch < 1) ch = 1;
     const int t = threadIdx.x;
-    for (int q0 = 0; q0 < nsum; q0 += nrow) {
-        const int nr = nsum - q0 < nrow ? nsum - q0 : nrow;
-        if (staged) {
-            for (int r = 0; r < nr; r++) {
-                const double* a = src(q0 + r);
-                for (int i = t; i < nwet; i += RED_THREADS) rows[r * stride + i] = a[i];
-            }
+    const double sc = M.g.rVtot;
+    double tot0 = 0.0, tot1 = 0.0;  // sums t and t + RED_THREADS
+    for (int c0 = 0; c0 < nwet; c0 += ch) {
+        const int n = nwet - c0 < ch ? nwet - c0 : ch;
+        for (int q = 0; q < nsum; q++) {
+            const AS_GLOBAL double* a = src(q) + c0;
+            for (int i = t; i < n; i += RED_THREADS) rows[q * ch + i] = a[i];
         }
         __syncthreads();
-        if (t < nr) {
-            const double* a = staged ? rows + t * stride : src(q0 + t);
-            const bool scaled = (q0 + t) == 0;
-            const double sc = M.g.rVtot;
-            double tot = 0.0;
-            int i = 0;
-            for (; i + 8 <= nwet; i += 8) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = a[i + u];
-#pragma unroll
-                for (int u = 0; u < 8; u++) tot += scaled ? v[u] * sc : v[u];
-            }
-            for (; i < nwet; i++) tot += scaled ? a[i] * sc : a[i];
-            s_val[q0 + t] = tot;
+        if (t < nsum) {
+            const double* r = rows + t * ch;
+            if (t == 0)
+                for (int i = 0; i < n; i++) tot0 += r[i] * sc;
+            else
+                for (int i = 0; i < n; i++) tot0 += r[i];
+        }
+        if (t + RED_THREADS < nsum) {
+            const double* r = rows + (t + RED_THREADS) * ch;
+            for (int i = 0; i < n; i++) tot1 += r[i];
         }
         __syncthreads();
     }
+    if (t < nsum) s_val[t] = tot0;
+    if (t + RED_THREADS < nsum) s_val[t + RED_THREADS] = tot1;
+    __syncthreads();
     for (int l = 2 + t; l < L; l += RED_THREADS) {
         const double nw = s_val[l - 1];
         const double rnew = fabs(nw) < NULLSMALL ? 0.0 : 1.0 / nw;
-        M.scal[SC_FACT0 + l] = s_val[L - 3 + l] * rnew;
+        gp(M.scal)[SC_FACT0 + l] = s_val[L - 3 + l] * rnew;
     }
     if (t == 0) {
-        const double Sold = M.scal[SC_SMEAN];
+        const double Sold = gp(M.scal)[SC_SMEAN];
         const double Snew = s_val[0];
         const double Sratio = Snew / Sold;
-        M.scal[SC_SOLD] = Sold;
-        M.scal[SC_SNEW] = Snew;
-        M.scal[SC_SMEAN] = Snew;
-        M.scal[SC_SRATIO] = Sratio;
-        M.scal[SC_RSRATIO] = 1.0 / Sratio;
+        gp(M.scal)[SC_SOLD] = Sold;
+        gp(M.scal)[SC_SNEW] = Snew;
+        gp(M.scal)[SC_SMEAN] = Snew;
+        gp(M.scal)[SC_SRATIO] = Sratio;
+        gp(M.scal)[SC_RSRATIO] = 1.0 / Sratio;
     }
 }
 
-// the rescale of biogem_tracercoupling (biogem.f90:2001-2061).  LDS holds the
-// old (ocn) and new (state) absolute salinity of the block's cells, which the
-// tracer pass reads before the cell pass overwrites both.
-constexpr int APPLY_THREADS = 256;
-constexpr int APPLY_COLS = 64;
-constexpr int APPLY_UNROLL = 8;
-
-template <typename R>
-__global__ __launch_bounds__(APPLY_THREADS) void k_couple_apply(const MemberDev* __restrict__ mems,
-                                                                const int32_t* __restrict__ list) {
-    extern __shared__ double sal[];  // [2][maxk][APPLY_COLS]: ocn S (old), state S (new)
-    __shared__ int32_t s_col[APPLY_COLS], s_kb[APPLY_COLS];
+// biogem_tracercoupling's rescale (biogem.f90:2001-2061): tracers l >= 3
+// (2033-2046) with the OLD partial sums of the next coupling, then T and S
+// (2001-2013), the new cell masses (2049-2052) and the particulates
+// (2042-2043).  The block owns its cells, so the old salinity and masses the
+// tracers need are read before the same threads rewrite them.
+template <typename R, int NJ, int TG>  // NJ cells per thread (NJ * CPL_LG >= maxk), TG >= tg
+__global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* __restrict__ mems,
+                                                              const int32_t* __restrict__ list) {
+    extern __shared__ double prod[];  // [tg][maxk][CPL_COLS]
+    __shared__ CplTile T;
     const MemberDev& M = mems[list[blockIdx.y]];
     const GridDev& g = M.g;
-    const int w0 = blockIdx.x * APPLY_COLS;
+    const int w0 = blockIdx.x * CPL_COLS;
     if (w0 >= g.nwet) return;  // uniform over the block
+    cpl_tile(g, w0, T);
     const BgcParams& p = M.bp;
-    const int maxi = g.maxi, maxk = g.maxk, L = M.L;
+    const int c = threadIdx.x % CPL_COLS, lg = threadIdx.x / CPL_COLS;
+    const int maxk = g.maxk, L = M.L, tg = cpl_tg(maxk) < TG ? cpl_tg(maxk) : TG;
+    const int col = T.col[c], kb = T.kb[c];
     const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell, tstride = (size_t)maxk * plane;
-    const int par = *M.parity;
+    const int par = *gp(M.parity);
     void* st = M.st[par];
     int16_t* lo = M.lo[par];
-    double* __restrict__ ocn = M.ocn;
-    double* sold = sal;
-    double* snew = sal + (size_t)maxk * APPLY_COLS;
-    const int t = threadIdx.x;
-    if (t < APPLY_COLS) {
-        const int w = w0 + t;
-        const int col = w < g.nwet ? g.wetsorted[w] : 0;
-        s_col[t] = col;
-        s_kb[t] = w < g.nwet ? k1at(g, col % maxi + 1, col / maxi + 1) : maxk + 1;
-    }
-    __syncthreads();
-    // (a) salinities of the block's cells
-    for (int idx = t; idx < APPLY_COLS * maxk; idx += APPLY_THREADS) {
-        const int cc = idx % APPLY_COLS, k = idx / APPLY_COLS + 1;
-        if (k < s_kb[cc]) continue;
-        const size_t c = (size_t)(k - 1) * plane + s_col[cc];
-        sold[idx] = ocn[ncell + c];
-        snew[idx] = StateIO<R>::ld(st, lo, tstride + c) + SALN0 + 0.0;
-    }
-    __syncthreads();
-    const double rSold = 1.0 / M.scal[SC_SOLD];
-    const double Snew = M.scal[SC_SNEW], Sratio = M.scal[SC_SRATIO], rSratio = M.scal[SC_RSRATIO];
-    // (b) tracers l >= 3 (2033-2046) and the OLD partials of the next coupling
+    AS_GLOBAL double* __restrict__ ocn = gp(M.ocn);
+    const double rSold = 1.0 / gp(M.scal)[SC_SOLD];
+    const double Snew = gp(M.scal)[SC_SNEW], Sratio = gp(M.scal)[SC_SRATIO], rSratio = gp(M.scal)[SC_RSRATIO];
+    // per cell: old salinity, Snew / new salinity, new mass
+    double so[NJ], rs[NJ], mn[NJ];
     {
-        const int cc = t % APPLY_COLS, wv = t / APPLY_COLS;
-        const int w = w0 + cc;
-        if (w < g.nwet) {
-            const int col = s_col[cc], kb = s_kb[cc], slot = g.vpos[w];
-            for (int l = 2 + wv; l < L; l += APPLY_THREADS / APPLY_COLS) {
-                int b = -1;
-                for (int q = 0; q < NBGC; q++)
-                    if (p.slot[q] == l) b = q;
-                const bool hasv = M.bgc && b >= 0;
-                const double* __restrict__ vdl = M.vd + (size_t)(b < 0 ? 0 : b) * ncell;
-                double* __restrict__ ocl = ocn + (size_t)(b < 0 ? 0 : p.ocn[b]) * ncell;
-                const double fact = M.scal[SC_FACT0 + l];
-                double acc = 0.0;
-                for (int k0 = kb; k0 <= maxk; k0 += APPLY_UNROLL) {
-                    double lv[APPLY_UNROLL], vv[APPLY_UNROLL], mn[APPLY_UNROLL];
+        typename StateIO<R>::Raw sr[NJ];
 #pragma unroll
-                    for (int u = 0; u < APPLY_UNROLL; u++) {
-                        const int k = k0 + u;
-                        if (k <= maxk) {
-                            const size_t c = (size_t)(k - 1) * plane + col;
-                            lv[u] = StateIO<R>::ld(st, lo, (size_t)l * tstride + c);
-                            vv[u] = hasv ? vdl[c] : 0.0;
-                            mn[u] = M.M[c];
-                        }
-                    }
+        for (int j = 0; j < NJ; j++) {
+            const int k = 1 + lg + CPL_LG * j;
+            if (k <= maxk && k >= kb) {
+                const size_t cc = (size_t)(k - 1) * plane + col;
+                so[j] = ocn[ncell + cc];
+                sr[j] = StateIO<R>::ldr(st, lo, tstride + cc);
+                mn[j] = gp(M.M)[cc];
+            }
+        }
 #pragma unroll
-                    for (int u = 0; u < APPLY_UNROLL; u++) {
-                        const int k = k0 + u;
-                        if (k <= maxk) {
-                            const size_t c = (size_t)(k - 1) * plane + col;
-                            const int li = (k - 1) * APPLY_COLS + cc;
-                            const double x = lv[u] * sold[li] * rSold;
-                            double v = fact * x + vv[u];
-                            v = Sratio * v;
-                            if (b >= 0) ocl[c] = v;
-                            StateIO<R>::st(st, lo, (size_t)l * tstride + c, (Snew / snew[li]) * v);
-                            acc += v * (rSratio * mn[u]);
-                        }
-                    }
-                }
-                M.colold[(size_t)l * g.nwet + slot] = acc;
+        for (int j = 0; j < NJ; j++) {
+            const int k = 1 + lg + CPL_LG * j;
+            if (k <= maxk && k >= kb) {
+                const double snew = StateIO<R>::val(sr[j], lo != nullptr) + SALN0 + 0.0;
+                rs[j] = Snew / snew;
+                mn[j] = rSratio * mn[j];
             }
         }
     }
-    __syncthreads();
-    // (c) T, S (2001-2013), the new cell masses (2049-2052) and particulates
-    for (int idx = t; idx < APPLY_COLS * maxk; idx += APPLY_THREADS) {
-        const int cc = idx % APPLY_COLS, k = idx / APPLY_COLS + 1;
-        if (k < s_kb[cc]) continue;
-        const size_t c = (size_t)(k - 1) * plane + s_col[cc];
-        const double lvT = StateIO<R>::ld(st, lo, c) + ZEROC + 0.0;
-        const double lvS = snew[idx];
-        ocn[c] = lvT;
-        ocn[ncell + c] = lvS;
-        StateIO<R>::st(st, lo, c, lvT - ZEROC);
-        StateIO<R>::st(st, lo, tstride + c, lvS - SALN0);
-        M.M[c] = rSratio * M.M[c];
-        M.rM[c] = Sratio * M.rM[c];
+    for (int l0 = 2; l0 < L; l0 += tg) {
+        // all loads of the group first, then the rescale
+        typename StateIO<R>::Raw lv[TG][NJ];
+        double vv[TG][NJ];
+        int bs[TG];
 #pragma unroll
-        for (int s = 0; s < NSED; s++) {
-            double* pp = &M.part[(size_t)s * ncell + c];
-            *pp = Sratio * (*pp + 0.0);
+        for (int lt = 0; lt < TG; lt++) {
+            const int l = l0 + lt;
+            int b = -1;
+            for (int q = 0; q < NBGC; q++)
+                if (p.slot[q] == l) b = q;
+            bs[lt] = b;
+            const bool hasv = M.bgc && b >= 0;
+            if (lt < tg && l < L) {
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int k = 1 + lg + CPL_LG * j;
+                    if (k <= maxk && k >= kb) {
+                        const size_t cc = (size_t)(k - 1) * plane + col;
+                        lv[lt][j] = StateIO<R>::ldr(st, lo, (size_t)l * tstride + cc);
+                        vv[lt][j] = hasv ? gp(M.vd)[(size_t)b * ncell + cc] : 0.0;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int lt = 0; lt < TG; lt++) {
+            const int l = l0 + lt;
+            if (lt < tg && l < L) {
+                const int b = bs[lt];
+                const double fact = gp(M.scal)[SC_FACT0 + l];
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int k = 1 + lg + CPL_LG * j;
+                    if (k <= maxk && k >= kb) {
+                        const size_t cc = (size_t)(k - 1) * plane + col;
+                        const double x = StateIO<R>::val(lv[lt][j], lo != nullptr) * so[j] * rSold;
+                        double v = fact * x + vv[lt][j];
+                        v = Sratio * v;
+                        if (b >= 0) ocn[(size_t)p.ocn[b] * ncell + cc] = v;
+                        StateIO<R>::st(st, lo, (size_t)l * tstride + cc, rs[j] * v);
+                        prod[((size_t)lt * maxk + (k - 1)) * CPL_COLS + c] = v * mn[j];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        {
+            const int lt = lg;
+            const int l = l0 + lt;
+            if (lt < tg && l < L && w0 + c < g.nwet) {
+                double acc = 0.0;
+                for (int k = kb; k <= maxk; k++) acc += prod[((size_t)lt * maxk + (k - 1)) * CPL_COLS + c];
+                gp(M.colold)[(size_t)l * g.nwet + T.slot[c]] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    // T, S, masses and particulates of the thread's cells
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int k = 1 + lg + CPL_LG * j;
+        if (k <= maxk && k >= kb) {
+            const size_t cc = (size_t)(k - 1) * plane + col;
+            const auto rT = StateIO<R>::ldr(st, lo, cc);
+            const auto rS = StateIO<R>::ldr(st, lo, tstride + cc);
+            const double lvT = StateIO<R>::val(rT, lo != nullptr) + ZEROC + 0.0;
+            const double lvS = StateIO<R>::val(rS, lo != nullptr) + SALN0 + 0.0;
+            ocn[cc] = lvT;
+            ocn[ncell + cc] = lvS;
+            StateIO<R>::st(st, lo, cc, lvT - ZEROC);
+            StateIO<R>::st(st, lo, tstride + cc, lvS - SALN0);
+            gp(M.M)[cc] = mn[j];
+            gp(M.rM)[cc] = Sratio * gp(M.rM)[cc];
+#pragma unroll
+            for (int s2 = 0; s2 < NSED; s2++) {
+                AS_GLOBAL double* pp = &gp(M.part)[(size_t)s2 * ncell + cc];
+                *pp = Sratio * (*pp + 0.0);
+            }
         }
     }
 }
 
 // ------------------------------------------------------------ launchers
-size_t apply_lds_bytes(int maxk) { return (size_t)2 * maxk * APPLY_COLS * 8; }
-
 template <typename R>
 static void launch_col(int W, dim3 grid, const MemberDev* mems, const int32_t* list, hipStream_t s) {
     if (W <= 4)
@@ -645,19 +717,39 @@ void launch_bio(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
         launch_col<float>(W, gcol, mems, list, s);
 }
 
-void launch_coupling(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax, int prec,
-                     hipStream_t s) {
-    const dim3 gapp((nwetmax + APPLY_COLS - 1) / APPLY_COLS, nlist);
-    const dim3 gsum((nwetmax + SUMS_COLS - 1) / SUMS_COLS, nlist);
-    if (prec == 0)
-        hipLaunchKernelGGL(k_couple_sums<double>, gsum, dim3(SUMS_THREADS), 0, s, mems, list);
-    else
-        hipLaunchKernelGGL(k_couple_sums<float>, gsum, dim3(SUMS_THREADS), 0, s, mems, list);
+template <typename R>
+static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax,
+                              hipStream_t s) {
+    const dim3 grid((nwetmax + CPL_COLS - 1) / CPL_COLS, nlist);
+    // LDS for the largest grid of the batch; every member's tg fits in it
+    const size_t lds = (size_t)cpl_tg(maxkmax) * maxkmax * CPL_COLS * 8;
+    size_t need = 0;
+    for (int k = 1; k <= maxkmax; k++) need = std::max(need, (size_t)cpl_tg(k) * k * CPL_COLS * 8);
+    const size_t b = need > lds ? need : lds;
+    if (maxkmax <= 4 * CPL_LG) {
+        hipLaunchKernelGGL((k_couple_sums<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    } else if (maxkmax <= 8 * CPL_LG) {
+        hipLaunchKernelGGL((k_couple_sums<R, 8, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    } else {
+        hipLaunchKernelGGL((k_couple_sums<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    }
     hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), RED_LDS_BYTES, s, mems, list);
+    if (maxkmax <= 4 * CPL_LG) {
+        hipLaunchKernelGGL((k_couple_apply<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    } else if (maxkmax <= 8 * CPL_LG) {
+        hipLaunchKernelGGL((k_couple_apply<R, 8, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    } else {
+        hipLaunchKernelGGL((k_couple_apply<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    }
+}
+
+void launch_coupling(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax, int maxL,
+                     int prec, hipStream_t s) {
+    (void)maxL;
     if (prec == 0)
-        hipLaunchKernelGGL(k_couple_apply<double>, gapp, dim3(APPLY_THREADS), apply_lds_bytes(maxkmax), s, mems, list);
+        launch_coupling_t<double>(mems, list, nlist, nwetmax, maxkmax, s);
     else
-        hipLaunchKernelGGL(k_couple_apply<float>, gapp, dim3(APPLY_THREADS), apply_lds_bytes(maxkmax), s, mems, list);
+        launch_coupling_t<float>(mems, list, nlist, nwetmax, maxkmax, s);
 }
 
 }  // namespace cgo

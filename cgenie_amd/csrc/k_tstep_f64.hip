@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_inventory(const MemberDev* __restrict__
                                                    const double* __restrict__ w) {
     const MemberDev& M = mems[m];
     const int l = blockIdx.x;
-    const int par = *M.parity;
+    const int par = *gp(M.parity);
     const size_t ncell = M.g.ncell;
     const size_t base = (size_t)l * ncell;
     double s = 0.0;

@@ -38,24 +38,50 @@ __device__ __forceinline__ double carry_unit(float x) {
 
 // state access as double (FP32 form: hi + carry)
 template <typename R> struct StateIO;
+// (all state pointers are global memory: the casts keep the accesses off
+// the flat path, whose loads also count against the LDS wait counter)
+// ldr/val split a load from its conversion so that a thread can issue the
+// loads of many cells before it waits for any of them.
 template <> struct StateIO<double> {
+    struct Raw {
+        double v;
+    };
+    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t*, size_t i) {
+        return Raw{((const AS_GLOBAL double*)s)[i]};
+    }
+    static __device__ __forceinline__ double val(Raw r, bool) { return r.v; }
     static __device__ __forceinline__ double ld(const void* s, const int16_t*, size_t i) {
-        return static_cast<const double*>(s)[i];
+        return ((const AS_GLOBAL double*)s)[i];
     }
     static __device__ __forceinline__ void st(void* s, int16_t*, size_t i, double x) {
-        static_cast<double*>(s)[i] = x;
+        ((AS_GLOBAL double*)s)[i] = x;
     }
 };
 template <> struct StateIO<float> {
+    struct Raw {
+        float h;
+        int16_t c;
+    };
+    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t* lo, size_t i) {
+        Raw r;
+        r.h = ((const AS_GLOBAL float*)s)[i];
+        r.c = lo ? ((const AS_GLOBAL int16_t*)lo)[i] : (int16_t)0;
+        return r;
+    }
+    static __device__ __forceinline__ double val(Raw r, bool carry) {
+        double x = (double)r.h;
+        if (carry) x += (double)r.c * carry_unit(r.h);
+        return x;
+    }
     static __device__ __forceinline__ double ld(const void* s, const int16_t* lo, size_t i) {
-        float h = static_cast<const float*>(s)[i];
+        float h = ((const AS_GLOBAL float*)s)[i];
         double x = (double)h;
-        if (lo) x += (double)lo[i] * carry_unit(h);
+        if (lo) x += (double)((const AS_GLOBAL int16_t*)lo)[i] * carry_unit(h);
         return x;
     }
     static __device__ __forceinline__ void st(void* s, int16_t* lo, size_t i, double x) {
         float h = (float)x;
-        static_cast<float*>(s)[i] = h;
+        ((AS_GLOBAL float*)s)[i] = h;
         if (lo) {
             // x - h in units of carry_unit(h): scaling by the exact power of two
             // 2^(39 - e) rounds exactly as dividing by 2^(e - 39) would
@@ -63,7 +89,7 @@ template <> struct StateIO<float> {
             if (e < -126) e = -126;
             double r = rint(ldexp(x - (double)h, 23 + 16 - e));
             r = fmin(fmax(r, -32767.0), 32767.0);
-            lo[i] = (int16_t)r;
+            ((AS_GLOBAL int16_t*)lo)[i] = (int16_t)r;
         }
     }
 };
@@ -136,10 +162,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     if (kmin > maxk) return;  // every column of the block is land
 
     constexpr bool ANOM = sizeof(R) == 4;  // FP32 form: fluxes of q - q_cell plus q_cell * div(u)
-    const int par = *M.parity;
+    const int par = *gp(M.parity);
     const AS_GLOBAL R* __restrict__ src = gp(static_cast<const R*>(M.st[par]));
     AS_GLOBAL R* __restrict__ dst = gp(static_cast<R*>(M.st[par ^ 1]));
-    const int16_t* __restrict__ lo_src = M.lo[par];
+    const AS_GLOBAL int16_t* __restrict__ lo_src = M.lo[par] ? gp(M.lo[par]) : nullptr;
     int16_t* __restrict__ lo_dst = M.lo[par ^ 1];
     const AS_GLOBAL R* __restrict__ u = gp(static_cast<const R*>(M.u));
     AS_GLOBAL double* __restrict__ rhop = gp(M.rho);
@@ -164,6 +190,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R dphi = (R)g.dphi, rdphi = (R)g.rdphi;
     const R diff1 = sizeof(R) == 8 ? (R)M.diff1 : (R)M.diff1f;
     const R diffv = sizeof(R) == 8 ? (R)M.diff2 : (R)M.diff2f;
+    const R rdiff1 = (R)1.0 / diff1, rdiffv = (R)1.0 / diffv;
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
 
@@ -238,12 +265,23 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const R ua_ = u[2 * ncell + o + col];
             const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k];
             const R dt = mk[MK_DT * KS + k], rdz = mk[MK_RDZ * KS + k];
-            R pec;
-            pec = ue_ * dphi / diff1; const R upe = pec / ((R)2.0 + fabs(pec));
-            pec = uw_ * dphi / diff1; const R upw = pec / ((R)2.0 + fabs(pec));
-            pec = un_ * dsvn / diff1; const R upn = pec / ((R)2.0 + fabs(pec));
-            pec = us_ * dsvs / diff1; const R ups = pec / ((R)2.0 + fabs(pec));
-            pec = ua_ * dza / diffv; const R upa = pec / ((R)2.0 + fabs(pec));
+            R upe, upw, upn, ups, upa;
+            if constexpr (ANOM) {
+                // FP32 form: pec = u dx / kappa with the reciprocal of kappa
+                auto upw_ = [](float pc) { return __fdividef(pc, 2.0f + fabsf(pc)); };
+                upe = upw_(ue_ * dphi * rdiff1);
+                upw = upw_(uw_ * dphi * rdiff1);
+                upn = upw_(un_ * dsvn * rdiff1);
+                ups = upw_(us_ * dsvs * rdiff1);
+                upa = upw_(ua_ * dza * rdiffv);
+            } else {
+                R pec;
+                pec = ue_ * dphi / diff1; upe = pec / ((R)2.0 + fabs(pec));
+                pec = uw_ * dphi / diff1; upw = pec / ((R)2.0 + fabs(pec));
+                pec = un_ * dsvn / diff1; upn = pec / ((R)2.0 + fabs(pec));
+                pec = us_ * dsvs / diff1; ups = pec / ((R)2.0 + fabs(pec));
+                pec = ua_ * dza / diffv; upa = pec / ((R)2.0 + fabs(pec));
+            }
             const bool oe = k >= max(k1c, k1e), ow = k >= max(k1w, k1c);
             const bool on = k >= max(k1c, k1n), os = k >= max(k1s, k1c);
             // isoneutral coefficients from T and S (2489-2501, 2561-2605)
@@ -291,6 +329,70 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const R divu = ANOM ? u[3 * ncell + o + col] : (R)0;
             const R isoc = (R)0.25 * slim * diff1;
             double newT = 0, newS = 0;
+            if constexpr (ANOM) {
+                // FP32 form in coefficient form: each face flux is the
+                // neighbour difference times a coefficient of the cell, and the
+                // isoneutral flux a sum of corner differences times
+                // coefficients, formed once per cell for all tracers
+                const R ce = oe ? (ue_ * rc * ((R)1.0 - upe) * (R)0.5 - rc2 * diff1) * rdphi : (R)0;
+                const R cw = ow ? (uw_ * rc * ((R)1.0 + upw) * (R)0.5 + rc2 * diff1) * rdphi : (R)0;
+                const R cn = on ? (cvn * un_ * ((R)1.0 - upn) * (R)0.5 - cv2n * diff1) * rds : (R)0;
+                const R cs = os ? (cvs * us_ * ((R)1.0 + ups) * (R)0.5 + cv2s * diff1) * rds : (R)0;
+                const R ca = top ? (R)0 : ua_ * ((R)1.0 - upa) * (R)0.5 - rdza * diffv;
+                const R cbn = top ? (R)0 : -(ua_ * ((R)1.0 + upa) * (R)0.5) - rdza * diffv;
+                R ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, qq = 0;
+                if (iso) {
+                    const R f = (R)2 * isoc / dzrho;
+                    R s2 = 0;
+#pragma unroll
+                    for (int n = 0; n < 4; n++) {
+                        const bool east = n & 1;
+                        ax[n] = f * dxr[n] * rc * rdphi * (east ? (R)1 : (R)-1);
+                        ay[n] = f * dyr[n] * (east ? cvn * rdsvn : -(cvs * rdsvs));
+                        s2 = s2 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
+                    }
+                    qq = isoc * s2 / (dzrho * dzrho) * rdza;
+                }
+#pragma unroll 1
+                for (int q = 0; q < nl; q++) {
+                    const R Tc = SP(cb, q, p);
+                    const R dE = D(cb, q, pe, p), dW = D(cb, q, pw, p), dN = D(cb, q, pn, p), dS = D(cb, q, ps, p);
+                    R fa, fbn = 0;
+                    if (top) {
+                        fa = M.sflux ? (R)gp(M.sflux)[(l0 + q) * plane + col] : (R)0;
+                    } else {
+                        const R dU = DK(ub, cb, q, p);
+                        R tv = 0;
+                        if (iso) {
+                            // corner differences with the sign of the west/south
+                            // corners folded into ax[0], ax[2], ay[0], ay[2]
+                            tv = ax[0] * dW + ax[1] * dE + ay[0] * dS + ay[1] * dN;
+                            tv = tv + ax[2] * D(ub, q, pw, p) + ax[3] * D(ub, q, pe, p) + ay[2] * D(ub, q, ps, p) +
+                                 ay[3] * D(ub, q, pn, p);
+                            tv = tv - qq * dU;
+                        }
+                        fa = dU * ca + tv;
+                        fbn = dU * cbn + tv;
+                    }
+                    const R div = ((dE * ce - dW * cw) + (dN * cn - dS * cs)) + (fa - FB(q)) * rdz;
+                    const int idx = (l0 + q) * tstride + o + col;
+                    const R inc = -(dt * (div + Tc * divu));
+                    double nv;
+                    if (lo_src) {
+                        // compensated accumulation: the rounding residual of every
+                        // update is carried to the next step in the int16 plane
+                        nv = ((double)Tc + (double)SR(cb, q, p)) + (double)inc;
+                        StateIO<R>::st((R*)dst, lo_dst, idx, nv);
+                    } else {
+                        const R v = Tc + inc;
+                        dst[idx] = v;
+                        nv = (double)v;
+                    }
+                    FB(q) = fbn;
+                    if (l0 + q == 0) newT = nv;
+                    if (l0 + q == 1) newS = nv;
+                }
+            } else
 #pragma unroll 1
             for (int q = 0; q < nl; q++) {
                 const R Tc = SP(cb, q, p), Te = SP(cb, q, pe), Tw = SP(cb, q, pw), Tn = SP(cb, q, pn), Ts = SP(cb, q, ps);
@@ -390,12 +492,12 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
     const int I = col % maxi + 1, J = col / maxi + 1;
     const int k1 = k1at(g, I, J);
-    const int par = *M.parity;
+    const int par = *gp(M.parity);
     void* st = M.st[par ^ 1];
     int16_t* lo = M.lo[par ^ 1];
     const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
-    double* rho = M.rho;
-    const double* dz = g.mk + MK_DZ * (maxk + 2);
+    AS_GLOBAL double* rho = gp(M.rho);
+    const AS_GLOBAL double* dz = gp(g.mk) + MK_DZ * (maxk + 2);
     const double e0 = g.ec[0], e1 = g.ec[1], e2 = g.ec[2], e3 = g.ec[3];
     int kk[MAXK_DEV + 2];
     double dzm[MAXK_DEV + 2];
@@ -465,7 +567,7 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     }
     if (mixed || cost > 0.0) {
         for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
-        if (cost > 0.0) M.cost[col] += cost;
+        if (cost > 0.0) gp(M.cost)[col] += cost;
     }
 }
 
