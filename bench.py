@@ -118,12 +118,12 @@ def run_gpu(args):
                         p50_year_s=float(np.percentile(per_year, 50)), p99_year_s=float(np.percentile(per_year, 99)),
                         flux_ms=flux_ms, co_ms=co_ms, bgc_coupling_ms=bgc_ms, flux_bytes=B,
                         flux_gbs=B / (flux_ms * 1e-3) / 1e9, carb_fail=int(d.carb_fail), n_wet=n_wet, L=tset.L,
-                        tracer_block=tuned[tag]["tracer_block"])
+                        tracer_block=tuned[tag]["tracer_block"], flux_cols=tuned[tag]["cols"])
         eng.drop_graph(graph)
         for m in ids:
             eng.remove_member(m)
     # plain FP64 single member, eager launches (the reference-shaped path)
-    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"])
+    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"])
     ids, tset, n_wet, g = build_ensemble(eng, E, 1, E.FP64)
     eng.step(ids, STEPS_PER_YEAR)
     eng.sync()
@@ -229,9 +229,10 @@ def main():
         fp64_single_member_years_per_s=round(r["fp64_single"]["member_years_per_s"], 3),
         speedup_vs_fp64_single=round(main_["member_years_per_s"] / r["fp64_single"]["member_years_per_s"], 1),
         kernels_ms=dict(fp32c=dict(flux=main_["flux_ms"], co=main_["co_ms"], bgc_coupling=main_["bgc_coupling_ms"],
-                                   tracer_block=main_["tracer_block"]),
+                                   tracer_block=main_["tracer_block"], flux_cols=main_["flux_cols"]),
                         fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
-                                  bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"])),
+                                  bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"],
+                                  flux_cols=r["fp64"]["flux_cols"])),
         roofline=roof, cpu_baseline=cpu, serving=serving)
     print(json.dumps(line), flush=True)
 

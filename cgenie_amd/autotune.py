@@ -19,6 +19,7 @@ from . import engine as E
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 TRACER_BLOCKS = (2, 4, 6, 8)
+FLUX_COLS = (128, 256)
 
 
 def _key(grid_dims, L, precision, n_members) -> str:
@@ -45,24 +46,24 @@ def save_cache(c: dict) -> None:
 
 def tune(eng: E.Engine, members, grid_dims, L: int, precision: int, reps: int = 10,
          use_cache: bool = True, write_cache: bool = True) -> dict:
-    """Pick the tracer-block size of the flux kernel for this batch.
+    """Pick the flux kernel's tile shape for this batch.
 
-    Returns {"tracer_block": tb, "times_ms": {tb: ms}} and applies the choice
-    to the engine."""
+    Returns {"tracer_block": tb, "cols": cols, "times_ms": {"tbxcols": ms}}
+    and applies the choice to the engine."""
     key = _key(grid_dims, L, precision, len(members))
     cache = load_cache() if use_cache else {}
     if key in cache:
-        tb = int(cache[key]["tracer_block"])
-        eng.set_tile(precision, tb)
+        eng.set_tile(precision, int(cache[key]["tracer_block"]), int(cache[key].get("cols", 128)))
         return cache[key]
     times = {}
-    for tb in TRACER_BLOCKS:
-        eng.set_tile(precision, tb)
-        eng.time_kernel(0, members, 2)  # warm
-        times[tb] = eng.time_kernel(0, members, reps)
+    for cols in FLUX_COLS:
+        for tb in TRACER_BLOCKS:
+            eng.set_tile(precision, tb, cols)
+            eng.time_kernel(0, members, 2)  # warm
+            times[(tb, cols)] = eng.time_kernel(0, members, reps)
     best = min(times, key=times.get)
-    eng.set_tile(precision, best)
-    res = {"tracer_block": best, "times_ms": {str(k): v for k, v in times.items()}}
+    eng.set_tile(precision, best[0], best[1])
+    res = {"tracer_block": best[0], "cols": best[1], "times_ms": {f"{k[0]}x{k[1]}": v for k, v in times.items()}}
     if write_cache:
         cache = load_cache()
         cache[key] = res

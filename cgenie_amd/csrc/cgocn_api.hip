@@ -16,8 +16,10 @@
 #include "cgocn_dev.h"
 
 namespace cgo {
-void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, hipStream_t);
-void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, hipStream_t);
+void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, int nt,
+                   size_t lds_cap, hipStream_t);
+void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, int nt,
+                   size_t lds_cap, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
@@ -84,7 +86,9 @@ struct cgo_engine {
     std::vector<MemberH*> members;
     MemberDev* d_table = nullptr;
     bool table_dirty = false;
-    int lb[2] = {4, 4};
+    int lb[2] = {4, 4};       // flux tracer block per precision
+    int nt[2] = {128, 128};   // flux columns per block per precision
+    size_t lds_cap = 65536;   // LDS one workgroup may take
     std::vector<Graph> graphs;
     int32_t* d_list = nullptr;  // scratch lists for eager steps
     size_t d_list_cap = 0;
@@ -138,6 +142,9 @@ int cgo_create(int32_t device, cgo_engine** out) {
         delete e;
         return CGO_E_HIP;
     }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        e->lds_cap = (size_t)lds;
     *out = e;
     return CGO_OK;
 }
@@ -946,9 +953,11 @@ static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_list
         if (!ls.n[p]) continue;
         const int32_t* l = d_lists + ls.off[p];
         if (p == 0)
-            launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
+            launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->nt[0],
+                          e->lds_cap, e->s);
         else
-            launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
+            launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->nt[1],
+                          e->lds_cap, e->s);
         launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
     }
     launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
@@ -1126,6 +1135,13 @@ int cgo_set_tile(cgo_engine* e, int32_t precision, int32_t tb) {
     return CGO_OK;
 }
 
+int cgo_set_flux_cols(cgo_engine* e, int32_t precision, int32_t cols) {
+    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (cols != 128 && cols != 256))
+        return fail(e, CGO_E_ARG, "set_flux_cols: precision %d columns %d", precision, cols);
+    e->nt[precision == CGO_FP64 ? 0 : 1] = cols;
+    return CGO_OK;
+}
+
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
     if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 2) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
@@ -1141,9 +1157,11 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             const int32_t* l = e->d_list + ls.off[p];
             if (which == 0) {
                 if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->s);
+                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->nt[0],
+                          e->lds_cap, e->s);
                 else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->s);
+                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->nt[1],
+                          e->lds_cap, e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
             } else if (ls.n[2 + p]) {

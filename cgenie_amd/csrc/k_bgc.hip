@@ -527,9 +527,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_couple_reduce(const MemberDev* 
     double tot0 = 0.0, tot1 = 0.0;  // sums t and t + RED_THREADS
     for (int c0 = 0; c0 < nwet; c0 += ch) {
         const int n = nwet - c0 < ch ? nwet - c0 : ch;
-        for (int q = 0; q < nsum; q++) {
-            const AS_GLOBAL double* a = src(q) + c0;
-            for (int i = t; i < n; i += RED_THREADS) rows[q * ch + i] = a[i];
+        // stage nsum x n values, eight loads in flight per thread
+        const int tot = nsum * n;
+        for (int b0 = t; b0 < tot; b0 += RED_THREADS * 8) {
+            double v[8];
+            int at[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = b0 + u * RED_THREADS;
+                if (idx < tot) {
+                    const int q = idx / n, i = idx - q * n;
+                    v[u] = src(q)[c0 + i];
+                    at[u] = q * ch + i;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (b0 + u * RED_THREADS < tot) rows[at[u]] = v[u];
         }
         __syncthreads();
         if (t < nsum) {

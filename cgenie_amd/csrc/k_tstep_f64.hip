@@ -6,20 +6,23 @@
 
 namespace cgo {
 
-void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int maxi, int lb, hipStream_t s) {
-    const int tw = FLUX_THREADS + 2 * maxi;
-    // keep the workgroup within 64 KiB of LDS (wide grids take smaller tracer blocks)
-    while (lb > 2 && flux_lds_bytes(tw, lb, sizeof(double)) > 65536) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
-    const dim3 grid((maxcol + FLUX_THREADS - 1) / FLUX_THREADS, (maxL + lb - 1) / lb, n);
-    const size_t lds = flux_lds_bytes(tw, lb, sizeof(double));
-    if (lb == 2)
-        hipLaunchKernelGGL((k_flux<double, 2>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
-    else if (lb == 8)
-        hipLaunchKernelGGL((k_flux<double, 8>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
-    else if (lb == 6)
-        hipLaunchKernelGGL((k_flux<double, 6>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
-    else
-        hipLaunchKernelGGL((k_flux<double, 4>), grid, dim3(FLUX_THREADS), lds, s, t, l, tw);
+void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int maxi, int lb, int nt, size_t lds_cap,
+                   hipStream_t s) {
+    if (nt != 256) nt = 128;
+    const int tw = nt + 2 * maxi;
+    // keep the workgroup within the LDS the device grants one block
+    while (lb > 2 && flux_lds_bytes(tw, lb, sizeof(double), nt) > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
+    if (nt == 256 && flux_lds_bytes(tw, lb, sizeof(double), nt) > lds_cap) nt = 128;
+    const int tw2 = nt + 2 * maxi;
+    const dim3 grid((maxcol + nt - 1) / nt, (maxL + lb - 1) / lb, n);
+    const size_t lds = flux_lds_bytes(tw2, lb, sizeof(double), nt);
+#define CGO_FLUX(LB_, NT_) hipLaunchKernelGGL((k_flux<double, LB_, NT_>), grid, dim3(NT_), lds, s, t, l, tw2)
+    if (nt == 256) {
+        if (lb == 2) CGO_FLUX(2, 256); else if (lb == 8) CGO_FLUX(8, 256); else if (lb == 6) CGO_FLUX(6, 256); else CGO_FLUX(4, 256);
+    } else {
+        if (lb == 2) CGO_FLUX(2, 128); else if (lb == 8) CGO_FLUX(8, 128); else if (lb == 6) CGO_FLUX(6, 128); else CGO_FLUX(4, 128);
+    }
+#undef CGO_FLUX
 }
 
 void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int prec, hipStream_t s) {

@@ -17,7 +17,6 @@
 
 namespace cgo {
 
-constexpr int FLUX_THREADS = 128;
 
 template <typename R> struct Met;
 template <> struct Met<double> {
@@ -104,12 +103,15 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
 // flattened column range [c0 - maxi, c0 + FLUX_THREADS + maxi) -- every
 // east/west (periodic wrap included, offsets below maxi) and north/south
 // neighbour of the block's columns -- and the carried bottom-face flux.
+// Block widths (columns per block): 128 or 256; a wider block shares its
+// halo among more columns.  FLUX_PF(nt) tile positions per thread cover the
+// tile of nt + 2*maxi columns for maxi <= 128.
 constexpr int FLUX_NBUF = 3;
-constexpr int FLUX_PF = 3;  // tile positions per thread (TW <= 3 * FLUX_THREADS)
+__host__ __device__ constexpr int flux_pf(int nt) { return nt == 128 ? 3 : 2; }
 // The FP32 form also stages the carry of every value, as a float residual,
 // so differences of neighbours are taken at the stored (~40-bit) precision.
-__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es) {
-    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * 128 * es +
+__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es, int nt) {
+    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * nt * es +
            (es == 4 ? (size_t)FLUX_NBUF * (lb + 2) * tw * 4 : 0);
 }
 
@@ -120,7 +122,7 @@ __device__ __forceinline__ float carry_residual(float h, int16_t c) {
     return ldexpf((float)c, e - 23 - 16);
 }
 
-template <typename R, int LB>
+template <typename R, int LB, int FLUX_THREADS>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
                                                        const int32_t* __restrict__ list, int tw) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     auto FB = [&](int q) -> R& { return sfb[q * FLUX_THREADS + tid]; };
     auto slot_tracer = [&](int q) { return q < LB ? l0 + q : q - LB; };
     // prefetch of one plane into registers, then into the LDS ring
+    constexpr int FLUX_PF = flux_pf(FLUX_THREADS);
     R pf[LB + 2][FLUX_PF];
     int16_t pc[ANOM ? LB + 2 : 1][FLUX_PF];
     auto fetch = [&](int kz) {

@@ -93,6 +93,7 @@ def load() -> C.CDLL:
         "cgo_graph_launch": ([P, _i32, _i32], C.c_int),
         "cgo_graph_destroy": ([P, _i32], C.c_int),
         "cgo_set_tile": ([P, _i32, _i32], C.c_int),
+        "cgo_set_flux_cols": ([P, _i32, _i32], C.c_int),
         "cgo_time_kernel": ([P, _i32, ip, _i32, _i32, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -272,8 +273,10 @@ class Engine:
     def sync(self):
         self._chk(self.L.cgo_sync(self.h), "sync")
 
-    def set_tile(self, precision: int, tracer_block: int):
+    def set_tile(self, precision: int, tracer_block: int, cols: int | None = None):
         self._chk(self.L.cgo_set_tile(self.h, precision, tracer_block), "set_tile")
+        if cols is not None:
+            self._chk(self.L.cgo_set_flux_cols(self.h, precision, cols), "set_flux_cols")
 
     def time_kernel(self, which: int, members, reps: int) -> float:
         arr, n = _ids(members)
