@@ -356,8 +356,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     }
                     qq = isoc * s2 / (dzrho * dzrho) * rdza;
                 }
-#pragma unroll 1
-                for (int q = 0; q < nl; q++) {
+#pragma unroll
+                for (int q = 0; q < LB; q++) {
+                    if (q >= nl) break;
                     const R Tc = SP(cb, q, p);
                     const R dE = D(cb, q, pe, p), dW = D(cb, q, pw, p), dN = D(cb, q, pn, p), dS = D(cb, q, ps, p);
                     R fa, fbn = 0;
@@ -396,8 +397,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     if (l0 + q == 1) newS = nv;
                 }
             } else
-#pragma unroll 1
-            for (int q = 0; q < nl; q++) {
+#pragma unroll
+            for (int q = 0; q < LB; q++) {
+                if (q >= nl) break;
                 const R Tc = SP(cb, q, p), Te = SP(cb, q, pe), Tw = SP(cb, q, pw), Tn = SP(cb, q, pn), Ts = SP(cb, q, ps);
                 R fe, fw, fn, fs, fa, fbn = 0;
                 if (!ANOM) {
