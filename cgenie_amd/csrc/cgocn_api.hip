@@ -280,7 +280,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     const size_t o_k1 = take(g->k1.size() * 4), o_mj = take(g->mj.size() * 8), o_mk = take(g->mk.size() * 8);
     const size_t o_mjf = take(g->mj.size() * 4), o_mkf = take(g->mk.size() * 4), o_wet = take(g->wet.size() * 4 + 4);
     const size_t o_wets = take(g->wet.size() * 4 + 4), o_vpos = take(g->wet.size() * 4 + 4);
-    const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8);
+    const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8), o_cs = take((size_t)ncol * 4);
     hipSetDevice(e->device);
     if (hipMalloc(&g->dbuf, off) != hipSuccess) {
         delete g;
@@ -300,6 +300,11 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     }
     HIPCHK(e, hipMemcpy(b + o_V, g->V.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b + o_w, g->invw.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
+    {
+        std::vector<int32_t> cs(ncol, -1);
+        for (size_t q = 0; q < g->wet.size(); q++) cs[g->wet[q]] = (int32_t)q;
+        HIPCHK(e, hipMemcpy(b + o_cs, cs.data(), (size_t)ncol * 4, hipMemcpyHostToDevice));
+    }
     GridDev& gd = g->dev;
     gd.maxi = mi;
     gd.maxj = mj;
@@ -316,6 +321,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     gd.wetsorted = reinterpret_cast<const int32_t*>(b + o_wets);
     gd.vpos = reinterpret_cast<const int32_t*>(b + o_vpos);
     gd.V = reinterpret_cast<const double*>(b + o_V);
+    gd.cslot = reinterpret_cast<const int32_t*>(b + o_cs);
     g->d_invw = reinterpret_cast<double*>(b + o_w);
     gd.dphi = d->dphi;
     gd.rdphi = d->rdphi;
@@ -351,7 +357,7 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
     m.sflux = take(L * ncol * 8);
     m.rho = take(ncell * 8);
     m.cost = take(ncol * 8);
-    m.ctrl = take(256);  // parity, steps, diag[4], dmax
+    m.ctrl = take(256);  // parity, steps, diag[4], dmax, sfresh
     if (bgc) {
         m.ocn = take((2 + NBGC) * ncell * 8);
         m.part = take(NSED * ncell * 8);
@@ -420,6 +426,7 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
     d.steps = reinterpret_cast<int64_t*>(b + ly.ctrl + 8);
     d.diag = reinterpret_cast<int32_t*>(b + ly.ctrl + 16);
     d.dmax = reinterpret_cast<double*>(b + ly.ctrl + 32);
+    d.sfresh = reinterpret_cast<int32_t*>(b + ly.ctrl + 40);
     if (m->bgc) {
         d.ocn = reinterpret_cast<double*>(b + ly.ocn);
         d.part = reinterpret_cast<double*>(b + ly.part);
@@ -592,6 +599,7 @@ int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
     }
     const int32_t zero = 0;
     HIPCHK(e, hipMemcpy(m->dev.parity, &zero, 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(m->dev.sfresh, &zero, 4, hipMemcpyHostToDevice));
     return CGO_OK;
 }
 
@@ -869,6 +877,8 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
     HIPCHK(e, hipMemcpy(d.mld, ml.data(), ncol * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.colold, colold.data(), colold.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.scal, scal.data(), scal.size() * 8, hipMemcpyHostToDevice));
+    const int32_t zero = 0;
+    HIPCHK(e, hipMemcpy(d.sfresh, &zero, 4, hipMemcpyHostToDevice));
     e->table_dirty = true;
     return CGO_OK;
 }

@@ -44,6 +44,7 @@ struct GridDev {
     const int32_t* wetcols; // [nwet] column ids i0 + maxi*j0, reference vector order (i outer, j inner)
     const int32_t* wetsorted; // [nwet] the same ids ascending (i fastest): the thread order of column kernels
     const int32_t* vpos;    // [nwet] reference vector position of wetsorted[w] (partial-sum slot)
+    const int32_t* cslot;   // [ncol] partial-sum slot of each column, -1 for land
     const double* V;        // [ncell] cell volumes (biogem_data.f90:1128)
     double dphi, rdphi, rVtot;
     double ec[5];
@@ -90,7 +91,8 @@ struct MemberDev {
     double* scal;           // member scalars, see SC_*
     const double* rtab;     // [RT_N][maxk+2] remineralisation factors per level (host-computed)
     const int32_t* mink;    // [maxk+2] deepest-reach level of particles leaving level k (0 = bottom)
-    int32_t* diag;          // [4] limps, carb_fail, carb_iters, spare
+    int32_t* diag;          // [4] limps, carb_fail, carb_iters, co guard hits
+    int32_t* sfresh;        // device flag: colsum holds the sums of the current state
     double* dmax;           // device scalar
 };
 

@@ -421,7 +421,8 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_sums(const MemberDev* __
     const MemberDev& M = mems[list[blockIdx.y]];
     const GridDev& g = M.g;
     const int w0 = blockIdx.x * CPL_COLS;
-    if (w0 >= g.nwet) return;  // uniform over the block
+    if (w0 >= g.nwet) return;        // uniform over the block
+    if (*gp(M.sfresh)) return;        // the transport step already formed them
     cpl_tile(g, w0, T);
     const int c = threadIdx.x % CPL_COLS, lg = threadIdx.x / CPL_COLS;
     const int maxk = g.maxk, L = M.L, nsum = L - 1, tg = cpl_tg(maxk) < TG ? cpl_tg(maxk) : TG;
@@ -687,6 +688,8 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
         }
         __syncthreads();
     }
+    // the state changes below: the next coupling's sums come from the next transport step
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gp(M.sfresh) = 0;
     // T, S, masses and particulates of the thread's cells
 #pragma unroll
     for (int j = 0; j < NJ; j++) {

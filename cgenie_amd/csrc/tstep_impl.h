@@ -132,6 +132,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const int c0 = blockIdx.x * FLUX_THREADS;
     const int l0 = blockIdx.y * LB;
     if (c0 >= ncol || l0 >= L) return;  // uniform for the block
+    // every column's sums are formed below (land columns have none)
+    if (M.bgc && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
     const int nl = (L - l0) < LB ? (L - l0) : LB;
     const int t0 = c0 - maxi > 0 ? c0 - maxi : 0;
     const int t1 = c0 + FLUX_THREADS + maxi < ncol ? c0 + FLUX_THREADS + maxi : ncol;
@@ -247,6 +249,23 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     for (int q = 0; q < LB; q++) FB(q) = (R)0;
     int32_t limps = 0;
     R dmaxv = (R)0;
+    // Members with BIOGEM: the coupling's column sums of the new state
+    // (biogem.f90:1984-1997, 2016-2025), accumulated up the column in the
+    // order the coupling adds them; co redoes the columns it mixes.
+    const bool sums = M.bgc != 0;
+    const double rSold = sums ? 1.0 / gp(M.scal)[SC_SMEAN] : 0.0;
+    double accs[LB], accS = 0.0;
+#pragma unroll
+    for (int q = 0; q < LB; q++) accs[q] = 0.0;
+    auto add_sum = [&](int q, double nv, double os, double mm, double vv) {
+        const int l = l0 + q;
+        if (l == 1) {
+            accS += (nv + SALN0 + 0.0) * vv;
+        } else if (l >= 2) {
+            const double lv = nv * os * rSold;
+            accs[q] += lv * mm;
+        }
+    };
     fetch(kmin - 1);
     commit(kmin % FLUX_NBUF);
     if (kmin < maxk) {
@@ -332,6 +351,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const R divu = ANOM ? u[3 * ncell + o + col] : (R)0;
             const R isoc = (R)0.25 * slim * diff1;
             double newT = 0, newS = 0;
+            double s_os = 0.0, s_mm = 0.0, s_vv = 0.0;
+            if (sums) {
+                s_os = gp(M.ocn)[ncell + o + col];
+                s_mm = gp(M.M)[o + col];
+                if (l0 == 0) s_vv = gp(g.V)[o + col];
+            }
             if constexpr (ANOM) {
                 // FP32 form in coefficient form: each face flux is the
                 // neighbour difference times a coefficient of the cell, and the
@@ -395,6 +420,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     FB(q) = fbn;
                     if (l0 + q == 0) newT = nv;
                     if (l0 + q == 1) newS = nv;
+                    if (sums) add_sum(q, nv, s_os, s_mm, s_vv);
                 }
             } else
 #pragma unroll
@@ -473,6 +499,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 }
                 if (l0 + q == 0) newT = nv;
                 if (l0 + q == 1) newS = nv;
+                if (sums) add_sum(q, nv, s_os, s_mm, s_vv);
             }
             if (l0 == 0) rhop[o + col] = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
         }
@@ -482,6 +509,17 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     if (l0 == 0 && M.diso && active) {
         if (limps) atomicAdd(&M.diag[0], limps);
         if (dmaxv > (R)0) atomic_max_pos(M.dmax, (double)dmaxv);
+    }
+    if (sums && active && k1c <= maxk) {
+        const int slot = gp(g.cslot)[col];
+        AS_GLOBAL double* cs = gp(M.colsum);
+#pragma unroll
+        for (int q = 0; q < LB; q++) {
+            const int l = l0 + q;
+            if (q >= nl) break;
+            if (l == 1) cs[slot] = accS;
+            else if (l >= 2) cs[(size_t)l * g.nwet + slot] = accs[q];
+        }
     }
 }
 
@@ -568,6 +606,27 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
             cost += 1.0;
         } else {
             m = m - 1;
+        }
+    }
+    if ((mixed || cost > 0.0) && M.bgc) {
+        // the coupling's column sums of this column after mixing and fill-in
+        const double rSold = 1.0 / gp(M.scal)[SC_SMEAN];
+        const size_t ncl = (size_t)g.ncell, slot = (size_t)gp(g.vpos)[w];
+        AS_GLOBAL double* cs = gp(M.colsum);
+        double aS = 0.0;
+        for (int q = k1; q <= maxk; q++) {
+            const size_t c = (size_t)(q - 1) * plane + col;
+            aS += (StateIO<R>::ld(st, lo, tstride + c) + SALN0 + 0.0) * gp(g.V)[c];
+        }
+        cs[slot] = aS;
+        for (int l = 2; l < L; l++) {
+            double a = 0.0;
+            for (int q = k1; q <= maxk; q++) {
+                const size_t c = (size_t)(q - 1) * plane + col;
+                const double lv = StateIO<R>::ld(st, lo, (size_t)l * tstride + c) * gp(M.ocn)[ncl + c] * rSold;
+                a += lv * gp(M.M)[c];
+            }
+            cs[(size_t)l * g.nwet + slot] = a;
         }
     }
     if (mixed || cost > 0.0) {
