@@ -454,6 +454,10 @@ int cgo_member_create(cgo_engine* e, int32_t grid_id, int32_t maxl, int32_t prec
     if (grid_id < 0 || grid_id >= (int)e->grids.size()) return fail(e, CGO_E_ARG, "member_create: bad grid %d", grid_id);
     if (maxl < 2 || maxl > 256) return fail(e, CGO_E_ARG, "member_create: maxl %d outside 2..256", maxl);
     if (precision != CGO_FP64 && precision != CGO_FP32C) return fail(e, CGO_E_ARG, "member_create: bad precision");
+    // kernels address per-member arrays with 32-bit byte offsets
+    if ((size_t)maxl * e->grids[grid_id]->dev.ncell * sizeof(double) >= ((size_t)1 << 31))
+        return fail(e, CGO_E_ARG, "member_create: %d tracers x %d cells exceed 2 GiB per state array", maxl,
+                    e->grids[grid_id]->dev.ncell);
     int slot = -1;
     for (size_t i = 0; i < e->members.size(); i++)
         if (!e->members[i] || !e->members[i]->alive) {

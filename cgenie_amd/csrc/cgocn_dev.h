@@ -115,6 +115,25 @@ __device__ __forceinline__ AS_GLOBAL T* gp(T* p) { return (AS_GLOBAL T*)p; }
 template <typename T>
 __device__ __forceinline__ const AS_GLOBAL T* gp(const T* p) { return (const AS_GLOBAL T*)p; }
 
+// Element i of a per-member array, addressed as the array's base plus a
+// 32-bit byte offset: with a base uniform over the wave the load or store is
+// issued as scalar base + vector offset, with no 64-bit address arithmetic per
+// lane.  Every per-member array is below 2 GiB (checked at member creation).
+template <typename T>
+__device__ __forceinline__ const AS_GLOBAL T& at(const AS_GLOBAL T* b, uint32_t i) {
+    return *(const AS_GLOBAL T*)((const AS_GLOBAL char*)b + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ AS_GLOBAL T& at(AS_GLOBAL T* b, uint32_t i) {
+    return *(AS_GLOBAL T*)((AS_GLOBAL char*)b + i * (uint32_t)sizeof(T));
+}
+
+// unbiased binary exponent of a float, zero and subnormals taken as -126
+__device__ __forceinline__ int carry_exp(float x) {
+    const int eb = (int)((__float_as_uint(x) >> 23) & 0xffu);
+    return (eb > 1 ? eb : 1) - 127;
+}
+
 __device__ __forceinline__ int k1at(const GridDev& g, int I, int J) { return ((const AS_GLOBAL int32_t*)g.k1)[J * (g.maxi + 2) + I]; }
 
 // eos, goldstein.f90:3048-3061 (ieos = 0)

@@ -29,13 +29,14 @@ template <> struct Met<float> {
 };
 
 // weight of one unit of the int16 carry of a float value x: 2^-16 ulp(x)
-__device__ __forceinline__ double carry_unit(float x) {
-    int e = (x == 0.0f) ? -126 : ilogbf(x);
-    if (e < -126) e = -126;
-    return ldexp(1.0, e - 23 - 16);
-}
+__device__ __forceinline__ double carry_unit(float x) { return ldexp(1.0, carry_exp(x) - 23 - 16); }
 
 // state access as double (FP32 form: hi + carry)
+// a / b: the IEEE quotient in FP64; in the FP32 form the hardware reciprocal
+// (1 ulp), for the per-cell isoneutral coefficients
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <typename R> struct StateIO;
 // (all state pointers are global memory: the casts keep the accesses off
 // the flat path, whose loads also count against the LDS wait counter)
@@ -45,15 +46,15 @@ template <> struct StateIO<double> {
     struct Raw {
         double v;
     };
-    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t*, size_t i) {
-        return Raw{((const AS_GLOBAL double*)s)[i]};
+    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t*, uint32_t i) {
+        return Raw{at((const AS_GLOBAL double*)s, i)};
     }
     static __device__ __forceinline__ double val(Raw r, bool) { return r.v; }
-    static __device__ __forceinline__ double ld(const void* s, const int16_t*, size_t i) {
-        return ((const AS_GLOBAL double*)s)[i];
+    static __device__ __forceinline__ double ld(const void* s, const int16_t*, uint32_t i) {
+        return at((const AS_GLOBAL double*)s, i);
     }
-    static __device__ __forceinline__ void st(void* s, int16_t*, size_t i, double x) {
-        ((AS_GLOBAL double*)s)[i] = x;
+    static __device__ __forceinline__ void st(void* s, int16_t*, uint32_t i, double x) {
+        at((AS_GLOBAL double*)s, i) = x;
     }
 };
 template <> struct StateIO<float> {
@@ -61,10 +62,10 @@ template <> struct StateIO<float> {
         float h;
         int16_t c;
     };
-    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t* lo, size_t i) {
+    static __device__ __forceinline__ Raw ldr(const void* s, const int16_t* lo, uint32_t i) {
         Raw r;
-        r.h = ((const AS_GLOBAL float*)s)[i];
-        r.c = lo ? ((const AS_GLOBAL int16_t*)lo)[i] : (int16_t)0;
+        r.h = at((const AS_GLOBAL float*)s, i);
+        r.c = lo ? at((const AS_GLOBAL int16_t*)lo, i) : (int16_t)0;
         return r;
     }
     static __device__ __forceinline__ double val(Raw r, bool carry) {
@@ -72,23 +73,23 @@ template <> struct StateIO<float> {
         if (carry) x += (double)r.c * carry_unit(r.h);
         return x;
     }
-    static __device__ __forceinline__ double ld(const void* s, const int16_t* lo, size_t i) {
-        float h = ((const AS_GLOBAL float*)s)[i];
+    static __device__ __forceinline__ double ld(const void* s, const int16_t* lo, uint32_t i) {
+        float h = at((const AS_GLOBAL float*)s, i);
         double x = (double)h;
-        if (lo) x += (double)((const AS_GLOBAL int16_t*)lo)[i] * carry_unit(h);
+        if (lo) x += (double)at((const AS_GLOBAL int16_t*)lo, i) * carry_unit(h);
         return x;
     }
-    static __device__ __forceinline__ void st(void* s, int16_t* lo, size_t i, double x) {
+    static __device__ __forceinline__ void st(void* s, int16_t* lo, uint32_t i, double x) {
         float h = (float)x;
-        ((AS_GLOBAL float*)s)[i] = h;
+        at((AS_GLOBAL float*)s, i) = h;
         if (lo) {
             // x - h in units of carry_unit(h): scaling by the exact power of two
-            // 2^(39 - e) rounds exactly as dividing by 2^(e - 39) would
-            int e = (h == 0.0f) ? -126 : ilogbf(h);
-            if (e < -126) e = -126;
-            double r = rint(ldexp(x - (double)h, 23 + 16 - e));
-            r = fmin(fmax(r, -32767.0), 32767.0);
-            ((AS_GLOBAL int16_t*)lo)[i] = (int16_t)r;
+            // 2^(39 - e) rounds exactly as dividing by 2^(e - 39) would; the
+            // rounded value is within +-2^15, so the clamp is taken on the integer
+            const double r = rint(ldexp(x - (double)h, 23 + 16 - carry_exp(h)));
+            int ri = (int)r;
+            ri = ri < -32767 ? -32767 : (ri > 32767 ? 32767 : ri);
+            at((AS_GLOBAL int16_t*)lo, i) = (int16_t)ri;
         }
     }
 };
@@ -116,10 +117,10 @@ __host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es, int 
 }
 
 // the carry c of a float h as a float: c * 2^-16 ulp(h) (exact unless |h| < 2^-87)
+// (c 2^-39 is exact; times 2^e, the exponent bits of h, it rounds once, as
+// ldexpf(c, e - 39) does; zero and subnormal h give 0, as ldexpf(c, -165) does)
 __device__ __forceinline__ float carry_residual(float h, int16_t c) {
-    int e = (h == 0.0f) ? -126 : ilogbf(h);
-    if (e < -126) e = -126;
-    return ldexpf((float)c, e - 23 - 16);
+    return ((float)c * 0x1p-39f) * __uint_as_float(__float_as_uint(h) & 0x7f800000u);
 }
 
 template <typename R, int LB, int FLUX_THREADS>
@@ -226,8 +227,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             for (int q = 0; q < LB + 2; q++)
                 if (q < nslot || q >= LB) {
                     const bool ld = ok && (q < nslot);
-                    pf[q][r] = ld ? src[slot_tracer(q) * tstride + o + pp] : (R)0;
-                    if (ANOM) pc[ANOM ? q : 0][r] = (ld && lo_src) ? lo_src[slot_tracer(q) * tstride + o + pp] : (int16_t)0;
+                    const uint32_t i = (uint32_t)(slot_tracer(q) * tstride + o + pp);
+                    pf[q][r] = ld ? at(src, i) : (R)0;
+                    if (ANOM) pc[ANOM ? q : 0][r] = (ld && lo_src) ? at(lo_src, i) : (int16_t)0;
                 }
         }
     };
@@ -282,9 +284,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         if (active && k >= k1c) {
             const int o = kz * plane;
             // face velocities and Peclet-weighted upwinding (goldstein.f90:2489-2523)
-            const R ue_ = u[o + col], uw_ = u[o + cw];
-            const R un_ = u[ncell + o + col], us_ = u[ncell + o + cs];
-            const R ua_ = u[2 * ncell + o + col];
+            const uint32_t oc = (uint32_t)(o + col);
+            const R ue_ = at(u, oc), uw_ = at(u, (uint32_t)(o + cw));
+            const R un_ = at(u, ncell + oc), us_ = at(u, (uint32_t)(ncell + o + cs));
+            const R ua_ = at(u, 2 * ncell + oc);
             const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k];
             const R dt = mk[MK_DT * KS + k], rdz = mk[MK_RDZ * KS + k];
             R upe, upw, upn, ups, upa;
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const bool on = k >= max(k1c, k1n), os = k >= max(k1s, k1c);
             // isoneutral coefficients from T and S (2489-2501, 2561-2605)
             bool iso = false;
-            R dzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            R dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
             if (!top) {
                 const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p);
                 const R tatw = (R)0.5 * (Tc + Tu);
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 dzrho = (e1 * DK(ub, cb, qS, p) - tec * DK(ub, cb, qT, p)) * rdza;
                 if (M.diso && dzrho < (R)-1.0E-12) {
                     iso = true;
-                    const R rdzrho = (R)1.0 / dzrho;
+                    rdzrho = fdiv((R)1.0, dzrho);
                     R tv1 = 0;
                     // ina order (knp, nnp) = (0,0), (0,1), (1,0), (1,1)
 #pragma unroll
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
                     const R ssmax = mk[MK_SSMAX * KS + k];
                     if (tv1 > ssmax) {
-                        slim = ssmax * ssmax / (tv1 * tv1);
+                        slim = fdiv(ssmax * ssmax, tv1 * tv1);
                         limps++;
                     } else {
                         slim = 1;
@@ -348,14 +351,14 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             }
             // discrete divergence of the open faces (FP32 form only)
             // (evaluated in FP64 from the FP64 velocities when they are set)
-            const R divu = ANOM ? u[3 * ncell + o + col] : (R)0;
+            const R divu = ANOM ? at(u, 3 * ncell + oc) : (R)0;
             const R isoc = (R)0.25 * slim * diff1;
             double newT = 0, newS = 0;
             double s_os = 0.0, s_mm = 0.0, s_vv = 0.0;
             if (sums) {
-                s_os = gp(M.ocn)[ncell + o + col];
-                s_mm = gp(M.M)[o + col];
-                if (l0 == 0) s_vv = gp(g.V)[o + col];
+                s_os = at(gp(M.ocn), ncell + oc);
+                s_mm = at(gp(M.M), oc);
+                if (l0 == 0) s_vv = at(gp(g.V), oc);
             }
             if constexpr (ANOM) {
                 // FP32 form in coefficient form: each face flux is the
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 const R cbn = top ? (R)0 : -(ua_ * ((R)1.0 + upa) * (R)0.5) - rdza * diffv;
                 R ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, qq = 0;
                 if (iso) {
-                    const R f = (R)2 * isoc / dzrho;
+                    const R f = (R)2 * isoc * rdzrho;
                     R s2 = 0;
 #pragma unroll
                     for (int n = 0; n < 4; n++) {
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         ay[n] = f * dyr[n] * (east ? cvn * rdsvn : -(cvs * rdsvs));
                         s2 = s2 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
                     }
-                    qq = isoc * s2 / (dzrho * dzrho) * rdza;
+                    qq = isoc * s2 * (rdzrho * rdzrho) * rdza;
                 }
 #pragma unroll
                 for (int q = 0; q < LB; q++) {
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     const R dE = D(cb, q, pe, p), dW = D(cb, q, pw, p), dN = D(cb, q, pn, p), dS = D(cb, q, ps, p);
                     R fa, fbn = 0;
                     if (top) {
-                        fa = M.sflux ? (R)gp(M.sflux)[(l0 + q) * plane + col] : (R)0;
+                        fa = M.sflux ? (R)at(gp(M.sflux), (uint32_t)((l0 + q) * plane + col)) : (R)0;
                     } else {
                         const R dU = DK(ub, cb, q, p);
                         R tv = 0;
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         StateIO<R>::st((R*)dst, lo_dst, idx, nv);
                     } else {
                         const R v = Tc + inc;
-                        dst[idx] = v;
+                        at(dst, (uint32_t)idx) = v;
                         nv = (double)v;
                     }
                     FB(q) = fbn;
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     fs = os ? cvs * us_ * ((R)1.0 + ups) * dS * (R)0.5 + cv2s * dS * diff1 : (R)0;
                 }
                 if (top) {
-                    fa = M.sflux ? (R)gp(M.sflux)[(l0 + q) * plane + col] : (R)0;
+                    fa = M.sflux ? (R)at(gp(M.sflux), (uint32_t)((l0 + q) * plane + col)) : (R)0;
                 } else {
                     const R Tu = SP(ub, q, p);
                     const R dUc = DK(ub, cb, q, p);
@@ -487,13 +490,13 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         StateIO<R>::st((R*)dst, lo_dst, idx, nv);
                     } else {
                         const R v = Tc + inc;
-                        dst[idx] = v;
+                        at(dst, (uint32_t)idx) = v;
                         nv = (double)v;
                     }
                     FB(q) = fbn;
                 } else {
                     const R v = Tc - dt * div;
-                    dst[idx] = v;
+                    at(dst, (uint32_t)idx) = v;
                     nv = (double)v;
                     FB(q) = fa;
                 }
@@ -501,7 +504,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 if (l0 + q == 1) newS = nv;
                 if (sums) add_sum(q, nv, s_os, s_mm, s_vv);
             }
-            if (l0 == 0) rhop[o + col] = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
+            if (l0 == 0) at(rhop, oc) = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
         }
         if (more) commit((k + 2) % FLUX_NBUF);
         __syncthreads();
