@@ -86,6 +86,45 @@ def timed_years(eng, graph, years):
     return out
 
 
+def run_probe(args):
+    """Counter runs (rocprofv3 --pmc): `args.steps` eager steps of the ensemble
+    at the given tile, then a 1 GiB fill and a 1 GiB device-to-device copy
+    whose FETCH_SIZE / WRITE_SIZE readings calibrate the byte counters."""
+    import torch
+
+    from cgenie_amd import engine as E
+
+    prec = E.FP32C if args.probe == "fp32c" else E.FP64
+    eng = E.Engine(0)
+    ids, tset, n_wet, g = build_ensemble(eng, E, args.members, prec)
+    eng.set_tile(prec, args.probe_tb, args.probe_cols, args.probe_ring)
+    eng.step(ids, args.steps)
+    eng.sync()
+    print(f"# probe: {args.members} members x {args.steps} steps done", file=sys.stderr, flush=True)
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB written
+    y = torch.empty_like(x)
+    y.copy_(x)                                                    # 1 GiB read, 1 GiB written
+    torch.cuda.synchronize()
+    print("# probe: calibration fill and copy done", file=sys.stderr, flush=True)
+    eng.close()
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "flux_traffic_fp32c_36x36x16x24.txt")
+
+
+def measured_traffic():
+    """HBM bytes per flux launch from the committed counter summary (two
+    rocprofv3 --pmc runs of `bench.py --probe fp32c`, scaled by the copy
+    calibration), or None when the summary is absent."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith("bytes_per_launch"):
+                return int(line.split("=")[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
 def run_gpu(args):
     from cgenie_amd import engine as E
 
@@ -118,12 +157,13 @@ def run_gpu(args):
                         p50_year_s=float(np.percentile(per_year, 50)), p99_year_s=float(np.percentile(per_year, 99)),
                         flux_ms=flux_ms, co_ms=co_ms, bgc_coupling_ms=bgc_ms, flux_bytes=B,
                         flux_gbs=B / (flux_ms * 1e-3) / 1e9, carb_fail=int(d.carb_fail), n_wet=n_wet, L=tset.L,
-                        tracer_block=tuned[tag]["tracer_block"], flux_cols=tuned[tag]["cols"])
+                        tracer_block=tuned[tag]["tracer_block"], flux_cols=tuned[tag]["cols"],
+                        flux_ring=tuned[tag]["ring"])
         eng.drop_graph(graph)
         for m in ids:
             eng.remove_member(m)
     # plain FP64 single member, eager launches (the reference-shaped path)
-    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"])
+    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"], tuned["fp64"]["ring"])
     ids, tset, n_wet, g = build_ensemble(eng, E, 1, E.FP64)
     eng.step(ids, STEPS_PER_YEAR)
     eng.sync()
@@ -203,11 +243,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--probe", choices=("fp32c", "fp64"), default=None,
+                    help="counter-run mode: eager steps of the ensemble and a copy calibration, no JSON line")
+    ap.add_argument("--probe-tb", type=int, default=4)
+    ap.add_argument("--probe-cols", type=int, default=256)
+    ap.add_argument("--probe-ring", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(ensemble members shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        run_probe(args)
+        return
     cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
     cpu = None if args.no_cpu else run_cpu_baseline(cores, args.cpu_steps)
     print(f"# cpu baseline done: {cpu}", file=sys.stderr, flush=True)
@@ -215,7 +263,7 @@ def main():
     serving = run_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     main_ = r["fp32c"]
     roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4), traffic=None)
+                frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic())
     line = dict(
         metric="member-years/s at 36x36x16x24 tracers (FP32-comp batched ensemble); p50/p99 year latency",
         value=round(main_["member_years_per_s"], 2), unit="member-years/s", n_gpus=1, steps=args.steps,
@@ -229,10 +277,11 @@ def main():
         fp64_single_member_years_per_s=round(r["fp64_single"]["member_years_per_s"], 3),
         speedup_vs_fp64_single=round(main_["member_years_per_s"] / r["fp64_single"]["member_years_per_s"], 1),
         kernels_ms=dict(fp32c=dict(flux=main_["flux_ms"], co=main_["co_ms"], bgc_coupling=main_["bgc_coupling_ms"],
-                                   tracer_block=main_["tracer_block"], flux_cols=main_["flux_cols"]),
+                                   tracer_block=main_["tracer_block"], flux_cols=main_["flux_cols"],
+                                   flux_ring=main_["flux_ring"]),
                         fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
                                   bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"],
-                                  flux_cols=r["fp64"]["flux_cols"])),
+                                  flux_cols=r["fp64"]["flux_cols"], flux_ring=r["fp64"]["flux_ring"])),
         roofline=roof, cpu_baseline=cpu, serving=serving)
     print(json.dumps(line), flush=True)
 

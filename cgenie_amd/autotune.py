@@ -1,7 +1,8 @@
 """Tile-shape autotuner for the flux kernel.
 
 For a batch of members (grid, tracer count, precision, member count) the
-autotuner times every compiled tracer-block size of the flux kernel with HIP
+autotuner times every compiled tile shape of the flux kernel -- tracers per
+block, wet columns per block, level planes resident in LDS -- with HIP
 events on the engine's stream (cgo_time_kernel) and keeps the fastest.  The
 flux kernel only reads the current state and writes the other ping-pong
 buffer, so timing it repeatedly leaves the model state unchanged.
@@ -20,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 TRACER_BLOCKS = (2, 4, 6, 8)
 FLUX_COLS = (128, 256)
+FLUX_RINGS = (2, 3)
 
 
 def _key(grid_dims, L, precision, n_members) -> str:
@@ -48,22 +50,25 @@ def tune(eng: E.Engine, members, grid_dims, L: int, precision: int, reps: int = 
          use_cache: bool = True, write_cache: bool = True) -> dict:
     """Pick the flux kernel's tile shape for this batch.
 
-    Returns {"tracer_block": tb, "cols": cols, "times_ms": {"tbxcols": ms}}
-    and applies the choice to the engine."""
+    Returns {"tracer_block": tb, "cols": cols, "ring": planes,
+    "times_ms": {"tbxcolsxplanes": ms}} and applies the choice to the engine."""
     key = _key(grid_dims, L, precision, len(members))
     cache = load_cache() if use_cache else {}
     if key in cache:
-        eng.set_tile(precision, int(cache[key]["tracer_block"]), int(cache[key].get("cols", 128)))
-        return cache[key]
+        c = cache[key]
+        eng.set_tile(precision, int(c["tracer_block"]), int(c.get("cols", 128)), int(c.get("ring", 3)))
+        return c
     times = {}
-    for cols in FLUX_COLS:
-        for tb in TRACER_BLOCKS:
-            eng.set_tile(precision, tb, cols)
-            eng.time_kernel(0, members, 2)  # warm
-            times[(tb, cols)] = eng.time_kernel(0, members, reps)
+    for ring in FLUX_RINGS:
+        for cols in FLUX_COLS:
+            for tb in TRACER_BLOCKS:
+                eng.set_tile(precision, tb, cols, ring)
+                eng.time_kernel(0, members, 2)  # warm
+                times[(tb, cols, ring)] = eng.time_kernel(0, members, reps)
     best = min(times, key=times.get)
-    eng.set_tile(precision, best[0], best[1])
-    res = {"tracer_block": best[0], "cols": best[1], "times_ms": {f"{k[0]}x{k[1]}": v for k, v in times.items()}}
+    eng.set_tile(precision, *best)
+    res = {"tracer_block": best[0], "cols": best[1], "ring": best[2],
+           "times_ms": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in times.items()}}
     if write_cache:
         cache = load_cache()
         cache[key] = res

@@ -16,16 +16,16 @@
 #include "cgocn_dev.h"
 
 namespace cgo {
-void launch_flux64(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, int nt,
+void launch_flux64(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
                    size_t lds_cap, hipStream_t);
-void launch_flux32(const MemberDev*, const int32_t*, int nlist, int maxcol, int maxL, int maxi, int lb, int nt,
+void launch_flux32(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
                    size_t lds_cap, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
 void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int maxL, int prec,
                      hipStream_t);
-void launch_inventory(const MemberDev*, int32_t member, double* out, const double* w, int prec, hipStream_t);
+void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 }  // namespace cgo
 
 using namespace cgo;
@@ -46,6 +46,7 @@ struct GridH {
     std::vector<int32_t> wets, vpos; // ascending ids and their vocn positions
     std::vector<double> V;        // [ncell]
     std::vector<double> invw;     // [ncell] inventory weights dphi*ds(j)*dz(k)
+    int ftw[2] = {0, 0};          // widest flux tile for 128- and 256-thread blocks
     double Vtot = 0;
     void* dbuf = nullptr;
     double* d_invw = nullptr;
@@ -88,6 +89,7 @@ struct cgo_engine {
     bool table_dirty = false;
     int lb[2] = {4, 4};       // flux tracer block per precision
     int nt[2] = {128, 128};   // flux columns per block per precision
+    int nbuf[2] = {3, 3};     // flux level planes resident in LDS per precision
     size_t lds_cap = 65536;   // LDS one workgroup may take
     std::vector<Graph> graphs;
     int32_t* d_list = nullptr;  // scratch lists for eager steps
@@ -193,6 +195,9 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     if (!e || !d || !grid_id || !d->k1) return fail(e, CGO_E_ARG, "grid_create: null argument");
     const int mi = d->maxi, mj = d->maxj, mk = d->maxk;
     if (mi < 3 || mj < 3 || mk < 2 || mk > MAXK_DEV) return fail(e, CGO_E_ARG, "grid_create: bad dims %d %d %d", mi, mj, mk);
+    // a flux tile of one column and its neighbours (2 maxi + 1) must fit the
+    // 3 x 128 positions a 128-thread block stages
+    if (mi > 191) return fail(e, CGO_E_ARG, "grid_create: maxi %d above 191", mi);
     const double* jarr[] = {d->rc, d->rc2, d->cv, d->cv2, d->rds, d->dsv, d->rdsv, d->ds, d->sv};
     const double* karr[] = {d->dt, d->rdz, d->rdza, d->dza, d->ssmax, d->dz};
     for (auto* p : jarr)
@@ -281,6 +286,23 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     const size_t o_mjf = take(g->mj.size() * 4), o_mkf = take(g->mk.size() * 4), o_wet = take(g->wet.size() * 4 + 4);
     const size_t o_wets = take(g->wet.size() * 4 + 4), o_vpos = take(g->wet.size() * 4 + 4);
     const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8), o_cs = take((size_t)ncol * 4);
+    // flux blocks: up to nt consecutive wet columns (ascending), cut early so
+    // the staged range with its neighbours stays within flux_pf(nt) * nt
+    std::vector<FluxBlk> fb[2];
+    for (int v = 0; v < 2; v++) {
+        const int nt = v ? 256 : 128, cap = flux_pf(nt) * nt;
+        const int nw = (int)g->wets.size();
+        auto t0of = [&](int w) { return std::max(g->wets[w] - mi, 0); };
+        auto t1of = [&](int w) { return std::min(g->wets[w] + mi + 1, ncol); };
+        for (int w = 0; w < nw;) {
+            int e2 = w + 1;
+            while (e2 < nw && e2 - w < nt && t1of(e2) - t0of(w) <= cap) e2++;
+            fb[v].push_back(FluxBlk{w, e2, t0of(w), t1of(e2 - 1)});
+            g->ftw[v] = std::max(g->ftw[v], t1of(e2 - 1) - t0of(w));
+            w = e2;
+        }
+    }
+    const size_t o_fb0 = take(fb[0].size() * sizeof(FluxBlk) + 16), o_fb1 = take(fb[1].size() * sizeof(FluxBlk) + 16);
     hipSetDevice(e->device);
     if (hipMalloc(&g->dbuf, off) != hipSuccess) {
         delete g;
@@ -300,6 +322,8 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     }
     HIPCHK(e, hipMemcpy(b + o_V, g->V.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b + o_w, g->invw.data(), (size_t)ncell * 8, hipMemcpyHostToDevice));
+    if (!fb[0].empty()) HIPCHK(e, hipMemcpy(b + o_fb0, fb[0].data(), fb[0].size() * sizeof(FluxBlk), hipMemcpyHostToDevice));
+    if (!fb[1].empty()) HIPCHK(e, hipMemcpy(b + o_fb1, fb[1].data(), fb[1].size() * sizeof(FluxBlk), hipMemcpyHostToDevice));
     {
         std::vector<int32_t> cs(ncol, -1);
         for (size_t q = 0; q < g->wet.size(); q++) cs[g->wet[q]] = (int32_t)q;
@@ -322,6 +346,10 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     gd.vpos = reinterpret_cast<const int32_t*>(b + o_vpos);
     gd.V = reinterpret_cast<const double*>(b + o_V);
     gd.cslot = reinterpret_cast<const int32_t*>(b + o_cs);
+    gd.fblk[0] = reinterpret_cast<const FluxBlk*>(b + o_fb0);
+    gd.fblk[1] = reinterpret_cast<const FluxBlk*>(b + o_fb1);
+    gd.nfblk[0] = (int)fb[0].size();
+    gd.nfblk[1] = (int)fb[1].size();
     g->d_invw = reinterpret_cast<double*>(b + o_w);
     gd.dphi = d->dphi;
     gd.rdphi = d->rdphi;
@@ -653,7 +681,7 @@ int cgo_inventory(cgo_engine* e, int32_t id, double* inv) {
     if (rc) return rc;
     double* d_out = nullptr;
     HIPCHK(e, hipMalloc(&d_out, (size_t)m->L * 8));
-    launch_inventory(e->d_table, id, d_out, e->grids[m->grid]->d_invw, m->prec, e->s);
+    launch_inventory(e->d_table, id, m->L, d_out, e->grids[m->grid]->d_invw, m->prec, e->s);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(e->s));
     HIPCHK(e, hipMemcpy(inv, d_out, (size_t)m->L * 8, hipMemcpyDeviceToHost));
@@ -932,6 +960,7 @@ struct Lists {
     int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
     int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
+    FluxShape fs[2] = {};
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
@@ -947,6 +976,10 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
         ls.maxi[p] = std::max(ls.maxi[p], m->dev.g.maxi);
         ls.maxwet[p] = std::max(ls.maxwet[p], m->dev.g.nwet);
+        for (int v = 0; v < 2; v++) {
+            ls.fs[p].nblk[v] = std::max(ls.fs[p].nblk[v], m->dev.g.nfblk[v]);
+            ls.fs[p].tw[v] = std::max(ls.fs[p].tw[v], e->grids[m->grid]->ftw[v]);
+        }
         if (m->bgc) {
             ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
             ls.maxk_bgc[p] = std::max(ls.maxk_bgc[p], m->dev.g.maxk);
@@ -967,11 +1000,11 @@ static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_list
         if (!ls.n[p]) continue;
         const int32_t* l = d_lists + ls.off[p];
         if (p == 0)
-            launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->nt[0],
-                          e->lds_cap, e->s);
+            launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->lds_cap,
+                          e->s);
         else
-            launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->nt[1],
-                          e->lds_cap, e->s);
+            launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->lds_cap,
+                          e->s);
         launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
     }
     launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
@@ -1156,6 +1189,13 @@ int cgo_set_flux_cols(cgo_engine* e, int32_t precision, int32_t cols) {
     return CGO_OK;
 }
 
+int cgo_set_flux_ring(cgo_engine* e, int32_t precision, int32_t planes) {
+    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (planes != 2 && planes != 3))
+        return fail(e, CGO_E_ARG, "set_flux_ring: precision %d planes %d", precision, planes);
+    e->nbuf[precision == CGO_FP64 ? 0 : 1] = planes;
+    return CGO_OK;
+}
+
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
     if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 2) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
@@ -1171,11 +1211,11 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             const int32_t* l = e->d_list + ls.off[p];
             if (which == 0) {
                 if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[0], e->nt[0],
-                          e->lds_cap, e->s);
+                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->lds_cap,
+                          e->s);
                 else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.maxcol[p], ls.maxL[p], ls.maxi[p], e->lb[1], e->nt[1],
-                          e->lds_cap, e->s);
+                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->lds_cap,
+                          e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
             } else if (ls.n[2 + p]) {

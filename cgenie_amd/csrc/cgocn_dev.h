@@ -34,8 +34,22 @@ constexpr double NULLSMALL = +0.999999E-19;  // gem_cmn.f90:719
 constexpr double ZEROC = 273.15;
 constexpr double SALN0 = 34.9;
 
+// One block of the flux kernel: wet columns [w0, w1) of the ascending order,
+// staged (with every neighbour they read) over flattened columns [t0, t1).
+struct FluxBlk {
+    int32_t w0, w1, t0, t1;
+};
+// tile positions staged per thread of a flux block of nt threads
+__host__ __device__ constexpr int flux_pf(int nt) { return nt == 128 ? 3 : 2; }
+// the flux grid of a batch: blocks and widest tile, per block width (128, 256)
+struct FluxShape {
+    int nblk[2], tw[2];
+};
+
 struct GridDev {
     int maxi, maxj, maxk, ncol, ncell, nwet;
+    const FluxBlk* fblk[2];  // flux blocks for 128- and 256-thread workgroups
+    int nfblk[2];
     const int32_t* k1;      // [(maxj+2)*(maxi+2)]
     const double* mj;       // [NMJ*(maxj+2)]
     const double* mk;       // [NMK*(maxk+2)]

@@ -6,23 +6,9 @@
 
 namespace cgo {
 
-void launch_flux64(const MemberDev* t, const int32_t* l, int n, int maxcol, int maxL, int maxi, int lb, int nt, size_t lds_cap,
-                   hipStream_t s) {
-    if (nt != 256) nt = 128;
-    const int tw = nt + 2 * maxi;
-    // keep the workgroup within the LDS the device grants one block
-    while (lb > 2 && flux_lds_bytes(tw, lb, sizeof(double), nt) > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
-    if (nt == 256 && flux_lds_bytes(tw, lb, sizeof(double), nt) > lds_cap) nt = 128;
-    const int tw2 = nt + 2 * maxi;
-    const dim3 grid((maxcol + nt - 1) / nt, (maxL + lb - 1) / lb, n);
-    const size_t lds = flux_lds_bytes(tw2, lb, sizeof(double), nt);
-#define CGO_FLUX(LB_, NT_) hipLaunchKernelGGL((k_flux<double, LB_, NT_>), grid, dim3(NT_), lds, s, t, l, tw2)
-    if (nt == 256) {
-        if (lb == 2) CGO_FLUX(2, 256); else if (lb == 8) CGO_FLUX(8, 256); else if (lb == 6) CGO_FLUX(6, 256); else CGO_FLUX(4, 256);
-    } else {
-        if (lb == 2) CGO_FLUX(2, 128); else if (lb == 8) CGO_FLUX(8, 128); else if (lb == 6) CGO_FLUX(6, 128); else CGO_FLUX(4, 128);
-    }
-#undef CGO_FLUX
+void launch_flux64(const MemberDev* t, const int32_t* l, int n, const FluxShape& fs, int maxL, int lb, int nt,
+                   int nbuf, size_t lds_cap, hipStream_t s) {
+    launch_flux_t<double>(t, l, n, fs, maxL, lb, nt, nbuf, lds_cap, s);
 }
 
 void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int prec, hipStream_t s) {
@@ -61,14 +47,12 @@ __global__ __launch_bounds__(256) void k_inventory(const MemberDev* __restrict__
     if (threadIdx.x == 0) out[l] = red[0];
 }
 
-void launch_inventory(const MemberDev* t, int32_t m, double* out, const double* w, int prec, hipStream_t s) {
-    // one block per tracer; the member's L bounds the grid via the caller's buffer
-    MemberDev h;
-    hipMemcpy(&h, t + m, sizeof(MemberDev), hipMemcpyDeviceToHost);
+void launch_inventory(const MemberDev* t, int32_t m, int L, double* out, const double* w, int prec, hipStream_t s) {
+    // one block per tracer
     if (prec == 0)
-        hipLaunchKernelGGL(k_inventory<double>, dim3(h.L), dim3(256), 0, s, t, m, out, w);
+        hipLaunchKernelGGL(k_inventory<double>, dim3(L), dim3(256), 0, s, t, m, out, w);
     else
-        hipLaunchKernelGGL(k_inventory<float>, dim3(h.L), dim3(256), 0, s, t, m, out, w);
+        hipLaunchKernelGGL(k_inventory<float>, dim3(L), dim3(256), 0, s, t, m, out, w);
 }
 
 }  // namespace cgo

@@ -99,21 +99,21 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
 }
 
-// LDS layout of k_flux: the k1 tile, a ring of three level planes (k, k+1
-// and the prefetched k+2) of LB tracer slots plus T and S, each covering the
-// flattened column range [c0 - maxi, c0 + FLUX_THREADS + maxi) -- every
-// east/west (periodic wrap included, offsets below maxi) and north/south
-// neighbour of the block's columns -- and the carried bottom-face flux.
-// Block widths (columns per block): 128 or 256; a wider block shares its
-// halo among more columns.  FLUX_PF(nt) tile positions per thread cover the
-// tile of nt + 2*maxi columns for maxi <= 128.
-constexpr int FLUX_NBUF = 3;
-__host__ __device__ constexpr int flux_pf(int nt) { return nt == 128 ? 3 : 2; }
+// Work of k_flux: a block takes up to FLUX_THREADS consecutive wet columns
+// (ascending column order; land columns get no thread) and the tracer block
+// [l0, l0 + LB).  LDS layout: the k1 tile, a ring of NBUF level planes of LB
+// tracer slots plus T and S, each covering the block's flattened column range
+// [t0, t1) -- every east/west (periodic wrap included, offsets below maxi)
+// and north/south neighbour of its columns -- and the carried bottom-face
+// flux.  With NBUF = 3 the planes k, k+1 and the prefetched k+2 are
+// resident; with NBUF = 2 the prefetched plane replaces plane k after a
+// second barrier per level (less LDS, more workgroups per CU).  The host cuts
+// blocks so that t1 - t0 <= flux_pf(FLUX_THREADS) * FLUX_THREADS.
 // The FP32 form also stages the carry of every value, as a float residual,
 // so differences of neighbours are taken at the stored (~40-bit) precision.
-__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es, int nt) {
-    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)FLUX_NBUF * (lb + 2) * tw * es + (size_t)lb * nt * es +
-           (es == 4 ? (size_t)FLUX_NBUF * (lb + 2) * tw * 4 : 0);
+__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es, int nt, int nbuf) {
+    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)nbuf * (lb + 2) * tw * es + (size_t)lb * nt * es +
+           (es == 4 ? (size_t)nbuf * (lb + 2) * tw * 4 : 0);
 }
 
 // the carry c of a float h as a float: c * 2^-16 ulp(h) (exact unless |h| < 2^-87)
@@ -123,22 +123,21 @@ __device__ __forceinline__ float carry_residual(float h, int16_t c) {
     return ((float)c * 0x1p-39f) * __uint_as_float(__float_as_uint(h) & 0x7f800000u);
 }
 
-template <typename R, int LB, int FLUX_THREADS>
+template <typename R, int LB, int FLUX_THREADS, int FLUX_NBUF>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
                                                        const int32_t* __restrict__ list, int tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     const MemberDev& M = mems[list[blockIdx.z]];
     const GridDev& g = M.g;
-    const int maxi = g.maxi, maxj = g.maxj, maxk = g.maxk, L = M.L, ncol = g.ncol;
-    const int c0 = blockIdx.x * FLUX_THREADS;
+    const int maxi = g.maxi, maxj = g.maxj, maxk = g.maxk, L = M.L;
+    constexpr int V = FLUX_THREADS == 256 ? 1 : 0;
     const int l0 = blockIdx.y * LB;
-    if (c0 >= ncol || l0 >= L) return;  // uniform for the block
-    // every column's sums are formed below (land columns have none)
+    if ((int)blockIdx.x >= g.nfblk[V] || l0 >= L) return;  // uniform for the block
+    // every wet column's sums are formed below
     if (M.bgc && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
     const int nl = (L - l0) < LB ? (L - l0) : LB;
-    const int t0 = c0 - maxi > 0 ? c0 - maxi : 0;
-    const int t1 = c0 + FLUX_THREADS + maxi < ncol ? c0 + FLUX_THREADS + maxi : ncol;
-    const int TW = t1 - t0;
+    const AS_GLOBAL FluxBlk* blk = gp(g.fblk[V]) + blockIdx.x;
+    const int w0 = blk->w0, t0 = blk->t0, TW = blk->t1 - t0, nw = blk->w1 - w0;
     int* sk1 = reinterpret_cast<int*>(smem);
     R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
     R* sfb = sp + FLUX_NBUF * (LB + 2) * tw;
@@ -154,9 +153,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         sk1[pp] = k1at(g, c % maxi + 1, c / maxi + 1);
     }
     __syncthreads();
-    const int col = c0 + tid;
-    const bool active = col < ncol;
-    const int i0 = active ? col % maxi : 0, j0 = active ? col / maxi : 0;
+    const bool active = tid < nw;
+    const int col = active ? gp(g.wetsorted)[w0 + tid] : t0;
+    const int i0 = col % maxi, j0 = col / maxi;
     const int I = i0 + 1, J = j0 + 1;
     const int k1c = active ? k1at(g, I, J) : maxk + 1;
     const int k1e = k1at(g, I + 1, J), k1w = k1at(g, I - 1, J);
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     int16_t* __restrict__ lo_dst = M.lo[par ^ 1];
     const AS_GLOBAL R* __restrict__ u = gp(static_cast<const R*>(M.u));
     AS_GLOBAL double* __restrict__ rhop = gp(M.rho);
-    const int plane = ncol, tstride = maxk * ncol, ncell = g.ncell;
+    const int plane = g.ncol, tstride = maxk * g.ncol, ncell = g.ncell;
     // neighbour positions in the tile
     const int p = col - t0;
     const int pe = p + (i0 + 1 == maxi ? 1 - maxi : 1);
@@ -506,6 +505,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             }
             if (l0 == 0) at(rhop, oc) = eos<double>(g.ec[0], g.ec[1], g.ec[2], g.ec[3], newT, newS);
         }
+        if (FLUX_NBUF == 2) __syncthreads();  // plane k is read by every thread before it is replaced
         if (more) commit((k + 2) % FLUX_NBUF);
         __syncthreads();
     }
@@ -524,6 +524,41 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             else if (l >= 2) cs[(size_t)l * g.nwet + slot] = accs[q];
         }
     }
+}
+
+// Launch of k_flux over a batch: nt threads per block (128 or 256), lb
+// tracers per block, nbuf level planes; the tracer block, then the block
+// width, then the ring shrink until the workgroup fits the LDS the device
+// grants one block.
+template <typename R>
+void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape& fs, int maxL, int lb, int nt,
+                   int nbuf, size_t lds_cap, hipStream_t s) {
+    constexpr int es = sizeof(R);
+    if (nt != 256) nt = 128;
+    if (nbuf != 2) nbuf = 3;
+    auto bytes = [&]() { return flux_lds_bytes(fs.tw[nt == 256], lb, es, nt, nbuf); };
+    while (lb > 2 && bytes() > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
+    if (nt == 256 && bytes() > lds_cap) nt = 128;
+    if (nbuf == 3 && bytes() > lds_cap) nbuf = 2;
+    const int v = nt == 256, tw = fs.tw[v];
+    if (fs.nblk[v] == 0) return;
+    const dim3 grid(fs.nblk[v], (maxL + lb - 1) / lb, n);
+    const size_t lds = bytes();
+#define CGO_FLUX(LB_, NT_, NB_) hipLaunchKernelGGL((k_flux<R, LB_, NT_, NB_>), grid, dim3(NT_), lds, s, t, l, tw)
+#define CGO_FLUX_LB(NT_, NB_)                 \
+    do {                                      \
+        if (lb == 2) CGO_FLUX(2, NT_, NB_);   \
+        else if (lb == 8) CGO_FLUX(8, NT_, NB_); \
+        else if (lb == 6) CGO_FLUX(6, NT_, NB_); \
+        else CGO_FLUX(4, NT_, NB_);           \
+    } while (0)
+    if (nt == 256) {
+        if (nbuf == 2) CGO_FLUX_LB(256, 2); else CGO_FLUX_LB(256, 3);
+    } else {
+        if (nbuf == 2) CGO_FLUX_LB(128, 2); else CGO_FLUX_LB(128, 3);
+    }
+#undef CGO_FLUX_LB
+#undef CGO_FLUX
 }
 
 // co, goldstein.f90:2657-2777 (iconv = 0, ieos = 0): one thread per wet

@@ -94,6 +94,7 @@ def load() -> C.CDLL:
         "cgo_graph_destroy": ([P, _i32], C.c_int),
         "cgo_set_tile": ([P, _i32, _i32], C.c_int),
         "cgo_set_flux_cols": ([P, _i32, _i32], C.c_int),
+        "cgo_set_flux_ring": ([P, _i32, _i32], C.c_int),
         "cgo_time_kernel": ([P, _i32, ip, _i32, _i32, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -273,10 +274,12 @@ class Engine:
     def sync(self):
         self._chk(self.L.cgo_sync(self.h), "sync")
 
-    def set_tile(self, precision: int, tracer_block: int, cols: int | None = None):
+    def set_tile(self, precision: int, tracer_block: int, cols: int | None = None, ring: int | None = None):
         self._chk(self.L.cgo_set_tile(self.h, precision, tracer_block), "set_tile")
         if cols is not None:
             self._chk(self.L.cgo_set_flux_cols(self.h, precision, cols), "set_flux_cols")
+        if ring is not None:
+            self._chk(self.L.cgo_set_flux_ring(self.h, precision, ring), "set_flux_ring")
 
     def time_kernel(self, which: int, members, reps: int) -> float:
         arr, n = _ids(members)

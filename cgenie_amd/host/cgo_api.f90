@@ -229,6 +229,12 @@ module cgo_api
        integer(c_int32_t), value :: precision, cols
      end function cgo_set_flux_cols
 
+     integer(c_int) function cgo_set_flux_ring(e, precision, planes) bind(C, name="cgo_set_flux_ring")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: precision, planes
+     end function cgo_set_flux_ring
+
      integer(c_int) function cgo_time_kernel(e, which, members, n, reps, avg_ms) bind(C, name="cgo_time_kernel")
        import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: e

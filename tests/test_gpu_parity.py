@@ -96,19 +96,44 @@ def test_inventory_conserved_96_steps(eng):
     eng.remove_member(m)
 
 
-@pytest.mark.parametrize("tb", [2, 4, 8])
-def test_tracer_block_shapes_agree(eng, tb):
+# (tracers per block, wet columns per block, level planes in LDS)
+SHAPES = [(2, 128, 3), (4, 256, 2), (8, 128, 2), (6, 256, 3), (4, 128, 3), (8, 256, 3)]
+DEFAULT_SHAPE = (4, 128, 3)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tracer_block_shapes_agree(eng, shape):
+    say("fp64 flux shape", shape)
     c = make_case(kind="ts", L=11, seed=21)
-    eng.set_tile(E.FP64, tb)
+    eng.set_tile(E.FP64, *shape)
     m = new_member(eng, c)
     eng.step([m], 5)
     got = eng.download(m, c.ts.shape)
-    eng.set_tile(E.FP64, 4)
+    eng.set_tile(E.FP64, *DEFAULT_SHAPE)
     gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
     gold.step(5)
     w = wet(c)
     assert np.count_nonzero(got[w] != gold.ts[w]) == 0
     eng.remove_member(m)
+
+
+def test_fp32_tile_shapes_identical(eng):
+    """The FP32 form gives the same bits for every flux tile shape (each
+    cell's arithmetic does not depend on how cells are grouped)."""
+    c = make_case(kind="config2", seed=23)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    outs = []
+    for shape in [DEFAULT_SHAPE] + SHAPES:
+        say("fp32 flux shape", shape)
+        eng.set_tile(E.FP32C, *shape)
+        m = new_member(eng, c, prec=E.FP32C)
+        eng.bgc_init(m, p, c.light, c.mld)
+        eng.step([m], 4)
+        outs.append(eng.download(m, c.ts.shape))
+        eng.remove_member(m)
+    eng.set_tile(E.FP32C, *DEFAULT_SHAPE)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
 
 
 @pytest.mark.parametrize("kind,mld", [("config2", 0.0), ("domc", 150.0)])
