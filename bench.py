@@ -109,16 +109,17 @@ def run_probe(args):
     eng.close()
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "flux_traffic_fp32c_36x36x16x24.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "counters_fp32c_36x36x16x24.txt")
 
 
-def measured_traffic():
-    """HBM bytes per flux launch from the committed counter summary (two
-    rocprofv3 --pmc runs of `bench.py --probe fp32c`, scaled by the copy
-    calibration), or None when the summary is absent."""
+def measured_traffic(shape: str):
+    """HBM bytes per flux launch at tile `shape` ("tb x cols x ring") from the
+    committed counter summary (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+    of `bench.py --probe fp32c`, scaled by the copy calibration), or None when
+    the summary holds no figure for that tile."""
     try:
         for line in open(TRAFFIC_SUMMARY):
-            if line.startswith("bytes_per_launch"):
+            if line.startswith(f"bytes_per_launch {shape} ="):
                 return int(line.split("=")[1].split()[0])
     except OSError:
         return None
@@ -263,7 +264,8 @@ def main():
     serving = run_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     main_ = r["fp32c"]
     roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic())
+                frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4),
+                traffic=measured_traffic(f'{main_["tracer_block"]}x{main_["flux_cols"]}x{main_["flux_ring"]}'))
     line = dict(
         metric="member-years/s at 36x36x16x24 tracers (FP32-comp batched ensemble); p50/p99 year latency",
         value=round(main_["member_years_per_s"], 2), unit="member-years/s", n_gpus=1, steps=args.steps,
