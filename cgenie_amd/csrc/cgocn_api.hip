@@ -24,7 +24,7 @@ void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int pre
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
 void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int maxL, int prec,
-                     hipStream_t);
+                     size_t lds_cap, hipStream_t);
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 }  // namespace cgo
 
@@ -1022,7 +1022,7 @@ static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_li
     for (int p = 0; p < 2; p++)
         if (ls.n[2 + p])
             launch_coupling(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],
-                            ls.maxL[p], p, e->s);
+                            ls.maxL[p], p, e->lds_cap, e->s);
 }
 
 // whole ocean steps in the genie.f90 order
@@ -1221,7 +1221,7 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             } else if (ls.n[2 + p]) {
                 launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
                 launch_coupling(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],
-                                ls.maxL[p], p, e->s);
+                                ls.maxL[p], p, e->lds_cap, e->s);
             }
         }
     HIPCHK(e, hipEventRecord(e->ev1, e->s));

@@ -502,12 +502,12 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_sums(const MemberDev* __
 // block stages a chunk of columns of every partial-sum row in LDS, then each
 // sum is one thread's sequential pass over its row of the chunk; chunks go in
 // column order, so every sum is added in the reference's order.
-constexpr int RED_THREADS = 256;
-constexpr int RED_LDS_BYTES = 56 * 1024;
+constexpr int RED_THREADS = 1024;  // one block per member: many loads in flight per chunk
+constexpr int RED_LDS_BYTES = 128 * 1024;
 constexpr int RED_MAXL = 256;
 
 __global__ __launch_bounds__(RED_THREADS) void k_couple_reduce(const MemberDev* __restrict__ mems,
-                                                               const int32_t* __restrict__ list) {
+                                                               const int32_t* __restrict__ list, int lds_bytes) {
     extern __shared__ double rows[];
     __shared__ double s_val[2 * RED_MAXL];
     const MemberDev& M = mems[list[blockIdx.x]];
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_couple_reduce(const MemberDev* 
         if (q < L - 1) return gp(M.colsum) + (size_t)(q + 1) * nwet;
         return gp(M.colold) + (size_t)(q - L + 3) * nwet;
     };
-    int ch = RED_LDS_BYTES / (8 * nsum);   // columns per chunk, odd (LDS banks)
+    int ch = lds_bytes / (8 * nsum);   // columns per chunk, odd (LDS banks)
     if (ch > nwet) ch = nwet;
     if ((ch & 1) == 0) ch -= 1;
     if (ch < 1) ch = 1;
@@ -547,17 +547,22 @@ __global__ __launch_bounds__(RED_THREADS) void k_couple_reduce(const MemberDev* 
                 if (b0 + u * RED_THREADS < tot) rows[at[u]] = v[u];
         }
         __syncthreads();
-        if (t < nsum) {
-            const double* r = rows + t * ch;
-            if (t == 0)
-                for (int i = 0; i < n; i++) tot0 += r[i] * sc;
-            else
-                for (int i = 0; i < n; i++) tot0 += r[i];
-        }
-        if (t + RED_THREADS < nsum) {
-            const double* r = rows + (t + RED_THREADS) * ch;
-            for (int i = 0; i < n; i++) tot1 += r[i];
-        }
+        // each sum adds its columns in order; the LDS reads of eight columns
+        // are issued before the first of their adds
+        auto run = [&](const double* r, double tot, double f, bool scaled) {
+            int i = 0;
+            for (; i + 8 <= n; i += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = r[i + u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) tot += scaled ? v[u] * f : v[u];
+            }
+            for (; i < n; i++) tot += scaled ? r[i] * f : r[i];
+            return tot;
+        };
+        if (t < nsum) tot0 = run(rows + t * ch, tot0, sc, t == 0);
+        if (t + RED_THREADS < nsum) tot1 = run(rows + (t + RED_THREADS) * ch, tot1, 1.0, false);
         __syncthreads();
     }
     if (t < nsum) s_val[t] = tot0;
@@ -736,7 +741,7 @@ void launch_bio(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
 
 template <typename R>
 static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax,
-                              hipStream_t s) {
+                              size_t lds_cap, hipStream_t s) {
     const dim3 grid((nwetmax + CPL_COLS - 1) / CPL_COLS, nlist);
     // LDS for the largest grid of the batch; every member's tg fits in it
     const size_t lds = (size_t)cpl_tg(maxkmax) * maxkmax * CPL_COLS * 8;
@@ -750,7 +755,9 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
     } else {
         hipLaunchKernelGGL((k_couple_sums<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
     }
-    hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), RED_LDS_BYTES, s, mems, list);
+    // the staging chunk takes what LDS the device grants a block, up to RED_LDS_BYTES
+    const int red_lds = (int)std::min<size_t>(RED_LDS_BYTES, lds_cap - 2 * RED_MAXL * sizeof(double));
+    hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), (size_t)red_lds, s, mems, list, red_lds);
     if (maxkmax <= 4 * CPL_LG) {
         hipLaunchKernelGGL((k_couple_apply<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
     } else if (maxkmax <= 8 * CPL_LG) {
@@ -761,12 +768,12 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
 }
 
 void launch_coupling(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax, int maxL,
-                     int prec, hipStream_t s) {
+                     int prec, size_t lds_cap, hipStream_t s) {
     (void)maxL;
     if (prec == 0)
-        launch_coupling_t<double>(mems, list, nlist, nwetmax, maxkmax, s);
+        launch_coupling_t<double>(mems, list, nlist, nwetmax, maxkmax, lds_cap, s);
     else
-        launch_coupling_t<float>(mems, list, nlist, nwetmax, maxkmax, s);
+        launch_coupling_t<float>(mems, list, nlist, nwetmax, maxkmax, lds_cap, s);
 }
 
 }  // namespace cgo
