@@ -142,6 +142,31 @@ __device__ __forceinline__ AS_GLOBAL T& at(AS_GLOBAL T* b, uint32_t i) {
     return *(AS_GLOBAL T*)((AS_GLOBAL char*)b + i * (uint32_t)sizeof(T));
 }
 
+// Workgroups of a launch are dealt to the 8 XCDs (each with its own L2)
+// round-robin in dispatch order.  xcd_block renumbers the dispatch index h of
+// an n-block launch so that every XCD runs one contiguous range of the
+// renumbered blocks: blocks that share data (the tiles and tracer blocks of
+// one member) then share an L2.  A bijection of [0, n) for any n.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block(int h, int n) {
+    const int q = n / NXCD, rem = n % NXCD, x = h % NXCD, r = h / NXCD;
+    return x * q + (x < rem ? x : rem) + r;
+}
+// the dispatch index of this block and the launch's block count
+__device__ __forceinline__ int dispatch_index() {
+    return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+__device__ __forceinline__ int dispatch_count() { return (int)(gridDim.x * gridDim.y * gridDim.z); }
+// this block's (x, y, z) after the renumbering (x fastest, as dispatched)
+struct BlockXYZ {
+    int x, y, z;
+};
+__device__ __forceinline__ BlockXYZ xcd_blk() {
+    const int v = xcd_block(dispatch_index(), dispatch_count());
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    return BlockXYZ{v % gx, (v / gx) % gy, v / (gx * gy)};
+}
+
 // unbiased binary exponent of a float, zero and subnormals taken as -126
 __device__ __forceinline__ int carry_exp(float x) {
     const int eb = (int)((__float_as_uint(x) >> 23) & 0xffu);

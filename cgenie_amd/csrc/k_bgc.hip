@@ -124,9 +124,10 @@ struct Streams {
 
 template <typename R, int W>
 __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list) {
-    const MemberDev& M = mems[list[blockIdx.y]];
+    const BlockXYZ B = xcd_blk();
+    const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int w = blockIdx.x * 64 + threadIdx.x;
+    const int w = B.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
     const BgcParams& p = M.bp;
     const int col = gp(g.wetsorted)[w];
@@ -418,9 +419,10 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_sums(const MemberDev* __
                                                              const int32_t* __restrict__ list) {
     extern __shared__ double prod[];  // [tg][maxk][CPL_COLS]
     __shared__ CplTile T;
-    const MemberDev& M = mems[list[blockIdx.y]];
+    const BlockXYZ B = xcd_blk();
+    const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int w0 = blockIdx.x * CPL_COLS;
+    const int w0 = B.x * CPL_COLS;
     if (w0 >= g.nwet) return;        // uniform over the block
     if (*gp(M.sfresh)) return;        // the transport step already formed them
     cpl_tile(g, w0, T);
@@ -595,9 +597,10 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
                                                               const int32_t* __restrict__ list) {
     extern __shared__ double prod[];  // [tg][maxk][CPL_COLS]
     __shared__ CplTile T;
-    const MemberDev& M = mems[list[blockIdx.y]];
+    const BlockXYZ B = xcd_blk();
+    const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int w0 = blockIdx.x * CPL_COLS;
+    const int w0 = B.x * CPL_COLS;
     if (w0 >= g.nwet) return;  // uniform over the block
     cpl_tile(g, w0, T);
     const BgcParams& p = M.bp;
@@ -694,7 +697,7 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
         __syncthreads();
     }
     // the state changes below: the next coupling's sums come from the next transport step
-    if (blockIdx.x == 0 && threadIdx.x == 0) *gp(M.sfresh) = 0;
+    if (B.x == 0 && threadIdx.x == 0) *gp(M.sfresh) = 0;
     // T, S, masses and particulates of the thread's cells
 #pragma unroll
     for (int j = 0; j < NJ; j++) {

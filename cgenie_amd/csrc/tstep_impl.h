@@ -127,16 +127,17 @@ template <typename R, int LB, int FLUX_THREADS, int FLUX_NBUF>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
                                                        const int32_t* __restrict__ list, int tw) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const MemberDev& M = mems[list[blockIdx.z]];
+    const BlockXYZ B = xcd_blk();
+    const MemberDev& M = mems[list[B.z]];
     const GridDev& g = M.g;
     const int maxi = g.maxi, maxj = g.maxj, maxk = g.maxk, L = M.L;
     constexpr int V = FLUX_THREADS == 256 ? 1 : 0;
-    const int l0 = blockIdx.y * LB;
-    if ((int)blockIdx.x >= g.nfblk[V] || l0 >= L) return;  // uniform for the block
+    const int l0 = B.y * LB;
+    if (B.x >= g.nfblk[V] || l0 >= L) return;  // uniform for the block
     // every wet column's sums are formed below
-    if (M.bgc && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
+    if (M.bgc && B.x == 0 && B.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
     const int nl = (L - l0) < LB ? (L - l0) : LB;
-    const AS_GLOBAL FluxBlk* blk = gp(g.fblk[V]) + blockIdx.x;
+    const AS_GLOBAL FluxBlk* blk = gp(g.fblk[V]) + B.x;
     const int w0 = blk->w0, t0 = blk->t0, TW = blk->t1 - t0, nw = blk->w1 - w0;
     int* sk1 = reinterpret_cast<int*>(smem);
     R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
@@ -565,9 +566,10 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
 // column of a member, FP64 arithmetic on the new state in place.
 template <typename R>
 __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list) {
-    const MemberDev& M = mems[list[blockIdx.y]];
+    const BlockXYZ B = xcd_blk();
+    const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int w = blockIdx.x * 64 + threadIdx.x;
+    const int w = B.x * 64 + threadIdx.x;
     if (w >= g.nwet) return;
     const int col = g.wetsorted[w];
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
