@@ -24,7 +24,7 @@ void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int pre
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
 void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int maxL, int prec,
-                     size_t lds_cap, hipStream_t);
+                     size_t lds_cap, int phase, hipStream_t);
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 }  // namespace cgo
 
@@ -82,6 +82,8 @@ double carry_unit_h(float x) {
 struct cgo_engine {
     int device = 0;
     hipStream_t s = nullptr;
+    hipStream_t s2 = nullptr;              // BIOGEM columns, concurrent with transport
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     std::vector<GridH*> grids;
     std::vector<MemberH*> members;
@@ -139,8 +141,11 @@ int cgo_create(int32_t device, cgo_engine** out) {
     cgo_engine* e = new cgo_engine();
     e->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&e->d_table, TABLE_CAP * sizeof(MemberDev)) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete e;
         return CGO_E_HIP;
     }
@@ -174,6 +179,9 @@ int cgo_destroy(cgo_engine* e) {
     if (e->d_table) hipFree(e->d_table);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->ev_fork) hipEventDestroy(e->ev_fork);
+    if (e->ev_join) hipEventDestroy(e->ev_join);
+    if (e->s2) hipStreamDestroy(e->s2);
     if (e->s) hipStreamDestroy(e->s);
     delete e;
     return CGO_OK;
@@ -1011,26 +1019,39 @@ static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_list
 }
 
 // step_biogem for the members with BIOGEM state (biogem.f90:528-1877)
-static void enqueue_bio(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+static void enqueue_bio(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
     for (int p = 0; p < 2; p++)
         if (ls.n[2 + p])
-            launch_bio(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
+            launch_bio(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, s);
 }
 
-// biogem_tracercoupling (biogem.f90:1885-2077)
-static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+// biogem_tracercoupling (biogem.f90:1885-2077); phase as launch_coupling
+static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int phase = 0) {
     for (int p = 0; p < 2; p++)
         if (ls.n[2 + p])
             launch_coupling(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],
-                            ls.maxL[p], p, e->lds_cap, e->s);
+                            ls.maxL[p], p, e->lds_cap, phase, e->s);
 }
 
-// whole ocean steps in the genie.f90 order
+// Whole ocean steps in the genie.f90 order.  step_biogem reads only BIOGEM's
+// own arrays, which the previous coupling wrote, and writes none that the
+// transport step or the coupling's sums read: it runs on a second stream,
+// concurrently with tstepo and the global sums, and joins before the rescale.
 static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int nsteps) {
+    const bool bgc = ls.n[2] || ls.n[3];
     for (int it = 0; it < nsteps; it++) {
+        if (bgc) {
+            hipEventRecord(e->ev_fork, e->s);
+            hipStreamWaitEvent(e->s2, e->ev_fork, 0);
+            enqueue_bio(e, ls, d_lists, e->s2);
+            hipEventRecord(e->ev_join, e->s2);
+        }
         enqueue_tstepo(e, ls, d_lists);
-        enqueue_bio(e, ls, d_lists);
-        enqueue_coupling(e, ls, d_lists);
+        if (bgc) {
+            enqueue_coupling(e, ls, d_lists, 1);
+            hipStreamWaitEvent(e->s, e->ev_join, 0);
+            enqueue_coupling(e, ls, d_lists, 2);
+        }
     }
 }
 
@@ -1095,7 +1116,7 @@ int cgo_bio_step(cgo_engine* e, const int32_t* ids, int32_t n) {
     Lists ls;
     int rc = check_not_due(e, ids, n, "bio_step");
     if (rc || (rc = prepare(e, ids, n, ls))) return rc;
-    enqueue_bio(e, ls, e->d_list);
+    enqueue_bio(e, ls, e->d_list, e->s);
     HIPCHK(e, hipGetLastError());
     for (int i = 0; i < n; i++) member(e, ids[i])->coupling_due = true;
     return CGO_OK;
@@ -1221,7 +1242,7 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             } else if (ls.n[2 + p]) {
                 launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
                 launch_coupling(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],
-                                ls.maxL[p], p, e->lds_cap, e->s);
+                                ls.maxL[p], p, e->lds_cap, 0, e->s);
             }
         }
     HIPCHK(e, hipEventRecord(e->ev1, e->s));

@@ -743,24 +743,29 @@ void launch_bio(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
 }
 
 template <typename R>
+// phase 1: column sums (if stale) and the global sums; phase 2: the rescale;
+// phase 0: both
 static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax,
-                              size_t lds_cap, hipStream_t s) {
+                              size_t lds_cap, int phase, hipStream_t s) {
     const dim3 grid((nwetmax + CPL_COLS - 1) / CPL_COLS, nlist);
     // LDS for the largest grid of the batch; every member's tg fits in it
     const size_t lds = (size_t)cpl_tg(maxkmax) * maxkmax * CPL_COLS * 8;
     size_t need = 0;
     for (int k = 1; k <= maxkmax; k++) need = std::max(need, (size_t)cpl_tg(k) * k * CPL_COLS * 8);
     const size_t b = need > lds ? need : lds;
-    if (maxkmax <= 4 * CPL_LG) {
-        hipLaunchKernelGGL((k_couple_sums<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
-    } else if (maxkmax <= 8 * CPL_LG) {
-        hipLaunchKernelGGL((k_couple_sums<R, 8, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
-    } else {
-        hipLaunchKernelGGL((k_couple_sums<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+    if (phase != 2) {
+        if (maxkmax <= 4 * CPL_LG) {
+            hipLaunchKernelGGL((k_couple_sums<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        } else if (maxkmax <= 8 * CPL_LG) {
+            hipLaunchKernelGGL((k_couple_sums<R, 8, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        } else {
+            hipLaunchKernelGGL((k_couple_sums<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        }
+        // the staging chunk takes what LDS the device grants a block, up to RED_LDS_BYTES
+        const int red_lds = (int)std::min<size_t>(RED_LDS_BYTES, lds_cap - 2 * RED_MAXL * sizeof(double));
+        hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), (size_t)red_lds, s, mems, list, red_lds);
     }
-    // the staging chunk takes what LDS the device grants a block, up to RED_LDS_BYTES
-    const int red_lds = (int)std::min<size_t>(RED_LDS_BYTES, lds_cap - 2 * RED_MAXL * sizeof(double));
-    hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), (size_t)red_lds, s, mems, list, red_lds);
+    if (phase == 1) return;
     if (maxkmax <= 4 * CPL_LG) {
         hipLaunchKernelGGL((k_couple_apply<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
     } else if (maxkmax <= 8 * CPL_LG) {
@@ -771,12 +776,12 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
 }
 
 void launch_coupling(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax, int maxL,
-                     int prec, size_t lds_cap, hipStream_t s) {
+                     int prec, size_t lds_cap, int phase, hipStream_t s) {
     (void)maxL;
     if (prec == 0)
-        launch_coupling_t<double>(mems, list, nlist, nwetmax, maxkmax, lds_cap, s);
+        launch_coupling_t<double>(mems, list, nlist, nwetmax, maxkmax, lds_cap, phase, s);
     else
-        launch_coupling_t<float>(mems, list, nlist, nwetmax, maxkmax, lds_cap, s);
+        launch_coupling_t<float>(mems, list, nlist, nwetmax, maxkmax, lds_cap, phase, s);
 }
 
 }  // namespace cgo
