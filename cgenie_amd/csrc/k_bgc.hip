@@ -158,12 +158,19 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         const double Ca = salt_Ca(Ssur), B = salt_B(Ssur), SO4 = salt_SO4(Ssur), F = salt_F(Ssur);
         const double DIC = ocn[(size_t)p.ocn[B_DIC] * ncell + cs], ALK = ocn[(size_t)p.ocn[B_ALK] * ncell + cs];
         const double PO4 = hasPO4 ? ocn[(size_t)p.ocn[B_PO4] * ncell + cs] : 0.0;
-        const int it = calc_carb(DIC, ALK, Ca, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
+        // FP64 members: the reference's plain iteration; FP32-form members:
+        // the same fixed point, accelerated
+        constexpr bool fast = sizeof(R) == 4;
+        const int it = fast ? calc_carb_fast(DIC, ALK, Ca, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit)
+                            : calc_carb(DIC, ALK, Ca, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
         if (it < 0)
             atomicAdd(&M.diag[1], 1);
         else
             atomicAdd(&M.diag[2], it);
-        calc_carb_RF0(DIC, ALK, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
+        if (fast)
+            calc_carb_RF0_fast(DIC, ALK, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
+        else
+            calc_carb_RF0(DIC, ALK, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
         ohm = carb[IC_OHM_CAL * plane];
     }
 

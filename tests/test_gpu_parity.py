@@ -250,6 +250,34 @@ def test_fp32_form_tracks_fp64(eng, carry):
     eng.remove_member(m64)
 
 
+def test_fp32_form_carbonate_solve_reaches_the_same_root(eng):
+    """FP32-form members solve the surface carbonate chemistry with the
+    accelerated fixed point: same root as the reference iteration of the FP64
+    member (to the solver tolerance; the inputs differ by the FP32 form's
+    ~1e-12 rounding of the state), in fewer sweeps."""
+    c = make_case(kind="config2", seed=65)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    m64 = new_member(eng, c)
+    m32 = new_member(eng, c, prec=E.FP32C)
+    for m in (m64, m32):
+        eng.bgc_init(m, p, c.light, c.mld)
+    for _ in range(3):
+        eng.bio_step([m64, m32])
+        eng.tracercoupling([m64, m32])
+    shape = (c.g.maxj, c.g.maxi, 10)
+    a, b = eng.bgc_download(m32, 2, shape), eng.bgc_download(m64, 2, shape)
+    wc = G.wet_mask(c.g, c.k1)[-1]
+    sl = [0, 1, 2, 3, 4, 5, 6]  # H, fCO2, CO2, CO3, HCO3, ohm calcite, ohm aragonite
+    e = np.abs(a[wc][:, sl] - b[wc][:, sl]) / np.abs(b[wc][:, sl])
+    d32, d64 = eng.diag(m32), eng.diag(m64)
+    say("carbonate FP32-form vs FP64 max rel", e.max(axis=0), "sweeps", d32.carb_iters, d64.carb_iters)
+    assert np.all(e <= 1e-9)
+    assert d32.carb_fail == 0 and d64.carb_fail == 0
+    assert d32.carb_iters < d64.carb_iters
+    eng.remove_member(m32)
+    eng.remove_member(m64)
+
+
 def test_split_calls_equal_step(eng):
     """tstepo + bio_step + tracercoupling, called one by one in the genie.f90
     order, give the bits of cgo_step; the coupling is refused out of order."""
