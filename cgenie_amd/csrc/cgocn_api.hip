@@ -867,6 +867,7 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
         ml[c] = mld[c];
     }
     scal[SC_SMEAN] = Smean;
+    scal[SC_PSCALE] = 1.0;
     // remineralisation tables, evaluated as sub_box_remin_part does per level
     const int KS = mk + 2;
     const double* Dbot = &g->mk[MK_DBOT * KS];
@@ -946,10 +947,15 @@ int cgo_bgc_download(cgo_engine* e, int32_t id, int32_t what, double* out) {
                 if (P.slot[b] >= 0) out[(size_t)P.slot[b] + (size_t)L * c] = h[(size_t)(anom ? b : P.ocn[b]) * ncell + c];
         }
     } else if (what == 1) {
+        // after a coupling the particulates carry its salinity ratio, not yet
+        // applied on the device (the next column step applies it when it reads them)
         std::vector<double> h((size_t)NSED * ncell);
         HIPCHK(e, hipMemcpy(h.data(), m->dev.part, h.size() * 8, hipMemcpyDeviceToHost));
+        double ps = 1.0;
+        if (!m->coupling_due) HIPCHK(e, hipMemcpy(&ps, m->dev.scal + SC_PSCALE, 8, hipMemcpyDeviceToHost));
         for (size_t c = 0; c < ncell; c++)
-            for (int s = 0; s < NSED; s++) out[s + (size_t)NSED * c] = h[(size_t)s * ncell + c];
+            for (int s = 0; s < NSED; s++)
+                out[s + (size_t)NSED * c] = m->coupling_due ? h[(size_t)s * ncell + c] : ps * (h[(size_t)s * ncell + c] + 0.0);
     } else if (what == 2) {
         std::vector<double> h((size_t)NCARB * ncol);
         HIPCHK(e, hipMemcpy(h.data(), m->dev.carb, h.size() * 8, hipMemcpyDeviceToHost));

@@ -116,8 +116,11 @@ enum { RT_LAYER = 0, RT_POC1, RT_POC2, RT_CAL1, RT_CAL2, RT_N };
 
 // member scalar slots (scal[])
 // SC_SMEAN is the mean salinity the next coupling starts from; the coupling's
-// reduction moves it to SC_SOLD and stores the new mean in both SC_SNEW and SC_SMEAN
-enum { SC_SMEAN = 0, SC_SNEW, SC_SRATIO, SC_RSRATIO, SC_SOLD, SC_FACT0 };  // SC_FACT0 + l : OLD_l * rNEW_l
+// reduction moves it to SC_SOLD and stores the new mean in both SC_SNEW and SC_SMEAN.
+// SC_PSCALE: the coupling's salinity ratio for the particulates, applied when
+// they are next read (the BIOGEM column step rewrites them all before the
+// following coupling), 1 after initialisation.
+enum { SC_SMEAN = 0, SC_SNEW, SC_SRATIO, SC_RSRATIO, SC_SOLD, SC_PSCALE, SC_FACT0 };  // SC_FACT0 + l : OLD_l * rNEW_l
 
 // ---------------------------------------------------------------- helpers
 // Pointers read from the member table are generic (flat) to the compiler;

@@ -225,6 +225,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
 
     // (3) the march, surface to k1
     AS_GLOBAL double* __restrict__ part = gp(M.part);
+    const double pscale = gp(M.scal)[SC_PSCALE];  // the last coupling's ratio, pending on part
     AS_GLOBAL double* __restrict__ vd = gp(M.vd);
     const AS_GLOBAL double* __restrict__ pO2 = hasO2 ? ocn + (size_t)p.ocn[B_O2] * ncell : nullptr;
     const AS_GLOBAL double* __restrict__ pDC = hasDOMC ? ocn + (size_t)p.ocn[B_DOMC] * ncell : nullptr;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         v.domc = hasDOMC ? pDC[c] : 0.0;
         v.domp = (hasDOMC && hasDOMP) ? pDP[c] : 0.0;
 #pragma unroll
-        for (int s = 0; s < NSED; s++) v.p[s] = k >= klim ? part[(size_t)s * ncell + c] : 0.0;
+        for (int s = 0; s < NSED; s++) v.p[s] = k >= klim ? pscale * (part[(size_t)s * ncell + c] + 0.0) : 0.0;
     };
     // settled flux at k1 (the kk = k1 term of the settle loop)
     auto settle_add = [&](const double* t) {
@@ -704,7 +705,11 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
         __syncthreads();
     }
     // the state changes below: the next coupling's sums come from the next transport step
-    if (B.x == 0 && threadIdx.x == 0) *gp(M.sfresh) = 0;
+    if (B.x == 0 && threadIdx.x == 0) {
+        *gp(M.sfresh) = 0;
+        // the particulates' rescale (biogem.f90:2042-2043) is left to their next reader
+        gp(M.scal)[SC_PSCALE] = Sratio;
+    }
     // T, S, masses and particulates of the thread's cells
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
@@ -720,12 +725,8 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
             StateIO<R>::st(st, lo, cc, lvT - ZEROC);
             StateIO<R>::st(st, lo, tstride + cc, lvS - SALN0);
             gp(M.M)[cc] = mn[j];
-            gp(M.rM)[cc] = Sratio * gp(M.rM)[cc];
-#pragma unroll
-            for (int s2 = 0; s2 < NSED; s2++) {
-                AS_GLOBAL double* pp = &gp(M.part)[(size_t)s2 * ncell + cc];
-                *pp = Sratio * (*pp + 0.0);
-            }
+            // 1/M is read only at k1 (the sediment return)
+            if (k == kb) gp(M.rM)[cc] = Sratio * gp(M.rM)[cc];
         }
     }
 }
