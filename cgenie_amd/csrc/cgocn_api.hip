@@ -1054,7 +1054,7 @@ static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists
         }
         enqueue_tstepo(e, ls, d_lists);
         if (bgc) {
-            enqueue_coupling(e, ls, d_lists, 1);
+            enqueue_coupling(e, ls, d_lists, 3);  // the sums are fresh: tstepo formed them
             hipStreamWaitEvent(e->s, e->ev_join, 0);
             enqueue_coupling(e, ls, d_lists, 2);
         }

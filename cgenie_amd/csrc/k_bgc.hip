@@ -752,7 +752,7 @@ void launch_bio(const MemberDev* mems, const int32_t* list, int nlist, int nwetm
 
 template <typename R>
 // phase 1: column sums (if stale) and the global sums; phase 2: the rescale;
-// phase 0: both
+// phase 0: both; phase 3: the global sums alone (right after a transport step)
 static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nlist, int nwetmax, int maxkmax,
                               size_t lds_cap, int phase, hipStream_t s) {
     const dim3 grid((nwetmax + CPL_COLS - 1) / CPL_COLS, nlist);
@@ -762,7 +762,9 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
     for (int k = 1; k <= maxkmax; k++) need = std::max(need, (size_t)cpl_tg(k) * k * CPL_COLS * 8);
     const size_t b = need > lds ? need : lds;
     if (phase != 2) {
-        if (maxkmax <= 4 * CPL_LG) {
+        if (phase == 3) {
+            // the transport step just formed the column sums
+        } else if (maxkmax <= 4 * CPL_LG) {
             hipLaunchKernelGGL((k_couple_sums<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
         } else if (maxkmax <= 8 * CPL_LG) {
             hipLaunchKernelGGL((k_couple_sums<R, 8, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
@@ -773,7 +775,7 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
         const int red_lds = (int)std::min<size_t>(RED_LDS_BYTES, lds_cap - 2 * RED_MAXL * sizeof(double));
         hipLaunchKernelGGL(k_couple_reduce, dim3(nlist), dim3(RED_THREADS), (size_t)red_lds, s, mems, list, red_lds);
     }
-    if (phase == 1) return;
+    if (phase == 1 || phase == 3) return;
     if (maxkmax <= 4 * CPL_LG) {
         hipLaunchKernelGGL((k_couple_apply<R, 4, 4>), grid, dim3(CPL_THREADS), b, s, mems, list);
     } else if (maxkmax <= 8 * CPL_LG) {
