@@ -1224,7 +1224,7 @@ int cgo_set_flux_ring(cgo_engine* e, int32_t precision, int32_t planes) {
 }
 
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
-    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 2) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
+    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 3) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
@@ -1232,7 +1232,31 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
     if ((rc = upload_lists(e, ls))) return rc;
     HIPCHK(e, hipStreamSynchronize(e->s));
     HIPCHK(e, hipEventRecord(e->ev0, e->s));
-    for (int r = 0; r < reps; r++)
+    if (which == 3) {
+        // the flux as it runs inside a whole step: the BIOGEM column step on
+        // the second stream beside it
+        const bool bgc = ls.n[2] || ls.n[3];
+        for (int r = 0; r < reps; r++) {
+            if (bgc) {
+                hipEventRecord(e->ev_fork, e->s);
+                hipStreamWaitEvent(e->s2, e->ev_fork, 0);
+                enqueue_bio(e, ls, e->d_list, e->s2);
+                hipEventRecord(e->ev_join, e->s2);
+            }
+            for (int p = 0; p < 2; p++) {
+                if (!ls.n[p]) continue;
+                const int32_t* l = e->d_list + ls.off[p];
+                if (p == 0)
+                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0],
+                                  e->lds_cap, e->s);
+                else
+                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1],
+                                  e->lds_cap, e->s);
+            }
+            if (bgc) hipStreamWaitEvent(e->s, e->ev_join, 0);
+        }
+    }
+    for (int r = 0; r < (which == 3 ? 0 : reps); r++)
         for (int p = 0; p < 2; p++) {
             if (!ls.n[p]) continue;
             const int32_t* l = e->d_list + ls.off[p];

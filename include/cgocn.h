@@ -150,8 +150,10 @@ int cgo_set_tile(cgo_engine *e, int32_t precision, int32_t tracer_block);   /* 2
 int cgo_set_flux_cols(cgo_engine *e, int32_t precision, int32_t cols);      /* 128 or 256 wet columns per block */
 int cgo_set_flux_ring(cgo_engine *e, int32_t precision, int32_t planes);    /* 2 or 3 level planes in LDS */
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
- * 1 = co, 2 = bgc column + coupling) over the listed members, from HIP events
- * on the engine stream; the state is advanced by the timed launches */
+ * 1 = co, 2 = bgc column + coupling, 3 = flux with the BIOGEM column step
+ * running beside it on the second stream, as inside a whole step) over the
+ * listed members, from HIP events on the engine stream; the state is
+ * advanced by the timed launches */
 int cgo_time_kernel(cgo_engine *e, int32_t which, const int32_t *members, int32_t n, int32_t reps, double *avg_ms);
 
 #ifdef __cplusplus
