@@ -97,7 +97,7 @@ def run_probe(args):
     prec = E.FP32C if args.probe == "fp32c" else E.FP64
     eng = E.Engine(0)
     ids, tset, n_wet, g = build_ensemble(eng, E, args.members, prec)
-    eng.set_tile(prec, args.probe_tb, args.probe_cols, args.probe_ring)
+    eng.set_tile(prec, args.probe_tb, args.probe_cols, args.probe_ring, args.probe_iso)
     eng.step(ids, args.steps)
     eng.sync()
     print(f"# probe: {args.members} members x {args.steps} steps done", file=sys.stderr, flush=True)
@@ -161,12 +161,13 @@ def run_gpu(args):
                         flux_bytes=B, flux_gbs=B / (flux_step_ms * 1e-3) / 1e9,
                         flux_alone_gbs=B / (flux_ms * 1e-3) / 1e9, carb_fail=int(d.carb_fail), n_wet=n_wet, L=tset.L,
                         tracer_block=tuned[tag]["tracer_block"], flux_cols=tuned[tag]["cols"],
-                        flux_ring=tuned[tag]["ring"])
+                        flux_ring=tuned[tag]["ring"], flux_iso_prepass=tuned[tag]["iso_prepass"])
         eng.drop_graph(graph)
         for m in ids:
             eng.remove_member(m)
     # plain FP64 single member, eager launches (the reference-shaped path)
-    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"], tuned["fp64"]["ring"])
+    eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"], tuned["fp64"]["ring"],
+                 tuned["fp64"]["iso_prepass"])
     ids, tset, n_wet, g = build_ensemble(eng, E, 1, E.FP64)
     eng.step(ids, STEPS_PER_YEAR)
     eng.sync()
@@ -251,6 +252,7 @@ def main():
     ap.add_argument("--probe-tb", type=int, default=4)
     ap.add_argument("--probe-cols", type=int, default=256)
     ap.add_argument("--probe-ring", type=int, default=3)
+    ap.add_argument("--probe-iso", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -269,8 +271,10 @@ def main():
     # it on the second stream); the same launch alone is reported next to it
     roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4),
-                traffic=measured_traffic(f'{main_["tracer_block"]}x{main_["flux_cols"]}x{main_["flux_ring"]}'),
-                kernel=f'k_flux<float, {main_["tracer_block"]}, {main_["flux_cols"]}, {main_["flux_ring"]}>',
+                traffic=measured_traffic(f'{main_["tracer_block"]}x{main_["flux_cols"]}x{main_["flux_ring"]}'
+                                         f'x{main_["flux_iso_prepass"]}'),
+                kernel=f'k_flux<float, {main_["tracer_block"]}, {main_["flux_cols"]}, {main_["flux_ring"]}, '
+                       f'{2 if main_["flux_iso_prepass"] else 0}>',
                 algorithmic_bytes=main_["flux_bytes"], launch_ms=round(main_["flux_step_ms"], 4),
                 alone_ms=round(main_["flux_ms"], 4), alone_achieved=round(main_["flux_alone_gbs"], 1))
     line = dict(
@@ -288,10 +292,11 @@ def main():
         kernels_ms=dict(fp32c=dict(flux=main_["flux_ms"], flux_in_step=main_["flux_step_ms"], co=main_["co_ms"],
                                    bgc_coupling=main_["bgc_coupling_ms"],
                                    tracer_block=main_["tracer_block"], flux_cols=main_["flux_cols"],
-                                   flux_ring=main_["flux_ring"]),
+                                   flux_ring=main_["flux_ring"], flux_iso_prepass=main_["flux_iso_prepass"]),
                         fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
                                   bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"],
-                                  flux_cols=r["fp64"]["flux_cols"], flux_ring=r["fp64"]["flux_ring"])),
+                                  flux_cols=r["fp64"]["flux_cols"], flux_ring=r["fp64"]["flux_ring"],
+                                  flux_iso_prepass=r["fp64"]["flux_iso_prepass"])),
         roofline=roof, cpu_baseline=cpu, serving=serving)
     print(json.dumps(line), flush=True)
 

@@ -2,7 +2,8 @@
 
 For a batch of members (grid, tracer count, precision, member count) the
 autotuner times every compiled tile shape of the flux kernel -- tracers per
-block, wet columns per block, level planes resident in LDS -- with HIP
+block, wet columns per block, level planes resident in LDS, isoneutral
+coefficients computed in every tracer block or once by a pre-pass -- with HIP
 events on the engine's stream (cgo_time_kernel) and keeps the fastest.  The
 flux kernel only reads the current state and writes the other ping-pong
 buffer, so timing it repeatedly leaves the model state unchanged.
@@ -22,6 +23,7 @@ CACHE = os.path.join(HERE, "tuning.json")
 TRACER_BLOCKS = (2, 4, 6, 8)
 FLUX_COLS = (128, 256)
 FLUX_RINGS = (2, 3)
+ISO_PREPASS = (0, 1)
 
 
 def _key(grid_dims, L, precision, n_members) -> str:
@@ -50,25 +52,27 @@ def tune(eng: E.Engine, members, grid_dims, L: int, precision: int, reps: int = 
          use_cache: bool = True, write_cache: bool = True) -> dict:
     """Pick the flux kernel's tile shape for this batch.
 
-    Returns {"tracer_block": tb, "cols": cols, "ring": planes,
-    "times_ms": {"tbxcolsxplanes": ms}} and applies the choice to the engine."""
+    Returns {"tracer_block": tb, "cols": cols, "ring": planes, "iso_prepass": 0|1,
+    "times_ms": {"tbxcolsxplanesxprepass": ms}} and applies the choice to the engine."""
     key = _key(grid_dims, L, precision, len(members))
     cache = load_cache() if use_cache else {}
     if key in cache:
         c = cache[key]
-        eng.set_tile(precision, int(c["tracer_block"]), int(c.get("cols", 128)), int(c.get("ring", 3)))
+        eng.set_tile(precision, int(c["tracer_block"]), int(c.get("cols", 128)), int(c.get("ring", 3)),
+                     int(c.get("iso_prepass", 0)))
         return c
     times = {}
-    for ring in FLUX_RINGS:
-        for cols in FLUX_COLS:
-            for tb in TRACER_BLOCKS:
-                eng.set_tile(precision, tb, cols, ring)
-                eng.time_kernel(0, members, 2)  # warm
-                times[(tb, cols, ring)] = eng.time_kernel(0, members, reps)
+    for pre in ISO_PREPASS:
+        for ring in FLUX_RINGS:
+            for cols in FLUX_COLS:
+                for tb in TRACER_BLOCKS:
+                    eng.set_tile(precision, tb, cols, ring, pre)
+                    eng.time_kernel(0, members, 2)  # warm
+                    times[(tb, cols, ring, pre)] = eng.time_kernel(0, members, reps)
     best = min(times, key=times.get)
     eng.set_tile(precision, *best)
-    res = {"tracer_block": best[0], "cols": best[1], "ring": best[2],
-           "times_ms": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in times.items()}}
+    res = {"tracer_block": best[0], "cols": best[1], "ring": best[2], "iso_prepass": best[3],
+           "times_ms": {f"{k[0]}x{k[1]}x{k[2]}x{k[3]}": v for k, v in times.items()}}
     if write_cache:
         cache = load_cache()
         cache[key] = res

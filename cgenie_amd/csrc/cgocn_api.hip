@@ -17,9 +17,9 @@
 
 namespace cgo {
 void launch_flux64(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
-                   size_t lds_cap, hipStream_t);
+                   int iso_split, size_t lds_cap, hipStream_t);
 void launch_flux32(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
-                   size_t lds_cap, hipStream_t);
+                   int iso_split, size_t lds_cap, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
@@ -92,6 +92,7 @@ struct cgo_engine {
     int lb[2] = {4, 4};       // flux tracer block per precision
     int nt[2] = {128, 128};   // flux columns per block per precision
     int nbuf[2] = {3, 3};     // flux level planes resident in LDS per precision
+    int isosplit[2] = {0, 0}; // isoneutral coefficients from a pre-pass, per precision
     size_t lds_cap = 65536;   // LDS one workgroup may take
     std::vector<Graph> graphs;
     int32_t* d_list = nullptr;  // scratch lists for eager steps
@@ -370,7 +371,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
 
 // byte layout of one member allocation
 struct MemLayout {
-    size_t st[2], lo[2], u, sflux, rho, cost, ctrl, ocn, part, settle, carb, M, rM, vd, light, mld, colsum, colold,
+    size_t st[2], lo[2], u, sflux, rho, cost, ctrl, isoc, ocn, part, settle, carb, M, rM, vd, light, mld, colsum, colold,
         scal, rtab, mink, total;
 };
 
@@ -394,6 +395,7 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
     m.rho = take(ncell * 8);
     m.cost = take(ncol * 8);
     m.ctrl = take(256);  // parity, steps, diag[4], dmax, sfresh
+    m.isoc = take((size_t)10 * ncell * 8);
     if (bgc) {
         m.ocn = take((2 + NBGC) * ncell * 8);
         m.part = take(NSED * ncell * 8);
@@ -458,6 +460,7 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
     d.sflux = nullptr;
     d.rho = reinterpret_cast<double*>(b + ly.rho);
     d.cost = reinterpret_cast<double*>(b + ly.cost);
+    d.isoc = reinterpret_cast<double*>(b + ly.isoc);
     d.parity = reinterpret_cast<int32_t*>(b + ly.ctrl);
     d.steps = reinterpret_cast<int64_t*>(b + ly.ctrl + 8);
     d.diag = reinterpret_cast<int32_t*>(b + ly.ctrl + 16);
@@ -1014,10 +1017,10 @@ static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_list
         if (!ls.n[p]) continue;
         const int32_t* l = d_lists + ls.off[p];
         if (p == 0)
-            launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->lds_cap,
+            launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0], e->lds_cap,
                           e->s);
         else
-            launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->lds_cap,
+            launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], e->lds_cap,
                           e->s);
         launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
     }
@@ -1223,6 +1226,13 @@ int cgo_set_flux_ring(cgo_engine* e, int32_t precision, int32_t planes) {
     return CGO_OK;
 }
 
+int cgo_set_flux_iso(cgo_engine* e, int32_t precision, int32_t prepass) {
+    if (!e || (precision != CGO_FP64 && precision != CGO_FP32C) || (prepass != 0 && prepass != 1))
+        return fail(e, CGO_E_ARG, "set_flux_iso: precision %d prepass %d", precision, prepass);
+    e->isosplit[precision == CGO_FP64 ? 0 : 1] = prepass;
+    return CGO_OK;
+}
+
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
     if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 3) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
@@ -1248,10 +1258,10 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
                 const int32_t* l = e->d_list + ls.off[p];
                 if (p == 0)
                     launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0],
-                                  e->lds_cap, e->s);
+                                  e->isosplit[0], e->lds_cap, e->s);
                 else
                     launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1],
-                                  e->lds_cap, e->s);
+                                  e->isosplit[1], e->lds_cap, e->s);
             }
             if (bgc) hipStreamWaitEvent(e->s, e->ev_join, 0);
         }
@@ -1262,10 +1272,10 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             const int32_t* l = e->d_list + ls.off[p];
             if (which == 0) {
                 if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->lds_cap,
+                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0], e->lds_cap,
                           e->s);
                 else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->lds_cap,
+                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], e->lds_cap,
                           e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);

@@ -86,6 +86,8 @@ struct MemberDev {
     const void* u;          // velocity (double, or float in the FP32 form)
     const double* sflux;    // [L][ncol] or null
     double* rho;            // [ncell]
+    double* isoc;           // isoneutral coefficients of the flux pre-pass: [10][ncell] doubles
+                            // (dzrho or +1, dxr[4], dyr[4], slim) or [9][ncell] floats (ax[4], ay[4], qq)
     double* cost;           // [ncol]
     int32_t* parity;        // device scalar: which st[] is current
     int64_t* steps;         // device scalar: completed steps

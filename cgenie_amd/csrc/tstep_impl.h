@@ -111,9 +111,11 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
 // blocks so that t1 - t0 <= flux_pf(FLUX_THREADS) * FLUX_THREADS.
 // The FP32 form also stages the carry of every value, as a float residual,
 // so differences of neighbours are taken at the stored (~40-bit) precision.
-__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int lb, int es, int nt, int nbuf) {
-    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)nbuf * (lb + 2) * tw * es + (size_t)lb * nt * es +
-           (es == 4 ? (size_t)nbuf * (lb + 2) * tw * 4 : 0);
+// ns level-plane slots (lb tracers, plus T and S when the block computes the
+// isoneutral coefficients itself), lb carried bottom fluxes
+__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int ns, int lb, int es, int nt, int nbuf) {
+    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)nbuf * ns * tw * es + (size_t)lb * nt * es +
+           (es == 4 ? (size_t)nbuf * ns * tw * 4 : 0);
 }
 
 // the carry c of a float h as a float: c * 2^-16 ulp(h) (exact unless |h| < 2^-87)
@@ -123,9 +125,15 @@ __device__ __forceinline__ float carry_residual(float h, int16_t c) {
     return ((float)c * 0x1p-39f) * __uint_as_float(__float_as_uint(h) & 0x7f800000u);
 }
 
-template <typename R, int LB, int FLUX_THREADS, int FLUX_NBUF>
+// IM selects where the isoneutral coefficients come from: 0 computed in every
+// tracer block from T and S staged beside its tracers; 1 the pre-pass (one
+// block per column block, T and S only, LB = 2) that computes them once per
+// cell into the member's isoc planes; 2 the tracer blocks after that pre-pass,
+// which load them and stage only their own tracers.
+template <typename R, int LB, int FLUX_THREADS, int FLUX_NBUF, int IM>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
                                                        const int32_t* __restrict__ list, int tw) {
+    constexpr int NS = IM == 0 ? LB + 2 : LB;  // slots per staged level plane
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockXYZ B = xcd_blk();
     const MemberDev& M = mems[list[B.z]];
@@ -135,17 +143,18 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const int l0 = B.y * LB;
     if (B.x >= g.nfblk[V] || l0 >= L) return;  // uniform for the block
     // every wet column's sums are formed below
-    if (M.bgc && B.x == 0 && B.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
+    if (IM != 1 && M.bgc && B.x == 0 && B.y == 0 && threadIdx.x == 0) *gp(M.sfresh) = 1;
     const int nl = (L - l0) < LB ? (L - l0) : LB;
     const AS_GLOBAL FluxBlk* blk = gp(g.fblk[V]) + B.x;
     const int w0 = blk->w0, t0 = blk->t0, TW = blk->t1 - t0, nw = blk->w1 - w0;
     int* sk1 = reinterpret_cast<int*>(smem);
     R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
-    R* sfb = sp + FLUX_NBUF * (LB + 2) * tw;
+    R* sfb = sp + FLUX_NBUF * NS * tw;
     float* sr = reinterpret_cast<float*>(sfb + LB * FLUX_THREADS);  // FP32 form: residual ring
-    // slot q < LB: tracer l0+q; T and S sit in slots 0/1 when l0 == 0, else LB/LB+1
-    const int qT = l0 == 0 ? 0 : LB, qS = l0 == 0 ? 1 : LB + 1;
-    const int nslot = l0 == 0 ? nl : LB + 2;  // slots staged per level
+    // slot q < LB: tracer l0+q; T and S sit in slots 0/1 when l0 == 0, else
+    // (coefficients computed here) LB/LB+1
+    const int qT = (l0 == 0 || IM != 0) ? 0 : LB, qS = (l0 == 0 || IM != 0) ? 1 : LB + 1;
+    const int nslot = (l0 == 0 || IM != 0) ? nl : LB + 2;  // slots staged per level
     __shared__ int s_kmin;
     const int tid = threadIdx.x;
     if (tid == 0) s_kmin = maxk + 1;
@@ -199,8 +208,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
 
-    auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * (LB + 2) + q) * tw + pp]; };
-    auto SR = [&](int buf, int q, int pp) -> float& { return sr[(buf * (LB + 2) + q) * tw + pp]; };
+    auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * NS + q) * tw + pp]; };
+    auto SR = [&](int buf, int q, int pp) -> float& { return sr[(buf * NS + q) * tw + pp]; };
     // difference of two staged values; the FP32 form adds the residuals
     auto D = [&](int buf, int q, int pa, int pb) -> R {
         if (ANOM) return (SP(buf, q, pa) - SP(buf, q, pb)) + (R)(SR(buf, q, pa) - SR(buf, q, pb));
@@ -215,8 +224,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     auto slot_tracer = [&](int q) { return q < LB ? l0 + q : q - LB; };
     // prefetch of one plane into registers, then into the LDS ring
     constexpr int FLUX_PF = flux_pf(FLUX_THREADS);
-    R pf[LB + 2][FLUX_PF];
-    int16_t pc[ANOM ? LB + 2 : 1][FLUX_PF];
+    R pf[NS][FLUX_PF];
+    int16_t pc[ANOM ? NS : 1][FLUX_PF];
     auto fetch = [&](int kz) {
         const int o = kz * plane + t0;
 #pragma unroll
@@ -224,8 +233,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const int pp = tid + r * FLUX_THREADS;
             const bool ok = pp < TW && kz + 1 >= sk1[pp < TW ? pp : 0];
 #pragma unroll
-            for (int q = 0; q < LB + 2; q++)
-                if (q < nslot || q >= LB) {
+            for (int q = 0; q < NS; q++)
+                if (q < nslot || (IM == 0 && q >= LB)) {
                     const bool ld = ok && (q < nslot);
                     const uint32_t i = (uint32_t)(slot_tracer(q) * tstride + o + pp);
                     pf[q][r] = ld ? at(src, i) : (R)0;
@@ -239,7 +248,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const int pp = tid + r * FLUX_THREADS;
             if (pp < TW) {
 #pragma unroll
-                for (int q = 0; q < LB + 2; q++)
+                for (int q = 0; q < NS; q++)
                     if (q < nslot) {
                         SP(buf, q, pp) = pf[q][r];
                         if (ANOM) SR(buf, q, pp) = carry_residual((float)pf[q][r], pc[ANOM ? q : 0][r]);
@@ -251,10 +260,71 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     for (int q = 0; q < LB; q++) FB(q) = (R)0;
     int32_t limps = 0;
     R dmaxv = (R)0;
+    // the isoneutral slope terms of a cell at level k < maxk from the staged
+    // T and S (2489-2501, 2561-2605): dzrho, the four corner gradients, the
+    // slope limiter; counts limps and the diffusive stability number
+    auto iso_coeffs = [&](int k, int cb, int ub, R rdza, R dt, bool& iso, R& dzrho, R& rdzrho, R& slim, R* dxr,
+                          R* dyr) {
+        const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p);
+        const R tatw = (R)0.5 * (Tc + Tu);
+        const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
+        dzrho = (e1 * DK(ub, cb, qS, p) - tec * DK(ub, cb, qT, p)) * rdza;
+        if (M.diso && dzrho < (R)-1.0E-12) {
+            iso = true;
+            rdzrho = fdiv((R)1.0, dzrho);
+            R tv1 = 0;
+            // ina order (knp, nnp) = (0,0), (0,1), (1,0), (1,1)
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const int b = n < 2 ? cb : ub;
+                const int kn = n < 2 ? k : k + 1;
+                const bool east = n & 1;
+                const bool mx = kn >= (east ? k1e : k1w), my = kn >= (east ? k1n : k1s);
+                const R dxT = mx ? (east ? D(b, qT, pe, p) : D(b, qT, p, pw)) * rc * rdphi : (R)0;
+                const R dxS = mx ? (east ? D(b, qS, pe, p) : D(b, qS, p, pw)) * rc * rdphi : (R)0;
+                const R dyT = my ? (east ? D(b, qT, pn, p) * cvn * rdsvn : D(b, qT, p, ps) * cvs * rdsvs) : (R)0;
+                const R dyS = my ? (east ? D(b, qS, pn, p) * cvn * rdsvn : D(b, qS, p, ps) * cvs * rdsvs) : (R)0;
+                dxr[n] = scc * dxS - tec * dxT;
+                dyr[n] = scc * dyS - tec * dyT;
+                tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
+            }
+            tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
+            const R ssmax = mk[MK_SSMAX * KS + k];
+            if (tv1 > ssmax) {
+                slim = fdiv(ssmax * ssmax, tv1 * tv1);
+                limps++;
+            } else {
+                slim = 1;
+            }
+            tv1 = tv1 * slim * diff1 * rdza;
+            const R tvd = tv1 * dt * rdza;
+            if (tvd > dmaxv) dmaxv = tvd;
+        }
+    };
+    // the FP32 form's isoneutral coefficients: corner differences times ax, ay
+    // (the sign of the west/south corners folded in) and the vertical qq
+    auto fp32_iso = [&](bool iso, R slim, R rdzrho, R rdza, const R* dxr, const R* dyr, R* ax, R* ay, R& qq) {
+#pragma unroll
+        for (int n = 0; n < 4; n++) ax[n] = ay[n] = (R)0;
+        qq = (R)0;
+        if (iso) {
+            const R isoc = (R)0.25 * slim * diff1;
+            const R f = (R)2 * isoc * rdzrho;
+            R s2 = 0;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const bool east = n & 1;
+                ax[n] = f * dxr[n] * rc * rdphi * (east ? (R)1 : (R)-1);
+                ay[n] = f * dyr[n] * (east ? cvn * rdsvn : -(cvs * rdsvs));
+                s2 = s2 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
+            }
+            qq = isoc * s2 * (rdzrho * rdzrho) * rdza;
+        }
+    };
     // Members with BIOGEM: the coupling's column sums of the new state
     // (biogem.f90:1984-1997, 2016-2025), accumulated up the column in the
     // order the coupling adds them; co redoes the columns it mixes.
-    const bool sums = M.bgc != 0;
+    const bool sums = IM != 1 && M.bgc != 0;
     const double rSold = sums ? 1.0 / gp(M.scal)[SC_SMEAN] : 0.0;
     double accs[LB], accS = 0.0;
 #pragma unroll
@@ -281,7 +351,36 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         const bool top = (k == maxk);
         const bool more = k + 2 <= maxk;
         if (more) fetch(kz + 2);
-        if (active && k >= k1c) {
+        if (IM == 1 && active && k >= k1c && !top && M.diso) {
+            // the pre-pass: the cell's isoneutral coefficients, once
+            const uint32_t oc = (uint32_t)(kz * plane + col);
+            const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k], dt = mk[MK_DT * KS + k];
+            (void)dza;
+            bool iso = false;
+            R dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            iso_coeffs(k, cb, ub, rdza, dt, iso, dzrho, rdzrho, slim, dxr, dyr);
+            if constexpr (ANOM) {
+                R ax[4], ay[4], qq;
+                fp32_iso(iso, slim, rdzrho, rdza, dxr, dyr, ax, ay, qq);
+                AS_GLOBAL float* ic = reinterpret_cast<AS_GLOBAL float*>(gp(M.isoc));
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    at(ic, (uint32_t)(n * ncell) + oc) = ax[n];
+                    at(ic, (uint32_t)((4 + n) * ncell) + oc) = ay[n];
+                }
+                at(ic, (uint32_t)(8 * ncell) + oc) = qq;
+            } else {
+                AS_GLOBAL double* ic = gp(M.isoc);
+                at(ic, oc) = iso ? (double)dzrho : 1.0;  // +1: no isoneutral flux in this cell
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    at(ic, (uint32_t)((1 + n) * ncell) + oc) = dxr[n];
+                    at(ic, (uint32_t)((5 + n) * ncell) + oc) = dyr[n];
+                }
+                at(ic, (uint32_t)(9 * ncell) + oc) = slim;
+            }
+        }
+        if (IM != 1 && active && k >= k1c) {
             const int o = kz * plane;
             // face velocities and Peclet-weighted upwinding (goldstein.f90:2489-2523)
             const uint32_t oc = (uint32_t)(o + col);
@@ -312,41 +411,37 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             // isoneutral coefficients from T and S (2489-2501, 2561-2605)
             bool iso = false;
             R dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            R ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, qq = 0;  // the FP32 form's coefficients
             if (!top) {
-                const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p);
-                const R tatw = (R)0.5 * (Tc + Tu);
-                const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
-                dzrho = (e1 * DK(ub, cb, qS, p) - tec * DK(ub, cb, qT, p)) * rdza;
-                if (M.diso && dzrho < (R)-1.0E-12) {
-                    iso = true;
-                    rdzrho = fdiv((R)1.0, dzrho);
-                    R tv1 = 0;
-                    // ina order (knp, nnp) = (0,0), (0,1), (1,0), (1,1)
+                if constexpr (IM == 2) {
+                    if (M.diso) {
+                        if constexpr (ANOM) {
+                            // zero coefficients give the flux without the isoneutral term exactly
+                            const AS_GLOBAL float* ic = reinterpret_cast<const AS_GLOBAL float*>(gp(M.isoc));
 #pragma unroll
-                    for (int n = 0; n < 4; n++) {
-                        const int b = n < 2 ? cb : ub;
-                        const int kn = n < 2 ? k : k + 1;
-                        const bool east = n & 1;
-                        const bool mx = kn >= (east ? k1e : k1w), my = kn >= (east ? k1n : k1s);
-                        const R dxT = mx ? (east ? D(b, qT, pe, p) : D(b, qT, p, pw)) * rc * rdphi : (R)0;
-                        const R dxS = mx ? (east ? D(b, qS, pe, p) : D(b, qS, p, pw)) * rc * rdphi : (R)0;
-                        const R dyT = my ? (east ? D(b, qT, pn, p) * cvn * rdsvn : D(b, qT, p, ps) * cvs * rdsvs) : (R)0;
-                        const R dyS = my ? (east ? D(b, qS, pn, p) * cvn * rdsvn : D(b, qS, p, ps) * cvs * rdsvs) : (R)0;
-                        dxr[n] = scc * dxS - tec * dxT;
-                        dyr[n] = scc * dyS - tec * dyT;
-                        tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
+                            for (int n = 0; n < 4; n++) {
+                                ax[n] = at(ic, (uint32_t)(n * ncell) + oc);
+                                ay[n] = at(ic, (uint32_t)((4 + n) * ncell) + oc);
+                            }
+                            qq = at(ic, (uint32_t)(8 * ncell) + oc);
+                            iso = true;
+                        } else {
+                            const AS_GLOBAL double* ic = gp(M.isoc);
+                            dzrho = at(ic, oc);
+                            iso = dzrho < 0.0;
+                            if (iso) {
+#pragma unroll
+                                for (int n = 0; n < 4; n++) {
+                                    dxr[n] = at(ic, (uint32_t)((1 + n) * ncell) + oc);
+                                    dyr[n] = at(ic, (uint32_t)((5 + n) * ncell) + oc);
+                                }
+                                slim = at(ic, (uint32_t)(9 * ncell) + oc);
+                            }
+                        }
                     }
-                    tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
-                    const R ssmax = mk[MK_SSMAX * KS + k];
-                    if (tv1 > ssmax) {
-                        slim = fdiv(ssmax * ssmax, tv1 * tv1);
-                        limps++;
-                    } else {
-                        slim = 1;
-                    }
-                    tv1 = tv1 * slim * diff1 * rdza;
-                    const R tvd = tv1 * dt * rdza;
-                    if (tvd > dmaxv) dmaxv = tvd;
+                } else {
+                    iso_coeffs(k, cb, ub, rdza, dt, iso, dzrho, rdzrho, slim, dxr, dyr);
+                    if constexpr (ANOM) fp32_iso(iso, slim, rdzrho, rdza, dxr, dyr, ax, ay, qq);
                 }
             }
             // discrete divergence of the open faces (FP32 form only)
@@ -371,19 +466,6 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 const R cs = os ? (cvs * us_ * ((R)1.0 + ups) * (R)0.5 + cv2s * diff1) * rds : (R)0;
                 const R ca = top ? (R)0 : ua_ * ((R)1.0 - upa) * (R)0.5 - rdza * diffv;
                 const R cbn = top ? (R)0 : -(ua_ * ((R)1.0 + upa) * (R)0.5) - rdza * diffv;
-                R ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, qq = 0;
-                if (iso) {
-                    const R f = (R)2 * isoc * rdzrho;
-                    R s2 = 0;
-#pragma unroll
-                    for (int n = 0; n < 4; n++) {
-                        const bool east = n & 1;
-                        ax[n] = f * dxr[n] * rc * rdphi * (east ? (R)1 : (R)-1);
-                        ay[n] = f * dyr[n] * (east ? cvn * rdsvn : -(cvs * rdsvs));
-                        s2 = s2 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
-                    }
-                    qq = isoc * s2 * (rdzrho * rdzrho) * rdza;
-                }
 #pragma unroll
                 for (int q = 0; q < LB; q++) {
                     if (q >= nl) break;
@@ -510,7 +592,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         if (more) commit((k + 2) % FLUX_NBUF);
         __syncthreads();
     }
-    if (l0 == 0 && M.diso && active) {
+    if (IM != 2 && l0 == 0 && M.diso && active) {
         if (limps) atomicAdd(&M.diag[0], limps);
         if (dmaxv > (R)0) atomic_max_pos(M.dmax, (double)dmaxv);
     }
@@ -528,35 +610,54 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
 }
 
 // Launch of k_flux over a batch: nt threads per block (128 or 256), lb
-// tracers per block, nbuf level planes; the tracer block, then the block
-// width, then the ring shrink until the workgroup fits the LDS the device
-// grants one block.
+// tracers per block, nbuf level planes; with iso_split the isoneutral
+// coefficients come from a pre-pass launch (IM 1) and the tracer blocks (IM 2)
+// stage only their tracers.  The tracer block, then the block width, then the
+// ring shrink until the workgroup fits the LDS the device grants one block.
 template <typename R>
 void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape& fs, int maxL, int lb, int nt,
-                   int nbuf, size_t lds_cap, hipStream_t s) {
+                   int nbuf, int iso_split, size_t lds_cap, hipStream_t s) {
     constexpr int es = sizeof(R);
     if (nt != 256) nt = 128;
     if (nbuf != 2) nbuf = 3;
-    auto bytes = [&]() { return flux_lds_bytes(fs.tw[nt == 256], lb, es, nt, nbuf); };
-    while (lb > 2 && bytes() > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
-    if (nt == 256 && bytes() > lds_cap) nt = 128;
-    if (nbuf == 3 && bytes() > lds_cap) nbuf = 2;
+    auto bytes = [&](int ns, int b) { return flux_lds_bytes(fs.tw[nt == 256], ns, b, es, nt, nbuf); };
+    auto slots = [&]() { return iso_split ? lb : lb + 2; };
+    while (lb > 2 && bytes(slots(), lb) > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
+    if (nt == 256 && bytes(slots(), lb) > lds_cap) nt = 128;
+    if (nbuf == 3 && bytes(slots(), lb) > lds_cap) nbuf = 2;
     const int v = nt == 256, tw = fs.tw[v];
     if (fs.nblk[v] == 0) return;
     const dim3 grid(fs.nblk[v], (maxL + lb - 1) / lb, n);
-    const size_t lds = bytes();
-#define CGO_FLUX(LB_, NT_, NB_) hipLaunchKernelGGL((k_flux<R, LB_, NT_, NB_>), grid, dim3(NT_), lds, s, t, l, tw)
-#define CGO_FLUX_LB(NT_, NB_)                 \
-    do {                                      \
-        if (lb == 2) CGO_FLUX(2, NT_, NB_);   \
-        else if (lb == 8) CGO_FLUX(8, NT_, NB_); \
-        else if (lb == 6) CGO_FLUX(6, NT_, NB_); \
-        else CGO_FLUX(4, NT_, NB_);           \
+    const size_t lds = bytes(slots(), lb);
+#define CGO_FLUX(LB_, NT_, NB_, IM_) \
+    hipLaunchKernelGGL((k_flux<R, LB_, NT_, NB_, IM_>), grid, dim3(NT_), lds, s, t, l, tw)
+#define CGO_FLUX_LB(NT_, NB_, IM_)                    \
+    do {                                              \
+        if (lb == 2) CGO_FLUX(2, NT_, NB_, IM_);      \
+        else if (lb == 8) CGO_FLUX(8, NT_, NB_, IM_); \
+        else if (lb == 6) CGO_FLUX(6, NT_, NB_, IM_); \
+        else CGO_FLUX(4, NT_, NB_, IM_);              \
     } while (0)
-    if (nt == 256) {
-        if (nbuf == 2) CGO_FLUX_LB(256, 2); else CGO_FLUX_LB(256, 3);
+    if (iso_split) {
+        // the pre-pass: one block per column block, T and S staged
+        const dim3 g1(fs.nblk[v], 1, n);
+        const size_t lds1 = bytes(2, 2);
+        if (nt == 256) {
+            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 256, 2, 1>), g1, dim3(256), lds1, s, t, l, tw);
+            else hipLaunchKernelGGL((k_flux<R, 2, 256, 3, 1>), g1, dim3(256), lds1, s, t, l, tw);
+        } else {
+            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 128, 2, 1>), g1, dim3(128), lds1, s, t, l, tw);
+            else hipLaunchKernelGGL((k_flux<R, 2, 128, 3, 1>), g1, dim3(128), lds1, s, t, l, tw);
+        }
+        if (nt == 256) {
+            if (nbuf == 2) CGO_FLUX_LB(256, 2, 2); else CGO_FLUX_LB(256, 3, 2);
+        } else {
+            if (nbuf == 2) CGO_FLUX_LB(128, 2, 2); else CGO_FLUX_LB(128, 3, 2);
+        }
+    } else if (nt == 256) {
+        if (nbuf == 2) CGO_FLUX_LB(256, 2, 0); else CGO_FLUX_LB(256, 3, 0);
     } else {
-        if (nbuf == 2) CGO_FLUX_LB(128, 2); else CGO_FLUX_LB(128, 3);
+        if (nbuf == 2) CGO_FLUX_LB(128, 2, 0); else CGO_FLUX_LB(128, 3, 0);
     }
 #undef CGO_FLUX_LB
 #undef CGO_FLUX

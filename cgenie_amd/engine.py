@@ -95,6 +95,7 @@ def load() -> C.CDLL:
         "cgo_set_tile": ([P, _i32, _i32], C.c_int),
         "cgo_set_flux_cols": ([P, _i32, _i32], C.c_int),
         "cgo_set_flux_ring": ([P, _i32, _i32], C.c_int),
+        "cgo_set_flux_iso": ([P, _i32, _i32], C.c_int),
         "cgo_time_kernel": ([P, _i32, ip, _i32, _i32, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -274,12 +275,15 @@ class Engine:
     def sync(self):
         self._chk(self.L.cgo_sync(self.h), "sync")
 
-    def set_tile(self, precision: int, tracer_block: int, cols: int | None = None, ring: int | None = None):
+    def set_tile(self, precision: int, tracer_block: int, cols: int | None = None, ring: int | None = None,
+                 iso_prepass: int | None = None):
         self._chk(self.L.cgo_set_tile(self.h, precision, tracer_block), "set_tile")
         if cols is not None:
             self._chk(self.L.cgo_set_flux_cols(self.h, precision, cols), "set_flux_cols")
         if ring is not None:
             self._chk(self.L.cgo_set_flux_ring(self.h, precision, ring), "set_flux_ring")
+        if iso_prepass is not None:
+            self._chk(self.L.cgo_set_flux_iso(self.h, precision, iso_prepass), "set_flux_iso")
 
     def time_kernel(self, which: int, members, reps: int) -> float:
         arr, n = _ids(members)

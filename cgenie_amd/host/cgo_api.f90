@@ -235,6 +235,12 @@ module cgo_api
        integer(c_int32_t), value :: precision, planes
      end function cgo_set_flux_ring
 
+     integer(c_int) function cgo_set_flux_iso(e, precision, prepass) bind(C, name="cgo_set_flux_iso")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: precision, prepass
+     end function cgo_set_flux_iso
+
      integer(c_int) function cgo_time_kernel(e, which, members, n, reps, avg_ms) bind(C, name="cgo_time_kernel")
        import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: e

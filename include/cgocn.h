@@ -149,6 +149,7 @@ int cgo_graph_destroy(cgo_engine *e, int32_t graph_id);
 int cgo_set_tile(cgo_engine *e, int32_t precision, int32_t tracer_block);   /* 2, 4, 6 or 8 */
 int cgo_set_flux_cols(cgo_engine *e, int32_t precision, int32_t cols);      /* 128 or 256 wet columns per block */
 int cgo_set_flux_ring(cgo_engine *e, int32_t precision, int32_t planes);    /* 2 or 3 level planes in LDS */
+int cgo_set_flux_iso(cgo_engine *e, int32_t precision, int32_t prepass);    /* 1: isoneutral coefficients from a pre-pass */
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
  * 1 = co, 2 = bgc column + coupling, 3 = flux with the BIOGEM column step
  * running beside it on the second stream, as inside a whole step) over the

@@ -96,9 +96,11 @@ def test_inventory_conserved_96_steps(eng):
     eng.remove_member(m)
 
 
-# (tracers per block, wet columns per block, level planes in LDS)
-SHAPES = [(2, 128, 3), (4, 256, 2), (8, 128, 2), (6, 256, 3), (4, 128, 3), (8, 256, 3)]
-DEFAULT_SHAPE = (4, 128, 3)
+# (tracers per block, wet columns per block, level planes in LDS,
+#  isoneutral coefficients from a pre-pass)
+SHAPES = [(2, 128, 3, 0), (4, 256, 2, 1), (8, 128, 2, 1), (6, 256, 3, 0), (4, 128, 3, 1), (8, 256, 3, 0),
+          (2, 256, 2, 1)]
+DEFAULT_SHAPE = (4, 128, 3, 0)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
