@@ -17,9 +17,9 @@
 
 namespace cgo {
 void launch_flux64(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
-                   int iso_split, size_t lds_cap, hipStream_t);
+                   int iso_split, int part, size_t lds_cap, hipStream_t);
 void launch_flux32(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
-                   int iso_split, size_t lds_cap, hipStream_t);
+                   int iso_split, int part, size_t lds_cap, hipStream_t);
 void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
@@ -1012,19 +1012,30 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
 }
 
 // tstepo: flux + co per precision, then the parity flip (goldstein.f90:2280-2432)
-static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+// the flux launches of both precisions; part as launch_flux_t
+static void enqueue_flux(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int part) {
     for (int p = 0; p < 2; p++) {
         if (!ls.n[p]) continue;
         const int32_t* l = d_lists + ls.off[p];
         if (p == 0)
-            launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0], e->lds_cap,
-                          e->s);
+            launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0],
+                          part, e->lds_cap, e->s);
         else
-            launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], e->lds_cap,
-                          e->s);
-        launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+            launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1],
+                          part, e->lds_cap, e->s);
     }
+}
+
+// co and the parity flip that end tstepo
+static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    for (int p = 0; p < 2; p++)
+        if (ls.n[p]) launch_co(e->d_table, d_lists + ls.off[p], ls.n[p], ls.maxwet[p], p, e->s);
     launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
+}
+
+static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    enqueue_flux(e, ls, d_lists, 3);
+    enqueue_co_flip(e, ls, d_lists);
 }
 
 // step_biogem for the members with BIOGEM state (biogem.f90:528-1877)
@@ -1049,13 +1060,17 @@ static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_li
 static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int nsteps) {
     const bool bgc = ls.n[2] || ls.n[3];
     for (int it = 0; it < nsteps; it++) {
+        // the isoneutral pre-pass first (short and latency-bound: it runs
+        // alone), then the BIOGEM step beside the flux's tracer blocks
+        enqueue_flux(e, ls, d_lists, 1);
         if (bgc) {
             hipEventRecord(e->ev_fork, e->s);
             hipStreamWaitEvent(e->s2, e->ev_fork, 0);
             enqueue_bio(e, ls, d_lists, e->s2);
             hipEventRecord(e->ev_join, e->s2);
         }
-        enqueue_tstepo(e, ls, d_lists);
+        enqueue_flux(e, ls, d_lists, 2);
+        enqueue_co_flip(e, ls, d_lists);
         if (bgc) {
             enqueue_coupling(e, ls, d_lists, 3);  // the sums are fresh: tstepo formed them
             hipStreamWaitEvent(e->s, e->ev_join, 0);
@@ -1243,26 +1258,18 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
     HIPCHK(e, hipStreamSynchronize(e->s));
     HIPCHK(e, hipEventRecord(e->ev0, e->s));
     if (which == 3) {
-        // the flux as it runs inside a whole step: the BIOGEM column step on
-        // the second stream beside it
+        // the flux as it runs inside a whole step: the isoneutral pre-pass, then
+        // the tracer blocks with the BIOGEM column step on the second stream
         const bool bgc = ls.n[2] || ls.n[3];
         for (int r = 0; r < reps; r++) {
+            enqueue_flux(e, ls, e->d_list, 1);
             if (bgc) {
                 hipEventRecord(e->ev_fork, e->s);
                 hipStreamWaitEvent(e->s2, e->ev_fork, 0);
                 enqueue_bio(e, ls, e->d_list, e->s2);
                 hipEventRecord(e->ev_join, e->s2);
             }
-            for (int p = 0; p < 2; p++) {
-                if (!ls.n[p]) continue;
-                const int32_t* l = e->d_list + ls.off[p];
-                if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0],
-                                  e->isosplit[0], e->lds_cap, e->s);
-                else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1],
-                                  e->isosplit[1], e->lds_cap, e->s);
-            }
+            enqueue_flux(e, ls, e->d_list, 2);
             if (bgc) hipStreamWaitEvent(e->s, e->ev_join, 0);
         }
     }
@@ -1272,10 +1279,10 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             const int32_t* l = e->d_list + ls.off[p];
             if (which == 0) {
                 if (p == 0)
-                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0], e->lds_cap,
+                    launch_flux64(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[0], e->nt[0], e->nbuf[0], e->isosplit[0], 3, e->lds_cap,
                           e->s);
                 else
-                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], e->lds_cap,
+                    launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], 3, e->lds_cap,
                           e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);

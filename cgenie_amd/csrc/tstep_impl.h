@@ -612,11 +612,11 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
 // Launch of k_flux over a batch: nt threads per block (128 or 256), lb
 // tracers per block, nbuf level planes; with iso_split the isoneutral
 // coefficients come from a pre-pass launch (IM 1) and the tracer blocks (IM 2)
-// stage only their tracers.  The tracer block, then the block width, then the
+// stage only their tracers.  part: 1 the pre-pass, 2 the tracer blocks, 3 both.  The tracer block, then the block width, then the
 // ring shrink until the workgroup fits the LDS the device grants one block.
 template <typename R>
 void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape& fs, int maxL, int lb, int nt,
-                   int nbuf, int iso_split, size_t lds_cap, hipStream_t s) {
+                   int nbuf, int iso_split, int part, size_t lds_cap, hipStream_t s) {
     constexpr int es = sizeof(R);
     if (nt != 256) nt = 128;
     if (nbuf != 2) nbuf = 3;
@@ -641,6 +641,7 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
     if (iso_split) {
         // the pre-pass: one block per column block, T and S staged
         const dim3 g1(fs.nblk[v], 1, n);
+        if (part & 1) {
         const size_t lds1 = bytes(2, 2);
         if (nt == 256) {
             if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 256, 2, 1>), g1, dim3(256), lds1, s, t, l, tw);
@@ -649,11 +650,15 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
             if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 128, 2, 1>), g1, dim3(128), lds1, s, t, l, tw);
             else hipLaunchKernelGGL((k_flux<R, 2, 128, 3, 1>), g1, dim3(128), lds1, s, t, l, tw);
         }
-        if (nt == 256) {
+        }
+        if (!(part & 2)) {
+        } else if (nt == 256) {
             if (nbuf == 2) CGO_FLUX_LB(256, 2, 2); else CGO_FLUX_LB(256, 3, 2);
         } else {
             if (nbuf == 2) CGO_FLUX_LB(128, 2, 2); else CGO_FLUX_LB(128, 3, 2);
         }
+    } else if (!(part & 2)) {
+        // no pre-pass: the tracer blocks compute their own coefficients
     } else if (nt == 256) {
         if (nbuf == 2) CGO_FLUX_LB(256, 2, 0); else CGO_FLUX_LB(256, 3, 0);
     } else {
