@@ -273,8 +273,9 @@ def main():
                 frac=round(main_["flux_gbs"] / HBM_PEAK_GBS, 4),
                 traffic=measured_traffic(f'{main_["tracer_block"]}x{main_["flux_cols"]}x{main_["flux_ring"]}'
                                          f'x{main_["flux_iso_prepass"]}'),
-                kernel=f'k_flux<float, {main_["tracer_block"]}, {main_["flux_cols"]}, {main_["flux_ring"]}, '
-                       f'{2 if main_["flux_iso_prepass"] else 0}>',
+                kernel=(f'k_flux<float, 2, {main_["flux_cols"]}, {main_["flux_ring"]}, 1> + ' if main_["flux_iso_prepass"]
+                        else '') + f'k_flux<float, {main_["tracer_block"]}, {main_["flux_cols"]}, {main_["flux_ring"]}, '
+                                   f'{2 if main_["flux_iso_prepass"] else 0}>',
                 algorithmic_bytes=main_["flux_bytes"], launch_ms=round(main_["flux_step_ms"], 4),
                 alone_ms=round(main_["flux_ms"], 4), alone_achieved=round(main_["flux_alone_gbs"], 1))
     line = dict(
