@@ -686,6 +686,19 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     int16_t* lo = M.lo[par ^ 1];
     const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
     AS_GLOBAL double* rho = gp(M.rho);
+    // A column whose density falls strictly from k1 up is left as it is: the
+    // loop below walks m down to k1 without mixing and the fill-in finds every
+    // level in place.  Most columns are; they skip the per-level arrays.
+    {
+        bool stable = true;
+        double rb = rho[(size_t)(maxk - 1) * plane + col];
+        for (int q = maxk - 1; q >= k1 && stable; q--) {
+            const double ra = rho[(size_t)(q - 1) * plane + col];
+            stable = rb < ra;
+            rb = ra;
+        }
+        if (stable) return;
+    }
     const AS_GLOBAL double* dz = gp(g.mk) + MK_DZ * (maxk + 2);
     const double e0 = g.ec[0], e1 = g.ec[1], e2 = g.ec[2], e3 = g.ec[3];
     int kk[MAXK_DEV + 2];
