@@ -122,6 +122,58 @@ __device__ __forceinline__ void carb_sweep(double H, double DIC, double ALK, dou
     o.H2 = cc[CC_K2] * o.HCO3 / o.CO3;
 }
 
+// 1/x to ~1 ulp: the hardware reciprocal and two Newton steps (the FP32 form's
+// members; FP64 members divide exactly as the reference)
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// The loop-invariant reciprocals of one solve (FP32 form)
+struct CarbRcp {
+    double rKB, rP2, rP3, rP1P2, rP2P3, rP1P2P3, P1, P1P2, P1P2P3, rK4, r2K4;
+};
+__device__ __forceinline__ void carb_rcp(const double* cc, CarbRcp& r) {
+    const double P1 = cc[CC_KP1], P2 = cc[CC_KP2], P3 = cc[CC_KP3];
+    r.P1 = P1;
+    r.P1P2 = P1 * P2;
+    r.P1P2P3 = P1 * P2 * P3;
+    r.rKB = 1.0 / cc[CC_KB];
+    r.rP2 = 1.0 / P2;
+    r.rP3 = 1.0 / P3;
+    r.rP1P2 = 1.0 / r.P1P2;
+    r.rP2P3 = 1.0 / (P2 * P3);
+    r.rP1P2P3 = 1.0 / r.P1P2P3;
+    r.rK4 = 1.0 / (cc[CC_K] - 4.0);
+    r.r2K4 = 1.0 / (2.0 * (cc[CC_K] - 4.0));
+}
+
+// the speciation sweep of carb_sweep with the divides replaced by products
+// with reciprocals (FP32 form); the same fixed point to rounding
+__device__ __forceinline__ void carb_sweep_fast(double H, double DIC, double ALK, double PO4t, double Bt, double SO4t,
+                                                double Ft, const double* cc, const CarbRcp& rc, Spec& o) {
+    const double rH = rcp_nr(H);
+    const double H2 = H * H, H3 = H * H2, rH2 = rH * rH, rH3 = rH2 * rH;
+    const double OH = cc[CC_KW] * rH;
+    const double H4BO4 = Bt * rcp_nr(1.0 + H * rc.rKB);
+    const double HSO4 = SO4t * rcp_nr(1.0 + cc[CC_KHSO4] * rH);
+    const double HF = Ft * rcp_nr(1.0 + cc[CC_KHF] * rH);
+    const double H3PO4 = PO4t * rcp_nr(1.0 + rc.P1 * rH + rc.P1P2 * rH2 + rc.P1P2P3 * rH3);
+    const double HPO4 = PO4t * rcp_nr(1.0 + H * rc.rP2 + H2 * rc.rP1P2 + cc[CC_KP3] * rH);
+    const double PO4 = PO4t * rcp_nr(1.0 + H * rc.rP3 + H2 * rc.rP2P3 + H3 * rc.rP1P2P3);
+    const double A = ALK - H4BO4 - OH - HPO4 - 2.0 * PO4 + H + HSO4 + HF + H3PO4;
+    const double K = cc[CC_K];
+    const double t = 4.0 * A + DIC * K - A * K;
+    const double zed = sqrt(t * t + 4.0 * (K - 4.0) * (A * A));
+    o.HCO3 = (DIC * K - zed) * rc.rK4;
+    o.CO3 = (A * K - DIC * K - 4.0 * A + zed) * rc.r2K4;
+    o.CO2 = DIC - A + o.CO3;
+    o.H1 = cc[CC_K1] * o.CO2 * rcp_nr(o.HCO3);
+    o.H2 = cc[CC_K2] * o.HCO3 * rcp_nr(o.CO3);
+}
+
 // sub_calc_carb: >0 iterations, -1 negative [H] estimate, -2 iteration limit
 // (ctrl_carbchem_fail: carb is left unchanged, no random re-seed)
 __device__ inline int calc_carb(double DIC, double ALK, double Ca, double PO4, double Bt, double SO4t, double Ft,
@@ -190,13 +242,15 @@ __device__ inline int calc_carb_fast(double DIC, double ALK, double Ca, double P
     int n = 0;
     double H = carb[IC_H * cs];
     Spec o;
+    CarbRcp rc;
+    carb_rcp(cc, rc);
     double h[3];
     for (;;) {
         h[0] = H;
         bool done = false;
         for (int j = 0; j < 2 && !done; j++) {
             const double Hin = H;
-            carb_sweep(Hin, DIC, ALK, PO4, Bt, SO4t, Ft, cc, o);
+            carb_sweep_fast(Hin, DIC, ALK, PO4, Bt, SO4t, Ft, cc, rc, o);
             n = n + 1;
             if ((o.H1 < NULLSMALL) || (o.H2 < NULLSMALL)) return -1;
             H = sqrt(o.H1 * o.H2);
@@ -225,13 +279,15 @@ __device__ inline int calc_carb_RF0_fast(double DIC, double ALK, double PO4, dou
     double H = carb[IC_H * cs];
     const double DICr = DIC + 1.0e-6;
     Spec o;
+    CarbRcp rc;
+    carb_rcp(cc, rc);
     double h[3];
     for (;;) {
         h[0] = H;
         bool done = false;
         for (int j = 0; j < 2 && !done; j++) {
             const double Hin = H;
-            carb_sweep(Hin, DICr, ALK, PO4, Bt, SO4t, Ft, cc, o);
+            carb_sweep_fast(Hin, DICr, ALK, PO4, Bt, SO4t, Ft, cc, rc, o);
             n = n + 1;
             H = sqrt(o.H1 * o.H2);
             h[j + 1] = H;
