@@ -89,10 +89,12 @@ struct cgo_engine {
     std::vector<MemberH*> members;
     MemberDev* d_table = nullptr;
     bool table_dirty = false;
-    int lb[2] = {4, 4};       // flux tracer block per precision
-    int nt[2] = {128, 128};   // flux columns per block per precision
-    int nbuf[2] = {3, 3};     // flux level planes resident in LDS per precision
-    int isosplit[2] = {0, 0}; // isoneutral coefficients from a pre-pass, per precision
+    // flux tile per precision (FP64, FP32C): the shapes the autotuner picks
+    // for a large ensemble on MI355X; cgo_set_tile and its companions change them
+    int lb[2] = {8, 6};       // tracers per block
+    int nt[2] = {256, 256};   // wet columns per block
+    int nbuf[2] = {2, 2};     // level planes resident in LDS
+    int isosplit[2] = {1, 1}; // isoneutral coefficients from a pre-pass
     size_t lds_cap = 65536;   // LDS one workgroup may take
     std::vector<Graph> graphs;
     int32_t* d_list = nullptr;  // scratch lists for eager steps

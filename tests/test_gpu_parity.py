@@ -100,7 +100,8 @@ def test_inventory_conserved_96_steps(eng):
 #  isoneutral coefficients from a pre-pass)
 SHAPES = [(2, 128, 3, 0), (4, 256, 2, 1), (8, 128, 2, 1), (6, 256, 3, 0), (4, 128, 3, 1), (8, 256, 3, 0),
           (2, 256, 2, 1)]
-DEFAULT_SHAPE = (4, 128, 3, 0)
+DEFAULT_SHAPE = (8, 256, 2, 1)      # the engine's FP64 default
+DEFAULT_SHAPE_32 = (6, 256, 2, 1)   # the engine's FP32-form default
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -119,13 +120,33 @@ def test_tracer_block_shapes_agree(eng, shape):
     eng.remove_member(m)
 
 
+@pytest.mark.parametrize("shape", [(6, 256, 2, 1), (4, 128, 3, 1), (8, 256, 2, 0)])
+def test_tile_shapes_with_sflux_random_u_no_diso(eng, shape):
+    """The isoneutral pre-pass and the tile shapes on a case with surface
+    fluxes and a random (divergent) velocity field, with and without
+    isoneutral diffusion: FP64 equal to the oracle bit for bit."""
+    for diso in (True, False):
+        say("fp64 flux shape", shape, "diso", diso)
+        c = make_case(kind="ts", L=7, seed=27, random_u=True, sflux=True)
+        eng.set_tile(E.FP64, *shape)
+        m = new_member(eng, c, diso=diso)
+        eng.step([m], 4)
+        got = eng.download(m, c.ts.shape)
+        eng.set_tile(E.FP64, *DEFAULT_SHAPE)
+        gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u, c.sflux, diso=diso)
+        gold.step(4)
+        w = wet(c)
+        assert np.count_nonzero(got[w] != gold.ts[w]) == 0
+        eng.remove_member(m)
+
+
 def test_fp32_tile_shapes_identical(eng):
     """The FP32 form gives the same bits for every flux tile shape (each
     cell's arithmetic does not depend on how cells are grouped)."""
     c = make_case(kind="config2", seed=23)
     p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
     outs = []
-    for shape in [DEFAULT_SHAPE] + SHAPES:
+    for shape in [DEFAULT_SHAPE_32] + SHAPES:
         say("fp32 flux shape", shape)
         eng.set_tile(E.FP32C, *shape)
         m = new_member(eng, c, prec=E.FP32C)
@@ -133,7 +154,7 @@ def test_fp32_tile_shapes_identical(eng):
         eng.step([m], 4)
         outs.append(eng.download(m, c.ts.shape))
         eng.remove_member(m)
-    eng.set_tile(E.FP32C, *DEFAULT_SHAPE)
+    eng.set_tile(E.FP32C, *DEFAULT_SHAPE_32)
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
 
