@@ -1062,11 +1062,15 @@ static void enqueue_coupling(cgo_engine* e, const Lists& ls, const int32_t* d_li
 static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists, int nsteps) {
     const bool bgc = ls.n[2] || ls.n[3];
     for (int it = 0; it < nsteps; it++) {
-        // the isoneutral pre-pass first (short and latency-bound: it runs
-        // alone), then the BIOGEM step beside the flux's tracer blocks
+        // the BIOGEM step depends only on the last coupling: it forks from
+        // there and runs beside the isoneutral pre-pass and the flux's tracer
+        // blocks.  Its launch is enqueued after the pre-pass's, which a
+        // captured graph keeps: the pre-pass (short, latency-bound) takes the
+        // CUs first, and BIOGEM's blocks fill the slots both flux kernels
+        // leave (enqueued first, they ran beside the pre-pass alone)
+        if (bgc) hipEventRecord(e->ev_fork, e->s);
         enqueue_flux(e, ls, d_lists, 1);
         if (bgc) {
-            hipEventRecord(e->ev_fork, e->s);
             hipStreamWaitEvent(e->s2, e->ev_fork, 0);
             enqueue_bio(e, ls, d_lists, e->s2);
             hipEventRecord(e->ev_join, e->s2);
@@ -1260,13 +1264,14 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
     HIPCHK(e, hipStreamSynchronize(e->s));
     HIPCHK(e, hipEventRecord(e->ev0, e->s));
     if (which == 3) {
-        // the flux as it runs inside a whole step: the isoneutral pre-pass, then
-        // the tracer blocks with the BIOGEM column step on the second stream
+        // the flux as it runs inside a whole step: the isoneutral pre-pass and
+        // the tracer blocks with the BIOGEM column step on the second stream,
+        // enqueued as a step enqueues them
         const bool bgc = ls.n[2] || ls.n[3];
         for (int r = 0; r < reps; r++) {
+            if (bgc) hipEventRecord(e->ev_fork, e->s);
             enqueue_flux(e, ls, e->d_list, 1);
             if (bgc) {
-                hipEventRecord(e->ev_fork, e->s);
                 hipStreamWaitEvent(e->s2, e->ev_fork, 0);
                 enqueue_bio(e, ls, e->d_list, e->s2);
                 hipEventRecord(e->ev_join, e->s2);
