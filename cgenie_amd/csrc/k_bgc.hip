@@ -111,9 +111,12 @@ struct Lvl {
 // leaves it; the stream is stepped through the levels below it until it
 // settles at min_k (NEWP) or reaches the sea floor.  At target level kk the
 // live streams are the sources kk+1 .. kk+W-1 (the host checks that no
-// stream spans W levels or more), kept in slot (k + off) % W; the march is
-// unrolled by W so every slot index is a compile-time constant and the
-// streams stay in registers.  The contributions to a level therefore arrive
+// stream spans W levels or more), source kk+1+j in slot j; after each level
+// the streams move down one slot, so every slot index is a compile-time
+// constant, the streams stay in registers and the level loop is not unrolled
+// (an unrolled march, W copies of W-1 stream steps, was ~120 KB of code:
+// more than the instruction cache two CUs share, and it evicted the flux
+// kernel running beside it).  The contributions to a level therefore arrive
 // in the reference's order: DOM first, then sources from the top down, then
 // the sediment return, then uptake.
 template <int W>
@@ -354,33 +357,38 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
             cur = nxt;
         }
     } else {
-        Streams<W> S;
+        // at level kk slot j holds the stream of source kk + 1 + j; after the
+        // level every stream moves down one slot and source kk enters slot 0
+        // (the stream of source kk + W - 1 has settled by then)
+        Streams<W - 1> S;
 #pragma unroll
-        for (int j = 0; j < W; j++) S.mk[j] = -2;
-        for (int base = maxk; base >= kb; base -= W) {
+        for (int j = 0; j < W - 1; j++) S.mk[j] = -2;
+        for (int kk = maxk; kk >= kb; kk--) {
+            if (kk > kb) load(kk - 1, nxt);
+            double pt[NSED] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            int pmk = -2;
+            level(
+                kk, cur,
+                [&](int k_, double rf, double* v, double* np) {
 #pragma unroll
-            for (int r = 0; r < W; r++) {
-                const int kk = base - r;
-                if (kk >= kb) {
-                    if (kk > kb) load(kk - 1, nxt);
-                    level(
-                        kk, cur,
-                        [&](int k_, double rf, double* v, double* np) {
+                    for (int j = W - 2; j >= 0; j--)  // sources from the top down
+                        if (S.mk[j] > -2) advance(S.t[j], S.mk[j], k_, rf, v, np);
+                },
+                [&](const double* t, int mk) {
 #pragma unroll
-                            for (int d = W - 1; d >= 1; d--) {
-                                const int j = (W - 1 - r + d) % W;  // slot of source kk + d
-                                if (S.mk[j] > -2) advance(S.t[j], S.mk[j], k_, rf, v, np);
-                            }
-                        },
-                        [&](const double* t, int mk) {
-                            const int j = W - 1 - r;
+                    for (int s = 0; s < NSED; s++) pt[s] = t[s];
+                    pmk = mk;
+                });
 #pragma unroll
-                            for (int s = 0; s < NSED; s++) S.t[j][s] = t[s];
-                            S.mk[j] = mk;
-                        });
-                    cur = nxt;
-                }
+            for (int j = W - 2; j >= 1; j--) {
+#pragma unroll
+                for (int s = 0; s < NSED; s++) S.t[j][s] = S.t[j - 1][s];
+                S.mk[j] = S.mk[j - 1];
             }
+#pragma unroll
+            for (int s = 0; s < NSED; s++) S.t[0][s] = pt[s];
+            S.mk[0] = pmk;
+            cur = nxt;
         }
     }
 #pragma unroll
