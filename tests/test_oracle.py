@@ -76,6 +76,39 @@ def test_carbchem_matches_compiled_reference(tol, maxit):
         assert c1[9] == c2[9]
 
 
+GOLDEN_CARB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "carbchem_ref.npz")
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_carbchem_matches_reference_vectors(case):
+    """og_calc_carbconst/og_calc_carb/og_calc_carb_RF0 against vectors the
+    reference's gem_carbchem.f90 wrote (compiled from its sources into
+    oracle/_ref by tests/golden/make_carbchem_golden.py): bit-identical, with
+    or without the reference tree at hand."""
+    G = np.load(GOLDEN_CARB)
+    L = O.lib()
+    tol, maxit = G["tol_maxit"][case]
+    maxit = int(maxit)
+    x = G["inputs"]
+    assert len(x) >= 64
+    for i in range(len(x)):
+        D, T, S, DIC, ALK, PO4, H0 = x[i]
+        a = np.zeros(17)
+        L.og_calc_carbconst(D, T, S, a)
+        assert np.array_equal(a, G["carbconst"][i])
+        tot = G["salt_totals"][i]
+        assert [L.og_calc_Ca(S), L.og_calc_Btot(S), L.og_calc_SO4tot(S), L.og_calc_Ftot(S)] == list(tot)
+        c1 = np.zeros(10)
+        c1[0] = H0
+        ca1 = np.zeros(13)
+        it = L.og_calc_carb(DIC, ALK, tot[0], PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, a, c1, ca1.ctypes.data,
+                            tol, maxit)
+        assert (it > 0) == (G[f"status_{case}"][i] == 0)
+        assert np.array_equal(c1, G[f"carb_{case}"][i]) and np.array_equal(ca1, G[f"carbalk_{case}"][i])
+        L.og_calc_carb_RF0(DIC, ALK, PO4, 0.0, tot[1], tot[2], tot[3], 0.0, 0.0, a, c1, tol, maxit)
+        assert c1[9] == G[f"rf0_{case}"][i]
+
+
 def test_carbonate_root_satisfies_alkalinity():
     """At the fixed point H1 = H2, so the carbonate alkalinity quadratic gives
     [HCO3]+2[CO3] = CA and HCO3^2/(CO2 CO3) = k1/k2 (independent check)."""
