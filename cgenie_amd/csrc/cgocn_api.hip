@@ -208,7 +208,7 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     if (mi < 3 || mj < 3 || mk < 2 || mk > MAXK_DEV) return fail(e, CGO_E_ARG, "grid_create: bad dims %d %d %d", mi, mj, mk);
     // a flux tile of one column and its neighbours (2 maxi + 1) must fit the
     // 3 x 128 positions a 128-thread block stages
-    if (mi > 191) return fail(e, CGO_E_ARG, "grid_create: maxi %d above 191", mi);
+    if (2 * mi + 1 > flux_tw(128)) return fail(e, CGO_E_ARG, "grid_create: maxi %d above %d", mi, (flux_tw(128) - 1) / 2);
     const double* jarr[] = {d->rc, d->rc2, d->cv, d->cv2, d->rds, d->dsv, d->rdsv, d->ds, d->sv};
     const double* karr[] = {d->dt, d->rdz, d->rdza, d->dza, d->ssmax, d->dz};
     for (auto* p : jarr)
@@ -298,10 +298,10 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     const size_t o_wets = take(g->wet.size() * 4 + 4), o_vpos = take(g->wet.size() * 4 + 4);
     const size_t o_V = take((size_t)ncell * 8), o_w = take((size_t)ncell * 8), o_cs = take((size_t)ncol * 4);
     // flux blocks: up to nt consecutive wet columns (ascending), cut early so
-    // the staged range with its neighbours stays within flux_pf(nt) * nt
+    // the staged range with its neighbours stays within flux_tw(nt)
     std::vector<FluxBlk> fb[2];
     for (int v = 0; v < 2; v++) {
-        const int nt = v ? 256 : 128, cap = flux_pf(nt) * nt;
+        const int nt = v ? 256 : 128, cap = flux_tw(nt);
         const int nw = (int)g->wets.size();
         auto t0of = [&](int w) { return std::max(g->wets[w] - mi, 0); };
         auto t1of = [&](int w) { return std::min(g->wets[w] + mi + 1, ncol); };
@@ -311,6 +311,12 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
             fb[v].push_back(FluxBlk{w, e2, t0of(w), t1of(e2 - 1)});
             g->ftw[v] = std::max(g->ftw[v], t1of(e2 - 1) - t0of(w));
             w = e2;
+        }
+        // the flux kernel's LDS slots hold cap positions
+        if (g->ftw[v] > cap) {
+            const int wide = g->ftw[v];
+            delete g;
+            return fail(e, CGO_E_ARG, "grid_create: a flux block stages %d positions (at most %d)", wide, cap);
         }
     }
     const size_t o_fb0 = take(fb[0].size() * sizeof(FluxBlk) + 16), o_fb1 = take(fb[1].size() * sizeof(FluxBlk) + 16);

@@ -41,6 +41,12 @@ struct FluxBlk {
 };
 // tile positions staged per thread of a flux block of nt threads
 __host__ __device__ constexpr int flux_pf(int nt) { return nt == 128 ? 3 : 2; }
+// positions of one LDS slot of a flux block (<= flux_pf(nt) * nt): the host
+// cuts blocks so that every staged range fits, and the fixed stride makes
+// each slot's offset an immediate of the LDS instruction.  Close to the
+// widest range of a 36x36 mask (266 and 439 positions) so the slots cost no
+// resident blocks; a single column spans 2 maxi + 1 positions.
+__host__ __device__ constexpr int flux_tw(int nt) { return nt == 128 ? 288 : 448; }
 // the flux grid of a batch: blocks and widest tile, per block width (128, 256)
 struct FluxShape {
     int nblk[2], tw[2];

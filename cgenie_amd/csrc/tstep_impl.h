@@ -108,14 +108,16 @@ __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
 // flux.  With NBUF = 3 the planes k, k+1 and the prefetched k+2 are
 // resident; with NBUF = 2 the prefetched plane replaces plane k after a
 // second barrier per level (less LDS, more workgroups per CU).  The host cuts
-// blocks so that t1 - t0 <= flux_pf(FLUX_THREADS) * FLUX_THREADS.
+// blocks so that t1 - t0 <= flux_tw(FLUX_THREADS).
 // The FP32 form also stages the carry of every value, as a float residual,
 // so differences of neighbours are taken at the stored (~40-bit) precision.
 // ns level-plane slots (lb tracers, plus T and S when the block computes the
-// isoneutral coefficients itself), lb carried bottom fluxes
-__host__ __device__ constexpr size_t flux_lds_bytes(int tw, int ns, int lb, int es, int nt, int nbuf) {
-    return (((size_t)tw * 4 + 15) & ~(size_t)15) + (size_t)nbuf * ns * tw * es + (size_t)lb * nt * es +
-           (es == 4 ? (size_t)nbuf * ns * tw * 4 : 0);
+// isoneutral coefficients itself), lb carried bottom fluxes.  Every slot
+// holds flux_tw(nt) positions, a compile-time stride, so each slot's offset
+// is an immediate of the LDS instruction.
+__host__ __device__ constexpr size_t flux_lds_bytes(int ns, int lb, int es, int nt, int nbuf) {
+    return (size_t)flux_tw(nt) * 4 + (size_t)nbuf * ns * flux_tw(nt) * es + (size_t)lb * nt * es +
+           (es == 4 ? (size_t)nbuf * ns * flux_tw(nt) * 4 : 0);
 }
 
 // the carry c of a float h as a float: c * 2^-16 ulp(h) (exact unless |h| < 2^-87)
@@ -132,8 +134,9 @@ __device__ __forceinline__ float carry_residual(float h, int16_t c) {
 // which load them and stage only their own tracers.
 template <typename R, int LB, int FLUX_THREADS, int FLUX_NBUF, int IM>
 __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restrict__ mems,
-                                                       const int32_t* __restrict__ list, int tw) {
+                                                       const int32_t* __restrict__ list) {
     constexpr int NS = IM == 0 ? LB + 2 : LB;  // slots per staged level plane
+    constexpr int TWS = flux_tw(FLUX_THREADS);  // positions per slot (>= every block's t1 - t0)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockXYZ B = xcd_blk();
     const MemberDev& M = mems[list[B.z]];
@@ -148,8 +151,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const AS_GLOBAL FluxBlk* blk = gp(g.fblk[V]) + B.x;
     const int w0 = blk->w0, t0 = blk->t0, TW = blk->t1 - t0, nw = blk->w1 - w0;
     int* sk1 = reinterpret_cast<int*>(smem);
-    R* sp = reinterpret_cast<R*>(smem + (((size_t)tw * 4 + 15) & ~(size_t)15));
-    R* sfb = sp + FLUX_NBUF * NS * tw;
+    R* sp = reinterpret_cast<R*>(smem + (size_t)TWS * 4);
+    R* sfb = sp + FLUX_NBUF * NS * TWS;
     float* sr = reinterpret_cast<float*>(sfb + LB * FLUX_THREADS);  // FP32 form: residual ring
     // slot q < LB: tracer l0+q; T and S sit in slots 0/1 when l0 == 0, else
     // (coefficients computed here) LB/LB+1
@@ -208,8 +211,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
 
-    auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * NS + q) * tw + pp]; };
-    auto SR = [&](int buf, int q, int pp) -> float& { return sr[(buf * NS + q) * tw + pp]; };
+    auto SP = [&](int buf, int q, int pp) -> R& { return sp[(buf * NS + q) * TWS + pp]; };
+    auto SR = [&](int buf, int q, int pp) -> float& { return sr[(buf * NS + q) * TWS + pp]; };
     // difference of two staged values; the FP32 form adds the residuals
     auto D = [&](int buf, int q, int pa, int pb) -> R {
         if (ANOM) return (SP(buf, q, pa) - SP(buf, q, pb)) + (R)(SR(buf, q, pa) - SR(buf, q, pb));
@@ -620,17 +623,17 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
     constexpr int es = sizeof(R);
     if (nt != 256) nt = 128;
     if (nbuf != 2) nbuf = 3;
-    auto bytes = [&](int ns, int b) { return flux_lds_bytes(fs.tw[nt == 256], ns, b, es, nt, nbuf); };
+    auto bytes = [&](int ns, int b) { return flux_lds_bytes(ns, b, es, nt, nbuf); };
     auto slots = [&]() { return iso_split ? lb : lb + 2; };
     while (lb > 2 && bytes(slots(), lb) > lds_cap) lb = lb == 8 ? 6 : (lb == 6 ? 4 : 2);
     if (nt == 256 && bytes(slots(), lb) > lds_cap) nt = 128;
     if (nbuf == 3 && bytes(slots(), lb) > lds_cap) nbuf = 2;
-    const int v = nt == 256, tw = fs.tw[v];
+    const int v = nt == 256;
     if (fs.nblk[v] == 0) return;
     const dim3 grid(fs.nblk[v], (maxL + lb - 1) / lb, n);
     const size_t lds = bytes(slots(), lb);
 #define CGO_FLUX(LB_, NT_, NB_, IM_) \
-    hipLaunchKernelGGL((k_flux<R, LB_, NT_, NB_, IM_>), grid, dim3(NT_), lds, s, t, l, tw)
+    hipLaunchKernelGGL((k_flux<R, LB_, NT_, NB_, IM_>), grid, dim3(NT_), lds, s, t, l)
 #define CGO_FLUX_LB(NT_, NB_, IM_)                    \
     do {                                              \
         if (lb == 2) CGO_FLUX(2, NT_, NB_, IM_);      \
@@ -644,11 +647,11 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
         if (part & 1) {
         const size_t lds1 = bytes(2, 2);
         if (nt == 256) {
-            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 256, 2, 1>), g1, dim3(256), lds1, s, t, l, tw);
-            else hipLaunchKernelGGL((k_flux<R, 2, 256, 3, 1>), g1, dim3(256), lds1, s, t, l, tw);
+            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 256, 2, 1>), g1, dim3(256), lds1, s, t, l);
+            else hipLaunchKernelGGL((k_flux<R, 2, 256, 3, 1>), g1, dim3(256), lds1, s, t, l);
         } else {
-            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 128, 2, 1>), g1, dim3(128), lds1, s, t, l, tw);
-            else hipLaunchKernelGGL((k_flux<R, 2, 128, 3, 1>), g1, dim3(128), lds1, s, t, l, tw);
+            if (nbuf == 2) hipLaunchKernelGGL((k_flux<R, 2, 128, 2, 1>), g1, dim3(128), lds1, s, t, l);
+            else hipLaunchKernelGGL((k_flux<R, 2, 128, 3, 1>), g1, dim3(128), lds1, s, t, l);
         }
         }
         if (!(part & 2)) {
