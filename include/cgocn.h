@@ -93,7 +93,7 @@ int cgo_last_error(const cgo_engine *e, char *buf, size_t n);
 int cgo_sync(cgo_engine *e);
 
 /* grids and members */
-int cgo_grid_create(cgo_engine *e, const cgo_grid_desc *g, int32_t *grid_id);
+int cgo_grid_create(cgo_engine *e, const cgo_grid_desc *g, int32_t *grid_id);   /* maxi <= 143 */
 int cgo_member_create(cgo_engine *e, int32_t grid_id, int32_t maxl, int32_t precision,
                       const cgo_member_params *p, int32_t *member_id);
 int cgo_member_destroy(cgo_engine *e, int32_t member_id);
