@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
 constexpr int CPL_THREADS = 256;
 constexpr int CPL_COLS = 64;
 constexpr int CPL_LG = CPL_THREADS / CPL_COLS;         // level groups
-constexpr int CPL_MAXTG = 2;                            // tracers per LDS round (2: fewer registers, more waves)
+constexpr int CPL_MAXTG = 1;                            // tracers per LDS round (1: fewer registers, more waves)
 constexpr size_t CPL_LDS_CAP = 64 * 1024;
 
 __host__ __device__ inline int cpl_tg(int maxk) {
@@ -773,11 +773,11 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
         if (phase == 3) {
             // the transport step just formed the column sums
         } else if (maxkmax <= 4 * CPL_LG) {
-            hipLaunchKernelGGL((k_couple_sums<R, 4, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+            hipLaunchKernelGGL((k_couple_sums<R, 4, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
         } else if (maxkmax <= 8 * CPL_LG) {
-            hipLaunchKernelGGL((k_couple_sums<R, 8, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+            hipLaunchKernelGGL((k_couple_sums<R, 8, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
         } else {
-            hipLaunchKernelGGL((k_couple_sums<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+            hipLaunchKernelGGL((k_couple_sums<R, 12, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
         }
         // the staging chunk takes what LDS the device grants a block, up to RED_LDS_BYTES
         const int red_lds = (int)std::min<size_t>(RED_LDS_BYTES, lds_cap - 2 * RED_MAXL * sizeof(double));
@@ -785,11 +785,11 @@ static void launch_coupling_t(const MemberDev* mems, const int32_t* list, int nl
     }
     if (phase == 1 || phase == 3) return;
     if (maxkmax <= 4 * CPL_LG) {
-        hipLaunchKernelGGL((k_couple_apply<R, 4, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        hipLaunchKernelGGL((k_couple_apply<R, 4, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
     } else if (maxkmax <= 8 * CPL_LG) {
-        hipLaunchKernelGGL((k_couple_apply<R, 8, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        hipLaunchKernelGGL((k_couple_apply<R, 8, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
     } else {
-        hipLaunchKernelGGL((k_couple_apply<R, 12, 2>), grid, dim3(CPL_THREADS), b, s, mems, list);
+        hipLaunchKernelGGL((k_couple_apply<R, 12, 1>), grid, dim3(CPL_THREADS), b, s, mems, list);
     }
 }
 
