@@ -206,8 +206,8 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     if (!e || !d || !grid_id || !d->k1) return fail(e, CGO_E_ARG, "grid_create: null argument");
     const int mi = d->maxi, mj = d->maxj, mk = d->maxk;
     if (mi < 3 || mj < 3 || mk < 2 || mk > MAXK_DEV) return fail(e, CGO_E_ARG, "grid_create: bad dims %d %d %d", mi, mj, mk);
-    // a flux tile of one column and its neighbours (2 maxi + 1) must fit the
-    // 3 x 128 positions a 128-thread block stages
+    // a flux tile of one column and its neighbours (2 maxi + 1) must fit one
+    // LDS slot of a 128-thread block (flux_tw(128) positions)
     if (2 * mi + 1 > flux_tw(128)) return fail(e, CGO_E_ARG, "grid_create: maxi %d above %d", mi, (flux_tw(128) - 1) / 2);
     const double* jarr[] = {d->rc, d->rc2, d->cv, d->cv2, d->rds, d->dsv, d->rdsv, d->ds, d->sv};
     const double* karr[] = {d->dt, d->rdz, d->rdza, d->dza, d->ssmax, d->dz};
