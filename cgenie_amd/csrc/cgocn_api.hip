@@ -55,7 +55,7 @@ struct GridH {
 struct MemberH {
     bool alive = false;
     int grid = -1, L = 0, prec = 0, bgc = 0;
-    int wstream = 4;  // particle-stream window of the BIOGEM march (4, 8 or 16)
+    int wstream = 4;  // particle-stream window of the BIOGEM march (4, 5, 6, 8 or 16)
     bool coupling_due = false;  // cgo_bio_step ran; cgo_tracercoupling must follow
     MemberDev dev{};
     cgo_member_params p{};
@@ -909,8 +909,13 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
         int span = 1;
         if (bp->dtyr * bp->sinkrate <= bp->dsc)
             for (int k = 2; k <= mk; k++) span = std::max(span, k - mink[k]);
-        int W = 4;
-        while (W < span + 1 && W < 16) W *= 2;
+        // the smallest compiled window (k_bgc_col<R, W>) that holds the span
+        int W = 16;
+        for (int w : {4, 5, 6, 8, 16})
+            if (w >= span + 1) {
+                W = w;
+                break;
+            }
         if (span + 1 > W)
             return fail(e, CGO_E_ARG, "bgc_init: particles sink through %d levels in one step (at most 15 supported)", span);
         m->wstream = W;

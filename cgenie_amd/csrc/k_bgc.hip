@@ -744,6 +744,10 @@ template <typename R>
 static void launch_col(int W, dim3 grid, const MemberDev* mems, const int32_t* list, hipStream_t s) {
     if (W <= 4)
         hipLaunchKernelGGL((k_bgc_col<R, 4>), grid, dim3(64), 0, s, mems, list);
+    else if (W <= 5)
+        hipLaunchKernelGGL((k_bgc_col<R, 5>), grid, dim3(64), 0, s, mems, list);
+    else if (W <= 6)
+        hipLaunchKernelGGL((k_bgc_col<R, 6>), grid, dim3(64), 0, s, mems, list);
     else if (W <= 8)
         hipLaunchKernelGGL((k_bgc_col<R, 8>), grid, dim3(64), 0, s, mems, list);
     else
