@@ -200,6 +200,34 @@ def test_biogem_fp64_matches_oracle(eng, kind, mld):
     eng.remove_member(m)
 
 
+@pytest.mark.parametrize("sinkrate", [19200.0, 57600.0, 76800.0, 192000.0, 500000.0])
+def test_biogem_stream_windows_match_oracle(eng, sinkrate):
+    """Sinking speeds (m/yr) whose per-step distance on the 16-level grid
+    selects each particle-stream window of the column march: 200, 600, 800
+    and 2000 m cross 2, 5, 6 and 10 levels (windows 4, 6, 8 and 16), and
+    5208 m leaves the column in one step (the single-source march)."""
+    say("bgc sinkrate", sinkrate)
+    c = make_case(kind="config2", seed=33)
+    m = new_member(eng, c)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000, sinkrate=sinkrate)
+    eng.bgc_init(m, p, c.light, c.mld)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset, sinkrate=sinkrate), c.light, c.mld)
+    n = 6
+    eng.step([m], n)
+    bio.step(n)
+    w = wet(c)
+    err = rel_err(eng.download(m, c.ts.shape), gold.ts, w)
+    say("  ts max rel err", err.max())
+    assert np.all(err <= 1e-12)
+    part = eng.bgc_download(m, 1, (c.g.maxk, c.g.maxj, c.g.maxi, 5))
+    pref = bio.part()
+    ep = np.abs(part - pref).max(axis=(0, 1, 2)) / (np.abs(pref).max(axis=(0, 1, 2)) + 1e-300)
+    say("  part err", ep)
+    assert np.all(ep <= 1e-12)
+    eng.remove_member(m)
+
+
 def test_batched_members_equal_alone(eng):
     """Members of different grids, tracer counts and parameters stepped in one
     launch per kernel give the bits each gives alone."""
