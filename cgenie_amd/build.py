@@ -14,6 +14,7 @@ All HIP code is built with packed FP32 arithmetic disabled
 from __future__ import annotations
 
 import concurrent.futures as cf
+import hashlib
 import os
 import shutil
 import subprocess
@@ -24,6 +25,17 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcgocn.so")
 OBJ = os.path.join(HERE, "_build")
 ARCH = os.environ.get("CGO_OFFLOAD_ARCH", "gfx950")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cgocn.h")
+
+
+def abi_stamp() -> str:
+    """Hash of include/cgocn.h.  The library is built with it (cgo_abi_info)
+    and the Python binding compares it with the header it was written
+    against, so a library built from another version of the header is refused
+    when it is loaded instead of being called with mismatched structs."""
+    with open(HEADER, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
@@ -44,7 +56,7 @@ def hipcc() -> str:
 
 def _compile(src: str, flags: list[str]) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *COMMON, *flags, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *COMMON, *flags, f'-DCGO_ABI_STAMP="{abi_stamp()}"', "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

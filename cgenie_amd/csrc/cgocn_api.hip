@@ -134,7 +134,21 @@ static int sync_table(cgo_engine* e) {
     return CGO_OK;
 }
 
+#ifndef CGO_ABI_STAMP
+#error "CGO_ABI_STAMP (hash of include/cgocn.h) is set by cgenie_amd/build.py"
+#endif
+
 extern "C" {
+
+int cgo_abi_info(int64_t* sizes, char* stamp, size_t n) {
+    if (!sizes || !stamp || n == 0) return CGO_E_ARG;
+    sizes[0] = (int64_t)sizeof(cgo_grid_desc);
+    sizes[1] = (int64_t)sizeof(cgo_member_params);
+    sizes[2] = (int64_t)sizeof(cgo_bgc_params);
+    sizes[3] = (int64_t)sizeof(cgo_diag);
+    snprintf(stamp, n, "%s", CGO_ABI_STAMP);
+    return CGO_OK;
+}
 
 int cgo_create(int32_t device, cgo_engine** out) {
     if (!out) return CGO_E_ARG;

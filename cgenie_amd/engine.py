@@ -67,6 +67,7 @@ def load() -> C.CDLL:
     P = C.c_void_p
     ip = C.POINTER(_i32)
     sig = {
+        "cgo_abi_info": ([C.POINTER(_i64), C.c_char_p, C.c_size_t], C.c_int),
         "cgo_create": ([_i32, C.POINTER(P)], C.c_int),
         "cgo_destroy": ([P], C.c_int),
         "cgo_last_error": ([P, C.c_char_p, C.c_size_t], C.c_int),
@@ -102,8 +103,27 @@ def load() -> C.CDLL:
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res
+    check_abi(L)
     _lib = L
     return L
+
+
+def check_abi(L: C.CDLL) -> None:
+    """Refuse a library built from another version of include/cgocn.h: its
+    struct sizes must equal the ctypes mirrors below and its build stamp the
+    hash of the header in this tree (cgenie_amd/build.py writes both)."""
+    from .build import abi_stamp
+
+    sizes = (_i64 * 4)()
+    stamp = C.create_string_buffer(64)
+    if L.cgo_abi_info(sizes, stamp, 64) != 0:
+        raise CgoError("cgo_abi_info failed")
+    want = [C.sizeof(GridDesc), C.sizeof(MemberParams), C.sizeof(BgcParams), C.sizeof(Diag)]
+    if list(sizes) != want:
+        raise CgoError(f"{LIB_PATH}: struct sizes {list(sizes)} differ from this binding's {want}; rebuild it")
+    if stamp.value.decode() != abi_stamp():
+        raise CgoError(f"{LIB_PATH} was built from another include/cgocn.h (stamp {stamp.value.decode()}, "
+                       f"header {abi_stamp()}); rebuild it with __graft_entry__.build()")
 
 
 def _ptr(a: np.ndarray):

@@ -87,6 +87,11 @@ typedef struct cgo_diag {
 } cgo_diag;
 
 /* lifecycle */
+/* Build check of the library against this header (no device call):
+ * sizes[0..3] = sizeof cgo_grid_desc, cgo_member_params, cgo_bgc_params,
+ * cgo_diag as the library was compiled; stamp = hash of this header at build
+ * time.  A binding compares both with its own view before any other call. */
+int cgo_abi_info(int64_t *sizes, char *stamp, size_t n);
 int cgo_create(int32_t device, cgo_engine **out);
 int cgo_destroy(cgo_engine *e);
 int cgo_last_error(const cgo_engine *e, char *buf, size_t n);

@@ -59,33 +59,55 @@ def build(force: bool = False) -> None:
 _lib = None
 
 
+def _bind(L: C.CDLL) -> C.CDLL:
+    L.og_grid_init.argtypes = [C.POINTER(OgGrid), _i, _i, _i, _i, _d, _d, _d, _d, _d]
+    L.og_goldstein_step.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp, C.c_void_p, _dp, _dp,
+                                    C.c_void_p, _i, C.POINTER(C.c_int32), C.POINTER(_d)]
+    L.og_inventory.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp]
+    L.og_calc_carbconst.argtypes = [_d, _d, _d, _dp]
+    L.og_calc_carb.argtypes = [_d] * 10 + [_dp, _dp, C.c_void_p, _d, _i]
+    L.og_calc_carb.restype = _i
+    L.og_calc_carb_RF0.argtypes = [_d] * 9 + [_dp, _dp, _d, _i]
+    L.og_calc_carb_RF0.restype = _i
+    for f in ("og_calc_Ca", "og_calc_Btot", "og_calc_SO4tot", "og_calc_Ftot"):
+        getattr(L, f).argtypes = [_d]
+        getattr(L, f).restype = _d
+    L.og_eos.argtypes = [_dp, _d, _d]
+    L.og_eos.restype = _d
+    L.og_bgc_default_params.argtypes = [C.POINTER(OgBgcParams), _d]
+    L.og_bgc_init.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i, _i32p,
+                              _dp, _dp, _dp]
+    L.og_bgc_free.argtypes = [C.POINTER(OgBgcState)]
+    L.og_biogem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i32p, _dp]
+    L.og_tracercoupling.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), _i32p, _dp, _dp]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(LIB)
-        L.og_grid_init.argtypes = [C.POINTER(OgGrid), _i, _i, _i, _i, _d, _d, _d, _d, _d]
-        L.og_goldstein_step.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp, C.c_void_p, _dp, _dp,
-                                        C.c_void_p, _i, C.POINTER(C.c_int32), C.POINTER(_d)]
-        L.og_inventory.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp]
-        L.og_calc_carbconst.argtypes = [_d, _d, _d, _dp]
-        L.og_calc_carb.argtypes = [_d] * 10 + [_dp, _dp, C.c_void_p, _d, _i]
-        L.og_calc_carb.restype = _i
-        L.og_calc_carb_RF0.argtypes = [_d] * 9 + [_dp, _dp, _d, _i]
-        L.og_calc_carb_RF0.restype = _i
-        for f in ("og_calc_Ca", "og_calc_Btot", "og_calc_SO4tot", "og_calc_Ftot"):
-            getattr(L, f).argtypes = [_d]
-            getattr(L, f).restype = _d
-        L.og_eos.argtypes = [_dp, _d, _d]
-        L.og_eos.restype = _d
-        L.og_bgc_default_params.argtypes = [C.POINTER(OgBgcParams), _d]
-        L.og_bgc_init.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i, _i32p,
-                                  _dp, _dp, _dp]
-        L.og_bgc_free.argtypes = [C.POINTER(OgBgcState)]
-        L.og_biogem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i32p, _dp]
-        L.og_tracercoupling.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), _i32p, _dp, _dp]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+class use_library:
+    """Context manager: route every oracle call to another build of
+    cgo_oracle.c (the mutation tests load copies with a planted mistake)."""
+
+    def __init__(self, path: str):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _bind(C.CDLL(self.path))
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def grid(maxi=36, maxj=36, maxk=16, nyear=96, yearlen=365.25, kappa_h=2000.0, kappa_v=2.7e-5,

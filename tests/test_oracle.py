@@ -442,3 +442,218 @@ def test_zonal_diffusion_discrete_eigenmode():
         got = o.ts[:, J - 1, :, 2] - ts[:, J - 1, :, 2]
         want = lam * np.cos(m * phi)[None, :]
         assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(lam) + 1e-15)
+
+
+def test_meridional_upwind_advection_shifts_one_row_at_courant_one():
+    """|pec| -> inf gives ups -> +-1, first-order upwind in the meridional
+    flux (goldstein.f90:2538-2546): with cv(j) v(j) dt rds(j) = 1 on every
+    interior face a zonally uniform profile moves one row north per step
+    (rds is the same for every row of the equal-sin grid, so the row flux
+    carries q ds); the southern row empties and the northern wall row keeps
+    what arrives.  With v < 0 it moves one row south the same way."""
+    c, og, k1, phi = _ring_case(1e-20)
+    mj = c.g.maxj
+    q = 1.0 + 0.4 * np.sin(np.arange(1, mj + 1) * 0.7) + 0.01 * np.arange(1, mj + 1)
+    ts = np.zeros_like(c.ts)
+    ts[..., 0] = 5.0
+    ts[..., 2] = q[None, :, None]
+    for sign in (1.0, -1.0):
+        u = np.zeros_like(c.u)
+        for J in range(1, mj):
+            u[:, J, :, 1] = sign / (og.cv[J] * og.dt[8] * og.rds[J])
+        o = O.GoldsteinOracle(og, k1, ts, u, diso=False)
+        o.step(1)
+        if sign > 0:
+            want = np.concatenate([[0.0], q[:-2], [q[-1] + q[-2]]])
+        else:
+            want = np.concatenate([[q[0] + q[1]], q[2:], [0.0]])
+        got = o.ts[..., 2]
+        assert np.abs(got - want[None, :, None]).max() <= 1e-13, sign
+
+
+def test_meridional_diffusion_sin_latitude_eigenvector():
+    """u = 0: the meridional diffusive flux -cv2(j) (q(j+1) - q(j)) diff(1)
+    (goldstein.f90:2545-2546) with cv2 = cv^2 rdsv.  On the equal-sin grid
+    q = s(j), the sine of latitude (the first Legendre mode), has
+    q(j+1) - q(j) = dsv(j), so the face flux is -cv(j)^2 diff(1), and
+    rds(j) (cv(j)^2 - cv(j-1)^2) = -2 s(j) because cv^2 = 1 - sv^2 and
+    s(j) = (sv(j) + sv(j-1)) / 2: one step changes q by -2 dt diff(1) s(j)
+    in every row, the polar rows included (cv = 0 at the poles)."""
+    c, og, k1, phi = _ring_case(2000.0)
+    g = c.g
+    ts = np.zeros_like(c.ts)
+    ts[..., 0] = 5.0
+    ts[..., 2] = g.s[1:g.maxj + 1][None, :, None]
+    o = O.GoldsteinOracle(og, k1, ts, np.zeros_like(c.u), diso=False)
+    o.step(1)
+    want = -2.0 * og.dt[8] * og.diff[0] * g.s[1:g.maxj + 1]
+    got = o.ts[..., 2] - ts[..., 2]
+    assert np.all(np.abs(got - want[None, :, None]) <= 1e-9 * np.abs(want)[None, :, None] + 1e-17)
+
+
+def _column_case(S, q):
+    """One wet column (k1 = 3) walled by land; T = 0 so the density is the
+    salinity term alone; kappa_h and kappa_v negligible."""
+    og = O.grid(8, 8, 16, kappa_h=1e-20, kappa_v=1e-20)
+    k1 = np.full((10, 10), G.LAND_K1, dtype=np.int32)
+    k1[4, 4] = 3
+    ts = np.zeros((16, 8, 8, 3))
+    ts[:, 3, 3, 1] = S
+    ts[:, 3, 3, 2] = q
+    return og, k1, ts
+
+
+def test_vertical_upwind_advection_empties_one_cell_at_courant_one():
+    """Vertical advective flux u(3) ((1-ups) q(k+1) + (1+ups) q(k)) / 2
+    (goldstein.f90:2548-2558) at |pec| -> inf: with w at the single face
+    above level k0 chosen so that w dt(k0) rdz(k0) = 1, all of q(k0) leaves
+    k0 and q(k0) dz(k0) arrives in k0+1; with w < 0 and
+    -w dt(k0+1) rdz(k0+1) = 1, all of q(k0+1) moves down into k0.  No other
+    level changes; the salinity stratification stays stable, so co leaves
+    the column alone."""
+    k0 = 8
+    rng = np.random.default_rng(4)
+    q = rng.uniform(0.5, 1.5, 16)
+    kk = np.arange(1, 17)
+    for up in (True, False):
+        if up:
+            S = np.where(kk < k0, 1.0 * (k0 - kk), np.where(kk == k0, 0.5, np.where(kk == k0 + 1, -1.0, -2.0 * (kk - k0))))
+        else:
+            S = np.where(kk < k0, 3.0 + (k0 - kk), np.where(kk == k0, 2.0, np.where(kk == k0 + 1, -0.5, -3.0 * (kk - k0))))
+        og, k1, ts = _column_case(S, q)
+        u = np.zeros((16, 9, 9, 3))
+        if up:
+            u[k0 - 1, 4, 4, 2] = 1.0 / (og.dt[k0] * og.rdz[k0])
+        else:
+            u[k0 - 1, 4, 4, 2] = -1.0 / (og.dt[k0 + 1] * og.rdz[k0 + 1])
+        o = O.GoldsteinOracle(og, k1, ts, u, diso=False)
+        o.step(1)
+        assert o.cost.sum() == 0.0
+        want = q.copy()
+        if up:
+            want[k0 - 1] = 0.0
+            want[k0] = q[k0] + q[k0 - 1] * (og.dt[k0 + 1] * og.rdz[k0 + 1]) / (og.dt[k0] * og.rdz[k0])
+        else:
+            want[k0] = 0.0
+            want[k0 - 1] = q[k0 - 1] + q[k0] * (og.dt[k0] * og.rdz[k0]) / (og.dt[k0 + 1] * og.rdz[k0 + 1])
+        got = o.ts[:, 3, 3, 2]
+        assert np.abs(got[2:] - want[2:]).max() <= 1e-13, up
+
+
+def test_caco3_remin_closed_form():
+    """A column whose only particulate is CaCO3 at the surface, with no
+    recalcitrant fraction (CaCO3_frac2 = 0), no light and no rain from
+    before.  The flux leaving level k is F0 exp(-(Dbot(k) - Dbot(maxk)) /
+    CaCO3_eL1) (biogem_box.f90:2577-2584, 2590-2596, 2604-2620), and each
+    mole of CaCO3 that dissolves returns one mole of DIC and two of ALK with
+    no O2 change (conv_sed_ocn, gem_util.f90:79-98), down to the level min_k
+    that one step of sinking reaches."""
+    c = make_case(kind="config2", seed=3)
+    g = c.g
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    p = O.bgc_params(1.0 / 96, c.tset, CaCO3_frac2=0.0)
+    bio = O.BiogemOracle(gold, p, np.zeros_like(c.light), c.mld)
+    k1 = c.k1[1:-1, 1:-1]
+    j, i = np.argwhere(k1 == k1.min())[0]
+    part = bio.arr("part", O.NSED).reshape(g.maxk, g.maxj, g.maxi, O.NSED)
+    part[:] = 0.0
+    X = 1.5e-6
+    part[g.maxk - 1, j, i, 2] = X            # CaCO3
+    bio.step(1, goldstein=False)
+    Dbot, dD = np.array(bio.st.Dbot), np.array(bio.st.dD)
+    mk, eL = g.maxk, p.CaCO3_eL1
+    assert eL != p.POC_eL1
+    reach = Dbot[mk] + p.dtyr * p.sinkrate
+    min_k = next(kk for kk in range(mk - 1, 0, -1) if Dbot[kk] > reach)
+    assert min_k >= k1[j, i]
+    F0 = X * dD[mk]
+    F = lambda kk: F0 * np.exp(-(Dbot[kk] - Dbot[mk]) / eL)
+    vd, t = bio.vdocn, c.tset
+    for kk in range(mk - 1, min_k - 1, -1):
+        Fin = F0 if kk == mk - 1 else F(kk + 1)
+        want = (Fin - F(kk)) / dD[kk]
+        dic, alk = vd[kk - 1, j, i, t.DIC], vd[kk - 1, j, i, t.ALK]
+        assert abs(dic - want) <= 1e-11 * abs(want), (kk, dic, want)
+        assert abs(alk - 2.0 * want) <= 1e-11 * abs(want), (kk, alk, want)
+        assert vd[kk - 1, j, i, t.O2] == 0.0 and vd[kk - 1, j, i, t.PO4] == 0.0
+    for kk in range(min_k - 1, k1[j, i] - 1, -1):
+        assert vd[kk - 1, j, i, t.DIC] == 0.0 and vd[kk - 1, j, i, t.ALK] == 0.0
+    left = bio.part()[min_k - 1, j, i, 2]
+    assert abs(left - F(min_k) / dD[min_k]) <= 1e-11 * left
+
+
+def _coupling_case(seed=13):
+    c = make_case(18, 18, 8, kind="config2", seed=seed)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset), c.light, c.mld)
+    g = c.g
+    M = lambda: bio.arr("M").reshape(g.maxk, g.maxj, g.maxi).copy()  # noqa: E731
+    V = bio.arr("V").reshape(g.maxk, g.maxj, g.maxi).copy()
+    return c, gold, bio, M, V
+
+
+def _couple(bio, gold, vd):
+    O.lib().og_tracercoupling(C.byref(bio.st), C.byref(gold.g), gold.k1, gold.ts, np.ascontiguousarray(vd))
+
+
+def test_coupling_identity_and_single_cell_anomaly():
+    """biogem_tracercoupling (biogem.f90:1885-2077) on a state whose ocn is
+    the normalised go_ts (ocn = go_ts S / S-bar, as after initialisation).
+    With zero anomalies the de-normalised inventories NEW equal OLD, S-bar
+    is unchanged, and the re-normalisation S-bar/S (2046) undoes the
+    de-normalisation by the local S (1975-1981): go_ts and ocn come back
+    unchanged.  An anomaly d of one tracer in one cell then changes ocn there
+    by d and go_ts by d S-bar / S, and nothing else."""
+    c, gold, bio, M, V = _coupling_case()
+    w, L, t = wet(c), c.tset.L, c.tset
+    ts0, ocn0 = gold.ts.copy(), bio.ocn()
+    S = ocn0[..., 1]
+    Sbar = (S * V)[w].sum() / V[w].sum()
+    # the initial ocn is the normalised go_ts
+    for l in range(2, L):
+        assert np.allclose(ocn0[..., l][w], (ts0[..., l] * S / Sbar)[w], rtol=1e-13, atol=0)
+    _couple(bio, gold, np.zeros_like(ts0))
+    # T and S go through the unit offsets (+273.15, +34.9 and back): an ulp of those
+    assert np.abs(gold.ts[..., :2] - ts0[..., :2])[w].max() <= 2e-13
+    assert np.all(np.abs(gold.ts[..., 2:] - ts0[..., 2:])[w] <= 1e-14 * np.abs(ts0[..., 2:])[w] + 1e-18)
+    assert np.all(np.abs(bio.ocn() - ocn0)[w] <= 1e-14 * np.abs(ocn0)[w] + 1e-18)
+    # one anomaly of a dye in one cell
+    kz, j, i = np.argwhere(w)[len(np.argwhere(w)) // 2]
+    l, d = t.L - 1, 3e-3
+    vd = np.zeros_like(ts0)
+    vd[kz, j, i, l] = d
+    ts1, ocn1 = gold.ts.copy(), bio.ocn()
+    _couple(bio, gold, vd)
+    dts, docn = gold.ts - ts1, bio.ocn() - ocn1
+    assert abs(docn[kz, j, i, l] - d) <= 1e-12 * d
+    assert abs(dts[kz, j, i, l] - d * Sbar / ocn1[kz, j, i, 1]) <= 1e-12 * d
+    dts[kz, j, i, l] = 0.0
+    assert np.abs(dts[..., :2])[w].max() <= 2e-13
+    assert np.all(np.abs(dts[..., 2:])[w] <= 1e-14 * np.abs(ts1[..., 2:])[w] + 1e-18)
+
+
+def test_coupling_uniform_salinity_anomaly():
+    """A uniform salinity anomaly dS: the mean salinity moves to S-bar + dS,
+    every tracer q >= 3 of ocn is scaled by Sratio = S-bar_new / S-bar_old
+    (2037-2040), go_ts becomes (S-bar_new / S_new) ocn_new (2046), and the
+    cell masses are divided by Sratio (2049-2052)."""
+    c, gold, bio, M, V = _coupling_case(17)
+    w, L = wet(c), c.tset.L
+    ts0, ocn0, M0 = gold.ts.copy(), bio.ocn(), M()
+    S = ocn0[..., 1]
+    Sold = (S * V)[w].sum() / V[w].sum()
+    dS = 0.37
+    vd = np.zeros_like(ts0)
+    vd[..., 1][w] = dS
+    _couple(bio, gold, vd)
+    Snew = Sold + dS
+    ratio = Snew / Sold
+    ocn1 = bio.ocn()
+    assert np.allclose(ocn1[..., 1][w], (S + dS)[w], rtol=1e-14, atol=0)
+    assert np.abs(gold.ts[..., 1][w] - (ts0[..., 1] + dS)[w]).max() <= 2e-13
+    for l in range(2, L):
+        a = ocn0[..., l][w]
+        assert np.all(np.abs(ocn1[..., l][w] - ratio * a) <= 1e-13 * np.abs(a) + 1e-20), l
+        want = (Snew / (S + dS))[w] * ratio * a
+        assert np.all(np.abs(gold.ts[..., l][w] - want) <= 1e-13 * np.abs(want) + 1e-20), l
+    assert np.allclose(M()[w], M0[w] / ratio, rtol=1e-14, atol=0)

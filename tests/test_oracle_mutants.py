@@ -1,0 +1,132 @@
+"""The oracle's pins catch planted mistakes.
+
+Each mutant below is a copy of oracle/cgo_oracle.c with one plausible
+mistake planted at a cited line of the restatement (a wrong metric index,
+swapped upwind weights, a dropped term or factor).  It is compiled on the
+CPU into a scratch library, every oracle call is routed to it, and the pin
+tests of tests/test_oracle.py run against it: at least one must fail.  A
+mutant whose text is not found fails this test too, so the list cannot go
+stale silently when the oracle changes.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+
+import pytest
+
+import oracle_ffi as O
+import test_oracle as T
+
+SRC = os.path.join(O.ORACLE_DIR, "cgo_oracle.c")
+
+# (name, text in cgo_oracle.c, replacement, occurrences)
+MUTANTS = [
+    # tstepo_flux, goldstein.f90:2525-2558
+    ("meridional diffusion metric cv2(j-1)",
+     "fn[l - 1] = fn[l - 1] - g->cv2[j] *", "fn[l - 1] = fn[l - 1] - g->cv2[j - 1] *", 1),
+    ("meridional upwind weights swapped",
+     "((1.0 - ups[2]) * TS1(c, l, i, j + 1, k) + (1.0 + ups[2]) * TS1(c, l, i, j, k))",
+     "((1.0 + ups[2]) * TS1(c, l, i, j + 1, k) + (1.0 - ups[2]) * TS1(c, l, i, j, k))", 1),
+    ("vertical upwind weights swapped",
+     "((1.0 - ups[3]) * TS1(c, l, i, j, k + 1) + (1.0 + ups[3]) * TS1(c, l, i, j, k))",
+     "((1.0 + ups[3]) * TS1(c, l, i, j, k + 1) + (1.0 - ups[3]) * TS1(c, l, i, j, k))", 1),
+    ("zonal upwind weights swapped",
+     "((1.0 - ups[1]) * TS1(c, l, i + 1, j, k) + (1.0 + ups[1]) * TS1(c, l, i, j, k))",
+     "((1.0 + ups[1]) * TS1(c, l, i + 1, j, k) + (1.0 - ups[1]) * TS1(c, l, i, j, k))", 1),
+    ("zonal diffusion metric rc instead of rc2",
+     "(TS1(c, l, i + 1, j, k) - TS1(c, l, i, j, k)) * g->rc2[j] * diff1",
+     "(TS1(c, l, i + 1, j, k) - TS1(c, l, i, j, k)) * g->rc[j] * diff1", 1),
+    ("vertical diffusion metric rdz instead of rdza",
+     "(TS1(c, l, i, j, k + 1) - TS1(c, l, i, j, k)) * g->rdza[k] * diffv",
+     "(TS1(c, l, i, j, k + 1) - TS1(c, l, i, j, k)) * g->rdz[k] * diffv", 1),
+    ("meridional update metric rdsv instead of rds",
+     "(fn[l - 1] - FS(l, i)) * g->rds[j]", "(fn[l - 1] - FS(l, i)) * g->rdsv[j]", 1),
+    ("isoneutral factor 2 dropped",
+     "tv = tv + (2 * dzrho * DXTS(l, ina)", "tv = tv + (dzrho * DXTS(l, ina)", 1),
+    # co, goldstein.f90:2732-2737
+    ("convective mix not thickness weighted",
+     "sum[l - 1] = TS(c, l, i, j, kk[m]) * dzm[kk[m]];", "sum[l - 1] = TS(c, l, i, j, kk[m]) * dzm[kk[m - 1]];", 1),
+    # sub_box_remin_part, biogem_box.f90:2555-2596, 2779-2792; gem_util.f90:79-98
+    ("CaCO3 dissolution returns 1 ALK",
+     "cv->c[OG_S_CACO3][3] = 2.0;", "cv->c[OG_S_CACO3][3] = 1.0;", 1),
+    ("CaCO3 e-folding length from POC",
+     "double f1 = (1.0 - exp(-dD / p->CaCO3_eL1));", "double f1 = (1.0 - exp(-dD / p->POC_eL1));", 1),
+    ("layer thickness ratio inverted",
+     "const double layerratio = st->dD[kk + 1] / st->dD[kk];",
+     "const double layerratio = st->dD[kk] / st->dD[kk + 1];", 1),
+    ("POC O2 demand without the POP share",
+     "p->red_POP_PO2 / p->red_POP_POC - cO2POP / p->red_POP_POC", "p->red_POP_PO2 / p->red_POP_POC", 1),
+    # biogem_tracercoupling, biogem.f90:1975-2061
+    ("coupling de-normalisation without the local salinity",
+     "go_ts[O3(st, l, i, j, k)] * st->ocn[O3(st, 1, i, j, k)] * rSold;", "go_ts[O3(st, l, i, j, k)] * rSold;", 1),
+    ("coupling re-normalisation dropped",
+     "(Snew / st->ocn[O3(st, 1, i, j, k)]) * st->ocn[O3(st, l, i, j, k)];", "st->ocn[O3(st, l, i, j, k)];", 1),
+    ("coupling salinity ratio on the tracers dropped",
+     "st->ocn[O3(st, l, i, j, k)] = Sratio * v;", "st->ocn[O3(st, l, i, j, k)] = v;", 1),
+    ("cell masses scaled by Sratio instead of 1/Sratio",
+     "st->M[C3(st, i, j, k)] = rSratio * st->M[C3(st, i, j, k)];",
+     "st->M[C3(st, i, j, k)] = Sratio * st->M[C3(st, i, j, k)];", 1),
+]
+
+# the pins (tests/test_oracle.py), cheapest first; the compiled-reference
+# comparison is left out (it needs oracle/_ref), its golden vectors are in
+PINS = [
+    T.test_eos_polynomial, T.test_grid_closed_forms, T.test_carbconst_closed_form_mehrbach,
+    T.test_carbonate_root_satisfies_alkalinity, T.test_convection_mixes_inversion_to_thickness_weighted_mean,
+    T.test_vertical_diffusion_matches_textbook_column, T.test_vertical_upwind_advection_empties_one_cell_at_courant_one,
+    T.test_zonal_upwind_advection_shifts_one_cell_at_courant_one, T.test_zonal_diffusion_discrete_eigenmode,
+    T.test_meridional_upwind_advection_shifts_one_row_at_courant_one,
+    T.test_meridional_diffusion_sin_latitude_eigenvector, T.test_uniform_tracer_stays_uniform,
+    T.test_oracle_random_velocity_conserves, T.test_isoneutral_flux_closed_form,
+    T.test_biogem_oracle_uptake_closed_form, T.test_dom_remin_closed_form, T.test_particulate_remin_closed_form,
+    T.test_caco3_remin_closed_form, T.test_coupling_identity_and_single_cell_anomaly,
+    T.test_coupling_uniform_salinity_anomaly, lambda: T.test_biogem_oracle_conserves_phosphorus("config2"),
+    lambda: T.test_carbchem_matches_reference_vectors(0), T.test_oracle_conserves_inventory,
+]
+
+
+def _build(tmp, idx, name, find, repl, count):
+    src = open(SRC).read()
+    n = src.count(find)
+    if n != count:
+        return idx, None, f"{name}: text found {n} times, expected {count}"
+    d = os.path.join(tmp, f"m{idx}")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "cgo_oracle.c"), "w") as f:
+        f.write(src.replace(find, repl))
+    with open(os.path.join(O.ORACLE_DIR, "cgo_oracle.h")) as h, open(os.path.join(d, "cgo_oracle.h"), "w") as f:
+        f.write(h.read())
+    so = os.path.join(d, "libmut.so")
+    r = subprocess.run(["gcc", "-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-shared", "-o", so,
+                        os.path.join(d, "cgo_oracle.c"), "-lm"], capture_output=True, text=True)
+    if r.returncode != 0:
+        return idx, None, f"{name}: gcc failed: {r.stderr[-400:]}"
+    return idx, so, None
+
+
+@pytest.fixture(scope="module")
+def mutant_libs(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("mutants"))
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        res = list(ex.map(lambda a: _build(tmp, *a), [(i, *m) for i, m in enumerate(MUTANTS)]))
+    return {i: (so, err) for i, so, err in res}
+
+
+def test_unmutated_oracle_passes_every_pin():
+    for pin in PINS:
+        pin()
+
+
+@pytest.mark.parametrize("idx", range(len(MUTANTS)), ids=[m[0] for m in MUTANTS])
+def test_pins_catch_planted_mistake(mutant_libs, idx):
+    so, err = mutant_libs[idx]
+    assert err is None, err
+    caught = None
+    with O.use_library(so):
+        for pin in PINS:
+            try:
+                pin()
+            except Exception as ex:  # an AssertionError, or the pin's own arithmetic failing on bad output
+                caught = (getattr(pin, "__name__", "pin"), type(ex).__name__)
+                break
+    assert caught is not None, f"no pin fails with the mistake planted: {MUTANTS[idx][0]}"
