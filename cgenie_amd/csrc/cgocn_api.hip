@@ -68,6 +68,12 @@ struct Graph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int32_t* d_lists = nullptr;
+    // the captured members and the generation of their slots at capture:
+    // launch extents, precision lists and stream windows are baked into the
+    // graph, so a replay is refused once any of them was destroyed,
+    // re-created or re-laid-out
+    std::vector<int32_t> ids;
+    std::vector<uint64_t> gens;
 };
 
 // host copy of the device carry weight: 2^-16 ulp(x)
@@ -89,6 +95,10 @@ struct cgo_engine {
     std::vector<MemberH*> members;
     MemberDev* d_table = nullptr;
     bool table_dirty = false;
+    // per member slot: bumped whenever the slot's member is created,
+    // destroyed or re-laid-out (BIOGEM state added), so captured graphs can
+    // tell that a slot no longer holds what they captured
+    std::vector<uint64_t> gen;
     // flux tile per precision (FP64, FP32C): the shapes the autotuner picks
     // for a large ensemble on MI355X; cgo_set_tile and its companions change them
     int lb[2] = {8, 6};       // tracers per block
@@ -124,8 +134,16 @@ static MemberH* member(cgo_engine* e, int32_t m) {
     return e->members[m];
 }
 
+static void bump_gen(cgo_engine* e, int32_t slot) {
+    if ((int32_t)e->gen.size() <= slot) e->gen.resize(slot + 1, 0);
+    e->gen[slot]++;
+}
+
 static int sync_table(cgo_engine* e) {
     if (!e->table_dirty) return CGO_OK;
+    // work already enqueued (an eager step or a graph replay) reads the
+    // table: let it finish before the entries are rewritten
+    HIPCHK(e, hipStreamSynchronize(e->s));
     std::vector<MemberDev> t(e->members.size());
     for (size_t i = 0; i < e->members.size(); i++)
         if (e->members[i] && e->members[i]->alive) t[i] = e->members[i]->dev;
@@ -542,6 +560,7 @@ int cgo_member_create(cgo_engine* e, int32_t grid_id, int32_t maxl, int32_t prec
     if (rc) return rc;
     m->alive = true;
     e->table_dirty = true;
+    bump_gen(e, slot);
     *member_id = slot;
     return CGO_OK;
 }
@@ -554,6 +573,7 @@ int cgo_member_destroy(cgo_engine* e, int32_t id) {
     m->dbuf = nullptr;
     m->alive = false;
     e->table_dirty = true;
+    bump_gen(e, id);
     return CGO_OK;
 }
 
@@ -794,6 +814,7 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
     if (!m->bgc) {
         m->bgc = 1;
         rc = alloc_member(e, m);
+        bump_gen(e, id);  // new layout, whatever follows
         if (rc) return rc;
     }
     m->bp = *bp;
@@ -951,6 +972,7 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
     const int32_t zero = 0;
     HIPCHK(e, hipMemcpy(d.sfresh, &zero, 4, hipMemcpyHostToDevice));
     e->table_dirty = true;
+    bump_gen(e, id);
     return CGO_OK;
 }
 
@@ -1204,6 +1226,8 @@ int cgo_graph_capture(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nste
     if (rc) return rc;
     if ((rc = sync_table(e))) return rc;
     Graph gr;
+    gr.ids.assign(ids, ids + n);
+    for (int i = 0; i < n; i++) gr.gens.push_back(ids[i] < (int32_t)e->gen.size() ? e->gen[ids[i]] : 0);
     HIPCHK(e, hipMalloc(&gr.d_lists, ls.v.size() * 4));
     HIPCHK(e, hipMemcpy(gr.d_lists, ls.v.data(), ls.v.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipStreamSynchronize(e->s));
@@ -1235,8 +1259,23 @@ int cgo_graph_capture(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nste
 
 int cgo_graph_launch(cgo_engine* e, int32_t gid, int32_t nreps) {
     if (!e || gid < 0 || gid >= (int)e->graphs.size() || !e->graphs[gid].exec) return fail(e, CGO_E_ARG, "graph_launch: bad graph");
-    if (e->table_dirty) return fail(e, CGO_E_STATE, "graph_launch: members changed since capture; capture again");
-    for (int r = 0; r < nreps; r++) HIPCHK(e, hipGraphLaunch(e->graphs[gid].exec, e->s));
+    if (nreps < 0) return fail(e, CGO_E_ARG, "graph_launch: nreps %d", nreps);
+    const Graph& gr = e->graphs[gid];
+    // every captured member must still be the one captured (same slot
+    // generation: not destroyed, re-created or re-laid-out since) and must
+    // not be between cgo_bio_step and cgo_tracercoupling
+    for (size_t i = 0; i < gr.ids.size(); i++) {
+        const int32_t id = gr.ids[i];
+        if (!member(e, id) || id >= (int32_t)e->gen.size() || e->gen[id] != gr.gens[i])
+            return fail(e, CGO_E_STATE, "graph_launch: member %d was destroyed, re-created or re-initialised since the capture; capture again", id);
+    }
+    int rc = check_not_due(e, gr.ids.data(), (int32_t)gr.ids.size(), "graph_launch");
+    if (rc) return rc;
+    hipSetDevice(e->device);
+    // the graph reads the member table at run time: bring it up to date
+    // (e.g. a surface-flux buffer attached since the capture)
+    if ((rc = sync_table(e))) return rc;
+    for (int r = 0; r < nreps; r++) HIPCHK(e, hipGraphLaunch(gr.exec, e->s));
     return CGO_OK;
 }
 

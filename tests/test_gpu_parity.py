@@ -278,6 +278,60 @@ def test_graph_replay_equals_eager(eng):
     eng.remove_member(m2)
 
 
+def test_graph_replay_refused_after_member_changes(eng):
+    """A captured graph bakes in its members' launch extents, precision
+    lists and stream windows.  Its replay is refused with CGO_E_STATE, before
+    any kernel is enqueued, while a captured member is between bio_step and
+    tracercoupling, and for good once a captured member was destroyed (also
+    after an eager step has brought the member table up to date), its slot
+    re-used by a new member, or re-initialised."""
+    c = make_case(18, 18, 8, kind="config2", seed=31)
+    p = E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000)
+    a, b = new_member(eng, c), new_member(eng, c)
+    for m in (a, b):
+        eng.bgc_init(m, p, c.light, c.mld)
+    g = eng.capture([a, b], 2)
+    eng.launch(g, 1)
+    eng.sync()
+    eng.bio_step([a])
+    with pytest.raises(E.CgoError, match="rc=-3"):
+        eng.launch(g, 1)
+    eng.tracercoupling([a])
+    eng.launch(g, 1)
+    eng.sync()
+
+    def refused():
+        eng.sync()
+        before, steps = eng.download(a, c.ts.shape), eng.diag(a).steps
+        with pytest.raises(E.CgoError, match="rc=-3"):
+            eng.launch(g, 1)
+        eng.sync()
+        assert eng.diag(a).steps == steps
+        assert np.array_equal(eng.download(a, c.ts.shape), before)
+
+    eng.remove_member(b)
+    eng.step([a], 1)          # an eager step syncs the member table
+    refused()
+    # the next members created fill the free slots, b's among them
+    extra = []
+    while True:
+        extra.append(new_member(eng, c, prec=E.FP32C))
+        if extra[-1] == b:
+            break
+        assert len(extra) < 4096
+    refused()
+    eng.drop_graph(g)
+    g2 = eng.capture([a, b], 1)   # a fresh capture of the new member set replays
+    eng.launch(g2, 1)
+    eng.sync()
+    eng.bgc_init(b, p, c.light, c.mld)
+    with pytest.raises(E.CgoError, match="rc=-3"):
+        eng.launch(g2, 1)
+    eng.drop_graph(g2)
+    for m in [a] + extra:
+        eng.remove_member(m)
+
+
 @pytest.mark.parametrize("carry", [True, False])
 def test_fp32_form_tracks_fp64(eng, carry):
     c = make_case(kind="config2", seed=61)
