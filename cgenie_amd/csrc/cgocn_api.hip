@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -113,6 +114,19 @@ struct cgo_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+// Device memory of cgo_inventory_noalloc: the per-tracer sums of one call,
+// in place of the buffer cgo_inventory allocates.  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code never uses
+// the array's name as an address: the device address of each GPU's copy is
+// taken once per device, with that device current, by
+// hipGetSymbolAddress (noalloc_setup, called from cgo_create), and only
+// that address reaches kernels and copies.
+constexpr int NOALLOC_MAXL = 64;  // tracers one allocation-free inventory takes
+static __device__ __align__(256) double scratch[NOALLOC_MAXL];
+constexpr int NOALLOC_MAXDEV = 64;
+static double* noalloc_ptr[NOALLOC_MAXDEV] = {};
+static std::mutex noalloc_mu;
+
 static int fail(cgo_engine* e, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -168,6 +182,18 @@ int cgo_abi_info(int64_t* sizes, char* stamp, size_t n) {
     return CGO_OK;
 }
 
+// the device address of this device's scratch array, once per device
+// (the device is current); false if the runtime cannot give it
+static bool noalloc_setup(int device) {
+    if (device >= NOALLOC_MAXDEV) return true;  // such devices just have no allocation-free path
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return true;
+    double* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return false;
+    noalloc_ptr[device] = ptr;
+    return true;
+}
+
 int cgo_create(int32_t device, cgo_engine** out) {
     if (!out) return CGO_E_ARG;
     *out = nullptr;
@@ -180,7 +206,7 @@ int cgo_create(int32_t device, cgo_engine** out) {
         hipMalloc(&e->d_table, TABLE_CAP * sizeof(MemberDev)) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess || !noalloc_setup(device)) {
         delete e;
         return CGO_E_HIP;
     }
@@ -739,6 +765,30 @@ int cgo_inventory(cgo_engine* e, int32_t id, double* inv) {
     HIPCHK(e, hipStreamSynchronize(e->s));
     HIPCHK(e, hipMemcpy(inv, d_out, (size_t)m->L * 8, hipMemcpyDeviceToHost));
     hipFree(d_out);
+    return CGO_OK;
+}
+
+int cgo_inventory_noalloc(cgo_engine* e, int32_t id, double* inv) {
+    MemberH* m = member(e, id);
+    if (!m || !inv) return fail(e, CGO_E_ARG, "inventory_noalloc: bad argument");
+    if (m->L > NOALLOC_MAXL)
+        return fail(e, CGO_E_ARG, "inventory_noalloc: member %d has %d tracers, at most %d without allocation", id, m->L,
+                    NOALLOC_MAXL);
+    double* ptr = e->device < NOALLOC_MAXDEV ? noalloc_ptr[e->device] : nullptr;
+    if (!ptr) return fail(e, CGO_E_STATE, "inventory_noalloc: no scratch address for device %d", e->device);
+    HIPCHK(e, hipSetDevice(e->device));
+    if (e->table_dirty) {
+        // bring this member's table entry up to date from the host copy (the
+        // whole-table refresh would build a host vector); work in flight
+        // reads the table, so it finishes first
+        HIPCHK(e, hipStreamSynchronize(e->s));
+        HIPCHK(e, hipMemcpy(e->d_table + id, &m->dev, sizeof(MemberDev), hipMemcpyHostToDevice));
+    }
+    // every one of the member's L sums is written by the launch: nothing to clear
+    launch_inventory(e->d_table, id, m->L, ptr, e->grids[m->grid]->d_invw, m->prec, e->s);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(inv, ptr, (size_t)m->L * 8, hipMemcpyDeviceToHost));
     return CGO_OK;
 }
 

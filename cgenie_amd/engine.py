@@ -82,6 +82,7 @@ def load() -> C.CDLL:
         "cgo_download_state": ([P, _i32, _dp], C.c_int),
         "cgo_download_rho": ([P, _i32, _dp], C.c_int),
         "cgo_inventory": ([P, _i32, _dp], C.c_int),
+        "cgo_inventory_noalloc": ([P, _i32, _dp], C.c_int),
         "cgo_get_diag": ([P, _i32, C.POINTER(Diag)], C.c_int),
         "cgo_bgc_default_params": ([C.POINTER(BgcParams), _d], None),
         "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
@@ -243,6 +244,12 @@ class Engine:
     def inventory(self, m: int) -> np.ndarray:
         out = np.zeros(self.members[m])
         self._chk(self.L.cgo_inventory(self.h, m, _ptr(out)), "inventory")
+        return out
+
+    def inventory_noalloc(self, m: int) -> np.ndarray:
+        """cgo_inventory through the library's static scratch (no device allocation)."""
+        out = np.zeros(self.members[m])
+        self._chk(self.L.cgo_inventory_noalloc(self.h, m, _ptr(out)), "inventory_noalloc")
         return out
 
     def diag(self, m: int) -> Diag:

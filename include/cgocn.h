@@ -111,6 +111,13 @@ int cgo_upload_state(cgo_engine *e, int32_t m, const double *go_ts);    /* go_ts
 int cgo_download_state(cgo_engine *e, int32_t m, double *go_ts);
 int cgo_download_rho(cgo_engine *e, int32_t m, double *rho);            /* rho(1:maxi,1:maxj,1:maxk) */
 int cgo_inventory(cgo_engine *e, int32_t m, double *inv);               /* maxl, diag2 weights */
+/* cgo_inventory without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The sums go through a static
+ * device array of 64 doubles per GPU (members with more tracers get
+ * CGO_E_ARG); concurrent calls on one GPU, from any engine, share that array
+ * and race. */
+int cgo_inventory_noalloc(cgo_engine *e, int32_t m, double *inv);
 int cgo_get_diag(cgo_engine *e, int32_t m, cgo_diag *d);
 
 /* BIOGEM: state from the current go_ts (ocn = ts*S/Smean, T+273.15,
