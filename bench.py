@@ -53,9 +53,13 @@ def ensemble_inputs(n_members: int, seed: int = 2024, kind: str = "config2"):
     return g, k1, tset, ts0, light, mld, members
 
 
-def algorithmic_bytes_per_member_step(n_wet: int, L: int, s_state: int, s_vel: int) -> int:
-    """Flux kernel: read and write every tracer once, read u, v, w, write rho."""
-    return n_wet * (2 * L * s_state + 3 * s_vel + 8)
+def algorithmic_bytes_per_member_step(n_wet: int, n_col: int, L: int, s_state: int, s_vel: int,
+                                      s_rho: int = 8) -> int:
+    """Bytes one ocean step of one member must move at least: read and write
+    every tracer of every wet cell once (ping-pong, no copy), read u, v and w
+    and write rho once per wet cell, and read the surface-flux slot of every
+    tracer of every wet column."""
+    return n_wet * (2 * L * s_state + 3 * s_vel + s_rho) + n_col * L * s_state
 
 
 def build_ensemble(eng, E, n, precision, carry=True, seed=2024):
@@ -69,14 +73,25 @@ def build_ensemble(eng, E, n, precision, carry=True, seed=2024):
         bp = E.default_bgc_params(1.0 / STEPS_PER_YEAR, tset, carb_tol=1e-12, carb_maxit=1000, POC_eL1=p["eL1"])
         eng.bgc_init(m, bp, light, mld)
         ids.append(m)
-    n_wet = int(((np.arange(1, 17)[:, None, None]) >= k1[None, 1:37, 1:37]).sum())
-    return ids, tset, n_wet, g
+    wet = (np.arange(1, 17)[:, None, None]) >= k1[None, 1:37, 1:37]
+    n_wet = int(wet.sum())
+    n_col = int((k1[1:37, 1:37] <= 16).sum())
+    return ids, tset, n_wet, n_col, g
+
+
+def replay_steps(eng, year_graph, rest_graph, nsteps):
+    """Launch exactly nsteps ocean steps: whole years on the one-year graph,
+    the remainder on a graph of that many steps."""
+    q, r = divmod(nsteps, STEPS_PER_YEAR)
+    if q:
+        eng.launch(year_graph, q)
+    if r:
+        eng.launch(rest_graph, 1)
 
 
 def timed_years(eng, graph, years):
-    """Replay the one-year graph; return per-year wall seconds."""
-    import ctypes as C
-
+    """Replay the one-year graph `years` times, one year at a time; per-year
+    wall seconds (each year ends with a synchronisation)."""
     out = []
     for _ in range(years):
         t0 = time.perf_counter()
@@ -96,7 +111,7 @@ def run_probe(args):
 
     prec = E.FP32C if args.probe == "fp32c" else E.FP64
     eng = E.Engine(0)
-    ids, tset, n_wet, g = build_ensemble(eng, E, args.members, prec)
+    ids, tset, n_wet, n_col, g = build_ensemble(eng, E, args.members, prec)
     eng.set_tile(prec, args.probe_tb, args.probe_cols, args.probe_ring, args.probe_iso)
     eng.step(ids, args.steps)
     eng.sync()
@@ -126,53 +141,77 @@ def measured_traffic(shape: str):
     return None
 
 
+def profile_figure(path: str, key: str):
+    """A figure `key = value` from a committed rocprofv3 summary, or None."""
+    try:
+        for line in open(path):
+            if line.startswith(key + " ="):
+                return float(line.split("=")[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
+BGC_FLOPS_SUMMARY = os.path.join(ROOT, "profiles", "fp64_counters_k_bgc_col_36x36x16x24.txt")
+
+
 def run_gpu(args):
     from cgenie_amd import engine as E
+    from cgenie_amd import autotune
 
     eng = E.Engine(0)
     res = {}
     n = args.members
-    years = max(1, args.steps // STEPS_PER_YEAR)
-    warm_years = max(1, args.warmup // STEPS_PER_YEAR) if args.warmup > 0 else 0
-    from cgenie_amd import autotune
-
+    K, W = args.steps, args.warmup
     tuned = {}
     for prec, tag in ((E.FP32C, "fp32c"), (E.FP64, "fp64")):
-        ids, tset, n_wet, g = build_ensemble(eng, E, n, prec)
+        ids, tset, n_wet, n_col, g = build_ensemble(eng, E, n, prec)
         tuned[tag] = autotune.tune(eng, ids, (36, 36, 16), tset.L, prec, use_cache=False, write_cache=False)
         print(f"# autotune {tag}: {tuned[tag]}", file=sys.stderr, flush=True)
-        graph = eng.capture(ids, STEPS_PER_YEAR)
-        if warm_years:
-            eng.launch(graph, warm_years)
+        gy = eng.capture(ids, STEPS_PER_YEAR)
+        rest = {r: eng.capture(ids, r) for r in {K % STEPS_PER_YEAR, W % STEPS_PER_YEAR} if r}
+        replay_steps(eng, gy, rest.get(W % STEPS_PER_YEAR), W)       # W untimed warm-up steps
         eng.sync()
-        per_year = timed_years(eng, graph, years)
-        tot = sum(per_year)
+        t0 = time.perf_counter()
+        replay_steps(eng, gy, rest.get(K % STEPS_PER_YEAR), K)       # exactly K timed steps
+        eng.sync()
+        tot = time.perf_counter() - t0
+        per_year = timed_years(eng, gy, args.latency_years)
+        print(f"# {tag}: {K} steps in {tot:.3f} s; {args.latency_years} replayed years", file=sys.stderr, flush=True)
         flux_ms = eng.time_kernel(0, ids, 20)
         flux_step_ms = eng.time_kernel(3, ids, 20)  # beside the BIOGEM stream, as in the step
         co_ms = eng.time_kernel(1, ids, 20)
-        bgc_ms = eng.time_kernel(2, ids, 5)
+        bgc_cpl_ms = eng.time_kernel(2, ids, 5)
         d = eng.diag(ids[0])
-        s_state = 8 if prec == E.FP64 else 6
+        carb_fail = int(d.carb_fail)
+        bgc_ms = eng.time_kernel(4, ids, 5)         # last: leaves the BIOGEM arrays for timing only
+        s_state = 8 if prec == E.FP64 else 4
         s_vel = 8 if prec == E.FP64 else 4
-        B = algorithmic_bytes_per_member_step(n_wet, tset.L, s_state, s_vel) * n
-        res[tag] = dict(member_years_per_s=n * years / tot, ms_per_step=1e3 * tot / (years * STEPS_PER_YEAR),
+        B = algorithmic_bytes_per_member_step(n_wet, n_col, tset.L, s_state, s_vel) * n
+        Bc = algorithmic_bytes_per_member_step(n_wet, n_col, tset.L, s_state + (2 if prec == E.FP32C else 0), s_vel) * n
+        res[tag] = dict(member_years_per_s=n * K / STEPS_PER_YEAR / tot, ms_per_step=1e3 * tot / K,
                         p50_year_s=float(np.percentile(per_year, 50)), p99_year_s=float(np.percentile(per_year, 99)),
-                        flux_ms=flux_ms, flux_step_ms=flux_step_ms, co_ms=co_ms, bgc_coupling_ms=bgc_ms,
-                        flux_bytes=B, flux_gbs=B / (flux_step_ms * 1e-3) / 1e9,
-                        flux_alone_gbs=B / (flux_ms * 1e-3) / 1e9, carb_fail=int(d.carb_fail), n_wet=n_wet, L=tset.L,
+                        latency_years=len(per_year), latency_member_years_per_s=n * len(per_year) / sum(per_year),
+                        flux_ms=flux_ms, flux_step_ms=flux_step_ms, co_ms=co_ms, bgc_coupling_ms=bgc_cpl_ms,
+                        bgc_ms=bgc_ms, flux_bytes=B, flux_bytes_with_carry=Bc,
+                        flux_gbs=B / (flux_step_ms * 1e-3) / 1e9, flux_alone_gbs=B / (flux_ms * 1e-3) / 1e9,
+                        carb_fail=carb_fail, n_wet=n_wet, n_col=n_col, L=tset.L,
                         tracer_block=tuned[tag]["tracer_block"], flux_cols=tuned[tag]["cols"],
                         flux_ring=tuned[tag]["ring"], flux_iso_prepass=tuned[tag]["iso_prepass"])
-        eng.drop_graph(graph)
+        eng.drop_graph(gy)
+        for gr in rest.values():
+            eng.drop_graph(gr)
         for m in ids:
             eng.remove_member(m)
-    # plain FP64 single member, eager launches (the reference-shaped path)
+    # plain FP64 single member, eager launches (the reference-shaped path:
+    # a host calling the step once per ocean step)
     eng.set_tile(E.FP64, tuned["fp64"]["tracer_block"], tuned["fp64"]["cols"], tuned["fp64"]["ring"],
                  tuned["fp64"]["iso_prepass"])
-    ids, tset, n_wet, g = build_ensemble(eng, E, 1, E.FP64)
+    ids, tset, n_wet, n_col, g = build_ensemble(eng, E, 1, E.FP64)
     eng.step(ids, STEPS_PER_YEAR)
     eng.sync()
+    ny = 3
     t0 = time.perf_counter()
-    ny = max(1, min(years, 3))
     eng.step(ids, STEPS_PER_YEAR * ny)
     eng.sync()
     dt = time.perf_counter() - t0
@@ -183,17 +222,19 @@ def run_gpu(args):
     return res
 
 
-def run_serving(seconds: float, rate: float = 4.0, seed: int = 5):
-    """The serving form: a seeded Poisson stream (mean `rate` requests/s) of
-    runs on 18x18x8, 36x36x16 and 72x72x32 with 2-40 tracers, 10-1000 model
-    years, FP32C or FP64, admitted against a 200 GB HBM budget and stepped
-    together; served for `seconds` of wall time."""
+def run_serving(seconds: float, rate: float = 4.0, seed: int = 5, uniform: bool = False):
+    """The serving form: a seeded Poisson stream (mean `rate` requests/s)
+    admitted against a 200 GB HBM budget and stepped together, served for
+    `seconds` of wall time.  Mixed: runs on 18x18x8, 36x36x16 and 72x72x32
+    with 2-40 tracers, 10-1000 model years, FP32C or FP64.  Uniform (the
+    like-for-like stream): every run 36x36x16 with 24 tracers in FP32C."""
     from cgenie_amd import engine as E
     from cgenie_amd import serving as S
 
     eng = E.Engine(0)
     srv = S.Server(eng, hbm_budget=200e9, round_steps=48)
-    for r in S.poisson_requests(int(rate * seconds * 2) + 8, rate, seed):
+    kw = dict(grids=((36, 36, 16),), L_range=(24, 24), precisions=(E.FP32C,)) if uniform else {}
+    for r in S.poisson_requests(int(rate * seconds * 2) + 8, rate, seed, **kw):
         srv.submit(r)
     st = srv.run(max_seconds=seconds, log=lambda m: print("# serve " + m, file=sys.stderr, flush=True))
     eng.close()
@@ -201,12 +242,16 @@ def run_serving(seconds: float, rate: float = 4.0, seed: int = 5):
 
 
 def _cpu_worker(args):
-    """One serial member-year of the CPU oracle (BIOGEM every step)."""
+    """One serial member of the CPU oracle's -O3 build, nsteps ocean steps
+    with BIOGEM every step; wall seconds."""
     q, nsteps = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+
     import oracle_ffi as O
     from cgenie_amd import grid as G
 
+    O._lib = O._bind(C.CDLL(os.path.join(ROOT, "oracle", "liboracle_o3.so")))
     g, k1, tset, ts0, light, mld, mem = ensemble_inputs(1, 2024 + q)
     og = O.grid(36, 36, 16, nyear=STEPS_PER_YEAR)
     og.diff[0], og.diff[1] = mem[0]["diff"]
@@ -217,24 +262,46 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
+def _compiler() -> str:
+    import subprocess
+
+    try:
+        v = subprocess.run(["gcc", "-dumpfullversion"], capture_output=True, text=True, timeout=10).stdout.strip()
+    except (OSError, subprocess.SubprocessError):
+        v = "?"
+    return f"gcc {v} -O3 -funroll-loops -ffp-contract=off (oracle/liboracle_o3.so)"
+
+
 def run_cpu_baseline(cores: int, nsteps: int):
-    """The oracle (a port of the reference arithmetic), one serial member per
-    core, each running nsteps ocean steps.  Runs before the GPU is touched."""
+    """The oracle (a C restatement of the reference arithmetic, -O3), one
+    serial member per core, each running nsteps ocean steps.  Runs before the
+    GPU is touched."""
     import multiprocessing as mp
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
 
     O.build()
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle_o3.so")):
+        import subprocess
+
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle_o3.so"], check=True,
+                       capture_output=True)
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
         times = pool.map(_cpu_worker, [(q, nsteps) for q in range(cores)])
     wall = time.perf_counter() - t0
     member_years = cores * nsteps / STEPS_PER_YEAR
-    return dict(value=member_years / max(times), unit="member-years/s", cores=cores, kind="port",
-                sample=f"{cores} members x {nsteps} steps (36x36x16, 24 tracers, BIOGEM every step), "
-                       f"one serial member per core; per-member {np.mean(times):.2f} s, wall {wall:.1f} s")
+    per_member_year = [t * STEPS_PER_YEAR / nsteps for t in times]
+    return dict(value=round(member_years / max(times), 3), unit="member-years/s", cores=cores, kind="port",
+                nproc=os.cpu_count(), compiler=_compiler(),
+                per_core_member_years_per_s=round(nsteps / STEPS_PER_YEAR / float(np.mean(times)), 4),
+                p50_year_s=round(float(np.percentile(per_member_year, 50)), 4),
+                p99_year_s=round(float(np.percentile(per_member_year, 99)), 4),
+                sample=f"{cores} members x {nsteps} steps (36x36x16, 24 tracers, BIOGEM every step), one serial "
+                       f"member per core on {cores} of the host's {os.cpu_count()} cores (the host cores one GPU's "
+                       f"jobs may use); per-member {np.mean(times):.2f} s, wall {wall:.1f} s")
 
 
 def main():
@@ -242,6 +309,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=960)
     ap.add_argument("--warmup", type=int, default=96)
+    ap.add_argument("--latency-years", type=int, default=20,
+                    help="model years replayed one at a time for the p50/p99 year latency (at least 10)")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("CGO_CPU_CORES", "16")))
     ap.add_argument("--cpu-steps", type=int, default=96)
@@ -249,10 +318,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--probe", choices=("fp32c", "fp64"), default=None,
                     help="counter-run mode: eager steps of the ensemble and a copy calibration, no JSON line")
-    ap.add_argument("--probe-tb", type=int, default=4)
+    ap.add_argument("--probe-tb", type=int, default=6)
     ap.add_argument("--probe-cols", type=int, default=256)
-    ap.add_argument("--probe-ring", type=int, default=3)
-    ap.add_argument("--probe-iso", type=int, default=0)
+    ap.add_argument("--probe-ring", type=int, default=2)
+    ap.add_argument("--probe-iso", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -261,12 +330,15 @@ def main():
     if args.probe:
         run_probe(args)
         return
+    args.latency_years = max(10, args.latency_years)
     cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
     cpu = None if args.no_cpu else run_cpu_baseline(cores, args.cpu_steps)
     print(f"# cpu baseline done: {cpu}", file=sys.stderr, flush=True)
     r = run_gpu(args)
     serving = run_serving(args.serve_seconds) if args.serve_seconds > 0 else None
+    serving_u = run_serving(args.serve_seconds, uniform=True) if args.serve_seconds > 0 else None
     main_ = r["fp32c"]
+    single = r["fp64_single"]["member_years_per_s"]
     # the flux launch as it runs inside the step (the BIOGEM column step beside
     # it on the second stream); the same launch alone is reported next to it
     roof = dict(bound="hbm", achieved=round(main_["flux_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -276,29 +348,43 @@ def main():
                 kernel=(f'k_flux<float, 2, {main_["flux_cols"]}, {main_["flux_ring"]}, 1> + ' if main_["flux_iso_prepass"]
                         else '') + f'k_flux<float, {main_["tracer_block"]}, {main_["flux_cols"]}, {main_["flux_ring"]}, '
                                    f'{2 if main_["flux_iso_prepass"] else 0}>',
-                algorithmic_bytes=main_["flux_bytes"], launch_ms=round(main_["flux_step_ms"], 4),
+                algorithmic_bytes=main_["flux_bytes"], algorithmic_bytes_with_carry=main_["flux_bytes_with_carry"],
+                launch_ms=round(main_["flux_step_ms"], 4),
                 alone_ms=round(main_["flux_ms"], 4), alone_achieved=round(main_["flux_alone_gbs"], 1))
+    # the BIOGEM column kernel is FP64-VALU / latency bound: FP64 operations
+    # per launch from the committed counter summary over its time alone
+    bgc_roof = {}
+    for tag in ("fp32c", "fp64"):
+        fl = profile_figure(BGC_FLOPS_SUMMARY, f"fp64_flops_per_launch {tag}")
+        bms = r[tag]["bgc_ms"]
+        bgc_roof[tag] = dict(bound="fp64-valu", peak=78.6, unit="TFLOP/s", launch_ms=round(bms, 4),
+                             flops=fl, achieved=None if fl is None else round(fl / (bms * 1e-3) / 1e12, 3),
+                             frac=None if fl is None else round(fl / (bms * 1e-3) / 1e12 / 78.6, 4))
     line = dict(
         metric="member-years/s at 36x36x16x24 tracers (FP32-comp batched ensemble); p50/p99 year latency",
         value=round(main_["member_years_per_s"], 2), unit="member-years/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(main_["ms_per_step"], 4), higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f32+carry/f64", data="synthetic",
         config=dict(workload=f"ensemble {args.members} members x 36x36x16 x 24 tracers, BIOGEM every step, "
-                             "FP32-compensated advection-diffusion, FP64 co/chemistry/sums, 1-year HIP graph",
+                             "FP32-compensated advection-diffusion, FP64 co/chemistry/sums, replayed HIP graphs "
+                             "(one model year, and the remainder of the step count)",
                     members=args.members, grid="36x36x16", tracers=24, steps_per_year=STEPS_PER_YEAR),
-        p50_year_s=main_["p50_year_s"], p99_year_s=main_["p99_year_s"],
+        p50_year_s=main_["p50_year_s"], p99_year_s=main_["p99_year_s"], latency_years=main_["latency_years"],
         fp64_batched_member_years_per_s=round(r["fp64"]["member_years_per_s"], 2),
-        fp64_single_member_years_per_s=round(r["fp64_single"]["member_years_per_s"], 3),
-        speedup_vs_fp64_single=round(main_["member_years_per_s"] / r["fp64_single"]["member_years_per_s"], 1),
+        fp64_batched_p50_year_s=r["fp64"]["p50_year_s"], fp64_batched_p99_year_s=r["fp64"]["p99_year_s"],
+        fp64_single_member_years_per_s=round(single, 3),
+        speedup_vs_fp64_single=round(main_["member_years_per_s"] / single, 1),
         kernels_ms=dict(fp32c=dict(flux=main_["flux_ms"], flux_in_step=main_["flux_step_ms"], co=main_["co_ms"],
-                                   bgc_coupling=main_["bgc_coupling_ms"],
+                                   bgc=main_["bgc_ms"], bgc_coupling=main_["bgc_coupling_ms"],
                                    tracer_block=main_["tracer_block"], flux_cols=main_["flux_cols"],
                                    flux_ring=main_["flux_ring"], flux_iso_prepass=main_["flux_iso_prepass"]),
-                        fp64=dict(flux=r["fp64"]["flux_ms"], co=r["fp64"]["co_ms"],
+                        fp64=dict(flux=r["fp64"]["flux_ms"], flux_in_step=r["fp64"]["flux_step_ms"],
+                                  co=r["fp64"]["co_ms"], bgc=r["fp64"]["bgc_ms"],
                                   bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"],
                                   flux_cols=r["fp64"]["flux_cols"], flux_ring=r["fp64"]["flux_ring"],
                                   flux_iso_prepass=r["fp64"]["flux_iso_prepass"])),
-        roofline=roof, cpu_baseline=cpu, serving=serving)
+        roofline=roof, roofline_bgc=bgc_roof, cpu_baseline=cpu, serving=serving, serving_uniform=serving_u,
+        serving_uniform_vs_fp64_single=(round(serving_u["member_years_per_s"] / single, 1) if serving_u else None))
     print(json.dumps(line), flush=True)
 
 

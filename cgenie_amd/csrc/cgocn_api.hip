@@ -27,6 +27,7 @@ void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W,
 void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int maxL, int prec,
                      size_t lds_cap, int phase, hipStream_t);
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
+void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
 }  // namespace cgo
 
 using namespace cgo;
@@ -34,6 +35,7 @@ using namespace cgo;
 namespace {
 
 constexpr int TABLE_CAP = 4096;
+constexpr int NCC_H = 17;  // carbonate constants per cell (n_carbconst, gem_cmn.f90:66)
 constexpr double CONST_PI = 3.141592653589793;  // gem_cmn.f90:688
 constexpr double CONST_REARTH = 6.37E+06;       // gem_cmn.f90:696
 constexpr double CONV_M3_KG = 1027.649;         // gem_cmn.f90:510
@@ -1063,6 +1065,20 @@ int cgo_bgc_download(cgo_engine* e, int32_t id, int32_t what, double* out) {
         HIPCHK(e, hipMemcpy(h.data(), m->dev.carb, h.size() * 8, hipMemcpyDeviceToHost));
         for (size_t c = 0; c < ncol; c++)
             for (int s = 0; s < NCARB; s++) out[s + (size_t)NCARB * c] = h[(size_t)s * ncol + c];
+    } else if (what == 4) {
+        int rc = sync_table(e);
+        if (rc) return rc;
+        std::vector<double> h((size_t)NCC_H * ncol, 0.0);
+        double* d = nullptr;
+        HIPCHK(e, hipMalloc(&d, h.size() * 8));
+        hipMemset(d, 0, h.size() * 8);
+        launch_carbconst_surface(e->d_table, id, g.nwet, d, e->s);
+        hipError_t r = hipStreamSynchronize(e->s);
+        if (r == hipSuccess) r = hipMemcpy(h.data(), d, h.size() * 8, hipMemcpyDeviceToHost);
+        hipFree(d);
+        if (r != hipSuccess) return fail(e, CGO_E_HIP, "bgc_download carbconst: %s", hipGetErrorString(r));
+        for (size_t c = 0; c < ncol; c++)
+            for (int s = 0; s < NCC_H; s++) out[s + (size_t)NCC_H * c] = h[(size_t)s * ncol + c];
     } else {
         return fail(e, CGO_E_ARG, "bgc_download: what %d", what);
     }
@@ -1369,7 +1385,7 @@ int cgo_set_flux_iso(cgo_engine* e, int32_t precision, int32_t prepass) {
 }
 
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
-    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 3) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
+    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 4) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
@@ -1407,6 +1423,9 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
                           e->s);
             } else if (which == 1) {
                 launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+            } else if (which == 4) {
+                if (ls.n[2 + p])
+                    launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
             } else if (ls.n[2 + p]) {
                 launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);
                 launch_coupling(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.maxk_bgc[p],

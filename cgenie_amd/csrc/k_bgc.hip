@@ -739,6 +739,26 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
     }
 }
 
+// The 17 carbonate constants (sub_calc_carbconst, gem_carbchem.f90:59-268)
+// at the surface cell of every wet column from the current ocn T and S, as
+// the column step evaluates them (a download for checking them).
+__global__ __launch_bounds__(64) void k_carbconst_surface(const MemberDev* __restrict__ mems, int32_t m,
+                                                          double* __restrict__ out) {
+    const MemberDev& M = mems[m];
+    const GridDev& g = M.g;
+    const int w = blockIdx.x * 64 + threadIdx.x;
+    if (w >= g.nwet) return;
+    const int col = gp(g.wetsorted)[w];
+    const size_t plane = (size_t)g.ncol, ncell = (size_t)g.ncell, cs = (size_t)(g.maxk - 1) * plane + col;
+    double cc[NCC];
+    calc_carbconst(gp(g.mk)[MK_DMID * (g.maxk + 2) + g.maxk], gp(M.ocn)[cs], gp(M.ocn)[ncell + cs], cc);
+    for (int q = 0; q < NCC; q++) out[(size_t)q * plane + col] = cc[q];
+}
+
+void launch_carbconst_surface(const MemberDev* mems, int32_t m, int nwet, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_carbconst_surface, dim3((nwet + 63) / 64), dim3(64), 0, s, mems, m, out);
+}
+
 // ------------------------------------------------------------ launchers
 template <typename R>
 static void launch_col(int W, dim3 grid, const MemberDev* mems, const int32_t* list, hipStream_t s) {

@@ -13,6 +13,11 @@ otherwise, as the reference's job configuration picks).  The server records
 the wall time at which each model year of each run completes; the per-year
 latency is the time between consecutive year completions of a run.
 
+Each round is replayed from a HIP graph of the round's steps over the active
+set, captured when the set (or the round length) changes, so a steady set of
+runs is stepped with one graph launch per round instead of one launch per
+kernel per step.
+
 Runs with at least seven tracers carry BIOGEM (T, S, DIC, ALK, PO4, O2,
 DOM_P, then dyes) and run the full step (tstepo, step_biogem,
 biogem_tracercoupling); smaller runs are T, S and dyes on GOLDSTEIN alone.
@@ -112,6 +117,8 @@ class Server:
         # host-side inputs of arrived requests are prepared off the stepping loop
         self.pool = ThreadPoolExecutor(max_workers=2)
         self.prep: dict = {}
+        self.graph, self.gkey = None, None   # the round graph and the (members, steps) it was captured for
+        self.captures = 0
 
     # -- requests
     def submit(self, req: Request):
@@ -182,7 +189,7 @@ class Server:
             # one round: every active run advances by up to round_steps
             n = min(self.round_steps, min(r.total_steps - r.steps_done for r in self.active.values()))
             ids = list(self.active.keys())
-            self.eng.step(ids, n)
+            self._round(ids, n)
             self.eng.sync()
             t = self.now()
             self.rounds += 1
@@ -196,18 +203,48 @@ class Server:
             if log and self.rounds % 20 == 0:
                 log(f"t={t:.1f}s active={len(self.active)} waiting={len(self.waiting)} done={len(self.done)} "
                     f"hbm={self.used / 1e9:.2f} GB")
+        self.close()
         return self.stats()
 
+    def _capture(self, ids, n):
+        if self.graph is not None:
+            self.eng.drop_graph(self.graph)
+            self.graph = None
+        self.graph, self.gkey = self.eng.capture(ids, n), (tuple(ids), n)
+        self.captures += 1
+
+    def _round(self, ids, n):
+        """n ocean steps of every active run, from the round graph."""
+        if self.gkey != (tuple(ids), n):
+            self._capture(ids, n)
+        try:
+            self.eng.launch(self.graph, 1)
+        except E.CgoError:
+            # the same slots now hold other runs (a retired run's slot reused):
+            # the engine refuses the old graph, so capture the set again
+            self._capture(ids, n)
+            self.eng.launch(self.graph, 1)
+
+    def close(self):
+        if self.graph is not None:
+            self.eng.drop_graph(self.graph)
+            self.graph, self.gkey = None, None
+
     def stats(self) -> dict:
-        lat, member_years = [], 0.0
+        lat, member_years, weighted = [], 0.0, 0.0
+        ref_work = 36 * 36 * 16 * 24
         for r in self.done + list(self.active.values()):
             prev = r.admitted
             for t in r.year_done:
                 lat.append(t - prev)
                 prev = t
             member_years += r.steps_done / r.nyear
+            # member-years in units of a 36x36x16, 24-tracer member-year:
+            # weighted by cells x tracers x steps per model year
+            weighted += r.steps_done * (r.dims[0] * r.dims[1] * r.dims[2] * r.L) / (ref_work * 96)
         wall = self.now()
         return dict(wall_s=wall, member_years=member_years, member_years_per_s=member_years / max(wall, 1e-9),
+                    equiv_36x36x16x24_member_years_per_s=weighted / max(wall, 1e-9), graph_captures=self.captures,
                     requests_done=len(self.done), requests_active=len(self.active),
                     requests_waiting=len(self.waiting), rounds=self.rounds,
                     p50_year_s=float(np.percentile(lat, 50)) if lat else None,
@@ -215,10 +252,11 @@ class Server:
 
 
 def poisson_requests(n: int, rate: float, seed: int, max_years: int = 1000, min_years: int = 10,
-                     grids=GRIDS, L_range=(2, 40)) -> list:
+                     grids=GRIDS, L_range=(2, 40), precisions=None) -> list:
     """A seeded request stream: Poisson arrivals, grids uniform over `grids`,
     tracer counts uniform in L_range, run lengths log-uniform in
-    [min_years, max_years], precision FP32C or FP64 with equal odds."""
+    [min_years, max_years], precision FP32C or FP64 with equal odds (or
+    uniform over `precisions`)."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     for q in range(n):
@@ -226,6 +264,9 @@ def poisson_requests(n: int, rate: float, seed: int, max_years: int = 1000, min_
         dims = grids[rng.integers(len(grids))]
         L = int(rng.integers(L_range[0], L_range[1] + 1))
         years = int(round(math.exp(rng.uniform(math.log(min_years), math.log(max_years)))))
-        prec = E.FP32C if rng.random() < 0.5 else E.FP64
+        if precisions is None:
+            prec = E.FP32C if rng.random() < 0.5 else E.FP64
+        else:
+            prec = precisions[int(rng.integers(len(precisions)))]
         out.append(Request(q, dims, L, years, prec, seed * 1000 + q, arrival=t))
     return out

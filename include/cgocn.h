@@ -127,7 +127,9 @@ void cgo_bgc_default_params(cgo_bgc_params *p, double dtyr);
 int cgo_bgc_init(cgo_engine *e, int32_t m, const cgo_bgc_params *p, const double *light, const double *mld);
 /* what: 0 = ocn (maxl,maxi,maxj,maxk: T, S and the carried species, other
  * slots zero), 1 = bio_part (5,maxi,maxj,maxk), 2 = carb (10,maxi,maxj),
- * 3 = last anomaly vdocn (maxl,maxi,maxj,maxk) */
+ * 3 = last anomaly vdocn (maxl,maxi,maxj,maxk), 4 = carbconst (17,maxi,maxj)
+ * of the surface cells from the current ocn T and S (sub_calc_carbconst,
+ * gem_carbchem.f90:59-268; land columns zero) */
 int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
 
 /* stepping: nsteps ocean steps of every listed member, batched into one
@@ -164,9 +166,11 @@ int cgo_set_flux_ring(cgo_engine *e, int32_t precision, int32_t planes);    /* 2
 int cgo_set_flux_iso(cgo_engine *e, int32_t precision, int32_t prepass);    /* 1: isoneutral coefficients from a pre-pass */
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
  * 1 = co, 2 = bgc column + coupling, 3 = flux with the BIOGEM column step
- * running beside it on the second stream, as inside a whole step) over the
- * listed members, from HIP events on the engine stream; the state is
- * advanced by the timed launches */
+ * running beside it on the second stream, as inside a whole step, 4 = the
+ * BIOGEM column step alone) over the listed members, from HIP events on the
+ * engine stream; the state is advanced by the timed launches (mode 4
+ * rewrites the BIOGEM arrays at every launch with no coupling between them:
+ * a member timed with it is for timing only) */
 int cgo_time_kernel(cgo_engine *e, int32_t which, const int32_t *members, int32_t n, int32_t reps, double *avg_ms);
 
 #ifdef __cplusplus

@@ -1,0 +1,229 @@
+"""The benchmark configurations at their full sizes on the GPU, through the C
+ABI, against the oracle (needs an MI355X).
+
+* Ten model years (960 ocean steps) of the FP64 path with BIOGEM at
+  36x36x16 with 24 tracers: state, BIOGEM ocn and particulates within 1e-12
+  per cell of the oracle, the carbonate system (RF0 included) and the
+  surface carbonate constants within the bars stated at each check.
+* 1000 model years (96,000 ocean steps) of the same configuration, FP64 and
+  FP32-compensated members stepped together as a replayed one-year graph:
+  the FP64 wet-cell state equal bit for bit to the oracle's (SHA-256 of the
+  float64 state at years 1, 10, 100 and 1000, and the full state at year
+  1000, from tests/golden/drift_1000y.npz), and the FP32-compensated form
+  within 1e-6 per tracer of it at every checkpoint.
+* The 256-member seeded ensemble of the benchmark, FP32-compensated and FP64
+  copies stepped together for one model year as one replayed graph:
+  sampled FP64 members within 1e-12 of the oracle, every FP32-compensated
+  member within 1e-6 per tracer of its FP64 copy.
+
+Relative error per cell is rel_err of tests/cases.py: |a - b| over
+max(|b|, 1e-3 max|b_l|) per tracer l, salinity compared as absolute
+salinity where FP32 and FP64 members are compared.
+"""
+import hashlib
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cgenie_amd import engine as E
+from cgenie_amd import grid as G
+import oracle_ffi as O
+from cases import make_case, rel_err, wet
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIFT = os.path.join(HERE, "golden", "drift_1000y.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def absS(x):
+    y = x.copy()
+    y[..., 1] += G.SALN0
+    return y
+
+
+def test_config2_ten_years_fp64_matches_oracle(eng):
+    """960 ocean steps, FP64, BIOGEM every step, 36x36x16, 24 tracers."""
+    c = make_case(kind="config2", seed=2024)
+    gid = eng.add_grid(c.g, c.k1)
+    m = eng.add_member(gid, c.tset.L, E.FP64, c.diff)
+    eng.set_velocity(m, c.u)
+    eng.upload(m, c.ts)
+    eng.bgc_init(m, E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000), c.light, c.mld)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset), c.light, c.mld)
+    eng.step([m], 960)
+    bio.step(960)
+    w = wet(c)
+    got = eng.download(m, c.ts.shape)
+    err = rel_err(got, gold.ts, w)
+    nbit = np.count_nonzero(got[w] == gold.ts[w])
+    say("config 2, 960 steps: ts max rel err", err.max(), "bit-identical cells", nbit, "of", got[w].size)
+    assert np.all(err <= 1e-12)
+    ocn, oref = eng.bgc_download(m, 0, c.ts.shape), bio.ocn()
+    t = c.tset
+    sl = [0, 1, t.DIC, t.ALK, t.PO4, t.O2, t.DOM_P]
+    eo = rel_err(ocn[..., sl], oref[..., sl], w)
+    say("  ocn max rel err", eo)
+    assert np.all(eo <= 1e-12)
+    part, pref = eng.bgc_download(m, 1, (c.g.maxk, c.g.maxj, c.g.maxi, 5)), bio.part()
+    ep = np.abs(part - pref).max(axis=(0, 1, 2)) / (np.abs(pref).max(axis=(0, 1, 2)) + 1e-300)
+    say("  part err", ep)
+    assert np.all(ep <= 1e-12)
+    # carbonate system of the last step (H, fCO2, CO2, CO3, HCO3, Omega
+    # calcite/aragonite, dCO3 calcite/aragonite, RF0)
+    wc = c.k1[1:-1, 1:-1] <= c.g.maxk
+    carb, cref = eng.bgc_download(m, 2, (c.g.maxj, c.g.maxi, 10))[wc], bio.carb()[wc]
+    ec = np.abs(carb - cref) / np.maximum(np.abs(cref), 1e-300)
+    say("  carb max rel err", ec.max(axis=0))
+    # the device's exp/log/pow are within 1 ulp of glibc's, so the constants
+    # differ in the last bits and the fixed point stops within the 1e-12
+    # tolerance of a slightly different root: H..HCO3 and Omega to 1e-11;
+    # dCO3 = CO3 - CO3_sat (a difference of near-equal numbers) and RF0 (a
+    # ratio of 1e-6 perturbation differences) lose what the cancellation costs
+    assert np.all(ec[:, :7] <= 1e-11)
+    assert np.all(ec[:, 7:9] <= 1e-9)
+    assert np.all(ec[:, 9] <= 1e-8)
+    # surface carbonate constants from the current T and S, against the oracle's formulas
+    cc = eng.bgc_download(m, 4, (c.g.maxj, c.g.maxi, 17))
+    L = O.lib()
+    Dmid = bio.st.Dmid[c.g.maxk]
+    emax = np.zeros(17)
+    for j, i in np.argwhere(wc):
+        ref = np.zeros(17)
+        L.og_calc_carbconst(Dmid, oref[-1, j, i, 0], oref[-1, j, i, 1], ref)
+        emax = np.maximum(emax, np.abs(cc[j, i] - ref) / np.maximum(np.abs(ref), 1e-300))
+    say("  carbconst max rel err", emax)
+    assert np.all(emax <= 1e-13)
+    d = eng.diag(m)
+    assert d.steps == 960 and d.carb_fail == 0 and bio.st.carb_fail == 0
+    eng.remove_member(m)
+
+
+def test_config3_thousand_years_fp64_bitwise_fp32c_within_1e6(eng):
+    """96,000 ocean steps of FP64 and FP32-compensated members against the
+    oracle's 1000-year run (tests/golden/make_drift_golden.py)."""
+    if not os.path.exists(DRIFT):
+        pytest.fail(f"{DRIFT} missing: python tests/golden/make_drift_golden.py")
+    gold = np.load(DRIFT)
+    years = [int(y) for y in gold["years"]]
+    g = G.make_grid(36, 36, 16)
+    k1 = G.make_k1(g, 2024)
+    tset = G.tracer_set("config2")
+    u = G.make_velocity(g, k1, 2025)
+    ts = G.make_tracers(g, k1, tset, 2026)
+    light, mld = G.make_light(g, 2027), G.make_mld(g, 2028, 0.0)
+    w = G.wet_mask(g, k1)
+    gid = eng.add_grid(g, k1)
+    p = E.default_bgc_params(1.0 / 96, tset, carb_tol=1e-12, carb_maxit=1000)
+    ms = []
+    for prec in (E.FP64, E.FP32C):
+        m = eng.add_member(gid, tset.L, prec, G.diffusivities(2000.0, 2.7e-5))
+        eng.set_velocity(m, u)
+        eng.upload(m, ts)
+        eng.bgc_init(m, p, light, mld)
+        ms.append(m)
+    graph = eng.capture(ms, 96)
+    done = 0
+    worst = 0.0
+    for q, y in enumerate(years):
+        eng.launch(graph, y - done)
+        eng.sync()
+        done = y
+        a64 = eng.download(ms[0], ts.shape)[w]
+        a32 = eng.download(ms[1], ts.shape)[w]
+        h = hashlib.sha256(np.ascontiguousarray(a64, dtype="<f8").tobytes()).hexdigest()
+        e32 = rel_err(absS(a32)[None], absS(a64)[None], np.ones((1,) + a64.shape[:1], bool))
+        worst = max(worst, float(e32.max()))
+        i64, i32 = eng.inventory(ms[0]), eng.inventory(ms[1])
+        big = np.abs(i64) > 1e-3 * np.abs(i64).max()
+        dinv = float((np.abs(i32 - i64)[big] / np.abs(i64)[big]).max())
+        say(f"year {y}: FP64 hash equal {h == gold['sha256'][q]}; FP32C max rel err {e32.max():.3e} "
+            f"(T {e32[0]:.2e} S {e32[1]:.2e} PO4 {e32[tset.PO4]:.2e}); FP32C inventory drift {dinv:.2e}")
+        assert h == gold["sha256"][q], f"FP64 state differs from the oracle at year {y}"
+        assert np.array_equal(i64, gold["inventory"][q]) or np.allclose(i64, gold["inventory"][q], rtol=1e-13, atol=0)
+        assert np.all(e32 <= 1e-6), (y, e32)
+        assert dinv <= 1e-6
+    assert np.array_equal(a64, gold["state"])
+    e = rel_err(absS(a32)[None], absS(gold["state"])[None], np.ones((1,) + a64.shape[:1], bool))
+    say("year 1000 FP32C vs oracle per tracer:", np.array2string(e, precision=2))
+    assert np.all(e <= 1e-6)
+    d64, d32 = eng.diag(ms[0]), eng.diag(ms[1])
+    assert d64.steps == 96000 and d32.steps == 96000 and d64.carb_fail == 0 and d32.carb_fail == 0
+    eng.drop_graph(graph)
+    for m in ms:
+        eng.remove_member(m)
+
+
+def ensemble_params(n, seed=2024):
+    """The benchmark's seeded per-member parameters (bench.py ensemble_inputs)."""
+    rng = np.random.default_rng(seed + 5)
+    out = []
+    for _ in range(n):
+        kh = rng.uniform(500.0, 5000.0)
+        kv = math.exp(rng.uniform(math.log(1e-5), math.log(1e-4)))
+        vs = rng.uniform(0.5, 1.5)
+        eL1 = rng.uniform(300.0, 900.0)
+        out.append((kh, kv, vs, eL1))
+    return out
+
+
+def test_config4_ensemble_256_members(eng):
+    n = 256
+    g = G.make_grid(36, 36, 16)
+    k1 = G.make_k1(g, 2024)
+    tset = G.tracer_set("config2")
+    u0 = G.make_velocity(g, k1, 2025)
+    ts = G.make_tracers(g, k1, tset, 2026)
+    light, mld = G.make_light(g, 2027), G.make_mld(g, 2028, 0.0)
+    w = G.wet_mask(g, k1)
+    gid = eng.add_grid(g, k1)
+    prm = ensemble_params(n)
+    m64, m32 = [], []
+    for kh, kv, vs, eL1 in prm:
+        p = E.default_bgc_params(1.0 / 96, tset, carb_tol=1e-12, carb_maxit=1000, POC_eL1=eL1)
+        for prec, lst in ((E.FP64, m64), (E.FP32C, m32)):
+            m = eng.add_member(gid, tset.L, prec, G.diffusivities(kh, kv))
+            eng.set_velocity(m, u0 * vs)
+            eng.upload(m, ts)
+            eng.bgc_init(m, p, light, mld)
+            lst.append(m)
+    graph = eng.capture(m64 + m32, 96)
+    eng.launch(graph, 1)
+    eng.sync()
+    worst = np.zeros(tset.L)
+    for q in range(n):
+        a = absS(eng.download(m32[q], ts.shape))
+        b = absS(eng.download(m64[q], ts.shape))
+        worst = np.maximum(worst, rel_err(a, b, w))
+        assert eng.diag(m64[q]).steps == 96 and eng.diag(m32[q]).carb_fail == 0
+    say("256 members, one year: FP32C vs FP64 max rel err per tracer", np.array2string(worst, precision=2))
+    assert np.all(worst <= 1e-6)
+    for q in (0, 85, 170, 255):
+        kh, kv, vs, eL1 = prm[q]
+        og = O.grid(36, 36, 16, kappa_h=kh, kappa_v=kv)
+        gold = O.GoldsteinOracle(og, k1, ts, u0 * vs)
+        bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, tset, POC_eL1=eL1), light, mld)
+        bio.step(96)
+        err = rel_err(eng.download(m64[q], ts.shape), gold.ts, w)
+        say(f"member {q} (kappa_h {kh:.0f}, kappa_v {kv:.2e}, u x{vs:.2f}, eL1 {eL1:.0f} m): "
+            f"FP64 vs oracle max rel err {err.max():.2e}")
+        assert np.all(err <= 1e-12)
+    eng.drop_graph(graph)
+    for m in m64 + m32:
+        eng.remove_member(m)

@@ -347,7 +347,9 @@ def test_fp32_form_tracks_fp64(eng, carry):
     b[..., 1] += G.SALN0
     err = rel_err(a, b, wet(c))
     say("FP32 (carry=%s) vs FP64 after one year: max rel err" % carry, err.max(), err[:7])
-    assert np.all(err <= (5e-6 if carry else 5e-5))
+    # the compensated form holds every tracer within 1e-6 of FP64; plain
+    # FP32 (no carry) is reported for comparison at its own, looser bar
+    assert np.all(err <= (1e-6 if carry else 5e-5))
     i32, i64 = eng.inventory(m32), eng.inventory(m64)
     big = np.abs(i64) > 1e-3 * np.abs(i64).max()
     say("  inventory rel diff", np.abs(i32 - i64)[big].max() / np.abs(i64)[big].max())

@@ -657,3 +657,21 @@ def test_coupling_uniform_salinity_anomaly():
         want = (Snew / (S + dS))[w] * ratio * a
         assert np.all(np.abs(gold.ts[..., l][w] - want) <= 1e-13 * np.abs(want) + 1e-20), l
     assert np.allclose(M()[w], M0[w] / ratio, rtol=1e-14, atol=0)
+
+
+def test_o3_build_gives_the_same_bits():
+    """The CPU baseline times oracle/liboracle_o3.so (-O3 -funroll-loops,
+    -ffp-contract=off): the same results as the checker build, bit for bit."""
+    fast = os.path.join(O.ORACLE_DIR, "liboracle_o3.so")
+    import subprocess
+    subprocess.run(["make", "-C", O.ORACLE_DIR, "liboracle_o3.so"], check=True, capture_output=True)
+    runs = []
+    for path in (O.LIB, fast):
+        with O.use_library(path):
+            c = make_case(18, 18, 8, kind="config2", seed=19)
+            gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+            bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset), c.light, c.mld)
+            bio.step(4)
+            runs.append((gold.ts.copy(), bio.ocn(), bio.carb()))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

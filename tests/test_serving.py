@@ -16,6 +16,7 @@ class CountingEngine:
 
     def __init__(self, bytes_per_member=100):
         self.bpm, self.members, self.nextm, self.calls = bytes_per_member, {}, 0, []
+        self.graphs, self.captures, self.dropped = [], 0, 0
 
     def add_grid(self, g, k1):
         return 0
@@ -41,6 +42,19 @@ class CountingEngine:
         self.calls.append((tuple(ids), n))
         for m in ids:
             self.members[m] += n
+
+    def capture(self, ids, n):
+        self.graphs.append((tuple(ids), n))
+        self.captures += 1
+        return len(self.graphs) - 1
+
+    def launch(self, g, reps=1):
+        ids, n = self.graphs[g]
+        for _ in range(reps):
+            self.step(ids, n)
+
+    def drop_graph(self, g):
+        self.dropped += 1
 
     def sync(self):
         pass
@@ -78,6 +92,9 @@ def test_scheduler_admits_by_budget_and_recycles(monkeypatch):
         assert r.steps_done == r.total_steps and len(r.year_done) == r.years
     assert st["member_years"] == sum(r.years for r in reqs)
     assert st["p50_year_s"] is not None
+    # rounds replay a graph captured per (member set, round length); every graph is dropped
+    assert eng.captures == len(set(eng.calls)) and eng.dropped == eng.captures
+    assert st["equiv_36x36x16x24_member_years_per_s"] * 6 == pytest.approx(st["member_years_per_s"])
 
 
 def test_poisson_stream_is_seeded():
