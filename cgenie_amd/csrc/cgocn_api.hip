@@ -794,6 +794,22 @@ int cgo_inventory_noalloc(cgo_engine* e, int32_t id, double* inv) {
     return CGO_OK;
 }
 
+int cgo_set_cost(cgo_engine* e, int32_t id, const double* cost) {
+    MemberH* m = member(e, id);
+    if (!m || !cost) return fail(e, CGO_E_ARG, "set_cost: bad argument");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(m->dev.cost, cost, (size_t)m->dev.g.ncol * 8, hipMemcpyHostToDevice));
+    return CGO_OK;
+}
+
+int cgo_download_cost(cgo_engine* e, int32_t id, double* cost) {
+    MemberH* m = member(e, id);
+    if (!m || !cost) return fail(e, CGO_E_ARG, "download_cost: bad argument");
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(cost, m->dev.cost, (size_t)m->dev.g.ncol * 8, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
 int cgo_get_diag(cgo_engine* e, int32_t id, cgo_diag* d) {
     MemberH* m = member(e, id);
     if (!m || !d) return fail(e, CGO_E_ARG, "get_diag: bad argument");
@@ -1065,6 +1081,11 @@ int cgo_bgc_download(cgo_engine* e, int32_t id, int32_t what, double* out) {
         HIPCHK(e, hipMemcpy(h.data(), m->dev.carb, h.size() * 8, hipMemcpyDeviceToHost));
         for (size_t c = 0; c < ncol; c++)
             for (int s = 0; s < NCARB; s++) out[s + (size_t)NCARB * c] = h[(size_t)s * ncol + c];
+    } else if (what == 5) {
+        std::vector<double> h((size_t)NSED * ncol);
+        HIPCHK(e, hipMemcpy(h.data(), m->dev.settle, h.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < ncol; c++)
+            for (int s = 0; s < NSED; s++) out[s + (size_t)NSED * c] = h[(size_t)s * ncol + c];
     } else if (what == 4) {
         int rc = sync_table(e);
         if (rc) return rc;

@@ -84,6 +84,8 @@ def load() -> C.CDLL:
         "cgo_inventory": ([P, _i32, _dp], C.c_int),
         "cgo_inventory_noalloc": ([P, _i32, _dp], C.c_int),
         "cgo_get_diag": ([P, _i32, C.POINTER(Diag)], C.c_int),
+        "cgo_set_cost": ([P, _i32, _dp], C.c_int),
+        "cgo_download_cost": ([P, _i32, _dp], C.c_int),
         "cgo_bgc_default_params": ([C.POINTER(BgcParams), _d], None),
         "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
         "cgo_bgc_download": ([P, _i32, _i32, _dp], C.c_int),

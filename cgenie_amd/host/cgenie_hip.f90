@@ -4,14 +4,24 @@
 ! It keeps the reference's calling pattern: module state set up once (as
 ! initialise_goldstein / initialise_biogem do, goldstein.f90:906-1075,
 ! biogem_data.f90:1098-1140), then per ocean step the calls that genie.f90
-! makes (genie.f90:306-313 GOLDSTEIN, 360-381 BIOGEM):
+! makes (genie.f90:306-313 GOLDSTEIN, 360-381 BIOGEM), with the reference's
+! argument lists, so the wrappers' CALL lines (genie_loop_wrappers.f90:122-151,
+! 310-322) compile unchanged against this module:
 !
-!   tstepo()                              goldstein.f90:2280-2432, called at 235
-!   step_goldstein_tracers(go_ts, go_ts1) the tracer part of step_goldstein
-!                                         (goldstein.f90:17-38, 167-187, 235, 443-444)
-!   step_biogem(dum_dts)                  biogem.f90:528-547 (biology, remineralisation
-!                                         and surface chemistry of the carried species)
+!   step_goldstein(istep, latent_ocn, ..., go_rho)   goldstein.f90:17-38: the
+!                                         surface-flux slot from the heat and
+!                                         freshwater fluxes (149-170), go_ts in,
+!                                         tstepo (235), go_ts/go_rho/go_cost and
+!                                         the surface and energy/water outputs
+!                                         back (427-478)
+!   step_biogem(dum_dts, dum_genie_clock, dum_sfcatm1, ..., dum_sfxsumrok1)
+!                                         biogem.f90:528-547 (biology,
+!                                         remineralisation and surface chemistry
+!                                         of the carried species; bottom-water
+!                                         composition and rain out, 1719-1754)
 !   biogem_tracercoupling(dum_ts, dum_ts1) biogem.f90:1885-1890
+!   tstepo()                              goldstein.f90:2280-2432, called at 235
+!   step_goldstein_tracers(go_ts, go_ts1) tstepo and the go_ts hand-back alone
 !
 ! The tracer state lives on the GPU.  With host mirroring on (the default,
 ! the reference's behaviour) go_ts / go_ts1 are refreshed after every call
@@ -28,7 +38,7 @@ module cgenie_hip
   public :: initialise_goldstein_hip, initialise_biogem_hip, end_cgenie_hip
   public :: tstepo, step_goldstein_tracers, step_biogem, biogem_tracercoupling
   public :: cgenie_hip_set_mirror, cgenie_hip_download, cgenie_hip_inventory, cgenie_hip_diag
-  public :: cgenie_hip_set_sflux
+  public :: cgenie_hip_set_sflux, step_goldstein
 
   type(c_ptr), save :: eng = c_null_ptr
   integer(c_int32_t), save :: grid_id = -1, mem(1) = -1
@@ -37,6 +47,28 @@ module cgenie_hip
   ! arrays whose addresses the grid descriptor holds during cgo_grid_create
   integer(c_int32_t), allocatable, target, save :: k1_c(:,:)
   real(c_double), allocatable, target, save :: jtab(:,:), ktab(:,:)
+  ! step_goldstein: the velocities last handed to the engine, the initial
+  ! energy and water inventories (goldstein.f90:79-96), the surface-flux slot
+  real(c_double), allocatable, target, save :: u_last(:,:,:,:), sflux_c(:,:,:)
+  real(c_double), save :: dphi_s = 0, dsc_s = 0, ini_energy = 0, ini_water = 0
+  logical, save :: sflux_on = .false.
+  ! step_biogem: go_ts positions of DIC, ALK, PO4, O2, DOM_C, DOM_P and the step length
+  integer, save :: bgc_slots(6) = 0
+  real(c_double), save :: bgc_dtyr = 0
+
+  ! the reference's REAL is 8 bytes (-fdefault-real-8, which also makes its
+  ! literals double): every literal here carries the kind
+  integer, parameter :: dp = c_double
+  ! GOLDSTEIN scales and conversions (goldstein_lib.f90:48-63, goldstein-defaults.nml:27, 60)
+  real(dp), parameter :: usc = 0.05_dp, rsc = 6.37E6_dp, dsc_p = 5.0E3_dp, rh0sc = 1.0E3_dp, cpsc = 3981.1_dp
+  real(dp), parameter :: saln0 = 34.9_dp, m2mm = 1000.0_dp, mm2m = 1.0_dp / m2mm, albocn = 0.05_dp
+  real(dp), parameter :: rfluxsc = rsc / (dsc_p * usc * rh0sc * cpsc)
+  ! BIOGEM constants (gem_cmn.f90:510-513, 688, 696, 719) and tracer indices (72-129, 190-255)
+  real(dp), parameter :: conv_yr_s = 3600.0_dp * (24.0_dp * 365.25_dp), const_pi = 3.141592653589793_dp
+  real(dp), parameter :: const_rEarth = 6.37E+06_dp, nullsmall = 0.999999E-19_dp
+  integer, parameter :: io_of_slot(6) = [3, 12, 8, 10, 15, 20]    ! DIC, ALK, PO4, O2, DOM_C, DOM_P
+  integer, parameter :: io_Ca = 35, io_B = 36, io_F = 37, io_SO4 = 38
+  integer, parameter :: is_of(5) = [3, 8, 14, 33, 34]              ! POC, POP, CaCO3, POC_frac2, CaCO3_frac2
 
 contains
 
@@ -103,7 +135,32 @@ contains
          'cgo_member_create')
     call check(cgo_set_velocity(eng, mem(1), go_u), 'cgo_set_velocity')
     call check(cgo_upload_state(eng, mem(1), go_ts), 'cgo_upload_state')
+    dphi_s = dphi; dsc_s = dsc
+    allocate (u_last(3, 0:maxi, 0:maxj, maxk), sflux_c(maxl, maxi, maxj))
+    u_last = go_u
+    sflux_c = 0
+    call energy_water(go_ts, ini_energy, ini_water)
   end subroutine initialise_goldstein_hip
+
+  ! The total heat and water diagnostics of step_goldstein
+  ! (goldstein.f90:79-96, 458-471), summed in the reference's order.
+  subroutine energy_water(ts, energy, water)
+    real(c_double), intent(in) :: ts(nl, nx, ny, nz)
+    real(c_double), intent(out) :: energy, water
+    real(c_double) :: vsc
+    integer :: j, k
+    vsc = dphi_s * rsc * rsc
+    energy = 0
+    water = 0
+    do k = 1, nz
+       do j = 1, ny
+          energy = energy + sum(ts(1, 1:nx, j, k)) * ktab(k, 6) * jtab(j, 8)
+          water = water - sum(ts(2, 1:nx, j, k)) * ktab(k, 6) * jtab(j, 8)
+       end do
+    end do
+    energy = energy * vsc * dsc_s * rh0sc * cpsc
+    water = m2mm * water * vsc * dsc_s / saln0
+  end subroutine energy_water
 
   ! BIOGEM state of the member from its current go_ts.  slots(1:6) are the
   ! 1-based go_ts positions of DIC, ALK, PO4, O2, DOM_C, DOM_P (0 = absent);
@@ -124,6 +181,8 @@ contains
     bp%caco3 = merge(1_c_int32_t, 0_c_int32_t, slots(1) > 0 .and. slots(2) > 0)
     if (present(carb_tol)) bp%carb_tol = carb_tol
     call check(cgo_bgc_init(eng, mem(1), bp, light, mld), 'cgo_bgc_init')
+    bgc_slots = slots
+    bgc_dtyr = dtyr
   end subroutine initialise_biogem_hip
 
   ! the surface boundary slot ts(:,1:maxi,1:maxj,maxk+1) (goldstein.f90:167-170)
@@ -168,12 +227,158 @@ contains
     if (mirror) call cgenie_hip_download(go_ts, go_ts1)
   end subroutine step_goldstein_tracers
 
-  ! step_biogem (biogem.f90:528-547); dum_dts is the BIOGEM step of the
-  ! parameters given to initialise_biogem_hip
-  subroutine step_biogem(dum_dts)
+  ! step_goldstein (goldstein.f90:17-478) for the tracer path.  The surface
+  ! boundary slot ts(:,:,:,maxk+1) is built from the heat and freshwater
+  ! fluxes as at 149-170 (no hosing or freshwater anomaly); the state comes
+  ! from go_ts when mirroring is on (176-177: the host's go_ts is the state;
+  ! go_ts1 equals go_ts between steps of the genie loop) and stays on the
+  ! device otherwise.  Velocities are the go_u given (3,maxi,maxj,maxk; the
+  ! periodic western face and the closed southern face of the engine's
+  ! u(3,0:maxi,0:maxj,maxk) follow from it), handed to the engine when they
+  ! change.  After tstepo: go_ts, go_ts1, go_rho and go_cost, the surface T,
+  ! S and velocities, albedo and the energy and water diagnostics (427-478).
+  ! The momentum outputs of the reference's velocity solve (go_tau, go_psi,
+  ! go_mldta) are left as given.
+  subroutine step_goldstein(istep, latent_ocn, sensible_ocn, netsolar_ocn, netlong_ocn, fx0sic_ocn, evap_ocn, &
+       pptn_ocn, runoff_ocn, fwsic_ocn, stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn, tsval_ocn, &
+       ssval_ocn, usval_ocn, vsval_ocn, albedo_ocn, test_energy_ocean, test_water_ocean, go_ts, go_ts1, go_cost, &
+       go_u, go_tau, go_psi, go_mldta, go_rho)
+    integer, intent(in) :: istep
+    real(c_double), dimension(:,:), intent(inout) :: latent_ocn, sensible_ocn, netsolar_ocn, netlong_ocn, &
+         fx0sic_ocn, evap_ocn, pptn_ocn, runoff_ocn, fwsic_ocn, stressxu_ocn, stressyu_ocn, stressxv_ocn, &
+         stressyv_ocn, tsval_ocn, ssval_ocn, usval_ocn, vsval_ocn, albedo_ocn
+    real(c_double), intent(out) :: test_energy_ocean, test_water_ocean
+    real(c_double), dimension(:,:,:,:), intent(inout) :: go_ts, go_ts1
+    real(c_double), dimension(:,:), intent(inout) :: go_cost, go_mldta
+    real(c_double), intent(inout) :: go_u(:,:,:,:), go_tau(:,:,:), go_psi(:,:), go_rho(:,:,:)
+    real(c_double) :: fx0neto(nx, ny), fwfxneto(nx, ny), tot_energy, tot_water
+    real(c_double), allocatable :: u(:,:,:,:)
+    integer :: i, j
+    logical :: wet
+
+    if (.not. c_associated(eng)) call check(CGO_E_STATE, 'step_goldstein: initialise_goldstein_hip first')
+    if (istep < 0) call check(CGO_E_ARG, 'step_goldstein: negative istep')
+    ! net heat and freshwater fluxes into the surface slot (149-170)
+    fx0neto = netsolar_ocn + sensible_ocn + netlong_ocn + latent_ocn + fx0sic_ocn
+    fwfxneto = pptn_ocn + evap_ocn + runoff_ocn + fwsic_ocn
+    fwfxneto = fwfxneto * mm2m
+    sflux_c = 0.0_dp
+    sflux_c(1, :, :) = -fx0neto * rfluxsc
+    if (nl >= 2) sflux_c(2, :, :) = fwfxneto * rpmesco()
+    if (any(fx0neto /= 0.0_dp) .or. any(fwfxneto /= 0.0_dp)) then
+       call check(cgo_set_sflux(eng, mem(1), c_loc(sflux_c)), 'cgo_set_sflux')
+       sflux_on = .true.
+    else if (sflux_on) then
+       call check(cgo_set_sflux(eng, mem(1), c_null_ptr), 'cgo_set_sflux')
+       sflux_on = .false.
+    end if
+    ! velocities, when the caller's differ from the engine's
+    ! (the engine reads the interior faces only; the west face of i = 1 is the
+    ! east face of i = maxi)
+    allocate (u(3, 0:nx, 0:ny, nz))
+    u = 0.0_dp
+    u(:, 1:nx, 1:ny, :) = go_u(:, 1:nx, 1:ny, 1:nz)
+    u(:, 0, 1:ny, :) = u(:, nx, 1:ny, :)
+    if (any(u(:, 1:nx, 1:ny, :) /= u_last(:, 1:nx, 1:ny, :))) then
+       call check(cgo_set_velocity(eng, mem(1), u), 'cgo_set_velocity')
+       u_last = u
+    end if
+    if (mirror) then
+       call check(cgo_upload_state(eng, mem(1), go_ts), 'cgo_upload_state')
+       call check(cgo_set_cost(eng, mem(1), go_cost), 'cgo_set_cost')
+    end if
+    call tstepo()
+    usval_ocn = go_u(1, 1:nx, 1:ny, nz)
+    vsval_ocn = go_u(2, 1:nx, 1:ny, nz)
+    if (.not. mirror) return
+    call cgenie_hip_download(go_ts, go_ts1)
+    call check(cgo_download_rho(eng, mem(1), go_rho), 'cgo_download_rho')
+    call check(cgo_download_cost(eng, mem(1), go_cost), 'cgo_download_cost')
+    do j = 1, ny
+       do i = 1, nx
+          wet = k1_c(i, j) <= nz
+          tsval_ocn(i, j) = merge(go_ts(1, i, j, nz), 0.0_c_double, wet)
+          ssval_ocn(i, j) = merge(go_ts(2, i, j, nz), 0.0_c_double, wet)
+          albedo_ocn(i, j) = merge(albocn, 0.0_c_double, wet)
+       end do
+    end do
+    call energy_water(go_ts, tot_energy, tot_water)
+    test_energy_ocean = tot_energy - ini_energy
+    test_water_ocean = tot_water - ini_water
+  end subroutine step_goldstein
+
+  pure function rpmesco() result(r)
+    real(c_double) :: r
+    r = rsc * saln0 / (dsc_p * usc)   ! goldstein.f90:904 (computed once per call: the reference's is a variable)
+  end function rpmesco
+
+  ! step_biogem (biogem.f90:528-547).  dum_dts is the BIOGEM step in seconds
+  ! (loc_dtyr = dum_dts / conv_yr_s, 622-624); it must be the step
+  ! initialise_biogem_hip was given.  With mirroring on, the interface arrays
+  ! BIOGEM writes are filled as at 1719-1754: dum_sfcocn1(io,i,j) the
+  ! bottom-water composition after this step's biology (ocn + the step's
+  ! anomaly at k1, Ca, B, F and SO4 from salinity), dum_sfxsed1(is,i,j) the
+  ! rain onto the sea floor in mol m-2 s-1 (the recalcitrant fractions
+  ! scaled by dtyr / dts).  Air-sea exchange and sediment / weathering inputs
+  ! are not part of this path: dum_sfxatm1, dum_sfxocn1 and dum_sfxsumrok1
+  ! are left as given.
+  subroutine step_biogem(dum_dts, dum_genie_clock, dum_sfcatm1, dum_sfxatm1, dum_sfcocn1, dum_sfxocn1, &
+       dum_sfcsed1, dum_sfxsed1, dum_sfxsumrok1)
     real(c_double), intent(in) :: dum_dts
+    integer(c_int64_t), intent(in) :: dum_genie_clock
+    real(c_double), intent(in), dimension(:,:,:) :: dum_sfcatm1, dum_sfcsed1
+    real(c_double), intent(inout), dimension(:,:,:) :: dum_sfxatm1, dum_sfxocn1, dum_sfxsed1, dum_sfxsumrok1
+    real(c_double), intent(out), dimension(:,:,:) :: dum_sfcocn1
+    real(c_double) :: dtyr, S, A
+    real(c_double), allocatable :: ocn(:,:,:,:), vd(:,:,:,:), settle(:,:,:)
+    integer :: i, j, kb, q, l, io, is
+
     if (.not. (dum_dts > 0.0_c_double)) call check(CGO_E_ARG, 'step_biogem: dum_dts must be positive')
+    dtyr = dum_dts / conv_yr_s
+    if (abs(dtyr - bgc_dtyr) > 1.0E-12_dp * bgc_dtyr) &
+         call check(CGO_E_ARG, 'step_biogem: dum_dts differs from the step initialise_biogem_hip was given')
     call check(cgo_bio_step(eng, mem, 1_c_int32_t), 'cgo_bio_step')
+    dum_sfcocn1 = 0.0_dp
+    if (.not. mirror) return
+    allocate (ocn(nl, nx, ny, nz), vd(nl, nx, ny, nz), settle(5, nx, ny))
+    call check(cgo_bgc_download(eng, mem(1), 0_c_int32_t, ocn), 'cgo_bgc_download ocn')
+    call check(cgo_bgc_download(eng, mem(1), 3_c_int32_t, vd), 'cgo_bgc_download anomalies')
+    call check(cgo_bgc_download(eng, mem(1), 5_c_int32_t, settle), 'cgo_bgc_download rain')
+    do j = 1, ny
+       do i = 1, nx
+          kb = k1_c(i, j)
+          if (kb > nz) cycle
+          ! (2) bottom-water tracers: T, S and the carried species
+          do l = 1, 2
+             if (l <= size(dum_sfcocn1, 1)) dum_sfcocn1(l, i, j) = ocn(l, i, j, kb) + vd(l, i, j, kb)
+          end do
+          do q = 1, 6
+             if (bgc_slots(q) <= 0) cycle
+             io = io_of_slot(q)
+             if (io <= size(dum_sfcocn1, 1)) dum_sfcocn1(io, i, j) = ocn(bgc_slots(q), i, j, kb) + &
+                  vd(bgc_slots(q), i, j, kb)
+          end do
+          if (size(dum_sfcocn1, 1) >= io_SO4 .and. size(dum_sfcocn1, 1) >= 2) then
+             S = dum_sfcocn1(2, i, j)
+             dum_sfcocn1(io_Ca, i, j) = max(0.01028_dp * S / 35.0_dp, nullsmall)    ! gem_carbchem.f90:834-890
+             dum_sfcocn1(io_B, i, j) = max(0.000416_dp * S / 35.0_dp, nullsmall)
+             dum_sfcocn1(io_SO4, i, j) = max(0.02824_dp * S / 35.0_dp, nullsmall)
+             dum_sfcocn1(io_F, i, j) = max(0.00007_dp * S / 35.0_dp, nullsmall)
+          end if
+          ! (3) ocean -> sediment flux (mol m-2 s-1); the bottom cell's area
+          ! as sub_init_phys_ocn forms it (biogem_data.f90:1120-1128)
+          A = 2.0_dp * const_pi * (const_rEarth * const_rEarth) * (1.0_dp / nx) * (jtab(j, 9) - jtab(j - 1, 9))
+          do q = 1, 5
+             is = is_of(q)
+             if (is > size(dum_sfxsed1, 1)) cycle
+             if (q >= 4) then
+                dum_sfxsed1(is, i, j) = settle(q, i, j) * (1.0_dp / dum_dts) * dtyr
+             else
+                dum_sfxsed1(is, i, j) = (1.0_dp / A) * settle(q, i, j) * (1.0_dp / dum_dts)
+             end if
+          end do
+       end do
+    end do
   end subroutine step_biogem
 
   ! biogem_tracercoupling(dum_ts, dum_ts1) (biogem.f90:1885-1890)
@@ -189,6 +394,7 @@ contains
     rc = cgo_destroy(eng)
     eng = c_null_ptr
     if (allocated(k1_c)) deallocate (k1_c, jtab, ktab)
+    if (allocated(u_last)) deallocate (u_last, sflux_c)
   end subroutine end_cgenie_hip
 
 end module cgenie_hip

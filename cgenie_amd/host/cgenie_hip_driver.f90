@@ -1,6 +1,11 @@
 ! cgenie_hip_driver.f90 -- runs the tracer path in the genie.f90 order
 ! (genie.f90:306-313 GOLDSTEIN every ocean step; 360-381 BIOGEM and the
-! tracer coupling every kbiogem ocean steps) through the cgenie_hip module.
+! tracer coupling every kbiogem ocean steps) through the cgenie_hip module,
+! with the argument lists of the reference's wrappers
+! (genie_loop_wrappers.f90:143-150, 313-315, 321): step_goldstein with zero
+! surface heat and freshwater fluxes and wind stress, step_biogem with the
+! BIOGEM step in seconds and the interface arrays of an ocean-only run,
+! biogem_tracercoupling.
 !
 !   cgenie_hip_driver INPUT OUTPUT
 !
@@ -11,7 +16,10 @@
 ! (each 0:maxk+1); float64 go_u(3,0:maxi,0:maxj,maxk),
 ! go_ts(maxl,maxi,maxj,maxk); if has_bgc: int32 slots(6) (1-based, 0 =
 ! absent), float64 dtyr, carb_tol, light(maxi,maxj), mld(maxi,maxj).
-! OUTPUT receives float64 go_ts(maxl,maxi,maxj,maxk) then inventory(maxl).
+! OUTPUT receives float64 go_ts(maxl,maxi,maxj,maxk), inventory(maxl),
+! go_rho(maxi,maxj,maxk), the energy and water diagnostics of the last
+! step_goldstein, dum_sfcocn1(1:38,maxi,maxj) and dum_sfxsed1(1:34,maxi,maxj)
+! of the last step_biogem (both zero without mirroring).
 program cgenie_hip_driver
   use, intrinsic :: iso_c_binding
   use cgenie_hip
@@ -24,6 +32,15 @@ program cgenie_hip_driver
   real(c_double), allocatable :: jt(:,:), kt(:,:), go_u(:,:,:,:), go_ts(:,:,:,:), go_ts1(:,:,:,:)
   real(c_double), allocatable :: light(:,:), mld(:,:), inv(:)
   integer(c_int32_t) :: slots(6)
+  ! the genie_global arrays of the calls (genie_global.f90): surface fluxes
+  ! and outputs (ilon1_ocn, ilat1_ocn), go_uvw, go_cost, go_tau, go_psi,
+  ! go_mldta, go_rho, and the BIOGEM interface arrays (tracers, lon, lat)
+  real(c_double), allocatable :: f2(:,:,:), go_uvw(:,:,:,:), go_cost(:,:), go_tau(:,:,:), go_psi(:,:)
+  real(c_double), allocatable :: go_mldta(:,:), go_rho(:,:,:), sfcatm(:,:,:), sfxatm(:,:,:), sfcocn(:,:,:)
+  real(c_double), allocatable :: sfxocn(:,:,:), sfcsed(:,:,:), sfxsed(:,:,:), sfxrok(:,:,:)
+  real(c_double) :: test_energy, test_water, dts
+  integer(c_int64_t) :: clock
+  real(c_double), parameter :: conv_yr_s = 3600.0_c_double * (24.0_c_double * 365.25_c_double)
   character(len=4096) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -57,10 +74,22 @@ program cgenie_hip_driver
   if (has_bgc) call initialise_biogem_hip(int(slots), dtyr, light, mld, carb_tol)
   call cgenie_hip_set_mirror(hdr(11) /= 0)
 
+  allocate (f2(maxi, maxj, 18), go_uvw(3, maxi, maxj, maxk), go_cost(maxi, maxj), go_tau(2, maxi, maxj))
+  allocate (go_psi(0:maxi, 0:maxj), go_mldta(maxi, maxj), go_rho(maxi, maxj, maxk))
+  allocate (sfcatm(19, maxi, maxj), sfxatm(19, maxi, maxj), sfcocn(95, maxi, maxj), sfxocn(95, maxi, maxj))
+  allocate (sfcsed(79, maxi, maxj), sfxsed(79, maxi, maxj), sfxrok(95, maxi, maxj))
+  f2 = 0; go_cost = 0; go_tau = 0; go_psi = 0; go_mldta = 0; go_rho = 0
+  sfcatm = 0; sfxatm = 0; sfcocn = 0; sfxocn = 0; sfcsed = 0; sfxsed = 0; sfxrok = 0
+  go_uvw = go_u(:, 1:maxi, 1:maxj, :)
+  dts = dtyr * conv_yr_s    ! dtyr is the BIOGEM step (kbiogem ocean steps)
+  clock = 0
   do koverall = 1, nsteps
-     call step_goldstein_tracers(go_ts, go_ts1)
+     call step_goldstein(koverall, f2(:,:,1), f2(:,:,2), f2(:,:,3), f2(:,:,4), f2(:,:,5), f2(:,:,6), &
+          f2(:,:,7), f2(:,:,8), f2(:,:,9), f2(:,:,10), f2(:,:,11), f2(:,:,12), f2(:,:,13), &
+          f2(:,:,14), f2(:,:,15), f2(:,:,16), f2(:,:,17), f2(:,:,18), test_energy, test_water, &
+          go_ts, go_ts1, go_cost, go_uvw, go_tau, go_psi, go_mldta, go_rho)
      if (has_bgc .and. mod(koverall, kbiogem) == 0) then
-        call step_biogem(dtyr)
+        call step_biogem(dts, clock, sfcatm, sfxatm, sfcocn, sfxocn, sfcsed, sfxsed, sfxrok)
         call biogem_tracercoupling(go_ts, go_ts1)
      end if
   end do
@@ -71,6 +100,10 @@ program cgenie_hip_driver
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
   write (u) go_ts
   write (u) inv
+  write (u) go_rho
+  write (u) test_energy, test_water
+  write (u) sfcocn(1:38, :, :)
+  write (u) sfxsed(1:34, :, :)
   close (u)
   write (*, '(a,i0,a)') 'cgenie_hip_driver: ', nsteps, ' ocean steps done'
 end program cgenie_hip_driver

@@ -148,6 +148,27 @@ module cgo_api
        type(cgo_diag), intent(out) :: d
      end function cgo_get_diag
 
+     integer(c_int) function cgo_set_cost(e, m, cost) bind(C, name="cgo_set_cost")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(in) :: cost(*)
+     end function cgo_set_cost
+
+     integer(c_int) function cgo_download_cost(e, m, cost) bind(C, name="cgo_download_cost")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(out) :: cost(*)
+     end function cgo_download_cost
+
+     integer(c_int) function cgo_inventory_noalloc(e, m, inv) bind(C, name="cgo_inventory_noalloc")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(out) :: inv(*)
+     end function cgo_inventory_noalloc
+
      subroutine cgo_bgc_default_params(p, dtyr) bind(C, name="cgo_bgc_default_params")
        import :: c_double, cgo_bgc_params
        type(cgo_bgc_params), intent(out) :: p

@@ -119,6 +119,10 @@ int cgo_inventory(cgo_engine *e, int32_t m, double *inv);               /* maxl,
  * and race. */
 int cgo_inventory_noalloc(cgo_engine *e, int32_t m, double *inv);
 int cgo_get_diag(cgo_engine *e, int32_t m, cgo_diag *d);
+/* the convection count cost(maxi,maxj) that co accumulates (goldstein.f90:2759);
+ * step_goldstein takes it in from go_cost and hands it back (goldstein.f90:99, 447) */
+int cgo_set_cost(cgo_engine *e, int32_t m, const double *cost);
+int cgo_download_cost(cgo_engine *e, int32_t m, double *cost);
 
 /* BIOGEM: state from the current go_ts (ocn = ts*S/Smean, T+273.15,
  * S+34.9), sub_init_phys_ocn geometry, [H] start 10**(-7.8).
@@ -129,7 +133,9 @@ int cgo_bgc_init(cgo_engine *e, int32_t m, const cgo_bgc_params *p, const double
  * slots zero), 1 = bio_part (5,maxi,maxj,maxk), 2 = carb (10,maxi,maxj),
  * 3 = last anomaly vdocn (maxl,maxi,maxj,maxk), 4 = carbconst (17,maxi,maxj)
  * of the surface cells from the current ocn T and S (sub_calc_carbconst,
- * gem_carbchem.f90:59-268; land columns zero) */
+ * gem_carbchem.f90:59-268; land columns zero), 5 = bio_settle at k1
+ * (5,maxi,maxj), the last step's rain onto the sea floor in mol per step
+ * (POC, POP, CaCO3) and the recalcitrant fractions (biogem.f90:1746-1754) */
 int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
 
 /* stepping: nsteps ocean steps of every listed member, batched into one

@@ -7,6 +7,7 @@ the C compiler gives the structs of include/cgocn.h.
 GPU: the driver, stepping in the genie.f90 order through tstepo /
 step_biogem / biogem_tracercoupling, reproduces the Python path through the
 same library bit for bit and the oracle to 1e-12."""
+import ctypes as C
 import os
 import shutil
 import subprocess
@@ -46,6 +47,48 @@ def test_bind_c_types_match_c_structs(tmp_path):
     assert [fsz["cgo_grid_desc"], fsz["cgo_member_params"], fsz["cgo_bgc_params"], fsz["cgo_diag"]] == csz
 
 
+REF_SRC = "/root/reference/src"
+
+
+@needs_flang
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="the reference tree is not present (checked where it is)")
+def test_reference_wrapper_calls_compile_against_module(tmp_path):
+    """The reference's goldstein_wrapper, biogem_wrapper and
+    biogem_tracercoupling_wrapper (genie_loop_wrappers.f90:122-151, 310-322),
+    taken from the reference file with only their `USE goldstein` /
+    `USE biogem` line swapped for `USE cgenie_hip`, compile against the module
+    with the reference's genie_global (compiled where it lies, with the
+    reference's -fdefault-real-8): the CALL lines of step_goldstein,
+    step_biogem and biogem_tracercoupling are unchanged."""
+    import re
+
+    _build_host()
+    flags = ["-cpp", "-fdefault-real-8", "-O0", "-module-dir", str(tmp_path), "-I", str(tmp_path)]
+    for f in ("common/gem_cmn.f90", "wrappers/genie_control.f90", "wrappers/genie_global.f90"):
+        r = subprocess.run(["amdflang", *flags, "-c", os.path.join(REF_SRC, f), "-o",
+                            str(tmp_path / (os.path.basename(f) + ".o"))], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+    src = open(os.path.join(REF_SRC, "wrappers", "genie_loop_wrappers.f90")).read()
+    parts = []
+    for name, use in (("goldstein_wrapper", "USE goldstein"), ("biogem_wrapper", "USE biogem"),
+                      ("biogem_tracercoupling_wrapper", "USE biogem")):
+        m = re.search(rf"^[ \t]*SUBROUTINE {name}[ \t]*$.*?^[ \t]*END SUBROUTINE {name}[ \t]*$", src, re.M | re.S)
+        assert m, name
+        body = m.group(0)
+        assert len(re.findall(rf"^[ \t]*{use}[ \t]*$", body, re.M)) == 1, name
+        assert "CALL " in body
+        parts.append(re.sub(rf"^([ \t]*){use}[ \t]*$", r"\1USE cgenie_hip", body, flags=re.M))
+    w = tmp_path / "wrappers_on_gpu.f90"
+    # IMPLICIT NONE (EXTERNAL): every CALL must resolve to a procedure with an
+    # explicit interface (the module's), not to an implicit external one
+    w.write_text("MODULE wrappers_on_gpu\n  USE genie_global\n  IMPLICIT NONE (TYPE, EXTERNAL)\nCONTAINS\n" +
+                 "\n".join(parts) +
+                 "\nEND MODULE wrappers_on_gpu\n")
+    r = subprocess.run(["amdflang", *flags, "-I", os.path.join(HOST, "_build"), "-c", str(w), "-o",
+                        str(tmp_path / "wrappers_on_gpu.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+
+
 def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, mirror=False, dtyr=1.0 / 96):
     g = c.g
     hdr = np.array([g.maxi, g.maxj, g.maxk, c.tset.L, nsteps, kbiogem, 1, precision, 1 if bgc else 0, 0,
@@ -70,11 +113,23 @@ def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, m
             f.write(np.ascontiguousarray(c.mld, dtype="<f8").tobytes())
 
 
+CONV_YR_S = 3600.0 * (24.0 * 365.25)              # gem_cmn.f90:511-513
+IO_OF = {"T": 1, "S": 2, "DIC": 3, "PO4": 8, "O2": 10, "ALK": 12, "DOM_C": 15, "DOM_P": 20}  # gem_cmn.f90:72-90
+IS_OF = (3, 8, 14, 33, 34)                        # POC, POP, CaCO3, POC_frac2, CaCO3_frac2 (gem_cmn.f90:190-255)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mirror", [False, True])
 def test_driver_genie_order_matches_engine_and_oracle(tmp_path, mirror):
+    """The driver calls step_goldstein, step_biogem and biogem_tracercoupling
+    with the reference's argument lists: the state equals the Python path
+    through the same library bit for bit and the oracle to 1e-12; with
+    mirroring on, go_rho, the energy and water diagnostics, the bottom-water
+    composition dum_sfcocn1 and the rain dum_sfxsed1 of the last step match
+    the oracle's (biogem.f90:1719-1754, goldstein.f90:458-471)."""
     assert os.path.exists(DRIVER), "build the host driver first (__graft_entry__.build())"
     c = make_case(kind="config2", seed=81)
+    g, t = c.g, c.tset
     nsteps = 6
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     write_driver_input(fin, c, nsteps, mirror=mirror)
@@ -83,23 +138,84 @@ def test_driver_genie_order_matches_engine_and_oracle(tmp_path, mirror):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     raw = np.fromfile(fout, dtype="<f8")
-    got = raw[: c.ts.size].reshape(c.ts.shape)
-    inv = raw[c.ts.size:]
+    n_ts, n_c, n_col = c.ts.size, g.maxi * g.maxj * g.maxk, g.maxi * g.maxj
+    o = 0
+    got = raw[o:o + n_ts].reshape(c.ts.shape); o += n_ts
+    inv = raw[o:o + t.L]; o += t.L
+    rho = raw[o:o + n_c].reshape(g.maxk, g.maxj, g.maxi); o += n_c
+    energy, water = raw[o], raw[o + 1]; o += 2
+    sfcocn = raw[o:o + 38 * n_col].reshape(g.maxj, g.maxi, 38); o += 38 * n_col
+    sfxsed = raw[o:o + 34 * n_col].reshape(g.maxj, g.maxi, 34); o += 34 * n_col
+    assert o == raw.size
     # the same library driven from Python
     eng = E.Engine(0)
-    m = eng.add_member(eng.add_grid(c.g, c.k1), c.tset.L, E.FP64, c.diff)
+    m = eng.add_member(eng.add_grid(c.g, c.k1), t.L, E.FP64, c.diff)
     eng.set_velocity(m, c.u)
     eng.upload(m, c.ts)
-    eng.bgc_init(m, E.default_bgc_params(1.0 / 96, c.tset, carb_tol=1e-12, carb_maxit=1000), c.light, c.mld)
+    eng.bgc_init(m, E.default_bgc_params(1.0 / 96, t, carb_tol=1e-12, carb_maxit=1000), c.light, c.mld)
     eng.step([m], nsteps)
     ref = eng.download(m, c.ts.shape)
     inv_ref = eng.inventory(m)
+    rho_ref = eng.rho(m, g)
     eng.close()
     assert np.array_equal(got, ref)
     assert np.array_equal(inv, inv_ref)
+    # the oracle, the last step split to see step_biogem's outputs before the coupling
     gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
-    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, c.tset), c.light, c.mld)
-    bio.step(nsteps)
-    err = rel_err(got, gold.ts, wet(c))
+    bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, t), c.light, c.mld)
+    bio.step(nsteps - 1)
+    gold.step(1)
+    ts_transported = gold.ts.copy()        # what the last step_goldstein handed back
+    L = O.lib()
+    L.og_biogem_step(C.byref(bio.st), C.byref(gold.g), C.byref(bio.p), gold.k1, bio.vdocn)
+    ocn_b, vd_b = bio.ocn(), bio.vdocn.copy()
+    settle = bio.arr("settle", O.NSED).reshape(g.maxk, g.maxj, g.maxi, O.NSED).copy()
+    L.og_tracercoupling(C.byref(bio.st), C.byref(gold.g), gold.k1, gold.ts, bio.vdocn)
+    w = wet(c)
+    err = rel_err(got, gold.ts, w)
     print("driver vs oracle max rel err", err.max(), flush=True)
     assert np.all(err <= 1e-12)
+    if not mirror:
+        assert not sfcocn.any() and not sfxsed.any()
+        return
+    assert np.array_equal(rho[w], rho_ref[w])
+    # energy and water relative to the start, summed as goldstein.f90:458-471
+    vsc = g.dphi * 6.37e6 * 6.37e6
+
+    def tot(ts):
+        e = wv = 0.0
+        for k in range(1, g.maxk + 1):
+            for j in range(1, g.maxj + 1):
+                e = e + ts[k - 1, j - 1, :, 0].sum() * g.dz[k] * g.ds[j]
+                wv = wv - ts[k - 1, j - 1, :, 1].sum() * g.dz[k] * g.ds[j]
+        return e * vsc * 5000.0 * 1000.0 * 3981.1, 1000.0 * wv * vsc * 5000.0 / 34.9
+
+    e0, w0 = tot(c.ts)
+    e1, w1 = tot(ts_transported)
+    print("energy, water diagnostics", energy, water, "oracle", e1 - e0, w1 - w0, flush=True)
+    assert abs(energy - (e1 - e0)) <= 1e-12 * abs(e0)
+    wscale = 1000.0 * vsc * 5000.0 / 34.9 * sum(np.abs(ts_transported[k - 1, j - 1, :, 1]).sum() * g.dz[k] * g.ds[j]
+                                             for k in range(1, g.maxk + 1) for j in range(1, g.maxj + 1))
+    assert abs(water - (w1 - w0)) <= 1e-12 * wscale
+    # bottom-water composition and rain of the last step_biogem
+    k1 = c.k1[1:-1, 1:-1]
+    dts = (1.0 / 96) * CONV_YR_S
+    dtyr = dts / CONV_YR_S
+    worst = 0.0
+    for j, i in np.argwhere(k1 <= g.maxk):
+        kb = k1[j, i] - 1
+        for name, io in IO_OF.items():
+            slot = {"T": 0, "S": 1}.get(name, getattr(t, name, -1))
+            if slot < 0:
+                continue
+            want = ocn_b[kb, j, i, slot] + vd_b[kb, j, i, slot]
+            worst = max(worst, abs(sfcocn[j, i, io - 1] - want) / max(abs(want), 1e-300))
+        S = sfcocn[j, i, 1]
+        assert sfcocn[j, i, 34] == max(0.01028 * S / 35.0, 0.999999e-19)    # Ca from salinity
+        A = 2.0 * 3.141592653589793 * (6.37e6 * 6.37e6) * (1.0 / g.maxi) * (g.sv[j + 1] - g.sv[j])
+        for q, isx in enumerate(IS_OF):
+            sv_ = settle[kb, j, i, q]
+            want = sv_ * (1.0 / dts) * dtyr if q >= 3 else (1.0 / A) * sv_ * (1.0 / dts)
+            worst = max(worst, abs(sfxsed[j, i, isx - 1] - want) / max(abs(want), 1e-300))
+    print("interface arrays vs oracle max rel err", worst, flush=True)
+    assert worst <= 1e-12
