@@ -51,6 +51,7 @@ struct GridH {
     std::vector<double> invw;     // [ncell] inventory weights dphi*ds(j)*dz(k)
     int ftw[2] = {0, 0};          // widest flux tile for 128- and 256-thread blocks
     double Vtot = 0;
+    uint64_t fingerprint = 0;     // of the dimensions, mask and metric tables (checkpoints)
     void* dbuf = nullptr;
     double* d_invw = nullptr;
 };
@@ -432,6 +433,20 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
     gd.rdphi = d->rdphi;
     gd.rVtot = g->Vtot > 0 ? 1.0 / g->Vtot : 0.0;
     for (int q = 0; q < 5; q++) gd.ec[q] = d->ec[q];
+    {
+        // FNV-1a over the dimensions, the mask and the metric tables
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](const void* p, size_t n) {
+            const unsigned char* b = static_cast<const unsigned char*>(p);
+            for (size_t q = 0; q < n; q++) h = (h ^ b[q]) * 1099511628211ull;
+        };
+        const int32_t dims[3] = {mi, mj, mk};
+        mix(dims, sizeof dims);
+        mix(g->k1.data(), g->k1.size() * 4);
+        mix(g->mj.data(), g->mj.size() * 8);
+        mix(g->mk.data(), g->mk.size() * 8);
+        g->fingerprint = h;
+    }
     e->grids.push_back(g);
     *grid_id = (int32_t)e->grids.size() - 1;
     return CGO_OK;
@@ -794,6 +809,146 @@ int cgo_inventory_noalloc(cgo_engine* e, int32_t id, double* inv) {
     return CGO_OK;
 }
 
+// the device copy of the BIOGEM parameters, with each carried species' slot
+// in the state and its plane in the ocn block
+static void set_bgc_params(MemberH* m, const cgo_bgc_params& bp) {
+    m->bp = bp;
+    BgcParams& P = m->dev.bp;
+    const int32_t slots[NBGC] = {bp.slot_DIC, bp.slot_PO4, bp.slot_O2, bp.slot_ALK, bp.slot_DOM_C, bp.slot_DOM_P};
+    int nplane = 2;
+    for (int b = 0; b < NBGC; b++) {
+        P.slot[b] = slots[b];
+        P.ocn[b] = slots[b] >= 0 ? nplane++ : -1;
+    }
+    P.caco3 = bp.caco3;
+    P.dtyr = bp.dtyr;
+    P.k0_PO4 = bp.k0_PO4;
+    P.c0_PO4 = bp.c0_PO4;
+    P.DOMfrac = bp.DOMfrac;
+    P.DOMlifetime = bp.DOMlifetime;
+    P.red_POP_POC = bp.red_POP_POC;
+    P.red_POP_PO2 = bp.red_POP_PO2;
+    P.red_PON_ALK = bp.red_PON_ALK;
+    P.red_POP_PON = bp.red_POP_PON;
+    P.red_POC_CaCO3 = bp.red_POC_CaCO3;
+    P.red_POC_CaCO3_pP = bp.red_POC_CaCO3_pP;
+    P.POC_frac2 = bp.POC_frac2;
+    P.POC_dfrac2 = bp.POC_dfrac2;
+    P.POC_c0frac2 = bp.POC_c0frac2;
+    P.POC_eL1 = bp.POC_eL1;
+    P.POC_eL2 = bp.POC_eL2;
+    P.CaCO3_frac2 = bp.CaCO3_frac2;
+    P.CaCO3_eL1 = bp.CaCO3_eL1;
+    P.CaCO3_eL2 = bp.CaCO3_eL2;
+    P.sinkrate = bp.sinkrate;
+    P.remin_k_O2 = bp.remin_k_O2;
+    P.remin_c0_O2 = bp.remin_c0_O2;
+    P.carb_tol = bp.carb_tol;
+    P.carb_maxit = bp.carb_maxit;
+    P.dsc = bp.dsc;
+}
+
+// ---------------------------------------------------------- checkpoints
+// blob = CkptHdr, then the member's whole device allocation as laid out by
+// layout() (state ping-pong and carry planes, velocities, surface flux,
+// rho, cost, control words, BIOGEM arrays and scalars): restoring it into a
+// member of the same grid, tracer count, precision and carry continues the
+// run bit for bit.
+struct CkptHdr {
+    char magic[8];
+    uint32_t version, hdr_bytes;
+    int32_t maxi, maxj, maxk, L, prec, carry, bgc, diso, wstream, coupling_due, has_sflux, pad;
+    uint64_t grid_fp, payload;
+    double diff[2];
+    cgo_bgc_params bp;
+};
+static const char CKPT_MAGIC[8] = {'C', 'G', 'O', 'C', 'K', 'P', 'T', '1'};
+
+int64_t cgo_checkpoint_bytes(cgo_engine* e, int32_t id) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "checkpoint_bytes: bad member %d", id);
+    return (int64_t)(sizeof(CkptHdr) + m->bytes);
+}
+
+int cgo_checkpoint(cgo_engine* e, int32_t id, void* buf, size_t n) {
+    MemberH* m = member(e, id);
+    if (!m || !buf) return fail(e, CGO_E_ARG, "checkpoint: bad argument");
+    if (n < sizeof(CkptHdr) + m->bytes)
+        return fail(e, CGO_E_ARG, "checkpoint: buffer of %zu bytes, %zu needed", n, sizeof(CkptHdr) + m->bytes);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    CkptHdr h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, CKPT_MAGIC, 8);
+    h.version = 1;
+    h.hdr_bytes = (uint32_t)sizeof(CkptHdr);
+    const GridDev& g = m->dev.g;
+    h.maxi = g.maxi;
+    h.maxj = g.maxj;
+    h.maxk = g.maxk;
+    h.L = m->L;
+    h.prec = m->prec;
+    h.carry = m->p.carry;
+    h.bgc = m->bgc;
+    h.diso = m->p.diso;
+    h.wstream = m->wstream;
+    h.coupling_due = m->coupling_due ? 1 : 0;
+    h.has_sflux = m->dev.sflux != nullptr;
+    h.grid_fp = e->grids[m->grid]->fingerprint;
+    h.payload = m->bytes;
+    h.diff[0] = m->p.diff[0];
+    h.diff[1] = m->p.diff[1];
+    h.bp = m->bp;
+    memcpy(buf, &h, sizeof h);
+    HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h, m->dbuf, m->bytes, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
+int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
+    MemberH* m = member(e, id);
+    if (!m || !buf || n < sizeof(CkptHdr)) return fail(e, CGO_E_ARG, "restore: bad argument");
+    CkptHdr h;
+    memcpy(&h, buf, sizeof h);
+    if (memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1 || h.hdr_bytes != sizeof(CkptHdr))
+        return fail(e, CGO_E_ARG, "restore: not a checkpoint of this library version");
+    const GridH* gh = e->grids[m->grid];
+    if (h.grid_fp != gh->fingerprint || h.L != m->L || h.prec != m->prec || h.carry != m->p.carry)
+        return fail(e, CGO_E_ARG, "restore: checkpoint of another grid, tracer count, precision or carry "
+                                  "(grid %dx%dx%d, %d tracers, precision %d)", h.maxi, h.maxj, h.maxk, h.L, h.prec);
+    if (n < sizeof h + h.payload) return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n,
+                                              (size_t)(sizeof h + h.payload));
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    int32_t slot = id;
+    if (h.bgc && !m->bgc) {
+        m->bgc = 1;
+        int rc = alloc_member(e, m);
+        if (rc) return rc;
+    }
+    if (h.payload != m->bytes)
+        return fail(e, CGO_E_STATE, "restore: the member's layout (%zu bytes) differs from the checkpoint's (%zu)",
+                    m->bytes, (size_t)h.payload);
+    HIPCHK(e, hipMemcpy(m->dbuf, static_cast<const char*>(buf) + sizeof h, h.payload, hipMemcpyHostToDevice));
+    m->p.diff[0] = h.diff[0];
+    m->p.diff[1] = h.diff[1];
+    m->p.diso = h.diso;
+    m->dev.diff1 = h.diff[0];
+    m->dev.diff2 = h.diff[1];
+    m->dev.diff1f = (float)h.diff[0];
+    m->dev.diff2f = (float)h.diff[1];
+    m->dev.diso = h.diso;
+    if (h.bgc) {
+        set_bgc_params(m, h.bp);
+        m->wstream = h.wstream;
+    }
+    m->coupling_due = h.coupling_due != 0;
+    const MemLayout ly = layout(gh, m->L, m->prec, m->p.carry, m->bgc);
+    m->dev.sflux = h.has_sflux ? reinterpret_cast<double*>(static_cast<char*>(m->dbuf) + ly.sflux) : nullptr;
+    e->table_dirty = true;
+    bump_gen(e, slot);
+    return CGO_OK;
+}
+
 int cgo_set_cost(cgo_engine* e, int32_t id, const double* cost) {
     MemberH* m = member(e, id);
     if (!m || !cost) return fail(e, CGO_E_ARG, "set_cost: bad argument");
@@ -885,39 +1040,8 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
         bump_gen(e, id);  // new layout, whatever follows
         if (rc) return rc;
     }
-    m->bp = *bp;
-    BgcParams& P = m->dev.bp;
-    int nplane = 2;
-    for (int b = 0; b < NBGC; b++) {
-        P.slot[b] = slots[b];
-        P.ocn[b] = slots[b] >= 0 ? nplane++ : -1;
-    }
-    P.caco3 = bp->caco3;
-    P.dtyr = bp->dtyr;
-    P.k0_PO4 = bp->k0_PO4;
-    P.c0_PO4 = bp->c0_PO4;
-    P.DOMfrac = bp->DOMfrac;
-    P.DOMlifetime = bp->DOMlifetime;
-    P.red_POP_POC = bp->red_POP_POC;
-    P.red_POP_PO2 = bp->red_POP_PO2;
-    P.red_PON_ALK = bp->red_PON_ALK;
-    P.red_POP_PON = bp->red_POP_PON;
-    P.red_POC_CaCO3 = bp->red_POC_CaCO3;
-    P.red_POC_CaCO3_pP = bp->red_POC_CaCO3_pP;
-    P.POC_frac2 = bp->POC_frac2;
-    P.POC_dfrac2 = bp->POC_dfrac2;
-    P.POC_c0frac2 = bp->POC_c0frac2;
-    P.POC_eL1 = bp->POC_eL1;
-    P.POC_eL2 = bp->POC_eL2;
-    P.CaCO3_frac2 = bp->CaCO3_frac2;
-    P.CaCO3_eL1 = bp->CaCO3_eL1;
-    P.CaCO3_eL2 = bp->CaCO3_eL2;
-    P.sinkrate = bp->sinkrate;
-    P.remin_k_O2 = bp->remin_k_O2;
-    P.remin_c0_O2 = bp->remin_c0_O2;
-    P.carb_tol = bp->carb_tol;
-    P.carb_maxit = bp->carb_maxit;
-    P.dsc = bp->dsc;
+    set_bgc_params(m, *bp);
+    const BgcParams& P = m->dev.bp;
     // host-side initial state, as sub_init_phys_ocn / sub_init_carb and the
     // inverse of the ts normalisation (biogem.f90:2046)
     const GridH* g = e->grids[m->grid];

@@ -85,6 +85,9 @@ def load() -> C.CDLL:
         "cgo_inventory_noalloc": ([P, _i32, _dp], C.c_int),
         "cgo_get_diag": ([P, _i32, C.POINTER(Diag)], C.c_int),
         "cgo_set_cost": ([P, _i32, _dp], C.c_int),
+        "cgo_checkpoint_bytes": ([P, _i32], _i64),
+        "cgo_checkpoint": ([P, _i32, C.c_void_p, C.c_size_t], C.c_int),
+        "cgo_restore": ([P, _i32, C.c_void_p, C.c_size_t], C.c_int),
         "cgo_download_cost": ([P, _i32, _dp], C.c_int),
         "cgo_bgc_default_params": ([C.POINTER(BgcParams), _d], None),
         "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
@@ -258,6 +261,19 @@ class Engine:
         d = Diag()
         self._chk(self.L.cgo_get_diag(self.h, m, C.byref(d)), "get_diag")
         return d
+
+    def checkpoint(self, m: int) -> bytes:
+        """The member's exact resident state (cgo_checkpoint)."""
+        n = int(self.L.cgo_checkpoint_bytes(self.h, m))
+        if n < 0:
+            self._chk(n, "checkpoint_bytes")
+        buf = C.create_string_buffer(n)
+        self._chk(self.L.cgo_checkpoint(self.h, m, buf, n), "checkpoint")
+        return buf.raw
+
+    def restore(self, m: int, blob: bytes):
+        """Continue member m from a checkpoint (cgo_restore)."""
+        self._chk(self.L.cgo_restore(self.h, m, blob, len(blob)), "restore")
 
     def bgc_init(self, m: int, p: BgcParams, light: np.ndarray, mld: np.ndarray):
         lt = np.ascontiguousarray(light, dtype=np.float64)

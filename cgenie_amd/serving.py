@@ -18,6 +18,12 @@ set, captured when the set (or the round length) changes, so a steady set of
 runs is stepped with one graph launch per round instead of one launch per
 kernel per step.
 
+A run can be suspended between rounds (``suspend``): its exact checkpoint
+(cgo_checkpoint) is kept on the host, its device memory is freed and it
+waits to be admitted again like a new request; on re-admission it is
+restored (cgo_restore) and continues to the bits it would have reached
+uninterrupted.
+
 Runs with at least seven tracers carry BIOGEM (T, S, DIC, ALK, PO4, O2,
 DOM_P, then dyes) and run the full step (tstepo, step_biogem,
 biogem_tracercoupling); smaller runs are T, S and dyes on GOLDSTEIN alone.
@@ -60,6 +66,8 @@ class Request:
     year_done: list = field(default_factory=list)
     result: np.ndarray | None = None
     inventory: np.ndarray | None = None
+    checkpoint: bytes | None = None   # held on the host while suspended
+    suspensions: int = 0
 
     @property
     def nyear(self) -> int:
@@ -119,6 +127,7 @@ class Server:
         self.prep: dict = {}
         self.graph, self.gkey = None, None   # the round graph and the (members, steps) it was captured for
         self.captures = 0
+        self.on_round = None                 # called with the server after every round (a scheduling hook)
 
     # -- requests
     def submit(self, req: Request):
@@ -152,17 +161,35 @@ class Server:
                 break                    # first come, first served: wait for memory
             heapq.heappop(self.waiting)
             gid = self.grids.get(req.dims)[0]
-            tset, ts, u, diff, light, mld = self.prep.pop(req.rid).result()
+            tset, ts, u, diff, light, mld = self.prep[req.rid].result()
             m = self.eng.add_member(gid, req.L, req.precision, diff, diso=True, carry=True)
-            self.eng.set_velocity(m, u)
-            self.eng.upload(m, ts)
-            if req.L >= 7:
-                p = E.default_bgc_params(1.0 / req.nyear, tset, carb_tol=1e-12, carb_maxit=1000)
-                self.eng.bgc_init(m, p, light, mld)
-            req.member, req.nbytes, req.admitted = m, nb, self.now()
+            if req.checkpoint is not None:
+                # a suspended run: its exact state, velocities and BIOGEM arrays included
+                self.eng.restore(m, req.checkpoint)
+                req.checkpoint = None
+            else:
+                self.eng.set_velocity(m, u)
+                self.eng.upload(m, ts)
+                if req.L >= 7:
+                    p = E.default_bgc_params(1.0 / req.nyear, tset, carb_tol=1e-12, carb_maxit=1000)
+                    self.eng.bgc_init(m, p, light, mld)
+                req.admitted = self.now()
+            req.member, req.nbytes = m, nb
             req._shape = ts.shape
             self.used += nb
             self.active[m] = req
+
+    def suspend(self, req: Request):
+        """Take an active run off the GPU between rounds: keep its exact
+        checkpoint on the host, free its device memory and queue it again."""
+        if req.member not in self.active:
+            raise ValueError(f"request {req.rid} is not active")
+        req.checkpoint = self.eng.checkpoint(req.member)
+        self.eng.remove_member(req.member)
+        self.used -= req.nbytes
+        del self.active[req.member]
+        req.member, req.suspensions = -1, req.suspensions + 1
+        heapq.heappush(self.waiting, (self.now(), req.rid, req))
 
     def _retire(self, req: Request):
         if self.keep:
@@ -172,6 +199,7 @@ class Server:
         self.used -= req.nbytes
         req.finished = self.now()
         del self.active[req.member]
+        self.prep.pop(req.rid, None)
         self.done.append(req)
 
     def run(self, until_idle: bool = True, max_seconds: float | None = None, log=None):
@@ -200,6 +228,8 @@ class Server:
                     r.year_done.append(t)
                 if r.steps_done >= r.total_steps:
                     self._retire(r)
+            if self.on_round is not None:
+                self.on_round(self)
             if log and self.rounds % 20 == 0:
                 log(f"t={t:.1f}s active={len(self.active)} waiting={len(self.waiting)} done={len(self.done)} "
                     f"hbm={self.used / 1e9:.2f} GB")

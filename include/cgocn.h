@@ -124,6 +124,18 @@ int cgo_get_diag(cgo_engine *e, int32_t m, cgo_diag *d);
 int cgo_set_cost(cgo_engine *e, int32_t m, const double *cost);
 int cgo_download_cost(cgo_engine *e, int32_t m, double *cost);
 
+/* Exact checkpoint of one member (no counterpart in the reference, whose
+ * restarts are the netCDF files of goldstein_data.f90:153-354 and
+ * biogem.f90:2280-2360; cgenie_amd/restart.py writes and reads those): a
+ * header (grid fingerprint, tracer count, precision, carry, parameters,
+ * BIOGEM state flags) and every device array of the member.  cgo_restore
+ * into a member of the same grid, tracer count, precision and carry (BIOGEM
+ * arrays are added if the checkpoint has them) continues the run bit for
+ * bit; anything else is refused with CGO_E_ARG. */
+int64_t cgo_checkpoint_bytes(cgo_engine *e, int32_t m);
+int cgo_checkpoint(cgo_engine *e, int32_t m, void *buf, size_t n);
+int cgo_restore(cgo_engine *e, int32_t m, const void *buf, size_t n);
+
 /* BIOGEM: state from the current go_ts (ocn = ts*S/Smean, T+273.15,
  * S+34.9), sub_init_phys_ocn geometry, [H] start 10**(-7.8).
  * light = solfor/solar_constant (maxi,maxj); mld in m (maxi,maxj). */

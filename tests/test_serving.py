@@ -56,6 +56,12 @@ class CountingEngine:
     def drop_graph(self, g):
         self.dropped += 1
 
+    def checkpoint(self, m):
+        return self.members[m].to_bytes(8, "little")
+
+    def restore(self, m, blob):
+        self.members[m] = int.from_bytes(blob, "little")
+
     def sync(self):
         pass
 
@@ -97,6 +103,36 @@ def test_scheduler_admits_by_budget_and_recycles(monkeypatch):
     assert st["equiv_36x36x16x24_member_years_per_s"] * 6 == pytest.approx(st["member_years_per_s"])
 
 
+def test_suspended_run_resumes_where_it_stopped(monkeypatch):
+    """A run suspended between rounds leaves the GPU (its memory is
+    returned to the budget), waits like a new request and, restored from its
+    checkpoint, finishes with exactly its own number of steps."""
+    monkeypatch.setattr(S, "request_inputs", lambda req, g, k1, u: (None, np.zeros(1), None, (0, 0), None, None))
+    monkeypatch.setattr(S.GridCache, "get", lambda self, dims: (0, None, None, None))
+    eng = CountingEngine(bytes_per_member=100)
+    srv = S.Server(eng, hbm_budget=1000, round_steps=48)
+    srv.pool = SyncPool()
+    reqs = [S.Request(q, (36, 36, 16), 4, years=3, precision=E.FP64, seed=q) for q in range(2)]
+    for r in reqs:
+        srv.submit(r)
+    seen = []
+
+    def hook(server):
+        r = reqs[0]
+        if r.member in server.active and r.suspensions == 0 and r.steps_done == 96:
+            used = server.used
+            server.suspend(r)
+            seen.append((used - server.used, r.member, len(eng.members)))
+
+    srv.on_round = hook
+    st = srv.run()
+    assert seen == [(400, -1, 1)]
+    assert st["requests_done"] == 2 and not eng.members and srv.used == 0
+    for r in reqs:
+        assert r.steps_done == r.total_steps and len(r.year_done) == r.years
+    assert reqs[0].suspensions == 1 and reqs[0].checkpoint is None
+
+
 def test_poisson_stream_is_seeded():
     a = S.poisson_requests(20, 4.0, seed=3)
     b = S.poisson_requests(20, 4.0, seed=3)
@@ -115,9 +151,18 @@ def test_served_runs_equal_alone():
     reqs = [S.Request(q, d, L, y, p, 700 + q, arrival=t) for q, (d, L, y, p, t) in enumerate(mix)]
     for r in reqs:
         srv.submit(r)
+
+    def suspend_once(server):
+        # take two runs off the GPU part-way through; they resume from their checkpoints
+        for r in (reqs[1], reqs[3]):
+            if r.member in server.active and r.suspensions == 0 and r.steps_done >= 48:
+                server.suspend(r)
+
+    srv.on_round = suspend_once
     st = srv.run()
     print("served:", st, flush=True)
     assert st["requests_done"] == len(reqs)
+    assert reqs[1].suspensions == 1 and reqs[3].suspensions == 1
     for r in reqs:
         gid, g, k1, base_u = srv.grids.get(r.dims)
         tset, ts, u, diff, light, mld = S.request_inputs(r, g, k1, base_u)
