@@ -28,6 +28,7 @@ void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, i
                      size_t lds_cap, int phase, hipStream_t);
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
+void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
 }  // namespace cgo
 
 using namespace cgo;
@@ -36,8 +37,6 @@ namespace {
 
 constexpr int TABLE_CAP = 4096;
 constexpr int NCC_H = 17;  // carbonate constants per cell (n_carbconst, gem_cmn.f90:66)
-constexpr double CONST_PI = 3.141592653589793;  // gem_cmn.f90:688
-constexpr double CONST_REARTH = 6.37E+06;       // gem_cmn.f90:696
 constexpr double CONV_M3_KG = 1027.649;         // gem_cmn.f90:510
 
 struct GridH {
@@ -454,8 +453,8 @@ int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
 
 // byte layout of one member allocation
 struct MemLayout {
-    size_t st[2], lo[2], u, sflux, rho, cost, ctrl, isoc, ocn, part, settle, carb, M, rM, vd, light, mld, colsum, colold,
-        scal, rtab, mink, total;
+    size_t st[2], lo[2], u, sflux, rho, cost, ctrl, isoc, ocn, part, settle, carb, M, rM, vd, light, mld, wspeed, atm,
+        fatm, colsum, colold, scal, rtab, mink, total;
 };
 
 static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
@@ -489,6 +488,9 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
         m.vd = take(NBGC * ncell * 8);
         m.light = take(ncol * 8);
         m.mld = take(ncol * 8);
+        m.wspeed = take(ncol * 8);
+        m.atm = take(2 * ncol * 8);
+        m.fatm = take(2 * ncol * 8);
         m.colsum = take((L + 1) * nwet * 8);
         m.colold = take((L + 1) * nwet * 8);
         m.scal = take((SC_FACT0 + L + 1) * 8);
@@ -559,6 +561,9 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
         d.vd = reinterpret_cast<double*>(b + ly.vd);
         d.light = reinterpret_cast<double*>(b + ly.light);
         d.mld = reinterpret_cast<double*>(b + ly.mld);
+        d.wspeed = reinterpret_cast<double*>(b + ly.wspeed);
+        d.atm = reinterpret_cast<double*>(b + ly.atm);
+        d.fatm = reinterpret_cast<double*>(b + ly.fatm);
         d.colsum = reinterpret_cast<double*>(b + ly.colsum);
         d.colold = reinterpret_cast<double*>(b + ly.colold);
         d.scal = reinterpret_cast<double*>(b + ly.scal);
@@ -846,6 +851,8 @@ static void set_bgc_params(MemberH* m, const cgo_bgc_params& bp) {
     P.carb_tol = bp.carb_tol;
     P.carb_maxit = bp.carb_maxit;
     P.dsc = bp.dsc;
+    P.airsea = bp.airsea;
+    P.gas_a = bp.gastransfer_a;
 }
 
 // ---------------------------------------------------------- checkpoints
@@ -1019,6 +1026,10 @@ void cgo_bgc_default_params(cgo_bgc_params* p, double dtyr) {
     p->carb_tol = 0.001;
     p->carb_maxit = 100;
     p->dsc = 5.0E3;
+    p->airsea = 0;
+    p->gastransfer_a = 0.310;  // biogem-defaults.nml:221
+    p->atm_pCO2 = 278.0E-6;    // atchem-defaults.nml:4
+    p->atm_pO2 = 0.2095;
 }
 
 int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const double* light, const double* mld) {
@@ -1159,6 +1170,15 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
     HIPCHK(e, hipMemcpy(d.rM, rMh.data(), ncell * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.light, lt.data(), ncol * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.mld, ml.data(), ncol * 8, hipMemcpyHostToDevice));
+    // the atmosphere starts uniform (sub_init_tracer_atm_comp, atchem_data.f90:230-260);
+    // the wind speed stays as cgo_set_windspeed last set it (zero until then)
+    {
+        std::vector<double> a(2 * ncol);
+        std::fill(a.begin(), a.begin() + ncol, bp->atm_pCO2);
+        std::fill(a.begin() + ncol, a.end(), bp->atm_pO2);
+        HIPCHK(e, hipMemcpy(d.atm, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemset(d.fatm, 0, 2 * ncol * 8));
+    }
     HIPCHK(e, hipMemcpy(d.colold, colold.data(), colold.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(d.scal, scal.data(), scal.size() * 8, hipMemcpyHostToDevice));
     const int32_t zero = 0;
@@ -1224,9 +1244,26 @@ int cgo_bgc_download(cgo_engine* e, int32_t id, int32_t what, double* out) {
         if (r != hipSuccess) return fail(e, CGO_E_HIP, "bgc_download carbconst: %s", hipGetErrorString(r));
         for (size_t c = 0; c < ncol; c++)
             for (int s = 0; s < NCC_H; s++) out[s + (size_t)NCC_H * c] = h[(size_t)s * ncol + c];
+    } else if (what == 6 || what == 7) {
+        std::vector<double> h((size_t)2 * ncol);
+        HIPCHK(e, hipMemcpy(h.data(), what == 6 ? m->dev.atm : m->dev.fatm, h.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < ncol; c++)
+            for (int q = 0; q < 2; q++) out[q + 2 * c] = h[(size_t)q * ncol + c];
     } else {
         return fail(e, CGO_E_ARG, "bgc_download: what %d", what);
     }
+    return CGO_OK;
+}
+
+int cgo_set_windspeed(cgo_engine* e, int32_t id, const double* wspeed) {
+    MemberH* m = member(e, id);
+    if (!m || !wspeed || !m->bgc) return fail(e, CGO_E_ARG, "set_windspeed: bad argument or no BIOGEM state");
+    const size_t ncol = m->dev.g.ncol;
+    for (size_t c = 0; c < ncol; c++)
+        if (!(wspeed[c] >= 0.0) || !std::isfinite(wspeed[c]))
+            return fail(e, CGO_E_ARG, "set_windspeed: wind speed %g at column %zu", wspeed[c], c);
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    HIPCHK(e, hipMemcpy(m->dev.wspeed, wspeed, ncol * 8, hipMemcpyHostToDevice));
     return CGO_OK;
 }
 
@@ -1237,6 +1274,7 @@ struct Lists {
     int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
     int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
+    int ncol_atm = 0;  // widest grid of the members with air-sea exchange, 0 if none
     FluxShape fs[2] = {};
 };
 
@@ -1261,6 +1299,7 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             ls.maxwet_bgc[p] = std::max(ls.maxwet_bgc[p], m->dev.g.nwet);
             ls.maxk_bgc[p] = std::max(ls.maxk_bgc[p], m->dev.g.maxk);
             ls.wstream[p] = std::max(ls.wstream[p], m->wstream);
+            if (m->dev.bp.airsea) ls.ncol_atm = std::max(ls.ncol_atm, m->dev.g.ncol);
         }
     }
     for (int q = 0; q < 5; q++) {
@@ -1298,11 +1337,17 @@ static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_list
     enqueue_co_flip(e, ls, d_lists);
 }
 
-// step_biogem for the members with BIOGEM state (biogem.f90:528-1877)
+// step_biogem for the members with BIOGEM state (biogem.f90:528-1877), then
+// for those with air-sea exchange the ocean -> atmosphere coupling and ATCHEM
+// (genie.f90:410-450; they read only the flux and the atmosphere, which
+// nothing else touches, so they may run before the tracer coupling)
 static void enqueue_bio(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
     for (int p = 0; p < 2; p++)
         if (ls.n[2 + p])
             launch_bio(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, s);
+    if (ls.ncol_atm)
+        for (int p = 0; p < 2; p++)
+            if (ls.n[2 + p]) launch_atchem(e->d_table, d_lists + ls.off[2 + p], ls.n[2 + p], ls.ncol_atm, s);
 }
 
 // biogem_tracercoupling (biogem.f90:1885-2077); phase as launch_coupling

@@ -33,6 +33,17 @@ constexpr int MAXK_DEV = 48;      // deepest grid the column kernels accept
 constexpr double NULLSMALL = +0.999999E-19;  // gem_cmn.f90:719
 constexpr double ZEROC = 273.15;
 constexpr double SALN0 = 34.9;
+// unit conversions and constants of gem_cmn.f90:510-545, 690-698
+constexpr double CONV_YR_HR = 24.0 * 365.25;
+constexpr double CONV_YR_S = 3600.0 * (24.0 * 365.25);
+constexpr double CONV_S_YR = 1.0 / (3600.0 * (24.0 * 365.25));
+constexpr double CONV_CM_M = 1.0 / 100.0;
+constexpr double CONV_PA_ATM = 1.0 / 1.01325E+05;
+constexpr double CONST_R_SI = 8.3145;
+constexpr double CONST_V = 0.022414;
+constexpr double CONST_PI = 3.141592653589793;
+constexpr double CONST_REARTH = 6.37E+06;
+constexpr double ATM_TH = 7777.0;  // par_atm_th, atchem_lib.f90:86
 
 // One block of the flux kernel: wet columns [w0, w1) of the ascending order,
 // staged (with every neighbour they read) over flattened columns [t0, t1).
@@ -80,6 +91,8 @@ struct BgcParams {
     double CaCO3_frac2, CaCO3_eL1, CaCO3_eL2;
     double sinkrate, remin_k_O2, remin_c0_O2, carb_tol, dsc;
     int32_t carb_maxit;
+    int32_t airsea;         // CO2 and O2 exchange with the well-mixed atmosphere
+    double gas_a;           // par_gastransfer_a
 };
 
 struct MemberDev {
@@ -108,6 +121,9 @@ struct MemberDev {
     double* vd;             // [NBGC][ncell] anomalies of the present species
     double* light;          // [ncol]
     double* mld;            // [ncol]
+    double* wspeed;         // [ncol] wind speed (m/s) of the air-sea exchange
+    double* atm;            // [2][ncol] atmospheric pCO2, pO2 (atm) over every column
+    double* fatm;           // [2][ncol] the last BIOGEM step's ocean -> atmosphere flux (mol yr-1)
     double* colsum;         // [(L+1)][nwet] column partial sums (NEW_l, S_new)
     double* colold;         // [L][nwet] column partial sums of OLD_l for the next coupling
     double* scal;           // member scalars, see SC_*

@@ -60,6 +60,56 @@ __device__ __forceinline__ double redfield(const BgcParams& p, double O2) {
     return p.remin_k_O2 * kO2 / lk;
 }
 
+// ---------------------------------------------------------------- air-sea
+// area of a cell in row J (biogem_data.f90:1156, atchem_data.f90:222)
+__device__ __forceinline__ double cell_area(const AS_GLOBAL double* sv, int maxi, int J) {
+    return 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / maxi) * (sv[J] - sv[J - 1]);
+}
+
+// fun_calc_solconst (gem_carbchem.f90:1367-1426) on the clamped T and S:
+// CO2 in mol kg-1 atm-1 (Weiss 1974), O2 from its Bunsen coefficient over
+// rho const_V; coefficient tables of gem_data.f90:68-141
+__device__ __forceinline__ double solconst(double b0, double b1, double b2, double b3, double b4, double b5,
+                                           bool bunsen, double T, double S, double rho) {
+    const double lT = T < ZEROC + 2.0 ? ZEROC + 2.0 : (T > (ZEROC + 35.0) ? ZEROC + 35.0 : T);
+    const double lS = S < 26.0 ? 26.0 : (S > 43.0 ? 43.0 : S);
+    const double rT = 1.0 / lT, Tr100 = lT / 100.0;
+    const double x = exp(b0 + b1 * (100 * rT) + b2 * log(Tr100) + lS * (b3 + b4 * (Tr100) + b5 * (Tr100 * Tr100)));
+    return bunsen ? x / (rho * CONST_V) : x;
+}
+
+// sub_calc_pv (biogem_box.f90:81-117): Wanninkhof (1992) piston velocity, m yr-1
+__device__ __forceinline__ double piston(double c0, double c1, double c2, double c3, double T, double u, double a) {
+    double TC = T - ZEROC;
+    if (TC < 0.0) TC = 0.0;
+    if (TC > 30.0) TC = 30.0;
+    const double TC2 = TC * TC, TC3 = TC2 * TC;
+    const double Sc = c0 - c1 * TC + c2 * TC2 - c3 * TC3;
+    return CONV_CM_M * CONV_YR_HR * a * (u * u) * pow(Sc * 1.515E-3, -0.5);
+}
+
+// fun_calc_ocnatm_flux for one gas (biogem_box.f90:123-299), ocean -> air
+// positive, mol yr-1: one step moves at most the surface layer's
+// disequilibrium (par_airsea_r_dflux_deqm_max = 1, biogem_lib.f90:708)
+__device__ __forceinline__ double ocnatm_flux(double pv, double A, double rho, double dD, double ocnv, double atmv,
+                                              double buff, double dtyr) {
+    if (ocnv < NULLSMALL) ocnv = 0.0;
+    if (atmv < NULLSMALL) atmv = 0.0;
+    const double lA = (1.0 - 0.0) * A;
+    double fo = pv * lA * rho * ocnv;
+    double fa = pv * lA * rho * atmv;
+    const double deqm = dD * A * rho * buff * fabs(atmv - ocnv);
+    const double dflux = dtyr * fabs(fo - fa);
+    if (deqm > NULLSMALL) {
+        const double r = dflux / deqm;
+        if (r > 1.00) {
+            fo = (1.00 / r) * fo;
+            fa = (1.00 / r) * fa;
+        }
+    }
+    return fo - fa;
+}
+
 // remineralised sediment -> ocean tracers: v[q] += (rf * conv[s][q]) * x[s]
 __device__ __forceinline__ void add_remin(const BgcParams& p, const Conv& cv, double rf, const double* x, double* v) {
 #pragma unroll
@@ -175,6 +225,44 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
         else
             calc_carb_RF0(DIC, ALK, PO4, B, SO4, F, cc, carb, plane, p.carb_tol, p.carb_maxit);
         ohm = carb[IC_OHM_CAL * plane];
+    }
+
+    // air-sea exchange (biogem.f90:1044-1046, 1605-1614) with the density of
+    // biogem_climate (fun_calc_rho, gem_util.f90:670-688; biogem.f90:2187)
+    double fair[2] = {0.0, 0.0};
+    if (p.airsea) {
+        const size_t cs = cell(maxk);
+        const double T = ocn[cs], S = ocn[ncell + cs];
+        const double TC = T - ZEROC;
+        const double rho = 1000.0 + (0.7968 * S - 0.0559 * TC - 0.0063 * (TC * TC) + 3.7315E-05 * (TC * (TC * TC)));
+        const int J = col / maxi + 1;
+        const double A = cell_area(gp(g.mj) + MJ_SV * (g.maxj + 2), maxi, J);
+        const double dD = gp(g.mk)[MK_DD * KS + maxk];
+        const double u = gp(M.wspeed)[col];
+        const AS_GLOBAL double* atm = gp(M.atm);
+        if (p.slot[B_DIC] >= 0 && p.slot[B_ALK] >= 0) {
+            const AS_GLOBAL double* carb = gp(M.carb) + col;
+            const double sol = solconst(-60.2409, 93.4517, 23.3585, 0.023517, -0.023656, 0.0047036, false, T, S, rho);
+            const double pv = piston(2073.1, 125.62, 3.6276, 0.043219, T, u, p.gas_a);
+            const double CO2 = carb[IC_CONC_CO2 * plane], RF0 = carb[IC_RF0 * plane];
+            double atmv = sol * atm[col], ocnv = CO2, buff;
+            if (RF0 > NULLSMALL) {
+                buff = 1.0 / (RF0 * CO2 / ocn[(size_t)p.ocn[B_DIC] * ncell + cs]);
+            } else {
+                ocnv = 0.0;
+                atmv = 0.0;
+                buff = 1.0;
+            }
+            fair[0] = ocnatm_flux(pv, A, rho, dD, ocnv, atmv, buff, p.dtyr);
+        }
+        if (hasO2) {
+            const double sol = solconst(-58.3877, 85.8079, 23.8439, -0.034892, 0.015568, -0.0019387, true, T, S, rho);
+            const double pv = piston(1953.4, 128.00, 3.9918, 0.050091, T, u, p.gas_a);
+            fair[1] = ocnatm_flux(pv, A, rho, dD, ocn[(size_t)p.ocn[B_O2] * ncell + cs], sol * atm[plane + col], 1.0,
+                                  p.dtyr);
+        }
+        gp(M.fatm)[col] = fair[0];
+        gp(M.fatm)[plane + col] = fair[1];
     }
 
     // (2) PO4 uptake of this column (biogem_box.f90:346-1515): the same
@@ -312,6 +400,13 @@ __global__ __launch_bounds__(64) void k_bgc_col(const MemberDev* __restrict__ me
             vdp = vdp + domP;
 #pragma unroll
             for (int q = 0; q < 4; q++) v[q] = v[q] - upt[q];
+        }
+        // vdocn = bio_remin + dtyr rM locijk_focn (biogem.f90:1821-1833): the
+        // surface takes minus the ocean -> atmosphere flux
+        if (p.airsea && kk == maxk) {
+            const double dtrM = p.dtyr * gp(M.rM)[c];
+            v[B_DIC] = v[B_DIC] + dtrM * (0.0 - fair[0]);
+            v[B_O2] = v[B_O2] + dtrM * (0.0 - fair[1]);
         }
 #pragma unroll
         for (int s = 0; s < NSED; s++) part[(size_t)s * ncell + c] = up ? upart[s] : np[s];
@@ -733,8 +828,8 @@ __global__ __launch_bounds__(CPL_THREADS) void k_couple_apply(const MemberDev* _
             StateIO<R>::st(st, lo, cc, lvT - ZEROC);
             StateIO<R>::st(st, lo, tstride + cc, lvS - SALN0);
             gp(M.M)[cc] = mn[j];
-            // 1/M is read only at k1 (the sediment return)
-            if (k == kb) gp(M.rM)[cc] = Sratio * gp(M.rM)[cc];
+            // 1/M is read only at k1 (the sediment return) and at the surface (the air-sea flux)
+            if (k == kb || k == maxk) gp(M.rM)[cc] = Sratio * gp(M.rM)[cc];
         }
     }
 }
@@ -757,6 +852,73 @@ __global__ __launch_bounds__(64) void k_carbconst_surface(const MemberDev* __res
 
 void launch_carbconst_surface(const MemberDev* mems, int32_t m, int nwet, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_carbconst_surface, dim3((nwet + 63) / 64), dim3(64), 0, s, mems, m, out);
+}
+
+// cpl_flux_ocnatm (atchem.f90:306-320) after BIOGEM's interface update
+// (biogem.f90:1719-1727), then step_atchem (atchem.f90:63-158) with no
+// decay, CH4 or 14C sources: one block per member.  Each cell's new partial
+// pressure is staged as its mole content, the contents are summed in the
+// reference's SUM order (i fastest) by one lane per gas, and the atmosphere
+// is left well mixed at 273.15 K (atchem_data.f90:248).
+constexpr int ATM_THREADS = 256;
+constexpr int ATM_LDS_COLS = 4096;  // columns staged in LDS; larger grids sum from global memory
+__global__ __launch_bounds__(ATM_THREADS) void k_atchem(const MemberDev* __restrict__ mems,
+                                                        const int32_t* __restrict__ list, int lds_cols) {
+    extern __shared__ double stage[];  // [2][lds_cols]
+    __shared__ double tot[3];
+    const MemberDev& M = mems[list[blockIdx.x]];
+    if (!M.bgc || !M.bp.airsea) return;  // uniform over the block
+    const GridDev& g = M.g;
+    const int maxi = g.maxi, ncol = g.ncol;
+    const AS_GLOBAL double* sv = gp(g.mj) + MJ_SV * (g.maxj + 2);
+    AS_GLOBAL double* atm = gp(M.atm);
+    const AS_GLOBAL double* fatm = gp(M.fatm);
+    const bool lds = ncol <= lds_cols;  // the launch sized the LDS for lds_cols columns
+    const double dts = M.bp.dtyr * CONV_YR_S;
+    for (int c = threadIdx.x; c < ncol; c += ATM_THREADS) {
+        const double A = cell_area(sv, maxi, c / maxi + 1);
+        const double V = ATM_TH * A;
+        const double cam = V / (CONV_PA_ATM * CONST_R_SI * ZEROC);
+        const double cma = 1.0 / cam;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double sfx = (1.0 / A) * CONV_S_YR * fatm[(size_t)q * ncol + c];
+            const double sum = 0.0 + dts * sfx;
+            const double a = atm[(size_t)q * ncol + c] + cma * A * sum + cma * 0.0;
+            if (lds)
+                stage[q * ncol + c] = cam * a;
+            else
+                atm[(size_t)q * ncol + c] = cam * a;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int q = threadIdx.x;
+        double acc = 0.0;
+        if (q == 2) {
+            for (int c = 0; c < ncol; c++) acc += ATM_TH * cell_area(sv, maxi, c / maxi + 1);
+        } else if (lds) {
+#pragma unroll 8
+            for (int c = 0; c < ncol; c++) acc += stage[q * ncol + c];
+        } else {
+#pragma unroll 8
+            for (int c = 0; c < ncol; c++) acc += atm[(size_t)q * ncol + c];
+        }
+        tot[q] = acc;
+    }
+    __syncthreads();
+    const double p0 = (tot[0] / tot[2]) * CONV_PA_ATM * CONST_R_SI * ZEROC;
+    const double p1 = (tot[1] / tot[2]) * CONV_PA_ATM * CONST_R_SI * ZEROC;
+    for (int c = threadIdx.x; c < ncol; c += ATM_THREADS) {
+        atm[c] = p0;
+        atm[(size_t)ncol + c] = p1;
+    }
+}
+
+void launch_atchem(const MemberDev* mems, const int32_t* list, int nlist, int ncolmax, hipStream_t s) {
+    const int lds_cols = ncolmax <= ATM_LDS_COLS ? ncolmax : 0;
+    hipLaunchKernelGGL(k_atchem, dim3(nlist), dim3(ATM_THREADS), (size_t)2 * lds_cols * sizeof(double), s, mems, list,
+                       lds_cols);
 }
 
 // ------------------------------------------------------------ launchers

@@ -41,7 +41,7 @@ class BgcParams(C.Structure):
                            "red_PON_ALK", "red_POP_PON", "red_POC_CaCO3", "red_POC_CaCO3_pP", "POC_frac2",
                            "POC_dfrac2", "POC_c0frac2", "POC_eL1", "POC_eL2", "CaCO3_frac2", "CaCO3_eL1",
                            "CaCO3_eL2", "sinkrate", "remin_k_O2", "remin_c0_O2", "carb_tol", "dsc")] + \
-        [("carb_maxit", _i32)]
+        [("carb_maxit", _i32), ("airsea", _i32), ("gastransfer_a", _d), ("atm_pCO2", _d), ("atm_pO2", _d)]
 
 
 class Diag(C.Structure):
@@ -92,6 +92,7 @@ def load() -> C.CDLL:
         "cgo_bgc_default_params": ([C.POINTER(BgcParams), _d], None),
         "cgo_bgc_init": ([P, _i32, C.POINTER(BgcParams), _dp, _dp], C.c_int),
         "cgo_bgc_download": ([P, _i32, _i32, _dp], C.c_int),
+        "cgo_set_windspeed": ([P, _i32, _dp], C.c_int),
         "cgo_step": ([P, ip, _i32, _i32], C.c_int),
         "cgo_tstepo": ([P, ip, _i32, _i32], C.c_int),
         "cgo_bio_step": ([P, ip, _i32], C.c_int),
@@ -284,6 +285,20 @@ class Engine:
         out = np.zeros(shape)
         self._chk(self.L.cgo_bgc_download(self.h, m, what, _ptr(out)), "bgc_download")
         return out
+
+    def set_windspeed(self, m: int, wspeed: np.ndarray):
+        """Wind speed (maxj, maxi) in m/s for the air-sea exchange
+        (par_phys_windspeed, biogem.f90:2228-2232)."""
+        w = np.ascontiguousarray(wspeed, dtype=np.float64)
+        self._chk(self.L.cgo_set_windspeed(self.h, m, _ptr(w)), "set_windspeed")
+
+    def atmosphere(self, m: int, maxi: int, maxj: int) -> np.ndarray:
+        """(maxj, maxi, 2) atmospheric pCO2 and pO2 (atm)."""
+        return self.bgc_download(m, 6, (maxj, maxi, 2))
+
+    def airsea_flux(self, m: int, maxi: int, maxj: int) -> np.ndarray:
+        """(maxj, maxi, 2) the last step's ocean -> atmosphere CO2 and O2 flux (mol yr-1)."""
+        return self.bgc_download(m, 7, (maxj, maxi, 2))
 
     # -- stepping
     def step(self, members, nsteps: int = 1):

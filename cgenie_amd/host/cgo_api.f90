@@ -37,6 +37,9 @@ module cgo_api
      real(c_double) :: CaCO3_frac2, CaCO3_eL1, CaCO3_eL2
      real(c_double) :: sinkrate, remin_k_O2, remin_c0_O2, carb_tol, dsc
      integer(c_int32_t) :: carb_maxit
+     ! air-sea exchange of CO2 and O2 with the well-mixed ATCHEM atmosphere
+     integer(c_int32_t) :: airsea
+     real(c_double) :: gastransfer_a, atm_pCO2, atm_pO2
   end type cgo_bgc_params
 
   type, bind(C) :: cgo_diag
@@ -189,6 +192,13 @@ module cgo_api
        integer(c_int32_t), value :: m, what
        real(c_double), intent(out) :: out(*)
      end function cgo_bgc_download
+
+     integer(c_int) function cgo_set_windspeed(e, m, wspeed) bind(C, name="cgo_set_windspeed")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(in) :: wspeed(*)
+     end function cgo_set_windspeed
 
      integer(c_int) function cgo_step(e, members, n, nsteps) bind(C, name="cgo_step")
        import :: c_int, c_ptr, c_int32_t

@@ -74,6 +74,12 @@ typedef struct cgo_bgc_params {
     double CaCO3_frac2, CaCO3_eL1, CaCO3_eL2;
     double sinkrate, remin_k_O2, remin_c0_O2, carb_tol, dsc;
     int32_t carb_maxit;
+    /* air-sea exchange of CO2 (needs DIC and ALK) and O2 with a well-mixed
+     * ATCHEM atmosphere (biogem_box.f90:27-299, atchem.f90:63-158, 306-320):
+     * on/off, par_gastransfer_a (biogem-defaults.nml:221) and the initial
+     * partial pressures in atm (atm_init, atchem-defaults.nml:4) */
+    int32_t airsea;
+    double gastransfer_a, atm_pCO2, atm_pO2;
 } cgo_bgc_params;
 
 typedef struct cgo_diag {
@@ -147,8 +153,16 @@ int cgo_bgc_init(cgo_engine *e, int32_t m, const cgo_bgc_params *p, const double
  * of the surface cells from the current ocn T and S (sub_calc_carbconst,
  * gem_carbchem.f90:59-268; land columns zero), 5 = bio_settle at k1
  * (5,maxi,maxj), the last step's rain onto the sea floor in mol per step
- * (POC, POP, CaCO3) and the recalcitrant fractions (biogem.f90:1746-1754) */
+ * (POC, POP, CaCO3) and the recalcitrant fractions (biogem.f90:1746-1754),
+ * 6 = the atmosphere (2,maxi,maxj): pCO2 and pO2 in atm over every column
+ * (ATCHEM's atm(ia_pCO2/ia_pO2,:,:), handed to BIOGEM as dum_sfcatm1),
+ * 7 = the last step's ocean -> atmosphere flux (2,maxi,maxj) of CO2 and O2
+ * in mol yr-1 (locij_focnatm, biogem.f90:1605; land zero) */
 int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
+/* wind speed (maxi,maxj) in m/s for the air-sea exchange: the prescribed
+ * field par_phys_windspeed that biogem_climate copies to ipoa_wspeed
+ * (biogem.f90:2228-2232); zero until set, kept across cgo_bgc_init */
+int cgo_set_windspeed(cgo_engine *e, int32_t m, const double *wspeed);
 
 /* stepping: nsteps ocean steps of every listed member, batched into one
  * launch per kernel (members with BIOGEM run the full coupling order:

@@ -28,6 +28,14 @@ static const double NULLSMALL = +0.999999E-19;
 static const double CONST_PI = 3.141592653589793;
 static const double CONST_REARTH = 6.37E+06;
 static const double SALN0 = 34.9; /* goldstein-defaults.nml:60 */
+/* gem_cmn.f90:511-545, 694-698 */
+static const double CONV_YR_HR = 24.0 * 365.25;
+static const double CONV_YR_S = 3600.0 * (24.0 * 365.25);
+static const double CONV_S_YR = 1.0 / (3600.0 * (24.0 * 365.25));
+static const double CONV_CM_M = 1.0 / 100.0;
+static const double CONV_PA_ATM = 1.0 / 1.01325E+05;
+static const double CONST_R_SI = 8.3145;
+static const double CONST_V = 0.022414;
 
 /* libgcc __powidf2: what gfortran calls for REAL**INTEGER (x**k with a
  * variable integer k), binary exponentiation. */
@@ -779,6 +787,10 @@ void og_bgc_default_params(og_bgc_params *p, double dtyr)
     p->carb_tol = 0.001;
     p->carb_maxit = 100;
     p->dsc = DSC;
+    p->airsea = 0;
+    p->gastransfer_a = 0.310;  /* biogem-defaults.nml:221 */
+    p->atm_pCO2 = 278.0E-6;    /* atchem-defaults.nml:4 */
+    p->atm_pO2 = 0.2095;       /* the modern atmosphere (the namelist default is 0) */
 }
 
 #define O3(st, n, i, j, k) ((size_t)(n) + (size_t)(st)->maxl * ((size_t)((i) - 1) + (size_t)(st)->maxi * ((size_t)((j) - 1) + (size_t)(st)->maxj * (size_t)((k) - 1))))
@@ -866,6 +878,15 @@ void og_bgc_init(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
     st->mld = calloc((size_t)maxi * maxj, sizeof(double));
     memcpy(st->light, light, sizeof(double) * maxi * maxj);
     memcpy(st->mld, mld, sizeof(double) * maxi * maxj);
+    st->wspeed = calloc((size_t)maxi * maxj, sizeof(double));
+    st->atm = calloc((size_t)2 * maxi * maxj, sizeof(double));
+    st->fatm = calloc((size_t)2 * maxi * maxj, sizeof(double));
+    st->sumatm = calloc((size_t)2 * maxi * maxj, sizeof(double));
+    /* sub_init_tracer_atm_comp (atchem_data.f90:230-260): every cell */
+    for (size_t c = 0; c < (size_t)maxi * maxj; c++) {
+        st->atm[2 * c] = p->atm_pCO2;
+        st->atm[2 * c + 1] = p->atm_pO2;
+    }
     /* biogem_data.f90:1098-1140 */
     double dzl[OG_MAXK + 3], dzal[OG_MAXK + 3];
     for (int k = 0; k <= maxk + 1; k++) dzl[k] = dzal[k] = 0.0;
@@ -930,6 +951,7 @@ void og_bgc_free(og_bgc_state *st)
 {
     free(st->ocn); free(st->part); free(st->settle); free(st->carb); free(st->carbconst);
     free(st->M); free(st->rM); free(st->V); free(st->light); free(st->mld);
+    free(st->wspeed); free(st->atm); free(st->fatm); free(st->sumatm);
     memset(st, 0, sizeof(*st));
 }
 
@@ -1017,6 +1039,151 @@ static void remin_part_col(og_bgc_state *st, const og_bgc_params *p, const og_co
             st->part[P3(st, s, i, j, k)] = NEWP[k][s];
             st->settle[P3(st, s, i, j, k)] = SET[k][s];
         }
+}
+
+/* ---------------------------------------------------------------- air-sea */
+/* Schmidt-number and Bunsen-coefficient tables for CO2 and O2
+ * (gem_data.f90:68-99, 105-141) */
+static const double SC_CO2[4] = {2073.1, 125.62, 3.6276, 0.043219};
+static const double SC_O2[4] = {1953.4, 128.00, 3.9918, 0.050091};
+static const double BUN_CO2[6] = {-60.2409, 93.4517, 23.3585, 0.023517, -0.023656, 0.0047036};
+static const double BUN_O2[6] = {-58.3877, 85.8079, 23.8439, -0.034892, 0.015568, -0.0019387};
+static const double AIRSEA_R_DFLUX_DEQM_MAX = 1.00; /* biogem_lib.f90:708 */
+static const double ATM_TH = 7777.0;                /* atchem_lib.f90:86 */
+
+/* fun_calc_rho, gem_util.f90:670-688 */
+double og_calc_rho(double T, double S)
+{
+    const double TC = T - CONST_ZEROC;
+    return 1000.0 + (0.7968 * S - 0.0559 * TC - 0.0063 * powi(TC, 2) + 3.7315E-05 * powi(TC, 3));
+}
+
+/* fun_calc_solconst, gem_carbchem.f90:1367-1426: CO2 in mol/(kg atm); O2
+ * (the default case) from its Bunsen coefficient over rho const_V */
+static double solconst(const double *b, int bunsen, double T, double S, double rho)
+{
+    double lT, lS;
+    if (T < CONST_ZEROC + 2.0)
+        lT = CONST_ZEROC + 2.0;
+    else if (T > (CONST_ZEROC + 35.0))
+        lT = CONST_ZEROC + 35.0;
+    else
+        lT = T;
+    if (S < 26.0)
+        lS = 26.0;
+    else if (S > 43.0)
+        lS = 43.0;
+    else
+        lS = S;
+    const double rT = 1.0 / lT, Tr100 = lT / 100.0;
+    const double x = exp(b[0] + b[1] * (100 * rT) + b[2] * log(Tr100) +
+                         lS * (b[3] + b[4] * (Tr100) + b[5] * powi(Tr100, 2)));
+    return bunsen ? x / (rho * CONST_V) : x;
+}
+
+/* sub_calc_pv, biogem_box.f90:81-117: Wanninkhof (1992) piston velocity, m/yr */
+static double piston(const double *c, double T, double wspeed, double gas_a)
+{
+    double TC = T - CONST_ZEROC;
+    if (TC < 0.0) TC = 0.0;
+    if (TC > 30.0) TC = 30.0;
+    const double TC2 = TC * TC, TC3 = TC2 * TC;
+    const double u2 = powi(wspeed, 2);
+    const double Sc = c[0] - c[1] * TC + c[2] * TC2 - c[3] * TC3;
+    return CONV_CM_M * CONV_YR_HR * gas_a * u2 * pow(Sc * 1.515E-3, -0.5);
+}
+
+/* fun_calc_ocnatm_flux for one gas (biogem_box.f90:123-299): the flux is
+ * limited so that one step moves at most the surface layer's disequilibrium */
+static double ocnatm_flux(double pv, double A, double rho, double dD, double ocnv, double atmv, double buff,
+                          double dtyr)
+{
+    if (ocnv < NULLSMALL) ocnv = 0.0;
+    if (atmv < NULLSMALL) atmv = 0.0;
+    const double lA = (1.0 - 0.0) * A;
+    double fo = pv * lA * rho * ocnv;
+    double fa = pv * lA * rho * atmv;
+    const double deqm = dD * A * rho * buff * fabs(atmv - ocnv);
+    const double dflux = dtyr * fabs(fo - fa);
+    if (deqm > NULLSMALL) {
+        const double r = dflux / deqm;
+        if (r > AIRSEA_R_DFLUX_DEQM_MAX) {
+            fo = (AIRSEA_R_DFLUX_DEQM_MAX / r) * fo;
+            fa = (AIRSEA_R_DFLUX_DEQM_MAX / r) * fa;
+        }
+    }
+    return fo - fa;
+}
+
+void og_airsea_cell(double T, double S, double rho, double dD, double A, double wspeed, double DIC, double O2,
+                    double CO2, double RF0, double pCO2, double pO2, double dtyr, double gas_a, double *out)
+{
+    const double sCO2 = solconst(BUN_CO2, 0, T, S, rho), sO2 = solconst(BUN_O2, 1, T, S, rho);
+    const double pvCO2 = piston(SC_CO2, T, wspeed, gas_a), pvO2 = piston(SC_O2, T, wspeed, gas_a);
+    /* CO2: the ocean side is [CO2], buffered by the Revelle factor */
+    double atmv = sCO2 * pCO2, ocnv = CO2, buff;
+    if (RF0 > NULLSMALL) {
+        buff = 1.0 / (RF0 * CO2 / DIC);
+    } else {
+        ocnv = 0.0;
+        atmv = 0.0;
+        buff = 1.0;
+    }
+    out[0] = sCO2;
+    out[1] = sO2;
+    out[2] = pvCO2;
+    out[3] = pvO2;
+    out[4] = ocnatm_flux(pvCO2, A, rho, dD, ocnv, atmv, buff, dtyr);
+    out[5] = ocnatm_flux(pvO2, A, rho, dD, O2, sO2 * pO2, 1.0, dtyr);
+}
+
+void og_bgc_set_windspeed(og_bgc_state *st, const double *wspeed)
+{
+    memcpy(st->wspeed, wspeed, sizeof(double) * st->maxi * st->maxj);
+}
+
+/* cpl_flux_ocnatm (atchem.f90:306-320) after BIOGEM's interface update
+ * (biogem.f90:1719-1727), then step_atchem (atchem.f90:63-158) with no decay,
+ * CH4 or 14C sources: the fluxes enter the atmospheric cells and the
+ * atmosphere mixes to one partial pressure at 273.15 K (atchem_data.f90:248) */
+void og_atchem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p)
+{
+    const int maxi = g->maxi, maxj = g->maxj;
+    const double dts = p->dtyr * CONV_YR_S;
+    for (int q = 0; q < 2; q++) {
+        for (int j = 1; j <= maxj; j++)
+            for (int i = 1; i <= maxi; i++) {
+                const size_t c = C2(st, i, j);
+                const double A = 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / maxi) * (g->sv[j] - g->sv[j - 1]);
+                const double sfx = (1.0 / A) * CONV_S_YR * st->fatm[2 * c + q];   /* 0 for land columns */
+                st->sumatm[2 * c + q] = st->sumatm[2 * c + q] + dts * sfx;
+            }
+        double tot = 0.0, totV = 0.0;
+        for (int j = 1; j <= maxj; j++)
+            for (int i = 1; i <= maxi; i++) {
+                const size_t c = C2(st, i, j);
+                const double A = 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / maxi) * (g->sv[j] - g->sv[j - 1]);
+                const double V = ATM_TH * A;
+                const double cam = V / (CONV_PA_ATM * CONST_R_SI * CONST_ZEROC);
+                const double cma = 1.0 / cam;
+                double *a = &st->atm[2 * c + q];
+                *a = *a + cma * A * st->sumatm[2 * c + q] + cma * 0.0;
+            }
+        for (int j = 1; j <= maxj; j++)
+            for (int i = 1; i <= maxi; i++) {
+                const size_t c = C2(st, i, j);
+                const double A = 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / maxi) * (g->sv[j] - g->sv[j - 1]);
+                const double V = ATM_TH * A;
+                tot += (V / (CONV_PA_ATM * CONST_R_SI * CONST_ZEROC)) * st->atm[2 * c + q];
+                totV += V;
+            }
+        for (int j = 1; j <= maxj; j++)
+            for (int i = 1; i <= maxi; i++) {
+                const size_t c = C2(st, i, j);
+                st->atm[2 * c + q] = (tot / totV) * CONV_PA_ATM * CONST_R_SI * CONST_ZEROC;
+                st->sumatm[2 * c + q] = 0.0;
+            }
+    }
 }
 
 void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
@@ -1115,6 +1282,21 @@ void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
                                  p->carb_maxit);
                 ohm = carb[OG_IC_OHM_CAL];
             }
+            /* air-sea exchange (biogem.f90:1044-1046, 1605-1614) with rho from
+             * biogem_climate (biogem.f90:2187), which genie calls after each
+             * coupling and at start-up (genie.f90:110, 389) */
+            if (p->airsea) {
+                const size_t c = C2(st, i, j);
+                const double A = 2.0 * CONST_PI * (CONST_REARTH * CONST_REARTH) * (1.0 / maxi) * (g->sv[j] - g->sv[j - 1]);
+                const double rho = og_calc_rho(oc[0], oc[1]);
+                const int co2 = p->l_DIC >= 0 && p->l_ALK >= 0;
+                double f[6];
+                og_airsea_cell(oc[0], oc[1], rho, st->dD[maxk], A, st->wspeed[c], co2 ? oc[p->l_DIC] : 0.0,
+                               p->l_O2 >= 0 ? oc[p->l_O2] : 0.0, carb[OG_IC_CONC_CO2], carb[OG_IC_RF0],
+                               st->atm[2 * c], st->atm[2 * c + 1], p->dtyr, p->gastransfer_a, f);
+                st->fatm[2 * c] = co2 ? f[4] : 0.0;
+                st->fatm[2 * c + 1] = p->l_O2 >= 0 ? f[5] : 0.0;
+            }
             if (p->l_PO4 < 0) continue;
             /* sub_calc_bio_uptake, 1N1T_PO4MM (biogem_box.f90:346-1515) */
             int k_mld = kb;
@@ -1185,6 +1367,23 @@ void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
                 }
             }
         }
+    /* vdocn = bio_remin + dtyr rM locijk_focn (biogem.f90:1821-1833), the
+     * surface locijk_focn being minus the ocean -> atmosphere flux */
+    if (p->airsea)
+        for (int i = 1; i <= maxi; i++)
+            for (int j = 1; j <= maxj; j++) {
+                if (KK1(i, j) > maxk) continue;
+                const size_t c = C2(st, i, j);
+                const double dtrM = p->dtyr * st->rM[C3(st, i, j, maxk)];
+                if (p->l_DIC >= 0 && p->l_ALK >= 0) {
+                    double *r = &vdocn[O3(st, p->l_DIC, i, j, maxk)];
+                    *r = *r + dtrM * (0.0 - st->fatm[2 * c]);
+                }
+                if (p->l_O2 >= 0) {
+                    double *r = &vdocn[O3(st, p->l_O2, i, j, maxk)];
+                    *r = *r + dtrM * (0.0 - st->fatm[2 * c + 1]);
+                }
+            }
     free(vrem);
 }
 

@@ -119,6 +119,10 @@ typedef struct og_bgc_params {
     double carb_tol;      /* par_carbchem_pH_tolerance */
     int carb_maxit;
     double dsc;           /* goldstein_dsc (m) */
+    /* air-sea exchange of CO2 and O2 with a well-mixed atmosphere */
+    int airsea;           /* on/off */
+    double gastransfer_a; /* par_gastransfer_a, biogem-defaults.nml:221 */
+    double atm_pCO2, atm_pO2; /* initial atmospheric partial pressures (atm) */
 } og_bgc_params;
 
 /* Reference defaults (biogem-defaults.nml:233-303, gem-defaults.nml:196-199). */
@@ -138,6 +142,11 @@ typedef struct og_bgc_state {
     double *mld;      /* (maxi, maxj) mixed-layer depth, m */
     int32_t carb_fail; /* count of cells whose carbonate solve failed */
     int32_t carb_iter; /* iterations of the last surface solve sweep (sum) */
+    /* with air-sea exchange: (2, maxi, maxj) CO2 then O2 of each column */
+    double *wspeed;   /* (maxi, maxj) wind speed (m/s), par_phys_windspeed */
+    double *atm;      /* atmospheric partial pressure (atm), dum_sfcatm1 */
+    double *fatm;     /* this step's ocean -> atmosphere flux (mol yr-1) */
+    double *sumatm;   /* the flux integrated since the last ATCHEM step (mol m-2) */
 } og_bgc_state;
 
 /* sub_init_phys_ocn geometry (biogem_data.f90:1098-1140) and ocn from go_ts
@@ -147,6 +156,22 @@ void og_bgc_init(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,
                  int maxl, const int32_t *k1, const double *go_ts,
                  const double *light, const double *mld);
 void og_bgc_free(og_bgc_state *st);
+
+/* fun_calc_rho, gem_util.f90:670-688 (T in K) */
+double og_calc_rho(double T, double S);
+/* One surface cell's air-sea exchange as step_biogem forms it
+ * (biogem.f90:1044-1046, 1605; biogem_box.f90: sub_calc_solconst 27-40,
+ * sub_calc_pv 81-117, fun_calc_ocnatm_flux 123-299; gem_carbchem.f90:
+ * fun_calc_solconst 1367-1426): out[0..1] solubility of CO2 and O2,
+ * out[2..3] piston velocity, out[4..5] ocean -> atmosphere flux (mol yr-1). */
+void og_airsea_cell(double T, double S, double rho, double dD, double A, double wspeed, double DIC, double O2,
+                    double CO2, double RF0, double pCO2, double pO2, double dtyr, double gas_a, double *out);
+/* wind speed (maxi, maxj) for the air-sea exchange */
+void og_bgc_set_windspeed(og_bgc_state *st, const double *wspeed);
+/* the ocean -> atmosphere coupling and ATCHEM step after a BIOGEM step
+ * (cpl_flux_ocnatm, atchem.f90:306-320; step_atchem, atchem.f90:63-158):
+ * well-mixed atmosphere over the ocean grid at 273.15 K */
+void og_atchem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p);
 
 /* step_biogem restricted to the selected species (biogem.f90:596-611,
  * 873-969, 985-1105, 1693, 1811-1844).  Writes vdocn (maxl,maxi,maxj,maxk). */

@@ -15,6 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_carbchem.so")
+REF_AIRSEA_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_airsea.so")
 
 OG_MAXJ, OG_MAXK = 256, 64
 NSED, NCARB, NCC = 5, 10, 17
@@ -41,14 +42,16 @@ class OgBgcParams(C.Structure):
                 ("red_POP_PON", _d), ("red_POC_CaCO3", _d), ("red_POC_CaCO3_pP", _d), ("POC_frac2", _d),
                 ("POC_dfrac2", _d), ("POC_c0frac2", _d), ("POC_eL1", _d), ("POC_eL2", _d),
                 ("CaCO3_frac2", _d), ("CaCO3_eL1", _d), ("CaCO3_eL2", _d), ("sinkrate", _d),
-                ("remin_k_O2", _d), ("remin_c0_O2", _d), ("carb_tol", _d), ("carb_maxit", _i), ("dsc", _d)]
+                ("remin_k_O2", _d), ("remin_c0_O2", _d), ("carb_tol", _d), ("carb_maxit", _i), ("dsc", _d),
+                ("airsea", _i), ("gastransfer_a", _d), ("atm_pCO2", _d), ("atm_pO2", _d)]
 
 
 class OgBgcState(C.Structure):
     _fields_ = [("maxi", _i), ("maxj", _i), ("maxk", _i), ("maxl", _i)] + \
         [(n, C.POINTER(_d)) for n in ("ocn", "part", "settle", "carb", "carbconst", "M", "rM", "V")] + \
         [(n, _d * (OG_MAXK + 2)) for n in ("Dbot", "Dmid", "dD")] + \
-        [("light", C.POINTER(_d)), ("mld", C.POINTER(_d)), ("carb_fail", C.c_int32), ("carb_iter", C.c_int32)]
+        [("light", C.POINTER(_d)), ("mld", C.POINTER(_d)), ("carb_fail", C.c_int32), ("carb_iter", C.c_int32)] + \
+        [(n, C.POINTER(_d)) for n in ("wspeed", "atm", "fatm", "sumatm")]
 
 
 def build(force: bool = False) -> None:
@@ -80,6 +83,11 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.og_bgc_free.argtypes = [C.POINTER(OgBgcState)]
     L.og_biogem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams), _i32p, _dp]
     L.og_tracercoupling.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), _i32p, _dp, _dp]
+    L.og_calc_rho.argtypes = [_d, _d]
+    L.og_calc_rho.restype = _d
+    L.og_airsea_cell.argtypes = [_d] * 14 + [_dp]
+    L.og_bgc_set_windspeed.argtypes = [C.POINTER(OgBgcState), _dp]
+    L.og_atchem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams)]
     return L
 
 
@@ -183,14 +191,30 @@ class BiogemOracle:
         n = (self.ncell if cells is None else cells) * per_cell
         return np.ctypeslib.as_array(getattr(self.st, name), shape=(n,))
 
+    def set_windspeed(self, wspeed: np.ndarray):
+        lib().og_bgc_set_windspeed(C.byref(self.st), np.ascontiguousarray(wspeed, dtype=np.float64))
+
     def step(self, n: int = 1, goldstein: bool = True):
-        """genie.f90 order: step_goldstein, step_biogem, tracercoupling."""
+        """genie.f90 order: step_goldstein, step_biogem, tracercoupling, then
+        with air-sea exchange the ocean -> atmosphere coupling and ATCHEM."""
         L = lib()
         for _ in range(n):
             if goldstein:
                 self.gold.step(1)
             L.og_biogem_step(C.byref(self.st), C.byref(self.gold.g), C.byref(self.p), self.gold.k1, self.vdocn)
             L.og_tracercoupling(C.byref(self.st), C.byref(self.gold.g), self.gold.k1, self.gold.ts, self.vdocn)
+            if self.p.airsea:
+                L.og_atchem_step(C.byref(self.st), C.byref(self.gold.g), C.byref(self.p))
+
+    def atm(self) -> np.ndarray:
+        """(maxj, maxi, 2) atmospheric pCO2 and pO2 (atm)."""
+        g = self.gold.g
+        return self.arr("atm", 2, g.maxi * g.maxj).reshape(g.maxj, g.maxi, 2).copy()
+
+    def fatm(self) -> np.ndarray:
+        """(maxj, maxi, 2) the last step's ocean -> atmosphere CO2 and O2 flux (mol yr-1)."""
+        g = self.gold.g
+        return self.arr("fatm", 2, g.maxi * g.maxj).reshape(g.maxj, g.maxi, 2).copy()
 
     def ocn(self) -> np.ndarray:
         g = self.gold.g
@@ -221,3 +245,18 @@ def ref_carbchem():
         R.ref_calc_carb_rf0.argtypes = [_d] * 9 + [_dp, _dp]
         _ref = R
     return _ref
+
+
+_ref_as = None
+
+
+def ref_airsea():
+    """The reference's BIOGEM air-sea routines compiled in place (oracle/_ref), or None."""
+    global _ref_as
+    if _ref_as is None and os.path.exists(REF_AIRSEA_LIB):
+        R = C.CDLL(REF_AIRSEA_LIB)
+        R.ref_calc_rho.argtypes = [_d, _d]
+        R.ref_calc_rho.restype = _d
+        R.ref_airsea_cell.argtypes = [_d] * 14 + [_dp]
+        _ref_as = R
+    return _ref_as

@@ -64,7 +64,7 @@ def test_struct_layouts_match_header(tmp_path):
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
     # the sizes the Fortran bind(C) mirrors are also checked against (test_fortran_host)
-    assert [got[(n, "size")] for n in STRUCTS] == [208, 24, 232, 48]
+    assert [got[(n, "size")] for n in STRUCTS] == [208, 24, 256, 48]
 
 
 class _FakeLib:

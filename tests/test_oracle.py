@@ -675,3 +675,145 @@ def test_o3_build_gives_the_same_bits():
             runs.append((gold.ts.copy(), bio.ocn(), bio.carb()))
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ air-sea
+def _airsea(T=293.15, S=35.0, dD=80.0, A=1e10, u=10.0, DIC=2.0e-3, O2=2.0e-4, CO2=1.0e-5, RF0=10.0,
+            pCO2=280e-6, pO2=0.2095, dtyr=1.0 / 96, a=0.31, rho=None):
+    L = O.lib()
+    rho = L.og_calc_rho(T, S) if rho is None else rho
+    out = np.zeros(6)
+    L.og_airsea_cell(T, S, rho, dD, A, u, DIC, O2, CO2, RF0, pCO2, pO2, dtyr, a, out)
+    return out, rho
+
+
+def test_airsea_matches_compiled_reference():
+    """og_calc_rho and og_airsea_cell against the reference's fun_calc_rho,
+    sub_calc_solconst, sub_calc_pv and fun_calc_ocnatm_flux compiled in place
+    (oracle/ref_airsea_shim.f90), bit for bit, over cells that clamp T and S,
+    switch the Revelle buffer off (RF0 = 0) and trip the flux limiter."""
+    R = O.ref_airsea()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree: make -C oracle ref)")
+    L = O.lib()
+    rng = np.random.default_rng(5)
+    limited = 0
+    for _ in range(4000):
+        T, S = 273.15 + rng.uniform(-3, 38), rng.uniform(20, 45)
+        rho = L.og_calc_rho(T, S)
+        assert rho == R.ref_calc_rho(T, S)
+        args = [T, S, rho, rng.uniform(10, 200), rng.uniform(1e9, 1e11), rng.uniform(0, 15),
+                rng.uniform(1.5e-3, 2.5e-3), rng.uniform(0, 4e-4), rng.uniform(0, 3e-5),
+                float(rng.choice([0.0, rng.uniform(5, 15)])), rng.uniform(0, 1e-3), rng.uniform(0, 0.3),
+                float(rng.choice([1.0 / 96, 1.0, 20.0])), 0.31]
+        a, b = np.zeros(6), np.zeros(6)
+        L.og_airsea_cell(*args, a)
+        R.ref_airsea_cell(*args, b)
+        assert np.array_equal(a, b), (args, a, b)
+        # the limiter binds: one step moves exactly the O2 disequilibrium of the surface layer
+        deqm = args[3] * args[4] * rho * abs(a[1] * args[11] - args[7])
+        limited += bool(deqm > 0 and abs(args[12] * a[5] - np.sign(a[5]) * deqm) <= 1e-12 * deqm)
+    assert limited > 100
+
+
+def test_airsea_solubility_closed_forms():
+    """CO2: Weiss (1974) K0 in mol kg-1 atm-1, ln K0 = -3.5617 at 25 C and
+    S = 35 (Dickson, Sabine and Christian 2007, SOP guide ch. 5); clamped to
+    2-35 C and S 26-43.  O2: a Bunsen coefficient over rho const_V, so
+    solubility times rho does not depend on rho; both fall with T and S."""
+    out, _ = _airsea(T=298.15, S=35.0)
+    assert abs(np.log(out[0]) - (-3.5617)) < 2e-4
+    lo, _ = _airsea(T=273.15 - 1.0, rho=1025.0)
+    at2, _ = _airsea(T=273.15 + 2.0, rho=1025.0)
+    assert lo[0] == at2[0] and lo[1] == at2[1]
+    hiS, _ = _airsea(S=50.0, rho=1025.0)
+    at43, _ = _airsea(S=43.0, rho=1025.0)
+    assert hiS[0] == at43[0] and hiS[1] == at43[1]
+    r1, _ = _airsea(rho=1020.0)
+    r2, _ = _airsea(rho=1030.0)
+    assert abs(r1[1] * 1020.0 - r2[1] * 1030.0) <= 1e-15 * r1[1] * 1020.0 and r1[0] == r2[0]
+    # about 213 umol/kg O2 at 25 C, 35 from a dry 0.2095 atm (Weiss 1970 Bunsen coefficient)
+    o2sat = _airsea(T=298.15)[0][1] * 0.2095
+    assert 2.0e-4 < o2sat < 2.2e-4
+    cold, warm = _airsea(T=278.15)[0], _airsea(T=298.15)[0]
+    fresh, salty = _airsea(S=30.0)[0], _airsea(S=40.0)[0]
+    assert cold[0] > warm[0] and cold[1] > warm[1] and fresh[0] > salty[0] and fresh[1] > salty[1]
+
+
+def test_airsea_piston_velocity_wanninkhof():
+    """k = a u^2 (Sc/660)^-1/2 cm/hr (Wanninkhof 1992), in m/yr; at 20 C the
+    seawater Schmidt numbers of his table A1 are 660 (CO2) and 568 (O2): the
+    fitted polynomials agree to 1 and 2.5 per cent.  k scales as u^2 and the
+    temperature is clamped to 0-30 C."""
+    cmhr = 0.31 * 10.0 ** 2 * np.array([(660.0 / 660.0) ** 0.5, (660.0 / 568.0) ** 0.5])
+    want = cmhr / 100.0 * 24.0 * 365.25
+    out, _ = _airsea(T=293.15, u=10.0)
+    assert abs(out[2] / want[0] - 1) < 0.01 and abs(out[3] / want[1] - 1) < 0.025
+    out2, _ = _airsea(T=293.15, u=20.0)
+    assert np.allclose(out2[2:4], 4 * out[2:4], rtol=1e-15, atol=0)
+    assert np.array_equal(_airsea(T=273.15 + 35.0)[0][2:4], _airsea(T=273.15 + 30.0)[0][2:4])
+    assert np.array_equal(_airsea(T=273.15 - 2.0)[0][2:4], _airsea(T=273.15)[0][2:4])
+    assert not _airsea(u=0.0)[0][2:].any()
+
+
+def test_airsea_bulk_flux_equilibrium_and_limit():
+    """Unlimited: F = k A rho (C - C_sat), the bulk formula, ocean to air
+    positive; zero at saturation; CO2 enters only through the aqueous [CO2]
+    and nothing moves without a Revelle factor.  Limited (a long step): the
+    step moves exactly the surface layer's disequilibrium, the CO2 one
+    scaled by the buffer DIC/(RF0 [CO2])."""
+    out, rho = _airsea(O2=2.5e-4)
+    sat = out[1] * 0.2095
+    assert abs(out[5] - out[3] * 1e10 * rho * (2.5e-4 - sat)) <= 1e-14 * abs(out[5]) and out[5] > 0
+    out, rho = _airsea(CO2=5e-6)
+    assert abs(out[4] - out[2] * 1e10 * rho * (5e-6 - out[0] * 280e-6)) <= 1e-14 * abs(out[4]) and out[4] < 0
+    s = _airsea()[0]
+    eq, _ = _airsea(CO2=s[0] * 280e-6, O2=s[1] * 0.2095)
+    assert eq[4] == 0.0 and eq[5] == 0.0
+    assert _airsea(RF0=0.0)[0][4] == 0.0
+    out, rho = _airsea(dtyr=50.0, CO2=5e-6, O2=1e-4)
+    deq_o2 = 80.0 * 1e10 * rho * abs(out[1] * 0.2095 - 1e-4)
+    deq_co2 = 80.0 * 1e10 * rho * (2.0e-3 / (10.0 * 5e-6)) * abs(out[0] * 280e-6 - 5e-6)
+    assert abs(50.0 * out[5] + deq_o2) <= 1e-13 * deq_o2
+    assert abs(50.0 * out[4] + deq_co2) <= 1e-13 * deq_co2
+
+
+def test_airsea_step_conserves_and_atmosphere_mixes():
+    """One BIOGEM step without transport, with and without air-sea exchange
+    from one state: the ocean tendencies differ only at the surface, in DIC
+    and O2, by -dtyr rM F; the ocean loses dtyr sum(F) moles, and the
+    well-mixed ATCHEM box gains them as an ideal gas at 273.15 K over
+    7777 m of the whole sphere, one partial pressure everywhere."""
+    c = make_case(18, 18, 8, kind="config2", seed=13)
+    g, t = c.g, c.tset
+    rng = np.random.default_rng(3)
+    wspeed = rng.uniform(2.0, 12.0, (g.maxj, g.maxi))
+    runs = []
+    for on in (0, 1):
+        gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u)
+        bio = O.BiogemOracle(gold, O.bgc_params(1.0 / 96, t, airsea=on, atm_pCO2=250e-6), c.light, c.mld)
+        bio.set_windspeed(wspeed)
+        atm0 = bio.atm()
+        vd = bio.vdocn
+        bio.step(1, goldstein=False)
+        runs.append((vd.copy(), bio.fatm(), atm0, bio.atm(), bio.arr("M").reshape(g.maxk, g.maxj, g.maxi).copy(),
+                     bio.arr("rM").reshape(g.maxk, g.maxj, g.maxi).copy()))
+    (vd0, f0, _, a_off, _, _), (vd1, f1, atm0, atm1, M, rM) = runs
+    assert not f0.any() and np.array_equal(a_off, atm0)
+    w = wet(c)
+    ws = w[-1]
+    assert np.abs(f1[ws]).min() > 0 and not f1[~ws].any()
+    d = vd1 - vd0
+    dt = 1.0 / 96
+    for q, l in ((0, t.DIC), (1, t.O2)):
+        want = -dt * rM[-1] * f1[..., q]
+        assert np.allclose(d[-1, ..., l][ws], want[ws], rtol=1e-9, atol=1e-22), l
+        d[-1, ..., l] = 0.0
+        moles = (M[-1] * want)[ws].sum()
+        assert abs(moles + dt * f1[..., q].sum()) <= 1e-9 * abs(dt * f1[..., q].sum())
+    assert np.abs(d[w]).max() <= 1e-22
+    Vtot = 7777.0 * 4.0 * np.pi * 6.37e6 ** 2
+    dp = dt * f1.sum(axis=(0, 1)) * 8.3145 * 273.15 / Vtot / 1.01325e5
+    for q in (0, 1):
+        assert np.all(atm1[..., q] == atm1[0, 0, q])
+        assert abs((atm1[0, 0, q] - atm0[0, 0, q]) - dp[q]) <= 1e-9 * abs(dp[q]), q

@@ -66,6 +66,18 @@ MUTANTS = [
     ("cell masses scaled by Sratio instead of 1/Sratio",
      "st->M[C3(st, i, j, k)] = rSratio * st->M[C3(st, i, j, k)];",
      "st->M[C3(st, i, j, k)] = Sratio * st->M[C3(st, i, j, k)];", 1),
+    # air-sea exchange, biogem_box.f90:81-299, gem_carbchem.f90:1367-1426; atchem.f90:63-158
+    ("Schmidt number cubic term with the wrong sign",
+     "c[2] * TC2 - c[3] * TC3", "c[2] * TC2 + c[3] * TC3", 1),
+    ("O2 solubility without the density",
+     "return bunsen ? x / (rho * CONST_V) : x;", "return bunsen ? x / CONST_V : x;", 1),
+    ("flux limiter ratio inverted",
+     "fo = (AIRSEA_R_DFLUX_DEQM_MAX / r) * fo;", "fo = r * fo;", 1),
+    ("air-sea flux enters the ocean with the wrong sign",
+     "*r = *r + dtrM * (0.0 - st->fatm[2 * c]);", "*r = *r + dtrM * st->fatm[2 * c];", 1),
+    ("atmosphere mixed without the gas temperature",
+     "st->atm[2 * c + q] = (tot / totV) * CONV_PA_ATM * CONST_R_SI * CONST_ZEROC;",
+     "st->atm[2 * c + q] = (tot / totV) * CONV_PA_ATM * CONST_R_SI;", 1),
 ]
 
 # the pins (tests/test_oracle.py), cheapest first; the compiled-reference
@@ -82,6 +94,8 @@ PINS = [
     T.test_caco3_remin_closed_form, T.test_coupling_identity_and_single_cell_anomaly,
     T.test_coupling_uniform_salinity_anomaly, lambda: T.test_biogem_oracle_conserves_phosphorus("config2"),
     lambda: T.test_carbchem_matches_reference_vectors(0), T.test_oracle_conserves_inventory,
+    T.test_airsea_solubility_closed_forms, T.test_airsea_piston_velocity_wanninkhof,
+    T.test_airsea_bulk_flux_equilibrium_and_limit, T.test_airsea_step_conserves_and_atmosphere_mixes,
 ]
 
 
