@@ -44,6 +44,7 @@ module cgenie_hip
   integer(c_int32_t), save :: grid_id = -1, mem(1) = -1
   integer, save :: nx = 0, ny = 0, nz = 0, nl = 0
   logical, save :: mirror = .true.
+  logical, save :: airsea = .false.   ! CO2 and O2 exchange with the device atmosphere
   ! arrays whose addresses the grid descriptor holds during cgo_grid_create
   integer(c_int32_t), allocatable, target, save :: k1_c(:,:)
   real(c_double), allocatable, target, save :: jtab(:,:), ktab(:,:)
@@ -65,6 +66,8 @@ module cgenie_hip
   real(dp), parameter :: rfluxsc = rsc / (dsc_p * usc * rh0sc * cpsc)
   ! BIOGEM constants (gem_cmn.f90:510-513, 688, 696, 719) and tracer indices (72-129, 190-255)
   real(dp), parameter :: conv_yr_s = 3600.0_dp * (24.0_dp * 365.25_dp), const_pi = 3.141592653589793_dp
+  real(dp), parameter :: conv_s_yr = 1.0_dp / (3600.0_dp * (24.0_dp * 365.25_dp))
+  integer, parameter :: ia_pCO2 = 3, ia_pO2 = 6    ! gem_cmn.f90 atmospheric tracer indices
   real(dp), parameter :: const_rEarth = 6.37E+06_dp, nullsmall = 0.999999E-19_dp
   integer, parameter :: io_of_slot(6) = [3, 12, 8, 10, 15, 20]    ! DIC, ALK, PO4, O2, DOM_C, DOM_P
   integer, parameter :: io_Ca = 35, io_B = 36, io_F = 37, io_SO4 = 38
@@ -165,11 +168,16 @@ contains
   ! BIOGEM state of the member from its current go_ts.  slots(1:6) are the
   ! 1-based go_ts positions of DIC, ALK, PO4, O2, DOM_C, DOM_P (0 = absent);
   ! light = solfor/solar_constant and mld (m) are (maxi,maxj).
-  subroutine initialise_biogem_hip(slots, dtyr, light, mld, carb_tol)
+  ! With wspeed (m/s, par_phys_windspeed) the carried CO2 and O2 exchange
+  ! with the well-mixed atmosphere of ATCHEM on the device, starting from
+  ! atm_init = (pCO2, pO2) in atm (atchem-defaults.nml) and stepped by every
+  ! step_biogem; gas_a is par_gastransfer_a (biogem-defaults.nml:221).
+  subroutine initialise_biogem_hip(slots, dtyr, light, mld, carb_tol, wspeed, atm_init, gas_a)
     integer, intent(in) :: slots(6)
     real(c_double), intent(in) :: dtyr
     real(c_double), intent(in) :: light(nx, ny), mld(nx, ny)
     real(c_double), intent(in), optional :: carb_tol
+    real(c_double), intent(in), optional :: wspeed(nx, ny), atm_init(2), gas_a
     type(cgo_bgc_params) :: bp
     call cgo_bgc_default_params(bp, dtyr)
     bp%slot_DIC = int(slots(1) - 1, c_int32_t)
@@ -180,7 +188,17 @@ contains
     bp%slot_DOM_P = int(slots(6) - 1, c_int32_t)
     bp%caco3 = merge(1_c_int32_t, 0_c_int32_t, slots(1) > 0 .and. slots(2) > 0)
     if (present(carb_tol)) bp%carb_tol = carb_tol
+    if (present(wspeed)) then
+       bp%airsea = 1
+       if (present(atm_init)) then
+          bp%atm_pCO2 = atm_init(1)
+          bp%atm_pO2 = atm_init(2)
+       end if
+       if (present(gas_a)) bp%gastransfer_a = gas_a
+    end if
     call check(cgo_bgc_init(eng, mem(1), bp, light, mld), 'cgo_bgc_init')
+    if (present(wspeed)) call check(cgo_set_windspeed(eng, mem(1), wspeed), 'cgo_set_windspeed')
+    airsea = present(wspeed)
     bgc_slots = slots
     bgc_dtyr = dtyr
   end subroutine initialise_biogem_hip
@@ -190,6 +208,12 @@ contains
     real(c_double), intent(in), target :: sflux(nl, nx, ny)
     call check(cgo_set_sflux(eng, mem(1), c_loc(sflux)), 'cgo_set_sflux')
   end subroutine cgenie_hip_set_sflux
+
+  ! the device atmosphere: atm(1,:,:) pCO2, atm(2,:,:) pO2 in atm (ATCHEM's atm)
+  subroutine cgenie_hip_atmosphere(atm)
+    real(c_double), intent(out) :: atm(2, nx, ny)
+    call check(cgo_bgc_download(eng, mem(1), 6_c_int32_t, atm), 'cgo_bgc_download atmosphere')
+  end subroutine cgenie_hip_atmosphere
 
   subroutine cgenie_hip_set_mirror(on)
     logical, intent(in) :: on
@@ -330,7 +354,7 @@ contains
     real(c_double), intent(inout), dimension(:,:,:) :: dum_sfxatm1, dum_sfxocn1, dum_sfxsed1, dum_sfxsumrok1
     real(c_double), intent(out), dimension(:,:,:) :: dum_sfcocn1
     real(c_double) :: dtyr, S, A
-    real(c_double), allocatable :: ocn(:,:,:,:), vd(:,:,:,:), settle(:,:,:)
+    real(c_double), allocatable :: ocn(:,:,:,:), vd(:,:,:,:), settle(:,:,:), fatm(:,:,:)
     integer :: i, j, kb, q, l, io, is
 
     if (.not. (dum_dts > 0.0_c_double)) call check(CGO_E_ARG, 'step_biogem: dum_dts must be positive')
@@ -340,8 +364,9 @@ contains
     call check(cgo_bio_step(eng, mem, 1_c_int32_t), 'cgo_bio_step')
     dum_sfcocn1 = 0.0_dp
     if (.not. mirror) return
-    allocate (ocn(nl, nx, ny, nz), vd(nl, nx, ny, nz), settle(5, nx, ny))
+    allocate (ocn(nl, nx, ny, nz), vd(nl, nx, ny, nz), settle(5, nx, ny), fatm(2, nx, ny))
     call check(cgo_bgc_download(eng, mem(1), 0_c_int32_t, ocn), 'cgo_bgc_download ocn')
+    if (airsea) call check(cgo_bgc_download(eng, mem(1), 7_c_int32_t, fatm), 'cgo_bgc_download air-sea flux')
     call check(cgo_bgc_download(eng, mem(1), 3_c_int32_t, vd), 'cgo_bgc_download anomalies')
     call check(cgo_bgc_download(eng, mem(1), 5_c_int32_t, settle), 'cgo_bgc_download rain')
     do j = 1, ny
@@ -368,6 +393,12 @@ contains
           ! (3) ocean -> sediment flux (mol m-2 s-1); the bottom cell's area
           ! as sub_init_phys_ocn forms it (biogem_data.f90:1120-1128)
           A = 2.0_dp * const_pi * (const_rEarth * const_rEarth) * (1.0_dp / nx) * (jtab(j, 9) - jtab(j - 1, 9))
+          ! (1) ocean -> atmosphere flux of pCO2 and pO2 in mol m-2 s-1
+          ! (biogem.f90:1721-1727); ATCHEM itself runs on the device
+          if (airsea .and. size(dum_sfxatm1, 1) >= ia_pO2) then
+             dum_sfxatm1(ia_pCO2, i, j) = (1.0_dp / A) * conv_s_yr * fatm(1, i, j)
+             dum_sfxatm1(ia_pO2, i, j) = (1.0_dp / A) * conv_s_yr * fatm(2, i, j)
+          end if
           do q = 1, 5
              is = is_of(q)
              if (is > size(dum_sfxsed1, 1)) cycle

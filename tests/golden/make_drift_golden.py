@@ -5,7 +5,7 @@ kappa_v 2.7e-5 m2/s; seed 2024; 96,000 ocean steps).
 
 Stored: the wet-cell state (N_wet, 24) in float64 after year 1000, the
 SHA-256 of the float64 wet-cell state and the per-tracer inventories at
-years 1, 10, 100 and 1000.  The GPU test (tests/test_gpu_long.py) runs the
+years 1, 10, 100 and 1000.  The GPU test (tests/test_gpu_configs.py) runs the
 same configuration in FP64 (bit-identical to these) and in the
 FP32-compensated form (within 1e-6 per tracer).
 
