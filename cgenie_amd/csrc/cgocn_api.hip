@@ -472,7 +472,8 @@ static MemLayout layout(const GridH* g, int L, int prec, int carry, int bgc) {
     const bool lo = prec == CGO_FP32C && carry;
     m.lo[0] = lo ? take(L * ncell * 2) : 0;
     m.lo[1] = lo ? take(L * ncell * 2) : 0;
-    m.u = take((prec == CGO_FP64 ? 3 : 4) * ncell * es);  // FP32C: u, v, w and the FP64 divergence
+    // FP64: u, v, w; FP32C: the six face coefficients of the flux and the divergence
+    m.u = take((prec == CGO_FP64 ? 3 : 7) * ncell * es);
     m.sflux = take(L * ncol * 8);
     m.rho = take(ncell * 8);
     m.cost = take(ncol * 8);
@@ -642,15 +643,20 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
     if (m->prec == CGO_FP64) {
         HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), h.data(), h.size() * 8, hipMemcpyHostToDevice));
     } else {
-        // FP32 form: float velocities plus the discrete divergence of the open
-        // faces of each wet cell evaluated in FP64 from the FP64 velocities
-        // (the flux kernel works with q - q_cell and adds q_cell * div u)
+        // FP32 form: per wet cell the six face coefficients of the flux in
+        // coefficient form (the face flux is the neighbour difference times
+        // the coefficient) with the Peclet-weighted upwinding of
+        // goldstein.f90:2489-2523 folded in, and the discrete divergence of
+        // the open faces, all evaluated in FP64 from the FP64 velocities and
+        // rounded once (the flux kernel works with q - q_cell and adds
+        // q_cell * div u)
         const GridH* gh = e->grids[m->grid];
         const int JS = mj + 2, KS = mk + 2;
         const size_t ncol = g.ncol;
         auto K1 = [&](int I, int J) { return gh->k1[(size_t)J * (mi + 2) + I]; };
-        std::vector<float> f(4 * ncell, 0.0f);
-        for (size_t x = 0; x < 3 * ncell; x++) f[x] = (float)h[x];
+        const double diff1 = m->p.diff[0], diffv = m->p.diff[1];
+        auto upwind = [](double pec) { return pec / (2.0 + std::fabs(pec)); };
+        std::vector<float> f(7 * ncell, 0.0f);
         for (int k = 1; k <= mk; k++)
             for (int J = 1; J <= mj; J++)
                 for (int I = 1; I <= mi; I++) {
@@ -670,7 +676,21 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
                     const double wb = k > k1c ? h[2 * ncell + c - ncol] : 0.0;
                     const double div = ((oe ? ue * rc : 0.0) - (ow ? uw * rc : 0.0)) * g.rdphi +
                                        ((on ? cvn * un : 0.0) - (os ? cvs * us : 0.0)) * rds + (ua - wb) * rdz;
-                    f[3 * ncell + c] = (float)div;
+                    const double rc2 = gh->mj[MJ_RC2 * JS + J];
+                    const double cv2n = gh->mj[MJ_CV2 * JS + J], cv2s = gh->mj[MJ_CV2 * JS + J - 1];
+                    const double dsvn = gh->mj[MJ_DSVM * JS + J], dsvs = gh->mj[MJ_DSVM * JS + J - 1];
+                    const double dza = gh->mk[MK_DZA * KS + k], rdza = gh->mk[MK_RDZA * KS + k];
+                    const double upe = upwind(ue * g.dphi / diff1), upw = upwind(uw * g.dphi / diff1);
+                    const double upn = upwind(un * dsvn / diff1), ups = upwind(us * dsvs / diff1);
+                    const double upa = upwind(ua * dza / diffv);
+                    const bool top = k == mk;
+                    f[c] = oe ? (float)((ue * rc * (1.0 - upe) * 0.5 - rc2 * diff1) * g.rdphi) : 0.0f;
+                    f[ncell + c] = ow ? (float)((uw * rc * (1.0 + upw) * 0.5 + rc2 * diff1) * g.rdphi) : 0.0f;
+                    f[2 * ncell + c] = on ? (float)((cvn * un * (1.0 - upn) * 0.5 - cv2n * diff1) * rds) : 0.0f;
+                    f[3 * ncell + c] = os ? (float)((cvs * us * (1.0 + ups) * 0.5 + cv2s * diff1) * rds) : 0.0f;
+                    f[4 * ncell + c] = top ? 0.0f : (float)(ua * (1.0 - upa) * 0.5 - rdza * diffv);
+                    f[5 * ncell + c] = top ? 0.0f : (float)(-(ua * (1.0 + upa) * 0.5) - rdza * diffv);
+                    f[6 * ncell + c] = (float)div;
                 }
         HIPCHK(e, hipMemcpy(const_cast<void*>(m->dev.u), f.data(), f.size() * 4, hipMemcpyHostToDevice));
     }

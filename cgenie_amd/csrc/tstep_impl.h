@@ -207,7 +207,6 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     const R dphi = (R)g.dphi, rdphi = (R)g.rdphi;
     const R diff1 = sizeof(R) == 8 ? (R)M.diff1 : (R)M.diff1f;
     const R diffv = sizeof(R) == 8 ? (R)M.diff2 : (R)M.diff2f;
-    const R rdiff1 = (R)1.0 / diff1, rdiffv = (R)1.0 / diffv;
     const R e0 = (R)g.ec[0], e1 = (R)g.ec[1], e2 = (R)g.ec[2], e3 = (R)g.ec[3];
     const R scc = M.diso ? e1 : (R)0;
 
@@ -262,20 +261,49 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
 
     for (int q = 0; q < LB; q++) FB(q) = (R)0;
     int32_t limps = 0;
-    R dmaxv = (R)0;
+    double dmaxv = 0.0;
+    // The isoneutral slope terms are formed in FP64 in both forms.  The
+    // density gradients they divide by are differences of nearly equal
+    // terms, so the FP32 form takes its staged values with their residuals
+    // and the FP64 metrics, and rounds the finished coefficients to float
+    // once; in float they lose up to 1e-4 of their value where the
+    // stratification is weak.  The FP64 form evaluates the restatement's
+    // expressions on the same values, bit for bit.
+    const double rcD = gp(g.mj)[MJ_RC * JS + J], cvnD = gp(g.mj)[MJ_CV * JS + J];
+    const double cvsD = gp(g.mj)[MJ_CV * JS + J - 1];
+    const double rdsvnD = gp(g.mj)[MJ_RDSV * JS + J], rdsvsD = gp(g.mj)[MJ_RDSV * JS + J - 1];
+    const double rdphiD = g.rdphi, diff1D = M.diff1;
+    const double e0D = g.ec[0], e1D = g.ec[1], e2D = g.ec[2], e3D = g.ec[3];
+    const double sccD = M.diso ? e1D : 0.0;
+    auto SPD = [&](int buf, int q, int pp) -> double {
+        if (ANOM) return (double)SP(buf, q, pp) + (double)SR(buf, q, pp);
+        return (double)SP(buf, q, pp);
+    };
+    auto DD = [&](int buf, int q, int pa, int pb) -> double {
+        if (ANOM)
+            return ((double)SP(buf, q, pa) - (double)SP(buf, q, pb)) + ((double)SR(buf, q, pa) - (double)SR(buf, q, pb));
+        return (double)(SP(buf, q, pa) - SP(buf, q, pb));
+    };
+    auto DKD = [&](int bu, int bc, int q, int pp) -> double {
+        if (ANOM)
+            return ((double)SP(bu, q, pp) - (double)SP(bc, q, pp)) + ((double)SR(bu, q, pp) - (double)SR(bc, q, pp));
+        return (double)(SP(bu, q, pp) - SP(bc, q, pp));
+    };
     // the isoneutral slope terms of a cell at level k < maxk from the staged
     // T and S (2489-2501, 2561-2605): dzrho, the four corner gradients, the
     // slope limiter; counts limps and the diffusive stability number
-    auto iso_coeffs = [&](int k, int cb, int ub, R rdza, R dt, bool& iso, R& dzrho, R& rdzrho, R& slim, R* dxr,
-                          R* dyr) {
-        const R Tc = SP(cb, qT, p), Tu = SP(ub, qT, p);
-        const R tatw = (R)0.5 * (Tc + Tu);
-        const R tec = -e0 - e2 * tatw * 2 - e3 * tatw * tatw * 3;
-        dzrho = (e1 * DK(ub, cb, qS, p) - tec * DK(ub, cb, qT, p)) * rdza;
-        if (M.diso && dzrho < (R)-1.0E-12) {
+    auto iso_coeffs = [&](int k, int cb, int ub, bool& iso, double& dzrho, double& rdzrho, double& slim,
+                          double* dxr, double* dyr) {
+        const AS_GLOBAL double* mkD = gp(g.mk);
+        const double rdza = mkD[MK_RDZA * KS + k], dt = mkD[MK_DT * KS + k];
+        const double Tc = SPD(cb, qT, p), Tu = SPD(ub, qT, p);
+        const double tatw = 0.5 * (Tc + Tu);
+        const double tec = -e0D - e2D * tatw * 2 - e3D * tatw * tatw * 3;
+        dzrho = (e1D * DKD(ub, cb, qS, p) - tec * DKD(ub, cb, qT, p)) * rdza;
+        if (M.diso && dzrho < -1.0E-12) {
             iso = true;
-            rdzrho = fdiv((R)1.0, dzrho);
-            R tv1 = 0;
+            rdzrho = fdiv(1.0, dzrho);
+            double tv1 = 0;
             // ina order (knp, nnp) = (0,0), (0,1), (1,0), (1,1)
 #pragma unroll
             for (int n = 0; n < 4; n++) {
@@ -283,45 +311,48 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 const int kn = n < 2 ? k : k + 1;
                 const bool east = n & 1;
                 const bool mx = kn >= (east ? k1e : k1w), my = kn >= (east ? k1n : k1s);
-                const R dxT = mx ? (east ? D(b, qT, pe, p) : D(b, qT, p, pw)) * rc * rdphi : (R)0;
-                const R dxS = mx ? (east ? D(b, qS, pe, p) : D(b, qS, p, pw)) * rc * rdphi : (R)0;
-                const R dyT = my ? (east ? D(b, qT, pn, p) * cvn * rdsvn : D(b, qT, p, ps) * cvs * rdsvs) : (R)0;
-                const R dyS = my ? (east ? D(b, qS, pn, p) * cvn * rdsvn : D(b, qS, p, ps) * cvs * rdsvs) : (R)0;
-                dxr[n] = scc * dxS - tec * dxT;
-                dyr[n] = scc * dyS - tec * dyT;
+                const double dxT = mx ? (east ? DD(b, qT, pe, p) : DD(b, qT, p, pw)) * rcD * rdphiD : 0.0;
+                const double dxS = mx ? (east ? DD(b, qS, pe, p) : DD(b, qS, p, pw)) * rcD * rdphiD : 0.0;
+                const double dyT = my ? (east ? DD(b, qT, pn, p) * cvnD * rdsvnD : DD(b, qT, p, ps) * cvsD * rdsvsD) : 0.0;
+                const double dyS = my ? (east ? DD(b, qS, pn, p) * cvnD * rdsvnD : DD(b, qS, p, ps) * cvsD * rdsvsD) : 0.0;
+                dxr[n] = sccD * dxS - tec * dxT;
+                dyr[n] = sccD * dyS - tec * dyT;
                 tv1 = tv1 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
             }
-            tv1 = (R)0.25 * tv1 * rdzrho * rdzrho;
-            const R ssmax = mk[MK_SSMAX * KS + k];
+            tv1 = 0.25 * tv1 * rdzrho * rdzrho;
+            const double ssmax = mkD[MK_SSMAX * KS + k];
             if (tv1 > ssmax) {
                 slim = fdiv(ssmax * ssmax, tv1 * tv1);
                 limps++;
             } else {
                 slim = 1;
             }
-            tv1 = tv1 * slim * diff1 * rdza;
-            const R tvd = tv1 * dt * rdza;
+            tv1 = tv1 * slim * diff1D * rdza;
+            const double tvd = tv1 * dt * rdza;
             if (tvd > dmaxv) dmaxv = tvd;
         }
     };
     // the FP32 form's isoneutral coefficients: corner differences times ax, ay
-    // (the sign of the west/south corners folded in) and the vertical qq
-    auto fp32_iso = [&](bool iso, R slim, R rdzrho, R rdza, const R* dxr, const R* dyr, R* ax, R* ay, R& qq) {
+    // (the sign of the west/south corners folded in) and the vertical qq,
+    // formed in FP64 and rounded once
+    auto fp32_iso = [&](int k, bool iso, double slim, double rdzrho, const double* dxr, const double* dyr, R* ax,
+                        R* ay, R& qq) {
 #pragma unroll
         for (int n = 0; n < 4; n++) ax[n] = ay[n] = (R)0;
         qq = (R)0;
         if (iso) {
-            const R isoc = (R)0.25 * slim * diff1;
-            const R f = (R)2 * isoc * rdzrho;
-            R s2 = 0;
+            const double rdza = gp(g.mk)[MK_RDZA * KS + k];
+            const double isoc = 0.25 * slim * diff1D;
+            const double f = 2 * isoc * rdzrho;
+            double s2 = 0;
 #pragma unroll
             for (int n = 0; n < 4; n++) {
                 const bool east = n & 1;
-                ax[n] = f * dxr[n] * rc * rdphi * (east ? (R)1 : (R)-1);
-                ay[n] = f * dyr[n] * (east ? cvn * rdsvn : -(cvs * rdsvs));
+                ax[n] = (R)(f * dxr[n] * rcD * rdphiD * (east ? 1.0 : -1.0));
+                ay[n] = (R)(f * dyr[n] * (east ? cvnD * rdsvnD : -(cvsD * rdsvsD)));
                 s2 = s2 + dxr[n] * dxr[n] + dyr[n] * dyr[n];
             }
-            qq = isoc * s2 * (rdzrho * rdzrho) * rdza;
+            qq = (R)(isoc * s2 * (rdzrho * rdzrho) * rdza);
         }
     };
     // Members with BIOGEM: the coupling's column sums of the new state
@@ -357,14 +388,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         if (IM == 1 && active && k >= k1c && !top && M.diso) {
             // the pre-pass: the cell's isoneutral coefficients, once
             const uint32_t oc = (uint32_t)(kz * plane + col);
-            const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k], dt = mk[MK_DT * KS + k];
-            (void)dza;
             bool iso = false;
-            R dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
-            iso_coeffs(k, cb, ub, rdza, dt, iso, dzrho, rdzrho, slim, dxr, dyr);
+            double dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            iso_coeffs(k, cb, ub, iso, dzrho, rdzrho, slim, dxr, dyr);
             if constexpr (ANOM) {
                 R ax[4], ay[4], qq;
-                fp32_iso(iso, slim, rdzrho, rdza, dxr, dyr, ax, ay, qq);
+                fp32_iso(k, iso, slim, rdzrho, dxr, dyr, ax, ay, qq);
                 AS_GLOBAL float* ic = reinterpret_cast<AS_GLOBAL float*>(gp(M.isoc));
 #pragma unroll
                 for (int n = 0; n < 4; n++) {
@@ -385,23 +414,19 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
         }
         if (IM != 1 && active && k >= k1c) {
             const int o = kz * plane;
-            // face velocities and Peclet-weighted upwinding (goldstein.f90:2489-2523)
+            // face velocities and Peclet-weighted upwinding (goldstein.f90:2489-2523);
+            // the FP32 form reads its face coefficients instead (below)
             const uint32_t oc = (uint32_t)(o + col);
-            const R ue_ = at(u, oc), uw_ = at(u, (uint32_t)(o + cw));
-            const R un_ = at(u, ncell + oc), us_ = at(u, (uint32_t)(ncell + o + cs));
-            const R ua_ = at(u, 2 * ncell + oc);
             const R dza = mk[MK_DZA * KS + k], rdza = mk[MK_RDZA * KS + k];
             const R dt = mk[MK_DT * KS + k], rdz = mk[MK_RDZ * KS + k];
-            R upe, upw, upn, ups, upa;
-            if constexpr (ANOM) {
-                // FP32 form: pec = u dx / kappa with the reciprocal of kappa
-                auto upw_ = [](float pc) { return __fdividef(pc, 2.0f + fabsf(pc)); };
-                upe = upw_(ue_ * dphi * rdiff1);
-                upw = upw_(uw_ * dphi * rdiff1);
-                upn = upw_(un_ * dsvn * rdiff1);
-                ups = upw_(us_ * dsvs * rdiff1);
-                upa = upw_(ua_ * dza * rdiffv);
-            } else {
+            R ue_ = 0, uw_ = 0, un_ = 0, us_ = 0, ua_ = 0;
+            R upe = 0, upw = 0, upn = 0, ups = 0, upa = 0;
+            if constexpr (!ANOM) {
+                ue_ = at(u, oc);
+                uw_ = at(u, (uint32_t)(o + cw));
+                un_ = at(u, ncell + oc);
+                us_ = at(u, (uint32_t)(ncell + o + cs));
+                ua_ = at(u, 2 * ncell + oc);
                 R pec;
                 pec = ue_ * dphi / diff1; upe = pec / ((R)2.0 + fabs(pec));
                 pec = uw_ * dphi / diff1; upw = pec / ((R)2.0 + fabs(pec));
@@ -413,7 +438,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
             const bool on = k >= max(k1c, k1n), os = k >= max(k1s, k1c);
             // isoneutral coefficients from T and S (2489-2501, 2561-2605)
             bool iso = false;
-            R dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
+            double dzrho = 0, rdzrho = 0, slim = 1, dxr[4] = {0, 0, 0, 0}, dyr[4] = {0, 0, 0, 0};
             R ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, qq = 0;  // the FP32 form's coefficients
             if (!top) {
                 if constexpr (IM == 2) {
@@ -443,13 +468,13 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                         }
                     }
                 } else {
-                    iso_coeffs(k, cb, ub, rdza, dt, iso, dzrho, rdzrho, slim, dxr, dyr);
-                    if constexpr (ANOM) fp32_iso(iso, slim, rdzrho, rdza, dxr, dyr, ax, ay, qq);
+                    iso_coeffs(k, cb, ub, iso, dzrho, rdzrho, slim, dxr, dyr);
+                    if constexpr (ANOM) fp32_iso(k, iso, slim, rdzrho, dxr, dyr, ax, ay, qq);
                 }
             }
             // discrete divergence of the open faces (FP32 form only)
             // (evaluated in FP64 from the FP64 velocities when they are set)
-            const R divu = ANOM ? at(u, 3 * ncell + oc) : (R)0;
+            const R divu = ANOM ? at(u, 6 * ncell + oc) : (R)0;
             const R isoc = (R)0.25 * slim * diff1;
             double newT = 0, newS = 0;
             double s_os = 0.0, s_mm = 0.0, s_vv = 0.0;
@@ -462,13 +487,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                 // FP32 form in coefficient form: each face flux is the
                 // neighbour difference times a coefficient of the cell, and the
                 // isoneutral flux a sum of corner differences times
-                // coefficients, formed once per cell for all tracers
-                const R ce = oe ? (ue_ * rc * ((R)1.0 - upe) * (R)0.5 - rc2 * diff1) * rdphi : (R)0;
-                const R cw = ow ? (uw_ * rc * ((R)1.0 + upw) * (R)0.5 + rc2 * diff1) * rdphi : (R)0;
-                const R cn = on ? (cvn * un_ * ((R)1.0 - upn) * (R)0.5 - cv2n * diff1) * rds : (R)0;
-                const R cs = os ? (cvs * us_ * ((R)1.0 + ups) * (R)0.5 + cv2s * diff1) * rds : (R)0;
-                const R ca = top ? (R)0 : ua_ * ((R)1.0 - upa) * (R)0.5 - rdza * diffv;
-                const R cbn = top ? (R)0 : -(ua_ * ((R)1.0 + upa) * (R)0.5) - rdza * diffv;
+                // coefficients.  The face coefficients depend on the velocity,
+                // the diffusivities and the mask only: cgo_set_velocity forms
+                // them in FP64 and rounds each once (closed faces are zero)
+                const R ce = at(u, oc), cw = at(u, ncell + oc);
+                const R cn = at(u, 2 * ncell + oc), cs = at(u, 3 * ncell + oc);
+                const R ca = at(u, 4 * ncell + oc), cbn = at(u, 5 * ncell + oc);
 #pragma unroll
                 for (int q = 0; q < LB; q++) {
                     if (q >= nl) break;
@@ -597,7 +621,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
     }
     if (IM != 2 && l0 == 0 && M.diso && active) {
         if (limps) atomicAdd(&M.diag[0], limps);
-        if (dmaxv > (R)0) atomic_max_pos(M.dmax, (double)dmaxv);
+        if (dmaxv > 0.0) atomic_max_pos(M.dmax, dmaxv);
     }
     if (sums && active && k1c <= maxk) {
         const int slot = gp(g.cslot)[col];
