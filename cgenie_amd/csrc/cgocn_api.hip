@@ -60,6 +60,9 @@ struct MemberH {
     int grid = -1, L = 0, prec = 0, bgc = 0;
     int wstream = 4;  // particle-stream window of the BIOGEM march (4, 5, 6, 8 or 16)
     bool coupling_due = false;  // cgo_bio_step ran; cgo_tracercoupling must follow
+    // a step, graph or kernel of this member may still be running on the
+    // engine stream (set when one is enqueued, cleared when the stream drains)
+    bool queued = false;
     MemberDev dev{};
     cgo_member_params p{};
     cgo_bgc_params bp{};
@@ -96,6 +99,9 @@ struct cgo_engine {
     std::string err;
     std::vector<GridH*> grids;
     std::vector<MemberH*> members;
+    // device buffers given up while other members may be stepping: hipFree
+    // waits for the whole device, so they are freed when the stream drains
+    std::vector<void*> graveyard;
     MemberDev* d_table = nullptr;
     bool table_dirty = false;
     // per member slot: bumped whenever the slot's member is created,
@@ -223,6 +229,7 @@ int cgo_destroy(cgo_engine* e) {
     if (!e) return CGO_E_ARG;
     hipSetDevice(e->device);
     hipStreamSynchronize(e->s);
+    for (void* p : e->graveyard) hipFree(p);
     for (auto& g : e->graphs) {
         if (g.exec) hipGraphExecDestroy(g.exec);
         if (g.graph) hipGraphDestroy(g.graph);
@@ -256,10 +263,30 @@ int cgo_last_error(const cgo_engine* e, char* buf, size_t n) {
     return CGO_OK;
 }
 
+}  // extern "C"
+
+// Wait for everything enqueued on the engine stream, then free the deferred
+// buffers: no member has work in flight afterwards.
+static int drain(cgo_engine* e) {
+    HIPCHK(e, hipStreamSynchronize(e->s));
+    for (auto* x : e->members)
+        if (x) x->queued = false;
+    for (void* p : e->graveyard) hipFree(p);
+    e->graveyard.clear();
+    return CGO_OK;
+}
+
+// Before the host reads or rewrites member m's device memory: wait for the
+// stream only if m may have work in flight.  A member being set up (created
+// since the stream last drained and not yet stepped) is filled while the
+// other members step.
+static int settle(cgo_engine* e, MemberH* m) { return m->queued ? drain(e) : CGO_OK; }
+
+extern "C" {
+
 int cgo_sync(cgo_engine* e) {
     if (!e) return CGO_E_ARG;
-    HIPCHK(e, hipStreamSynchronize(e->s));
-    return CGO_OK;
+    return drain(e);
 }
 
 int cgo_grid_create(cgo_engine* e, const cgo_grid_desc* d, int32_t* grid_id) {
@@ -523,7 +550,7 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
     HIPCHK(e, hipMemset(buf, 0, ly.total));
     if (old) {
         HIPCHK(e, hipMemcpy(buf, keep.data(), oldly.total, hipMemcpyHostToDevice));
-        hipFree(old);
+        e->graveyard.push_back(old);  // the caller settled the member: nothing in flight reads it
     }
     m->dbuf = buf;
     m->bytes = ly.total;
@@ -617,8 +644,9 @@ int cgo_member_create(cgo_engine* e, int32_t grid_id, int32_t maxl, int32_t prec
 int cgo_member_destroy(cgo_engine* e, int32_t id) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "member_destroy: bad member %d", id);
-    HIPCHK(e, hipStreamSynchronize(e->s));
-    if (m->dbuf) hipFree(m->dbuf);
+    int rc = settle(e, m);
+    if (rc) return rc;
+    if (m->dbuf) e->graveyard.push_back(m->dbuf);
     m->dbuf = nullptr;
     m->alive = false;
     e->table_dirty = true;
@@ -629,6 +657,7 @@ int cgo_member_destroy(cgo_engine* e, int32_t id) {
 int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
     MemberH* m = member(e, id);
     if (!m || !u) return fail(e, CGO_E_ARG, "set_velocity: bad argument");
+    if (int rc = settle(e, m)) return rc;
     const GridDev& g = m->dev.g;
     const int mi = g.maxi, mj = g.maxj, mk = g.maxk;
     const size_t ncell = g.ncell;
@@ -700,6 +729,7 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
 int cgo_set_sflux(cgo_engine* e, int32_t id, const double* sflux) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "set_sflux: bad member");
+    if (int rc = settle(e, m)) return rc;
     const GridH* g = e->grids[m->grid];
     MemLayout ly = layout(g, m->L, m->prec, m->p.carry, m->bgc);
     if (!sflux) {
@@ -730,22 +760,28 @@ static void to_device_order(const GridDev& g, int L, const double* go, std::vect
 int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
     MemberH* m = member(e, id);
     if (!m || !go_ts) return fail(e, CGO_E_ARG, "upload_state: bad argument");
-    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (int rc = settle(e, m)) return rc;
     const GridDev& g = m->dev.g;
     std::vector<double> h;
     to_device_order(g, m->L, go_ts, h);
     const size_t n = h.size();
+    std::vector<float> f;
+    std::vector<int16_t> lo;
+    if (m->prec != CGO_FP64) {
+        f.resize(n);
+        lo.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            f[i] = (float)h[i];
+            // (h - f) / carry_unit(f), the division by a power of two done exactly
+            const int ex = f[i] == 0.0f ? -126 : std::max(ilogbf(f[i]), -126);
+            const double r = rint(ldexp(h[i] - (double)f[i], 39 - ex));
+            lo[i] = (int16_t)std::min(std::max(r, -32767.0), 32767.0);
+        }
+    }
     for (int q = 0; q < 2; q++) {
         if (m->prec == CGO_FP64) {
             HIPCHK(e, hipMemcpy(m->dev.st[q], h.data(), n * 8, hipMemcpyHostToDevice));
         } else {
-            std::vector<float> f(n);
-            std::vector<int16_t> lo(n);
-            for (size_t i = 0; i < n; i++) {
-                f[i] = (float)h[i];
-                double r = rint((h[i] - (double)f[i]) / carry_unit_h(f[i]));
-                lo[i] = (int16_t)std::min(std::max(r, -32767.0), 32767.0);
-            }
             HIPCHK(e, hipMemcpy(m->dev.st[q], f.data(), n * 4, hipMemcpyHostToDevice));
             if (m->dev.lo[q]) HIPCHK(e, hipMemcpy(m->dev.lo[q], lo.data(), n * 2, hipMemcpyHostToDevice));
         }
@@ -760,7 +796,7 @@ static int download_dev(cgo_engine* e, MemberH* m, std::vector<double>& h) {
     const GridDev& g = m->dev.g;
     const size_t n = (size_t)m->L * g.ncell;
     int32_t par = 0;
-    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (int rc = settle(e, m)) return rc;
     HIPCHK(e, hipMemcpy(&par, m->dev.parity, 4, hipMemcpyDeviceToHost));
     h.resize(n);
     if (m->prec == CGO_FP64) {
@@ -945,7 +981,7 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
     if (n < sizeof h + h.payload) return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n,
                                               (size_t)(sizeof h + h.payload));
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (int rc = settle(e, m)) return rc;
     int32_t slot = id;
     if (h.bgc && !m->bgc) {
         m->bgc = 1;
@@ -1061,7 +1097,7 @@ int cgo_bgc_init(cgo_engine* e, int32_t id, const cgo_bgc_params* bp, const doub
             return fail(e, CGO_E_ARG, "bgc_init: species slot %d invalid for maxl %d", slots[b], m->L);
     if (bp->caco3 && (bp->slot_DIC < 0 || bp->slot_ALK < 0)) return fail(e, CGO_E_ARG, "bgc_init: CaCO3 needs DIC and ALK");
     if (m->L > 256) return fail(e, CGO_E_ARG, "bgc_init: at most 256 tracers with BIOGEM (maxl %d)", m->L);
-    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (int rc0 = settle(e, m)) return rc0;
     std::vector<double> ts;
     int rc = download_dev(e, m, ts);  // current state, device order
     if (rc) return rc;
@@ -1282,7 +1318,7 @@ int cgo_set_windspeed(cgo_engine* e, int32_t id, const double* wspeed) {
     for (size_t c = 0; c < ncol; c++)
         if (!(wspeed[c] >= 0.0) || !std::isfinite(wspeed[c]))
             return fail(e, CGO_E_ARG, "set_windspeed: wind speed %g at column %zu", wspeed[c], c);
-    HIPCHK(e, hipStreamSynchronize(e->s));
+    if (int rc = settle(e, m)) return rc;
     HIPCHK(e, hipMemcpy(m->dev.wspeed, wspeed, ncol * 8, hipMemcpyHostToDevice));
     return CGO_OK;
 }
@@ -1436,7 +1472,9 @@ static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
     if ((rc = sync_table(e))) return rc;
-    return upload_lists(e, ls);
+    if ((rc = upload_lists(e, ls))) return rc;
+    for (int i = 0; i < n; i++) member(e, ids[i])->queued = true;
+    return CGO_OK;
 }
 
 int cgo_step(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
@@ -1551,6 +1589,7 @@ int cgo_graph_launch(cgo_engine* e, int32_t gid, int32_t nreps) {
     // the graph reads the member table at run time: bring it up to date
     // (e.g. a surface-flux buffer attached since the capture)
     if ((rc = sync_table(e))) return rc;
+    for (int32_t id : gr.ids) member(e, id)->queued = true;
     for (int r = 0; r < nreps; r++) HIPCHK(e, hipGraphLaunch(gr.exec, e->s));
     return CGO_OK;
 }

@@ -110,9 +110,10 @@ def request_inputs(req: Request, g: G.Grid, k1: np.ndarray, base_u: np.ndarray):
 
 class Server:
     def __init__(self, eng: E.Engine, hbm_budget: float = 200e9, round_steps: int = 48,
-                 keep_results: bool = False):
+                 keep_results: bool = False, max_active: int | None = None):
         self.eng = eng
         self.budget = hbm_budget
+        self.max_active = max_active  # at most this many runs on the GPU at once (None: memory decides)
         self.round_steps = round_steps
         self.keep = keep_results
         self.grids = GridCache(eng)
@@ -127,6 +128,8 @@ class Server:
         self.prep: dict = {}
         self.graph, self.gkey = None, None   # the round graph and the (members, steps) it was captured for
         self.captures = 0
+        self.t_admit = self.t_capture = self.t_gpu = 0.0   # wall seconds spent admitting, capturing, stepping
+        self.t_calls: dict = {}       # wall seconds per engine call made while admitting and retiring
         self.on_round = None                 # called with the server after every round (a scheduling hook)
 
     # -- requests
@@ -147,6 +150,13 @@ class Server:
         return self.prep[req.rid]
 
     def _admit(self):
+        t0 = time.perf_counter()
+        try:
+            self._admit_ready()
+        finally:
+            self.t_admit += time.perf_counter() - t0
+
+    def _admit_ready(self):
         now = self.now()
         for _, _, r in self.waiting:
             if r.arrival <= now:
@@ -159,25 +169,35 @@ class Server:
             nb = self._bytes(req)
             if self.used + nb > self.budget:
                 break                    # first come, first served: wait for memory
+            if self.max_active is not None and len(self.active) >= self.max_active:
+                break                    # the batch is full: wait for a run to finish
             heapq.heappop(self.waiting)
             gid = self.grids.get(req.dims)[0]
             tset, ts, u, diff, light, mld = self.prep[req.rid].result()
-            m = self.eng.add_member(gid, req.L, req.precision, diff, diso=True, carry=True)
+            m = self._call("add_member", self.eng.add_member, gid, req.L, req.precision, diff, diso=True,
+                           carry=True)
             if req.checkpoint is not None:
                 # a suspended run: its exact state, velocities and BIOGEM arrays included
-                self.eng.restore(m, req.checkpoint)
+                self._call("restore", self.eng.restore, m, req.checkpoint)
                 req.checkpoint = None
             else:
-                self.eng.set_velocity(m, u)
-                self.eng.upload(m, ts)
+                self._call("set_velocity", self.eng.set_velocity, m, u)
+                self._call("upload", self.eng.upload, m, ts)
                 if req.L >= 7:
                     p = E.default_bgc_params(1.0 / req.nyear, tset, carb_tol=1e-12, carb_maxit=1000)
-                    self.eng.bgc_init(m, p, light, mld)
+                    self._call("bgc_init", self.eng.bgc_init, m, p, light, mld)
                 req.admitted = self.now()
             req.member, req.nbytes = m, nb
             req._shape = ts.shape
             self.used += nb
             self.active[m] = req
+
+    def _call(self, name, fn, *a, **kw):
+        t0 = time.perf_counter()
+        try:
+            return fn(*a, **kw)
+        finally:
+            self.t_calls[name] = self.t_calls.get(name, 0.0) + time.perf_counter() - t0
 
     def suspend(self, req: Request):
         """Take an active run off the GPU between rounds: keep its exact
@@ -195,7 +215,7 @@ class Server:
         if self.keep:
             req.result = self.eng.download(req.member, req._shape)
             req.inventory = self.eng.inventory(req.member)
-        self.eng.remove_member(req.member)
+        self._call("remove_member", self.eng.remove_member, req.member)
         self.used -= req.nbytes
         req.finished = self.now()
         del self.active[req.member]
@@ -206,7 +226,8 @@ class Server:
         """Serve until every submitted request is done (or max_seconds pass)."""
         self.t0 = time.perf_counter() if self.t0 is None else self.t0
         while True:
-            self._admit()
+            if not self.active:
+                self._admit()
             if not self.active:
                 if not self.waiting or (max_seconds is not None and self.now() > max_seconds):
                     break
@@ -218,10 +239,17 @@ class Server:
             n = min(self.round_steps, min(r.total_steps - r.steps_done for r in self.active.values()))
             ids = list(self.active.keys())
             self._round(ids, n)
+            # waiting runs are set up while the GPU steps this round (in the
+            # room the last round's finished runs left): the engine waits for
+            # its stream only for members with work in flight, and a new
+            # member has none.  They join the next round.
+            self._admit()
+            t1 = time.perf_counter()
             self.eng.sync()
+            self.t_gpu += time.perf_counter() - t1
             t = self.now()
             self.rounds += 1
-            for r in list(self.active.values()):
+            for r in [self.active[m] for m in ids]:
                 before = r.steps_done // r.nyear
                 r.steps_done += n
                 for _ in range(r.steps_done // r.nyear - before):
@@ -237,6 +265,13 @@ class Server:
         return self.stats()
 
     def _capture(self, ids, n):
+        t0 = time.perf_counter()
+        try:
+            self._capture_now(ids, n)
+        finally:
+            self.t_capture += time.perf_counter() - t0
+
+    def _capture_now(self, ids, n):
         if self.graph is not None:
             self.eng.drop_graph(self.graph)
             self.graph = None
@@ -275,6 +310,8 @@ class Server:
         wall = self.now()
         return dict(wall_s=wall, member_years=member_years, member_years_per_s=member_years / max(wall, 1e-9),
                     equiv_36x36x16x24_member_years_per_s=weighted / max(wall, 1e-9), graph_captures=self.captures,
+                    admit_s=self.t_admit, capture_s=self.t_capture, gpu_wait_s=self.t_gpu,
+                    call_s={k: round(v, 3) for k, v in self.t_calls.items()},
                     requests_done=len(self.done), requests_active=len(self.active),
                     requests_waiting=len(self.waiting), rounds=self.rounds,
                     p50_year_s=float(np.percentile(lat, 50)) if lat else None,

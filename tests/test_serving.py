@@ -103,6 +103,23 @@ def test_scheduler_admits_by_budget_and_recycles(monkeypatch):
     assert st["equiv_36x36x16x24_member_years_per_s"] * 6 == pytest.approx(st["member_years_per_s"])
 
 
+def test_batch_cap_limits_active_runs(monkeypatch):
+    """max_active: with memory for all, at most that many runs share a round;
+    the rest wait and join as runs finish, first come first served."""
+    monkeypatch.setattr(S, "request_inputs", lambda req, g, k1, u: (None, np.zeros(1), None, (0, 0), None, None))
+    monkeypatch.setattr(S.GridCache, "get", lambda self, dims: (0, None, None, None))
+    eng = CountingEngine(bytes_per_member=1)
+    srv = S.Server(eng, hbm_budget=1e9, round_steps=48, max_active=2)
+    srv.pool = SyncPool()
+    reqs = [S.Request(q, (36, 36, 16), 4, years=1 + q, precision=E.FP64, seed=q) for q in range(5)]
+    for r in reqs:
+        srv.submit(r)
+    st = srv.run()
+    assert st["requests_done"] == 5
+    assert max(len(ids) for ids, _ in eng.calls) == 2
+    assert [r.rid for r in sorted(reqs, key=lambda r: r.admitted)] == [0, 1, 2, 3, 4]
+
+
 def test_suspended_run_resumes_where_it_stopped(monkeypatch):
     """A run suspended between rounds leaves the GPU (its memory is
     returned to the budget), waits like a new request and, restored from its
