@@ -38,7 +38,7 @@ module cgenie_hip
   public :: initialise_goldstein_hip, initialise_biogem_hip, end_cgenie_hip
   public :: tstepo, step_goldstein_tracers, step_biogem, biogem_tracercoupling
   public :: cgenie_hip_set_mirror, cgenie_hip_download, cgenie_hip_inventory, cgenie_hip_diag
-  public :: cgenie_hip_set_sflux, step_goldstein
+  public :: cgenie_hip_set_sflux, step_goldstein, cgenie_hip_atmosphere
 
   type(c_ptr), save :: eng = c_null_ptr
   integer(c_int32_t), save :: grid_id = -1, mem(1) = -1
