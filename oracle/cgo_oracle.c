@@ -1485,3 +1485,566 @@ void og_tracercoupling(og_bgc_state *st, const og_grid *g, const int32_t *k1,
             }
     free(OLD); free(NEW); free(rNEW); free(lv);
 }
+
+/* ------------------------------------------------------------------------ */
+/* GOLDSTEIN momentum: the velocity solve                                    */
+/* ------------------------------------------------------------------------ */
+static const double MOM_USC = 0.05, MOM_FSC = 2 * 7.2921E-5, MOM_RH0SC = 1.0E3; /* goldstein_lib.f90:49-50 */
+
+#define MI (mo->maxi)
+#define MJ (mo->maxj)
+#define MK1(i, j) (k1[(i) + (MI + 2) * (j)])
+#define MH(l, i, j) (mo->h[(size_t)((l) - 1) + 3 * ((size_t)(i) + (size_t)(MI + 2) * (size_t)(j))])
+#define MRH(l, i, j) (mo->rh[(size_t)((l) - 1) + 3 * ((size_t)(i) + (size_t)(MI + 2) * (size_t)(j))])
+#define MKU(l, i, j) (mo->ku[(size_t)((l) - 1) + 2 * ((size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1))])
+#define MMK(i, j) (mo->mk[(size_t)(i) + (size_t)(MI + 2) * (size_t)(j)])
+#define MGETJ(i, j) (mo->getj[(size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1)])
+#define MDRAG(l, i, j) (mo->drag[(size_t)((l) - 1) + 2 * ((size_t)(i) + (size_t)(MI + 2) * (size_t)(j))])
+#define MRTV(i, j) (mo->rtv[(size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1)])
+#define MRTV3(i, j) (mo->rtv3[(size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1)])
+#define MGAP(k, l) (mo->gap[(size_t)((k) - 1) * (size_t)(2 * MI + 3) + (size_t)((l) - 1)])
+#define MRATM(k, l) (mo->ratm[(size_t)((k) - 1) * (size_t)(MI + 1) + (size_t)((l) - 1)])
+#define MGB(k) (mo->gb[(k) - 1])
+#define MGBOLD(k) (mo->gbold[(k) - 1])
+#define MBP(i, j, k) (mo->bp[(size_t)(i) + (size_t)(MI + 2) * ((size_t)(j) + (size_t)(MJ + 2) * (size_t)(k))])
+#define MSBP(i, j) (mo->sbp[(size_t)(i) + (size_t)(MI + 2) * (size_t)(j)])
+#define UBL(ubv, l, i, j) ((ubv)[(size_t)((l) - 1) + 2 * ((size_t)(i) + (size_t)(MI + 2) * (size_t)(j))])
+#define PSIL(psv, i, j) ((psv)[(size_t)(i) + (size_t)(MI + 1) * (size_t)(j)])
+#define MERISL(a, b) (mo->erisl[(size_t)((a) - 1) + (size_t)mo->isles * (size_t)((b) - 1)])
+#define MTAU(l, i, j) (mo->tau[(size_t)((l) - 1) + 2 * ((size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1))])
+#define MDZTAU(l, i, j) (mo->dztau[(size_t)((l) - 1) + 2 * ((size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1))])
+#define MDZTAV(l, i, j) (mo->dztav[(size_t)((l) - 1) + 2 * ((size_t)((i) - 1) + (size_t)MI * (size_t)((j) - 1))])
+#define MU(uv, l, i, j, k) ((uv)[(size_t)((l) - 1) + 3 * ((size_t)(i) + (size_t)(MI + 1) * ((size_t)(j) + (size_t)(MJ + 1) * (size_t)((k) - 1)))])
+#define MRHO(i, j, k) (rho[(size_t)(i) + (size_t)(MI + 2) * ((size_t)(j) + (size_t)(MJ + 2) * (size_t)(k))])
+#define ISIGN1(x) ((x) >= 0 ? 1 : -1)
+
+static int imax2(int a, int b) { return a > b ? a : b; }
+static int imin2(int a, int b) { return a < b ? a : b; }
+
+/* drgset, goldstein.f90:2845-2885: drag at psi points, raised twice by drgf
+ * near the equator and over shallow water (k1 > kmxdrg), then interpolated
+ * to the velocity points */
+static void mom_drgset(og_mom *mo, const int32_t *k1, double adrag, double drgf, int kmxdrg, int jeb)
+{
+    double *tmp = calloc((size_t)(MI + 1) * (MJ + 1), sizeof(double));
+#define TMPD(i, j) tmp[(size_t)(i) + (size_t)(MI + 1) * (size_t)(j)]
+    for (int j = 0; j <= MJ; j++)
+        for (int i = 0; i <= MI; i++) {
+            const int kloc2 = imax2(imax2(MK1(i, j), MK1(i + 1, j)), imax2(MK1(i, j + 1), MK1(i + 1, j + 1)));
+            int kloc4 = MK1(i, j);
+            for (int j1 = imax2(0, j - 1); j1 <= imin2(MJ + 1, j + 2); j1++)
+                for (int i1 = i - 1; i1 <= i + 2; i1++) {
+                    const int i1p = 1 + (MI + i1 - 1) % MI;
+                    kloc4 = imax2(kloc4, MK1(i1p, j1));
+                }
+            if (kloc2 > kmxdrg || abs(j - MJ / 2) <= jeb)
+                TMPD(i, j) = adrag * drgf * drgf;
+            else if (kloc4 > kmxdrg || abs(j - MJ / 2) == jeb + 1)
+                TMPD(i, j) = adrag * drgf;
+            else
+                TMPD(i, j) = adrag;
+        }
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            MDRAG(1, i, j) = 0.5 * (TMPD(i, j) + TMPD(i, j - 1));
+            MDRAG(2, i, j) = 0.5 * (TMPD(i, j) + TMPD(i - 1, j));
+        }
+    for (int j = 0; j <= MJ + 1; j++) MDRAG(2, MI + 1, j) = MDRAG(2, 1, j);
+#undef TMPD
+    free(tmp);
+}
+
+/* invert, goldstein.f90:3138-3204: the psi equation at psi points (i, j),
+ * i = 1..maxi (periodic), j = 0..maxj, as a band of half-width maxi+1, and
+ * its elimination without pivoting (multipliers kept in ratm) */
+static void mom_invert(og_mom *mo, const og_grid *g, const int32_t *k1)
+{
+    const int n = MI, m = MJ + 1, W = 2 * n + 3;
+    memset(mo->gap, 0, sizeof(double) * (size_t)n * m * W);
+    for (int i = 1; i <= MI; i++)
+        for (int j = 0; j <= MJ; j++) {
+            const int k = i + j * n;
+            if (imax2(imax2(MK1(i, j), MK1(i + 1, j)), imax2(MK1(i, j + 1), MK1(i + 1, j + 1))) <= g->maxk) {
+                const double tv = (g->s[j + 1] * MRH(1, i, j + 1) - g->s[j] * MRH(1, i, j)) / (2.0 * g->dsv[j] * g->dphi);
+                const double tv1 = (g->sv[j] * MRH(2, i + 1, j) - g->sv[j] * MRH(2, i, j)) / (2.0 * g->dphi * g->dsv[j]);
+                MGAP(k, 2) = MDRAG(1, i, j) * g->c[j] * g->c[j] * MRH(1, i, j) / (g->ds[j] * g->dsv[j]) + tv1;
+                int l = n + 1;
+                if (i == 1) l = 2 * n + 1;
+                MGAP(k, l) = MDRAG(2, i, j) * g->rcv[j] * g->rcv[j] * g->rdphi * g->rdphi * MRH(2, i, j) - tv;
+                MGAP(k, n + 2) = -(MDRAG(2, i, j) * MRH(2, i, j) + MDRAG(2, i + 1, j) * MRH(2, i + 1, j)) /
+                                     (g->cv[j] * g->cv[j] * g->dphi * g->dphi) -
+                                 (MDRAG(1, i, j) * g->c[j] * g->c[j] * MRH(1, i, j) / g->ds[j] +
+                                  MDRAG(1, i, j + 1) * g->c[j + 1] * g->c[j + 1] * MRH(1, i, j + 1) / g->ds[j + 1]) /
+                                     g->dsv[j];
+                l = n + 3;
+                if (i == MI) l = 3;
+                MGAP(k, l) = MDRAG(2, i + 1, j) * MRH(2, i + 1, j) / (g->cv[j] * g->cv[j] * g->dphi * g->dphi) + tv;
+                MGAP(k, 2 * n + 2) = MDRAG(1, i, j + 1) * g->c[j + 1] * g->c[j + 1] * MRH(1, i, j + 1) /
+                                         (g->ds[j + 1] * g->dsv[j]) -
+                                     tv1;
+            } else {
+                MGAP(k, n + 2) = 1;
+            }
+        }
+    memset(mo->ratm, 0, sizeof(double) * (size_t)n * m * (n + 1));
+    for (int i = 1; i <= n * m - 1; i++) {
+        const int im = imin2(i + n + 1, n * m);
+        for (int j = i + 1; j <= im; j++) {
+            const double rat = MGAP(j, n + 2 - j + i) / MGAP(i, n + 2);
+            MRATM(j, j - i) = rat;
+            if (rat != 0)
+                for (int k = n + 2 - j + i; k <= 2 * n + 3 - j + i; k++)
+                    MGAP(j, k) = MGAP(j, k) - rat * MGAP(i, k + j - i);
+        }
+    }
+}
+
+/* ubarsolv, goldstein.f90:3500-3565: forward elimination and back
+ * substitution of gb, psi from it, the barotropic velocities and their
+ * boundary values */
+static void mom_ubarsolv(og_mom *mo, const og_grid *g, double *ubv, double *psv)
+{
+    const int n = MI, m = MJ + 1;
+    for (int i = 1; i <= n * m - 1; i++) {
+        const int im = imin2(i + n + 1, n * m);
+        for (int j = i + 1; j <= im; j++) MGB(j) = MGB(j) - MRATM(j, j - i) * MGB(i);
+    }
+    MGB(n * m) = MGB(n * m) / MGAP(n * m, n + 2);
+    for (int i = n * m - 1; i >= 1; i--) {
+        const int km = imin2(n + 1, n * m - i);
+        for (int k = 1; k <= km; k++) MGB(i) = MGB(i) - MGAP(i, n + 2 + k) * MGB(i + k);
+        MGB(i) = MGB(i) / MGAP(i, n + 2);
+    }
+    for (int j = 0; j <= MJ; j++) {
+        for (int i = 1; i <= MI; i++) PSIL(psv, i, j) = MGB(i + j * n);
+        PSIL(psv, 0, j) = PSIL(psv, MI, j);
+    }
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++)
+            UBL(ubv, 1, i, j) = -MRH(1, i, j) * g->c[j] * (PSIL(psv, i, j) - PSIL(psv, i, j - 1)) * g->rds[j];
+    for (int j = 1; j <= MJ - 1; j++)
+        for (int i = 1; i <= MI; i++)
+            UBL(ubv, 2, i, j) = MRH(2, i, j) * (PSIL(psv, i, j) - PSIL(psv, i - 1, j)) * g->rcv[j] * g->rdphi;
+    for (int i = 1; i <= MI; i++) {
+        UBL(ubv, 2, i, MJ) = 0.0;
+        UBL(ubv, 2, i, 0) = 0.0;
+    }
+    for (int j = 1; j <= MJ; j++) {
+        UBL(ubv, 2, MI + 1, j) = UBL(ubv, 2, 1, j);
+        UBL(ubv, 1, 0, j) = UBL(ubv, 1, MI, j);
+        UBL(ubv, 1, MI + 1, j) = UBL(ubv, 1, 1, j);
+        UBL(ubv, 2, 0, j) = UBL(ubv, 2, MI, j);
+    }
+    UBL(ubv, 2, MI + 1, 0) = UBL(ubv, 2, 1, 0);
+    UBL(ubv, 2, 0, 0) = UBL(ubv, 2, MI, 0);
+}
+
+/* island, goldstein_lib.f90:186-241: the path integral round island isl of
+ * the barotropic momentum balance (indj = 1 adds wind and the bottom-pressure
+ * torque) */
+static double mom_island(const og_mom *mo, const og_grid *g, const double *ubv, int isl, int indj)
+{
+    double erisl1 = 0.0;
+    for (int p = 1; p <= mo->npi[isl - 1]; p++) {
+        const size_t q = (size_t)(p - 1) + (size_t)mo->mpi * (size_t)(isl - 1);
+        const int lpi = mo->lpisl[q], ipi = mo->ipisl[q], jpi = mo->jpisl[q];
+        const int al = abs(lpi);
+        double cor;
+        if (al == 1)
+            cor = -g->s[jpi] * 0.25 *
+                  (UBL(ubv, 2, ipi, jpi) + UBL(ubv, 2, ipi + 1, jpi) + UBL(ubv, 2, ipi, jpi - 1) + UBL(ubv, 2, ipi + 1, jpi - 1));
+        else
+            cor = g->sv[jpi] * 0.25 *
+                  (UBL(ubv, 1, ipi - 1, jpi) + UBL(ubv, 1, ipi, jpi) + UBL(ubv, 1, ipi - 1, jpi + 1) + UBL(ubv, 1, ipi, jpi + 1));
+        const double sgn = (double)ISIGN1(lpi);
+        const double rhv = al == 1 ? MRH(1, ipi, jpi) : MRH(2, ipi, jpi);
+        const double tauv = indj ? (al == 1 ? MTAU(1, ipi, jpi) : MTAU(2, ipi, jpi)) : 0.0;
+        erisl1 = erisl1 + sgn * (MDRAG(al, ipi, jpi) * UBL(ubv, al, ipi, jpi) + cor - (double)indj * tauv * rhv) *
+                              (g->c[jpi] * g->dphi * (2.0 - al) + g->rcv[jpi] * g->dsv[jpi] * (al - 1.0));
+        if (indj == 1) {
+            if (al == 1) {
+                double tv1 = 0.0;
+                for (int k = MKU(1, ipi, jpi); k <= MMK(ipi + 1, jpi); k++) tv1 = tv1 + MBP(ipi + 1, jpi, k) * g->dz[k];
+                for (int k = MKU(1, ipi, jpi); k <= MMK(ipi, jpi); k++) tv1 = tv1 - MBP(ipi, jpi, k) * g->dz[k];
+                erisl1 = erisl1 + (MSBP(ipi + 1, jpi) - MSBP(ipi, jpi) + tv1) * sgn * MRH(1, ipi, jpi);
+            } else {
+                double tv2 = 0.0;
+                for (int k = MKU(2, ipi, jpi); k <= MMK(ipi, jpi + 1); k++) tv2 = tv2 + MBP(ipi, jpi + 1, k) * g->dz[k];
+                for (int k = MKU(2, ipi, jpi); k <= MMK(ipi, jpi); k++) tv2 = tv2 - MBP(ipi, jpi, k) * g->dz[k];
+                erisl1 = erisl1 + (MSBP(ipi, jpi + 1) - MSBP(ipi, jpi) + tv2) * sgn * MRH(2, ipi, jpi);
+            }
+        }
+    }
+    return erisl1;
+}
+
+/* matinv_gold and matmult, goldstein.f90:3452-3492 (column-major amat) */
+static void mom_matinv(int nvar, double *a)
+{
+#define AM(i, j) a[(size_t)((i) - 1) + (size_t)nvar * (size_t)((j) - 1)]
+    for (int i = 1; i <= nvar - 1; i++)
+        for (int j = i + 1; j <= nvar; j++)
+            for (int k = i + 1; k <= nvar; k++) AM(j, k) = AM(i, i) * AM(j, k) - AM(j, i) * AM(i, k);
+}
+static void mom_matmult(int nvar, const double *a, double *rhs)
+{
+    for (int i = 1; i <= nvar - 1; i++)
+        for (int j = i + 1; j <= nvar; j++) rhs[j - 1] = AM(i, i) * rhs[j - 1] - AM(j, i) * rhs[i - 1];
+    rhs[nvar - 1] = rhs[nvar - 1] / AM(nvar, nvar);
+    for (int i = nvar - 1; i >= 1; i--) {
+        for (int j = i + 1; j <= nvar; j++) rhs[i - 1] = rhs[i - 1] - AM(i, j) * rhs[j - 1];
+        rhs[i - 1] = rhs[i - 1] / AM(i, i);
+    }
+#undef AM
+}
+
+void og_mom_free(og_mom *mo)
+{
+    free(mo->h); free(mo->rh); free(mo->ku); free(mo->mk); free(mo->getj); free(mo->drag);
+    free(mo->rtv); free(mo->rtv3); free(mo->gap); free(mo->ratm); free(mo->gb); free(mo->gbold);
+    free(mo->bp); free(mo->sbp); free(mo->ub); free(mo->psi); free(mo->ubisl); free(mo->psisl);
+    free(mo->erisl); free(mo->psibc); free(mo->npi); free(mo->lpisl); free(mo->ipisl); free(mo->jpisl);
+    free(mo->tau); free(mo->dztau); free(mo->dztav); free(mo->u1);
+    memset(mo, 0, sizeof(*mo));
+}
+
+int og_mom_init(og_mom *mo, const og_grid *g, const int32_t *k1, const int32_t *labels, int isles,
+                const int32_t *npi, const int32_t *paths, double adrag, double rel)
+{
+    memset(mo, 0, sizeof(*mo));
+    mo->maxi = g->maxi;
+    mo->maxj = g->maxj;
+    mo->maxk = g->maxk;
+    mo->isles = isles;
+    mo->rel = rel;
+    const int maxk = g->maxk, n = MI, N = MI * (MJ + 1);
+    const size_t P2 = (size_t)(MI + 2) * (MJ + 2);
+    mo->h = calloc(3 * P2, sizeof(double));
+    mo->rh = calloc(3 * P2, sizeof(double));
+    mo->ku = calloc((size_t)2 * MI * MJ, sizeof(int32_t));
+    mo->mk = calloc(P2, sizeof(int32_t));
+    mo->getj = calloc((size_t)MI * MJ, 1);
+    mo->drag = calloc(2 * P2, sizeof(double));
+    mo->rtv = calloc((size_t)MI * MJ, sizeof(double));
+    mo->rtv3 = calloc((size_t)MI * MJ, sizeof(double));
+    mo->gap = calloc((size_t)N * (2 * n + 3), sizeof(double));
+    mo->ratm = calloc((size_t)N * (n + 1), sizeof(double));
+    mo->gb = calloc(N, sizeof(double));
+    mo->gbold = calloc(N, sizeof(double));
+    mo->bp = calloc(P2 * (maxk + 1), sizeof(double));
+    mo->sbp = calloc(P2, sizeof(double));
+    mo->ub = calloc((size_t)2 * (MI + 2) * (MJ + 1), sizeof(double));
+    mo->psi = calloc((size_t)(MI + 1) * (MJ + 1), sizeof(double));
+    mo->tau = calloc((size_t)2 * MI * MJ, sizeof(double));
+    mo->dztau = calloc((size_t)2 * MI * MJ, sizeof(double));
+    mo->dztav = calloc((size_t)2 * MI * MJ, sizeof(double));
+    mo->u1 = calloc((size_t)3 * (MI + 1) * (MJ + 1) * maxk, sizeof(double));
+    /* seabed depth and its reciprocal at T, u and v points (1359-1384) */
+    for (int j = MJ + 1; j >= 0; j--)
+        for (int i = 0; i <= MI + 1; i++)
+            if (MK1(i, j) <= maxk) {
+                for (int k = MK1(i, j); k <= maxk; k++) MH(3, i, j) = MH(3, i, j) + g->dz[k];
+                MRH(3, i, j) = 1.0 / MH(3, i, j);
+            }
+    for (int j = 0; j <= MJ + 1; j++)
+        for (int i = 0; i <= MI; i++) {
+            MH(1, i, j) = fmin(MH(3, i, j), MH(3, i + 1, j));
+            if (imax2(MK1(i, j), MK1(i + 1, j)) <= maxk) MRH(1, i, j) = 1.0 / MH(1, i, j);
+        }
+    for (int j = 0; j <= MJ; j++)
+        for (int i = 0; i <= MI + 1; i++) {
+            MH(2, i, j) = fmin(MH(3, i, j), MH(3, i, j + 1));
+            if (imax2(MK1(i, j), MK1(i, j + 1)) <= maxk) MRH(2, i, j) = 1.0 / MH(2, i, j);
+        }
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            MKU(1, i, j) = imax2(MK1(i, j), MK1(i + 1, j));
+            MKU(2, i, j) = imax2(MK1(i, j), MK1(i, j + 1));
+        }
+    /* drag (1395-1405) and the reciprocal denominators (1409-1414) */
+    mom_drgset(mo, k1, 1.0 / (adrag * 86400 * MOM_FSC), 3.0, maxk / 2, 1);
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            MRTV(i, j) = 1.0 / (g->s[j] * g->s[j] + MDRAG(1, i, j) * MDRAG(1, i, j));
+            MRTV3(i, j) = 1.0 / (g->sv[j] * g->sv[j] + MDRAG(2, i, j) * MDRAG(2, i, j));
+        }
+    /* mk and getj (1460-1493) */
+#define WETF(i, j) ((1 + ISIGN1(maxk - MK1(i, j))) / 2)
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            int v = imax2(imax2(MK1(i, j) * WETF(i, j), MK1(i + 1, j) * WETF(i + 1, j)),
+                          imax2(imax2(MK1(i - 1, j) * WETF(i - 1, j), MK1(i, j + 1) * WETF(i, j + 1)),
+                                MK1(i, j - 1) * WETF(i, j - 1)));
+            MMK(i, j) = v * WETF(i, j);
+        }
+#undef WETF
+    for (int j = 1; j <= MJ; j++) MMK(MI + 1, j) = MMK(1, j);
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++)
+            MGETJ(i, j) = (imax2(imax2(MK1(i, j), MK1(i + 1, j)), imax2(MK1(i, j + 1), MK1(i + 1, j + 1))) <= maxk) &&
+                          (MK1(i, j) != MK1(i, j + 1) || MK1(i, j) != MK1(i + 1, j) || MK1(i, j) != MK1(i + 1, j + 1));
+    /* islands (1505-1575): labels as gbold, the paths checked as the reference checks them */
+    for (int j = 0; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) MGBOLD(i + j * MI) = (double)labels[(i - 1) + (size_t)MI * j];
+    int mpi = 1;
+    for (int q = 0; q < isles; q++) mpi = imax2(mpi, npi[q]);
+    mo->mpi = mpi;
+    const int nis = isles > 0 ? isles : 1;
+    mo->npi = calloc(nis, sizeof(int32_t));
+    mo->lpisl = calloc((size_t)mpi * nis, sizeof(int32_t));
+    mo->ipisl = calloc((size_t)mpi * nis, sizeof(int32_t));
+    mo->jpisl = calloc((size_t)mpi * nis, sizeof(int32_t));
+    mo->psisl = calloc((size_t)(MI + 1) * (MJ + 1) * nis, sizeof(double));
+    mo->ubisl = calloc((size_t)2 * (MI + 2) * (MJ + 1) * nis, sizeof(double));
+    mo->erisl = calloc((size_t)nis * (nis + 1), sizeof(double));
+    mo->psibc = calloc(nis, sizeof(double));
+    size_t o = 0;
+    for (int q = 0; q < isles; q++) {
+        mo->npi[q] = npi[q];
+        for (int p = 0; p < npi[q]; p++, o++) {
+            const int lpi = paths[3 * o], ipi = paths[3 * o + 1], jpi = paths[3 * o + 2];
+            /* the reference's checks (1555-1570), and the index ranges its
+             * arrays give island(): ku(2, maxi, maxj), ub(2, 0:maxi+1, 0:maxj) */
+            if ((abs(lpi) != 1 && abs(lpi) != 2) || ipi > MI || ipi < 1 || jpi > MJ || jpi < 1 || MK1(ipi, jpi) > maxk ||
+                (abs(lpi) == 2 && jpi + 1 > MJ))
+                return -1;
+            mo->lpisl[p + (size_t)mpi * q] = lpi;
+            mo->ipisl[p + (size_t)mpi * q] = ipi;
+            mo->jpisl[p + (size_t)mpi * q] = jpi;
+        }
+    }
+    /* the factorisation and the island basis (1788-1814) */
+    mom_invert(mo, g, k1);
+    for (int isol = 1; isol <= isles; isol++) {
+        for (int j = 0; j <= MJ; j++)
+            for (int i = 1; i <= MI; i++) {
+                const int k = i + j * MI;
+                MGB(k) = ((int)MGBOLD(k) == isol + 1) ? 1.0 : 0.0;
+            }
+        double *ubv = mo->ubisl + (size_t)2 * (MI + 2) * (MJ + 1) * (isol - 1);
+        double *psv = mo->psisl + (size_t)(MI + 1) * (MJ + 1) * (isol - 1);
+        mom_ubarsolv(mo, g, ubv, psv);
+        for (int isl = 1; isl <= isles; isl++) MERISL(isl, isol) = mom_island(mo, g, ubv, isl, 0);
+    }
+    if (isles > 0) mom_matinv(isles, mo->erisl);
+    return 0;
+}
+
+void og_mom_set_wind(og_mom *mo, const og_grid *g, const double *stress, double scf)
+{
+    const double dzz = g->dz[g->maxk] * g->dza[g->maxk - 1] / 2;
+    const double d = MOM_RH0SC * DSC * MOM_USC * MOM_FSC;
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            const size_t c = (size_t)4 * ((i - 1) + (size_t)MI * (j - 1));
+            MDZTAU(1, i, j) = scf * stress[c + 0] / d / dzz;
+            MDZTAU(2, i, j) = scf * stress[c + 1] / d / dzz;
+            MDZTAV(1, i, j) = scf * stress[c + 2] / d / dzz;
+            MDZTAV(2, i, j) = scf * stress[c + 3] / d / dzz;
+            MTAU(1, i, j) = MDZTAU(1, i, j) * dzz;
+            MTAU(2, i, j) = MDZTAV(2, i, j) * dzz;
+        }
+}
+
+void og_rho_from_ts(const og_grid *g, int maxl, const int32_t *k1, const double *go_ts, double *rho)
+{
+    og_mom tmp;
+    og_mom *mo = &tmp;
+    mo->maxi = g->maxi;
+    mo->maxj = g->maxj;
+    for (int k = 1; k <= g->maxk; k++)
+        for (int j = 1; j <= MJ; j++) {
+            for (int i = 1; i <= MI; i++)
+                if (k >= MK1(i, j)) {
+                    const size_t c = (size_t)maxl * ((i - 1) + (size_t)MI * ((j - 1) + (size_t)MJ * (k - 1)));
+                    MRHO(i, j, k) = og_eos(g->ec, go_ts[c], go_ts[c + 1]);
+                }
+            MRHO(0, j, k) = MRHO(MI, j, k);
+            MRHO(MI + 1, j, k) = MRHO(1, j, k);
+        }
+}
+
+void og_mom_step(og_mom *mo, const og_grid *g, const int32_t *k1, const double *rho, double *u)
+{
+    const int maxk = g->maxk, n = MI;
+    /* wind, 3685-3714 */
+    for (int i = 1; i <= MI; i++)
+        for (int j = 0; j <= MJ; j++) {
+            const int ip1 = i % MI + 1;
+            const int k = i + j * n;
+            if (imax2(imax2(MK1(i, j), MK1(i + 1, j)), imax2(MK1(i, j + 1), MK1(i + 1, j + 1))) <= maxk) {
+                MGB(k) = (MTAU(2, ip1, j) * MRH(2, i + 1, j) - MTAU(2, i, j) * MRH(2, i, j)) * g->rdphi * g->rcv[j] -
+                         (MTAU(1, i, j + 1) * g->c[j + 1] * MRH(1, i, j + 1) - MTAU(1, i, j) * g->c[j] * MRH(1, i, j)) *
+                             g->rdsv[j];
+            } else {
+                MGB(k) = 0;
+            }
+            MGBOLD(k) = MGB(k);
+        }
+    /* jbar, 3217-3315 */
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++)
+            if (MK1(i, j) <= maxk)
+                for (int k = MK1(i, j) + 1; k <= maxk; k++)
+                    MBP(i, j, k) = MBP(i, j, k - 1) - (MRHO(i, j, k) + MRHO(i, j, k - 1)) * g->dza[k - 1] * 0.5;
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++)
+            if (MMK(i, j) > 0) {
+                MSBP(i, j) = 0;
+                for (int k = MMK(i, j) + 1; k <= maxk; k++) MSBP(i, j) = MSBP(i, j) + MBP(i, j, k) * g->dz[k];
+            }
+    for (int j = 1; j <= MJ; j++) {
+        if (MK1(1, j) < maxk)
+            for (int k = MK1(1, j); k <= maxk; k++) MBP(MI + 1, j, k) = MBP(1, j, k);
+        if (MK1(1, j) <= maxk) MSBP(MI + 1, j) = MSBP(1, j);
+    }
+    for (int j = 1; j <= MJ - 1; j++)
+        for (int i = 1; i <= MI; i++) {
+            const int ip1 = i % MI + 1;
+            const int l = i + j * n;
+            if (MGETJ(i, j)) {
+                double tv1 = 0, tv2 = 0, tv3 = 0, tv4 = 0;
+                for (int k = MKU(2, ip1, j); k <= MMK(ip1, j + 1); k++) tv1 = tv1 + MBP(ip1, j + 1, k) * g->dz[k];
+                for (int k = MKU(2, ip1, j); k <= MMK(ip1, j); k++) tv2 = tv2 + MBP(ip1, j, k) * g->dz[k];
+                for (int k = MKU(2, i, j); k <= MMK(i, j + 1); k++) tv3 = tv3 + MBP(i, j + 1, k) * g->dz[k];
+                for (int k = MKU(2, i, j); k <= MMK(i, j); k++) tv4 = tv4 + MBP(i, j, k) * g->dz[k];
+                MGB(l) = MGBOLD(l) + ((tv3 + MSBP(i, j + 1) - tv4 - MSBP(i, j)) * MRH(2, i, j) -
+                                      (tv1 + MSBP(ip1, j + 1) - tv2 - MSBP(ip1, j)) * MRH(2, ip1, j)) *
+                                         g->rdphi * g->rdsv[j];
+                tv1 = 0;
+                for (int k = MKU(1, i, j + 1); k <= MMK(ip1, j + 1); k++) tv1 = tv1 + MBP(ip1, j + 1, k) * g->dz[k];
+                tv2 = 0;
+                for (int k = MKU(1, i, j); k <= MMK(ip1, j); k++) tv2 = tv2 + MBP(ip1, j, k) * g->dz[k];
+                tv3 = 0;
+                for (int k = MKU(1, i, j + 1); k <= MMK(i, j + 1); k++) tv3 = tv3 + MBP(i, j + 1, k) * g->dz[k];
+                tv4 = 0;
+                for (int k = MKU(1, i, j); k <= MMK(i, j); k++) tv4 = tv4 + MBP(i, j, k) * g->dz[k];
+                MGB(l) = MGB(l) + ((tv1 + MSBP(ip1, j + 1) - tv3 - MSBP(i, j + 1)) * MRH(1, i, j + 1) -
+                                   (tv2 + MSBP(ip1, j) - tv4 - MSBP(i, j)) * MRH(1, i, j)) *
+                                      g->rdphi * g->rdsv[j];
+            } else {
+                MGB(l) = MGBOLD(l);
+            }
+        }
+    /* ubarsolv, the island correction (198-229) */
+    mom_ubarsolv(mo, g, mo->ub, mo->psi);
+    const int isles = mo->isles;
+    if (isles > 0) {
+        for (int isl = 1; isl <= isles; isl++) MERISL(isl, isles + 1) = mom_island(mo, g, mo->ub, isl, 1);
+        if (isles > 1) {
+            mom_matmult(isles, mo->erisl, &MERISL(1, isles + 1));
+            for (int isl = 1; isl <= isles; isl++) mo->psibc[isl - 1] = -MERISL(isl, isles + 1);
+        } else {
+            mo->psibc[0] = -MERISL(1, 2) / MERISL(1, 1);
+        }
+        for (int j = 1; j <= MJ; j++)
+            for (int i = 0; i <= MI + 1; i++)
+                for (int l = 1; l <= 2; l++) {
+                    double sum = 0.0;
+                    for (int isl = 1; isl <= isles; isl++)
+                        sum = sum + UBL(mo->ubisl + (size_t)2 * (MI + 2) * (MJ + 1) * (isl - 1), l, i, j) * mo->psibc[isl - 1];
+                    UBL(mo->ub, l, i, j) = UBL(mo->ub, l, i, j) + sum;
+                }
+        for (int j = 0; j <= MJ; j++)
+            for (int i = 0; i <= MI; i++) {
+                double sum = 0.0;
+                for (int isl = 1; isl <= isles; isl++)
+                    sum = sum + PSIL(mo->psisl + (size_t)(MI + 1) * (MJ + 1) * (isl - 1), i, j) * mo->psibc[isl - 1];
+                PSIL(mo->psi, i, j) = PSIL(mo->psi, i, j) + sum;
+            }
+    }
+    /* velc, 3568-3679 */
+    double dzu[2][OG_MAXK + 2];
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            double sum[2] = {0, 0};
+            for (int k = MK1(i, j); k <= maxk; k++) {
+                double tv1, tv2, tv4, tv5;
+                if (MK1(i + 1, j) > k) {
+                    tv1 = 0;
+                    tv2 = 0;
+                } else {
+                    tv2 = -(MRHO(i + 1, j, k) - MRHO(i, j, k)) * g->rdphi * g->rc[j];
+                    if (imax2(imax2(MK1(i, j - 1), MK1(i, j + 1)), imax2(MK1(i + 1, j - 1), MK1(i + 1, j + 1))) <= k)
+                        tv1 = -g->c[j] * (MRHO(i + 1, j + 1, k) - MRHO(i + 1, j - 1, k) + MRHO(i, j + 1, k) - MRHO(i, j - 1, k)) *
+                              g->rds2[j] * 0.25;
+                    else if (imax2(MK1(i, j - 1), MK1(i + 1, j - 1)) <= k)
+                        tv1 = -g->c[j] * (MRHO(i + 1, j, k) - MRHO(i + 1, j - 1, k) + MRHO(i, j, k) - MRHO(i, j - 1, k)) *
+                              g->rdsv[j - 1] * 0.5;
+                    else if (imax2(MK1(i, j + 1), MK1(i + 1, j + 1)) <= k)
+                        tv1 = -g->c[j] * (MRHO(i + 1, j + 1, k) - MRHO(i + 1, j, k) + MRHO(i, j + 1, k) - MRHO(i, j, k)) *
+                              g->rdsv[j] * 0.5;
+                    else
+                        tv1 = 0;
+                }
+                if (MK1(i, j + 1) > k) {
+                    tv4 = 0;
+                    tv5 = 0;
+                } else {
+                    tv4 = -g->cv[j] * (MRHO(i, j + 1, k) - MRHO(i, j, k)) * g->rdsv[j];
+                    if (imax2(imax2(MK1(i - 1, j), MK1(i - 1, j + 1)), imax2(MK1(i + 1, j), MK1(i + 1, j + 1))) <= k)
+                        tv5 = -(MRHO(i + 1, j + 1, k) - MRHO(i - 1, j + 1, k) + MRHO(i + 1, j, k) - MRHO(i - 1, j, k)) *
+                              g->rdphi * 0.25 * g->rcv[j];
+                    else if (imax2(MK1(i - 1, j), MK1(i - 1, j + 1)) <= k)
+                        tv5 = -(MRHO(i, j + 1, k) - MRHO(i - 1, j + 1, k) + MRHO(i, j, k) - MRHO(i - 1, j, k)) * g->rdphi *
+                              0.5 * g->rcv[j];
+                    else if (imax2(MK1(i + 1, j), MK1(i + 1, j + 1)) <= k)
+                        tv5 = -(MRHO(i + 1, j + 1, k) - MRHO(i, j + 1, k) + MRHO(i + 1, j, k) - MRHO(i, j, k)) * g->rdphi *
+                              0.5 * g->rcv[j];
+                    else
+                        tv5 = 0;
+                }
+                if (k == maxk) {
+                    if (MK1(i + 1, j) <= k) {
+                        tv1 = tv1 - MDZTAU(2, i, j);
+                        tv2 = tv2 - MDZTAU(1, i, j);
+                    }
+                    if (MK1(i, j + 1) <= k) {
+                        tv4 = tv4 - MDZTAV(2, i, j);
+                        tv5 = tv5 - MDZTAV(1, i, j);
+                    }
+                }
+                dzu[0][k] = -(g->s[j] * tv1 + MDRAG(1, i, j) * tv2) * MRTV(i, j);
+                dzu[1][k] = -(MDRAG(2, i, j) * tv4 - g->sv[j] * tv5) * MRTV3(i, j);
+                for (int l = 1; l <= 2; l++) {
+                    if (k == MK1(i, j)) {
+                        MU(u, l, i, j, k) = 0;
+                    } else {
+                        MU(u, l, i, j, k) = MU(u, l, i, j, k - 1) + g->dza[k - 1] * (dzu[l - 1][k] + dzu[l - 1][k - 1]) * 0.5;
+                        sum[l - 1] = sum[l - 1] + g->dz[k] * MU(u, l, i, j, k);
+                    }
+                }
+            }
+            for (int k = MK1(i, j); k <= maxk; k++) {
+                if (MK1(i + 1, j) <= k) {
+                    MU(u, 1, i, j, k) = MU(u, 1, i, j, k) - sum[0] * MRH(1, i, j) + UBL(mo->ub, 1, i, j);
+                    MU(u, 1, i, j, k) = mo->rel * MU(mo->u1, 1, i, j, k) + (1.0 - mo->rel) * MU(u, 1, i, j, k);
+                    MU(mo->u1, 1, i, j, k) = MU(u, 1, i, j, k);
+                }
+                if (MK1(i, j + 1) <= k) {
+                    MU(u, 2, i, j, k) = MU(u, 2, i, j, k) - sum[1] * MRH(2, i, j) + UBL(mo->ub, 2, i, j);
+                    MU(u, 2, i, j, k) = mo->rel * MU(mo->u1, 2, i, j, k) + (1.0 - mo->rel) * MU(u, 2, i, j, k);
+                    MU(mo->u1, 2, i, j, k) = MU(u, 2, i, j, k);
+                }
+            }
+        }
+    for (int j = 1; j <= MJ; j++)
+        for (int k = MK1(1, j); k <= maxk; k++) MU(u, 1, 0, j, k) = MU(u, 1, MI, j, k);
+    for (int j = 1; j <= MJ; j++)
+        for (int i = 1; i <= MI; i++) {
+            double tv = 0;
+            for (int k = MK1(i, j); k <= maxk - 1; k++) {
+                const double tv1 = (MU(u, 1, i, j, k) - MU(u, 1, i - 1, j, k)) * g->rdphi * g->rc[j];
+                const double tv2 = (MU(u, 2, i, j, k) * g->cv[j] - MU(u, 2, i, j - 1, k) * g->cv[j - 1]) * g->rds[j];
+                MU(u, 3, i, j, k) = tv - g->dz[k] * (tv1 + tv2);
+                tv = MU(u, 3, i, j, k);
+            }
+        }
+}
+
+double og_mom_island(const og_mom *mo, const og_grid *g, int isl) { return mom_island(mo, g, mo->ub, isl, 1); }
+
+double og_mom_island_of(const og_mom *mo, const og_grid *g, const double *ubv, int isl, int indj)
+{
+    return mom_island(mo, g, ubv, isl, indj);
+}

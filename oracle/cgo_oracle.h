@@ -173,6 +173,63 @@ void og_bgc_set_windspeed(og_bgc_state *st, const double *wspeed);
  * well-mixed atmosphere over the ocean grid at 273.15 K */
 void og_atchem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p);
 
+/* ---------------- GOLDSTEIN momentum: the velocity solve ----------------
+ * Barotropic streamfunction with islands and baroclinic velocities
+ * (goldstein.f90: invert 3138-3204, jbar 3217-3315, ubarsolv 3500-3565,
+ * velc 3568-3679, wind 3685-3714, matinv_gold/matmult 3452-3492, drgset
+ * 2845-2885, set-up 1359-1530, 1788-1814, the step 198-233;
+ * goldstein_lib.f90: island 186-241).  Island data as the reference reads it:
+ * labels(maxi, 0:maxj) of the .psiles file (landmass numbers at psi points,
+ * 1 = the first landmass) and, per island, the path points (lpi, ipi, jpi)
+ * of the .paths file. */
+typedef struct og_mom {
+    int maxi, maxj, maxk, isles, mpi;
+    double rel;
+    double *h, *rh;          /* (3, 0:maxi+1, 0:maxj+1) */
+    int32_t *ku;             /* (2, maxi, maxj) */
+    int32_t *mk;             /* (0:maxi+1, 0:maxj+1) */
+    uint8_t *getj;           /* (maxi, maxj) */
+    double *drag;            /* (2, 0:maxi+1, 0:maxj+1) */
+    double *rtv, *rtv3;      /* (maxi, maxj) */
+    double *gap, *ratm;      /* (N, 2n+3) and (N, n+1), N = maxi (maxj+1), n = maxi */
+    double *gb, *gbold;      /* (N) */
+    double *bp;              /* (0:maxi+1, 0:maxj+1, 0:maxk) */
+    double *sbp;             /* (0:maxi+1, 0:maxj+1) */
+    double *ub;              /* (2, 0:maxi+1, 0:maxj) */
+    double *psi;             /* (0:maxi, 0:maxj) */
+    double *ubisl, *psisl;   /* (2, 0:maxi+1, 0:maxj, isles), (0:maxi, 0:maxj, isles) */
+    double *erisl, *psibc;   /* (isles, isles+1) column-major, (isles) */
+    int32_t *npi, *lpisl, *ipisl, *jpisl; /* (isles), (mpi, isles) */
+    double *tau, *dztau, *dztav;          /* (2, maxi, maxj) */
+    double *u1;              /* (3, 0:maxi, 0:maxj, maxk), the relaxation memory */
+} og_mom;
+
+/* Set-up of initialise_goldstein for the momentum: depths, drag (adrag in
+ * days, goldstein-defaults.nml:21), the banded factorisation of the psi
+ * equation and the island basis.  npi[isles] path lengths, then the paths
+ * concatenated as (lpi, ipi, jpi) triples.  Returns 0, or -1 for bad island
+ * data (a dry or out-of-range path point, as the reference's checks). */
+int og_mom_init(og_mom *mo, const og_grid *g, const int32_t *k1, const int32_t *labels, int isles,
+                const int32_t *npi, const int32_t *paths, double adrag, double rel);
+void og_mom_free(og_mom *mo);
+/* Wind stress (N m-2): stress(4, maxi, maxj) = x at u points, y at u points,
+ * x at v points, y at v points (stressxu/yu/xv/yv_ocn); scf the namelist's
+ * wind scale (goldstein.f90:102-107) */
+void og_mom_set_wind(og_mom *mo, const og_grid *g, const double *stress, double scf);
+/* One momentum step of step_goldstein (goldstein.f90:198-233): wind, jbar,
+ * ubarsolv, the island correction, velc.  rho (0:maxi+1, 0:maxj+1, 0:maxk)
+ * as tstepo leaves it; u (3, 0:maxi, 0:maxj, maxk) in/out. */
+void og_mom_step(og_mom *mo, const og_grid *g, const int32_t *k1, const double *rho, double *u);
+/* island(ub, ., isl, 1) with the current barotropic velocity: the
+ * circulation round island isl that the correction drives to zero */
+double og_mom_island(const og_mom *mo, const og_grid *g, int isl);
+/* the same path integral of a given ub (2, 0:maxi+1, 0:maxj); indj = 0 leaves
+ * out the wind and the bottom-pressure torque */
+double og_mom_island_of(const og_mom *mo, const og_grid *g, const double *ubv, int isl, int indj);
+/* rho of every wet cell and the periodic halo from go_ts (eos at
+ * initialisation, goldstein.f90:1430-1444, 1760-1765) */
+void og_rho_from_ts(const og_grid *g, int maxl, const int32_t *k1, const double *go_ts, double *rho);
+
 /* step_biogem restricted to the selected species (biogem.f90:596-611,
  * 873-969, 985-1105, 1693, 1811-1844).  Writes vdocn (maxl,maxi,maxj,maxk). */
 void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,

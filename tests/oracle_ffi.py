@@ -16,6 +16,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_carbchem.so")
 REF_AIRSEA_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_airsea.so")
+REF_GOLD_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_gold.so")
 
 OG_MAXJ, OG_MAXK = 256, 64
 NSED, NCARB, NCC = 5, 10, 17
@@ -54,6 +55,16 @@ class OgBgcState(C.Structure):
         [(n, C.POINTER(_d)) for n in ("wspeed", "atm", "fatm", "sumatm")]
 
 
+class OgMom(C.Structure):
+    _fields_ = [("maxi", _i), ("maxj", _i), ("maxk", _i), ("isles", _i), ("mpi", _i), ("rel", _d)] + \
+        [(n, C.POINTER(_d)) for n in ("h", "rh")] + [(n, C.POINTER(C.c_int32)) for n in ("ku", "mk")] + \
+        [("getj", C.POINTER(C.c_uint8))] + \
+        [(n, C.POINTER(_d)) for n in ("drag", "rtv", "rtv3", "gap", "ratm", "gb", "gbold", "bp", "sbp", "ub", "psi",
+                                      "ubisl", "psisl", "erisl", "psibc")] + \
+        [(n, C.POINTER(C.c_int32)) for n in ("npi", "lpisl", "ipisl", "jpisl")] + \
+        [(n, C.POINTER(_d)) for n in ("tau", "dztau", "dztav", "u1")]
+
+
 def build(force: bool = False) -> None:
     if force or not os.path.exists(LIB):
         subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True, capture_output=True)
@@ -88,6 +99,16 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.og_airsea_cell.argtypes = [_d] * 14 + [_dp]
     L.og_bgc_set_windspeed.argtypes = [C.POINTER(OgBgcState), _dp]
     L.og_atchem_step.argtypes = [C.POINTER(OgBgcState), C.POINTER(OgGrid), C.POINTER(OgBgcParams)]
+    L.og_mom_init.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _i32p, _i32p, _i, _i32p, _i32p, _d, _d]
+    L.og_mom_init.restype = _i
+    L.og_mom_free.argtypes = [C.POINTER(OgMom)]
+    L.og_mom_set_wind.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _dp, _d]
+    L.og_mom_step.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _i32p, _dp, _dp]
+    L.og_mom_island.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _i]
+    L.og_mom_island.restype = _d
+    L.og_mom_island_of.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _dp, _i, _i]
+    L.og_mom_island_of.restype = _d
+    L.og_rho_from_ts.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp]
     return L
 
 
@@ -260,3 +281,131 @@ def ref_airsea():
         R.ref_airsea_cell.argtypes = [_d] * 14 + [_dp]
         _ref_as = R
     return _ref_as
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def read_world(name: str = "worbe2", maxi: int = 36, maxj: int = 36):
+    """The reference's island data of a world as goldstein.f90 reads it
+    (1113-1120, 1500-1575): k1(0:maxi+1, 0:maxj+1) as (maxj+2, maxi+2) with
+    the periodic columns enforced, the .psiles landmass labels as
+    (maxj+1, maxi) (row j = 0 first), isles, and the .paths path lengths and
+    (lpi, ipi, jpi) triples."""
+    v = np.array(open(os.path.join(GOLDEN, name + ".k1")).read().split(), dtype=np.int32)
+    k1 = v[:(maxj + 2) * (maxi + 2)].reshape(maxj + 2, maxi + 2)[::-1].copy()
+    k1[:, 0] = k1[:, maxi]
+    k1[:, maxi + 1] = k1[:, 1]
+    lab = np.array(open(os.path.join(GOLDEN, name + ".psiles")).read().split(), dtype=float)
+    labels = lab[:(maxj + 1) * maxi].reshape(maxj + 1, maxi)[::-1].astype(np.int32).copy()
+    isles = int(labels.max()) - 1
+    tok = open(os.path.join(GOLDEN, name + ".paths")).read().split()
+    npi = np.array(tok[:isles], dtype=np.int32)
+    paths = np.array(tok[isles:isles + 3 * int(npi.sum())], dtype=np.int32)
+    return k1, labels, isles, npi, paths
+
+
+class MomOracle:
+    """The GOLDSTEIN velocity solve of one member (og_mom_*)."""
+
+    def __init__(self, g: OgGrid, k1, labels, isles, npi, paths, adrag=2.5, rel=0.9):
+        self.g = g
+        self.k1 = np.ascontiguousarray(k1, dtype=np.int32)
+        self.mo = OgMom()
+        rc = lib().og_mom_init(C.byref(self.mo), C.byref(g), self.k1, np.ascontiguousarray(labels, dtype=np.int32),
+                               isles, np.ascontiguousarray(npi, dtype=np.int32),
+                               np.ascontiguousarray(paths, dtype=np.int32), adrag, rel)
+        if rc != 0:
+            raise ValueError("og_mom_init: bad island data")
+
+    def __del__(self):
+        try:
+            lib().og_mom_free(C.byref(self.mo))
+        except Exception:
+            pass
+
+    def set_wind(self, stress: np.ndarray, scf: float = 2.0):
+        """stress (maxj, maxi, 4): x and y at u points, x and y at v points (N m-2)."""
+        lib().og_mom_set_wind(C.byref(self.mo), C.byref(self.g), np.ascontiguousarray(stress, dtype=np.float64), scf)
+
+    def step(self, rho: np.ndarray, u: np.ndarray):
+        lib().og_mom_step(C.byref(self.mo), C.byref(self.g), self.k1, rho, u)
+
+    def island(self, isl: int) -> float:
+        return lib().og_mom_island(C.byref(self.mo), C.byref(self.g), isl)
+
+    def island_of(self, ub: np.ndarray, isl: int, indj: int) -> float:
+        """The island path integral of a given ub ((maxj+1, maxi+2, 2), as ub() gives it)."""
+        return lib().og_mom_island_of(C.byref(self.mo), C.byref(self.g), np.ascontiguousarray(ub, dtype=np.float64),
+                                      isl, indj)
+
+    def _arr(self, name, shape, dtype=np.float64):
+        return np.ctypeslib.as_array(getattr(self.mo, name), shape=shape).copy()
+
+    def ub(self) -> np.ndarray:
+        """ub(2, 0:maxi+1, 0:maxj) as (maxj+1, maxi+2, 2)."""
+        g = self.g
+        return self._arr("ub", ((g.maxj + 1), (g.maxi + 2), 2))
+
+    def psi(self) -> np.ndarray:
+        g = self.g
+        return self._arr("psi", ((g.maxj + 1), (g.maxi + 1)))
+
+    def field(self, name: str) -> np.ndarray:
+        """(l, 0:maxi+1, 0:maxj+1) fields h, rh (l = 3) and drag (l = 2) as (maxj+2, maxi+2, l)."""
+        g = self.g
+        nl = 3 if name in ("h", "rh") else 2
+        return self._arr(name, ((g.maxj + 2), (g.maxi + 2), nl))
+
+
+_ref_gold = None
+
+
+def ref_gold():
+    """The reference's GOLDSTEIN island path integral compiled in place (oracle/_ref), or None."""
+    global _ref_gold
+    if _ref_gold is None and os.path.exists(REF_GOLD_LIB):
+        R = C.CDLL(REF_GOLD_LIB)
+        R.ref_gold_island.argtypes = [_i] * 5 + [C.c_void_p] * 4 + [C.c_void_p] * 6 + [_d] + \
+            [C.c_void_p] * 8 + [_i, _i, _dp]
+        _ref_gold = R
+    return _ref_gold
+
+
+def ref_island(m: "MomOracle", isl: int, indj: int = 1, ub: np.ndarray = None) -> float:
+    """goldstein_lib.f90's island() on the oracle member's arrays, each cut to
+    the bounds goldstein.f90 allocates it with; ub as MomOracle.ub() gives it
+    (default: the member's own)."""
+    R = ref_gold()
+    g, mo = m.g, m.mo
+    MI, MJ, MK = g.maxi, g.maxj, g.maxk
+    F = lambda a, dt=np.float64: np.asfortranarray(a, dtype=dt)
+    vec = lambda name, lo, hi: F(np.array(getattr(g, name)[lo:hi + 1]))
+    nis = max(mo.isles, 1)
+    npi = F(np.ctypeslib.as_array(mo.npi, shape=(nis,)), np.int32)
+    # (mpi, isles) column-major = the C arrays' layout p + mpi * q
+    paths = [F(np.ctypeslib.as_array(getattr(mo, n), shape=(nis, mo.mpi)).T, np.int32)
+             for n in ("lpisl", "ipisl", "jpisl")]
+    drag = m.field("drag").transpose(2, 1, 0)[:, 1:MI + 2, 1:MJ + 1]
+    rh = m.field("rh").transpose(2, 1, 0)
+    tau = np.ctypeslib.as_array(mo.tau, shape=(MJ, MI, 2)).transpose(2, 1, 0)
+    bp = np.ctypeslib.as_array(mo.bp, shape=(MK + 1, MJ + 2, MI + 2)).transpose(2, 1, 0)[1:MI + 2, 1:MJ + 1, 1:MK + 1]
+    sbp = np.ctypeslib.as_array(mo.sbp, shape=(MJ + 2, MI + 2)).T[1:MI + 2, 1:MJ + 1]
+    ku = np.ctypeslib.as_array(mo.ku, shape=(MJ, MI, 2)).transpose(2, 1, 0)
+    mk = np.ctypeslib.as_array(mo.mk, shape=(MJ + 2, MI + 2)).T[1:MI + 2, 1:MJ + 1]
+    ubv = (m.ub() if ub is None else ub).transpose(2, 1, 0)
+    keep = [F(drag), F(rh), F(tau), F(bp), F(sbp), F(ku, np.int32), F(mk, np.int32), F(ubv)]
+    vecs = [vec("s", 0, MJ), vec("sv", 0, MJ), vec("c", 0, MJ), vec("rcv", 1, MJ - 1), vec("dsv", 1, MJ - 1),
+            vec("dz", 1, MK)]
+    out = np.zeros(1)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    R.ref_gold_island(MI, MJ, MK, mo.isles, mo.mpi, ptr(npi), *[ptr(a) for a in paths], *[ptr(a) for a in vecs],
+                      g.dphi, *[ptr(a) for a in keep], isl, indj, out)
+    return float(out[0])
+
+
+def rho_from_ts(g: OgGrid, k1, go_ts) -> np.ndarray:
+    rho = np.zeros((g.maxk + 1, g.maxj + 2, g.maxi + 2))
+    lib().og_rho_from_ts(C.byref(g), go_ts.shape[-1], np.ascontiguousarray(k1, dtype=np.int32),
+                         np.ascontiguousarray(go_ts, dtype=np.float64), rho)
+    return rho

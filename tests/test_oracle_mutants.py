@@ -16,6 +16,7 @@ import pytest
 
 import oracle_ffi as O
 import test_oracle as T
+import test_velocity as V
 
 SRC = os.path.join(O.ORACLE_DIR, "cgo_oracle.c")
 
@@ -78,10 +79,31 @@ MUTANTS = [
     ("atmosphere mixed without the gas temperature",
      "st->atm[2 * c + q] = (tot / totV) * CONV_PA_ATM * CONST_R_SI * CONST_ZEROC;",
      "st->atm[2 * c + q] = (tot / totV) * CONV_PA_ATM * CONST_R_SI;", 1),
+    # the velocity solve, goldstein.f90:2845-3714, goldstein_lib.f90:186-241
+    ("psi equation: periodic west neighbour in the wrong column",
+     "if (i == 1) l = 2 * n + 1;", "if (i == MI) l = 2 * n + 1;", 1),
+    ("barotropic u without the cos(latitude) metric",
+     "UBL(ubv, 1, i, j) = -MRH(1, i, j) * g->c[j] * (PSIL(psv, i, j)",
+     "UBL(ubv, 1, i, j) = -MRH(1, i, j) * (PSIL(psv, i, j)", 1),
+    ("baroclinic shear: drag and Coriolis swapped",
+     "dzu[0][k] = -(g->s[j] * tv1 + MDRAG(1, i, j) * tv2) * MRTV(i, j);",
+     "dzu[0][k] = -(MDRAG(1, i, j) * tv1 + g->s[j] * tv2) * MRTV(i, j);", 1),
+    ("depth mean of the baroclinic velocity not removed",
+     "MU(u, 1, i, j, k) = MU(u, 1, i, j, k) - sum[0] * MRH(1, i, j) + UBL(mo->ub, 1, i, j);",
+     "MU(u, 1, i, j, k) = MU(u, 1, i, j, k) + UBL(mo->ub, 1, i, j);", 1),
+    ("island circulation: Coriolis term with the wrong sign",
+     "cor = -g->s[jpi] * 0.25 *", "cor = g->s[jpi] * 0.25 *", 1),
+    ("continuity: meridional divergence with rdsv",
+     "MU(u, 2, i, j - 1, k) * g->cv[j - 1]) * g->rds[j];", "MU(u, 2, i, j - 1, k) * g->cv[j - 1]) * g->rdsv[j];", 1),
+    ("bottom pressure integrated without the layer spacing",
+     "(MRHO(i, j, k) + MRHO(i, j, k - 1)) * g->dza[k - 1] * 0.5;", "(MRHO(i, j, k) + MRHO(i, j, k - 1)) * 0.5;", 1),
+    ("drag: equatorial band one row narrower",
+     "if (kloc2 > kmxdrg || abs(j - MJ / 2) <= jeb)", "if (kloc2 > kmxdrg || abs(j - MJ / 2) < jeb)", 1),
 ]
 
-# the pins (tests/test_oracle.py), cheapest first; the compiled-reference
-# comparison is left out (it needs oracle/_ref), its golden vectors are in
+# the pins (tests/test_oracle.py, tests/test_velocity.py), cheapest first;
+# the carbonate chemistry's compiled-reference comparison is represented by
+# its golden vectors (the first case of test_carbchem_matches_reference_vectors)
 PINS = [
     T.test_eos_polynomial, T.test_grid_closed_forms, T.test_carbconst_closed_form_mehrbach,
     T.test_carbonate_root_satisfies_alkalinity, T.test_convection_mixes_inversion_to_thickness_weighted_mean,
@@ -96,7 +118,18 @@ PINS = [
     lambda: T.test_carbchem_matches_reference_vectors(0), T.test_oracle_conserves_inventory,
     T.test_airsea_solubility_closed_forms, T.test_airsea_piston_velocity_wanninkhof,
     T.test_airsea_bulk_flux_equilibrium_and_limit, T.test_airsea_step_conserves_and_atmosphere_mixes,
+    V.test_channel_barotropic_flow_is_the_friction_wind_balance, V.test_channel_ekman_layer_and_depth_mean,
+    V.test_worbe2_island_circulation_vanishes, V.test_worbe2_solve_is_linear_in_the_forcing,
+    V.test_worbe2_velocity_leaves_a_uniform_tracer_uniform, V.test_worbe2_no_wind_and_level_density_give_no_flow,
+    V.test_drag_values, V.test_bottom_pressure_is_the_hydrostatic_integral,
 ]
+# the reference's own island() where it is built (oracle/_ref): the one pin of
+# the island integral's Coriolis term, which no closed form fixes (the
+# integral round an island is not path-independent in this discretisation)
+if O.ref_gold() is not None:
+    PINS += [V.test_island_integral_matches_the_reference_worbe2,
+             V.test_island_integral_matches_the_reference_u_and_v_path]
+NEEDS_REF = {"island circulation: Coriolis term with the wrong sign"}
 
 
 def _build(tmp, idx, name, find, repl, count):
@@ -135,6 +168,8 @@ def test_unmutated_oracle_passes_every_pin():
 def test_pins_catch_planted_mistake(mutant_libs, idx):
     so, err = mutant_libs[idx]
     assert err is None, err
+    if MUTANTS[idx][0] in NEEDS_REF and O.ref_gold() is None:
+        pytest.skip("pinned only by the reference build (oracle/_ref)")
     caught = None
     with O.use_library(so):
         for pin in PINS:
