@@ -3,8 +3,8 @@
 Each translation unit is compiled on its own (``hipcc -c``, parallel) and
 linked into ``cgenie_amd/libcgocn.so``:
 
-* k_tstep_f64.hip, k_bgc.hip  -ffp-contract=off: operations round as the
-  reference writes them (the FP64 path is bit-comparable with the oracle)
+* k_tstep_f64.hip, k_bgc.hip, k_mom.hip  -ffp-contract=off: operations round
+  as the reference writes them (the FP64 path is bit-comparable with the oracle)
 * k_tstep_f32.hip              -ffp-contract=fast: the FP32 form
 * cgocn_api.hip                host side of the C ABI
 
@@ -42,6 +42,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno
 UNITS = {
     "k_tstep_f64.hip": ["-ffp-contract=off"],
     "k_bgc.hip": ["-ffp-contract=off"],
+    "k_mom.hip": ["-ffp-contract=off"],
     "k_tstep_f32.hip": ["-ffp-contract=fast"],
     "cgocn_api.hip": ["-ffp-contract=off"],
 }

@@ -15,6 +15,7 @@
 
 #include "../../include/cgocn.h"
 #include "cgocn_dev.h"
+#include "mom_host.h"
 
 namespace cgo {
 void launch_flux64(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
@@ -29,6 +30,7 @@ void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, i
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
+void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, hipStream_t);
 }  // namespace cgo
 
 using namespace cgo;
@@ -53,6 +55,13 @@ struct GridH {
     uint64_t fingerprint = 0;     // of the dimensions, mask and metric tables (checkpoints)
     void* dbuf = nullptr;
     double* d_invw = nullptr;
+    // velocity solve (cgo_grid_set_momentum): the host set-up, its device
+    // copy (one allocation holding the MomGridDev struct and its arrays) and
+    // a fingerprint of its inputs (checkpoints)
+    MomSetup* mom = nullptr;
+    void* mbuf = nullptr;
+    const MomGridDev* d_mg = nullptr;
+    uint64_t mom_fp = 0;
 };
 
 struct MemberH {
@@ -68,6 +77,9 @@ struct MemberH {
     cgo_bgc_params bp{};
     void* dbuf = nullptr;
     size_t bytes = 0;
+    // velocity solve on (cgo_set_wind): the member's arrays of the solve
+    void* mbuf = nullptr;
+    size_t mbytes = 0;
 };
 
 struct Graph {
@@ -238,11 +250,14 @@ int cgo_destroy(cgo_engine* e) {
     for (auto* m : e->members)
         if (m) {
             if (m->dbuf) hipFree(m->dbuf);
+            if (m->mbuf) hipFree(m->mbuf);
             delete m;
         }
     for (auto* g : e->grids)
         if (g) {
             if (g->dbuf) hipFree(g->dbuf);
+            if (g->mbuf) hipFree(g->mbuf);
+            delete g->mom;
             delete g;
         }
     if (e->d_list) hipFree(e->d_list);
@@ -599,6 +614,7 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
         d.mink = reinterpret_cast<const int32_t*>(b + ly.mink);
     }
     m->dev.sflux = had_sflux ? reinterpret_cast<double*>(b + ly.sflux) : nullptr;
+    if (m->mbuf && m->prec == CGO_FP64) d.u64 = reinterpret_cast<double*>(b + ly.u);
     e->table_dirty = true;
     return CGO_OK;
 }
@@ -647,7 +663,9 @@ int cgo_member_destroy(cgo_engine* e, int32_t id) {
     int rc = settle(e, m);
     if (rc) return rc;
     if (m->dbuf) e->graveyard.push_back(m->dbuf);
+    if (m->mbuf) e->graveyard.push_back(m->mbuf);
     m->dbuf = nullptr;
+    m->mbuf = nullptr;
     m->alive = false;
     e->table_dirty = true;
     bump_gen(e, id);
@@ -658,6 +676,16 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
     MemberH* m = member(e, id);
     if (!m || !u) return fail(e, CGO_E_ARG, "set_velocity: bad argument");
     if (int rc = settle(e, m)) return rc;
+    if (m->mbuf) {  // prescribed velocities from now on: the member's solve is switched off
+        e->graveyard.push_back(m->mbuf);
+        m->mbuf = nullptr;
+        m->mbytes = 0;
+        m->dev.mg = nullptr;
+        m->dev.mw = nullptr;
+        m->dev.u64 = nullptr;
+        e->table_dirty = true;
+        bump_gen(e, id);
+    }
     const GridDev& g = m->dev.g;
     const int mi = g.maxi, mj = g.maxj, mk = g.maxk;
     const size_t ncell = g.ncell;
@@ -726,6 +754,248 @@ int cgo_set_velocity(cgo_engine* e, int32_t id, const double* u) {
     return CGO_OK;
 }
 
+// ------------------------------------------------------------ velocity solve
+
+static constexpr double MOM_FSC = 2 * 7.2921E-5;  // fsc, goldstein_lib.f90:49
+static constexpr double MOM_USC = 0.05;           // usc
+static constexpr double MOM_RH0SC = 1.0E3;        // rh0sc, goldstein_lib.f90:50
+
+int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_desc* d) {
+    if (!e || !d || grid_id < 0 || grid_id >= (int)e->grids.size()) return fail(e, CGO_E_ARG, "grid_set_momentum: bad argument");
+    if (!d->s || !d->c || !d->rcv || !d->rds2 || !d->labels || (d->isles > 0 && (!d->npi || !d->paths)))
+        return fail(e, CGO_E_ARG, "grid_set_momentum: null array");
+    GridH* g = e->grids[grid_id];
+    if (g->mom) return fail(e, CGO_E_STATE, "grid_set_momentum: grid %d already has its velocity solve", grid_id);
+    const int mi = g->d.maxi, mj = g->d.maxj, mk = g->d.maxk, JS = mj + 2, KS = mk + 2;
+    // the band solve runs on one wave: a step of the elimination updates the
+    // maxi+1 rows below the pivot, one per lane
+    if (mi + 1 > 64) return fail(e, CGO_E_ARG, "grid_set_momentum: maxi %d above 63", mi);
+    if ((long)mi * (mj + 1) > MOM_MAXN)
+        return fail(e, CGO_E_ARG, "grid_set_momentum: %d psi points above %d", mi * (mj + 1), MOM_MAXN);
+    if (!(d->adrag > 0.0) || !(d->rel >= 0.0 && d->rel <= 1.0))
+        return fail(e, CGO_E_ARG, "grid_set_momentum: adrag %g days, rel %g", d->adrag, d->rel);
+    const double* jrows[JT_N];
+    jrows[JT_S] = d->s;
+    jrows[JT_SV] = &g->mj[MJ_SV * JS];
+    jrows[JT_C] = d->c;
+    jrows[JT_CV] = &g->mj[MJ_CV * JS];
+    jrows[JT_RC] = &g->mj[MJ_RC * JS];
+    jrows[JT_RCV] = d->rcv;
+    jrows[JT_RDS] = &g->mj[MJ_RDS * JS];
+    jrows[JT_RDSV] = &g->mj[MJ_RDSV * JS];
+    jrows[JT_RDS2] = d->rds2;
+    jrows[JT_DS] = &g->mj[MJ_DS * JS];
+    jrows[JT_DSV] = &g->mj[MJ_DSVM * JS];  // dsv(j) at j = 1..maxj-1, the only rows the solve reads it at
+    MomSetup* M = new MomSetup();
+    const std::string err = mom_setup(*M, g->k1.data(), mi, mj, mk, jrows, &g->mk[MK_DZ * KS], &g->mk[MK_DZA * KS],
+                                      g->d.dphi, g->d.rdphi, d->labels, d->isles, d->npi, d->paths, d->adrag, MOM_FSC);
+    if (!err.empty()) {
+        delete M;
+        return fail(e, CGO_E_ARG, "grid_set_momentum: %s", err.c_str());
+    }
+    const int n = M->n, N = M->N, nis = std::max(M->isles, 1);
+    // the elimination's multipliers by pivot row and the eliminated band by row
+    std::vector<double> fwd((size_t)std::max(N - 1, 1) * (n + 1), 0.0), bwd((size_t)N * (n + 2), 0.0);
+    for (int i = 1; i <= N - 1; i++)
+        for (int q = 1; q <= n + 1 && i + q <= N; q++) fwd[(size_t)(i - 1) * (n + 1) + q - 1] = M->RATM(i + q, q);
+    for (int i = 1; i <= N; i++)
+        for (int k = 0; k <= n + 1 && i + k <= N; k++) bwd[(size_t)(i - 1) * (n + 2) + k] = M->GAP(i, n + 2 + k);
+    // one device buffer: the MomGridDev struct, then its arrays
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    struct Piece {
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Piece> pieces;
+    auto add = [&](const void* src, size_t bytes) {
+        pieces.push_back({src, bytes, take(bytes)});
+        return pieces.size() - 1;
+    };
+    const size_t o_struct = take(sizeof(MomGridDev));
+    const size_t p_jt = add(M->jt.data(), M->jt.size() * 8), p_dz = add(M->dz.data(), M->dz.size() * 8),
+                 p_dza = add(M->dza.data(), M->dza.size() * 8), p_rh = add(M->rh.data(), M->rh.size() * 8),
+                 p_drag = add(M->drag.data(), M->drag.size() * 8), p_rtv = add(M->rtv.data(), M->rtv.size() * 8),
+                 p_rtv3 = add(M->rtv3.data(), M->rtv3.size() * 8), p_ku = add(M->ku.data(), M->ku.size() * 4),
+                 p_mk = add(M->mkv.data(), M->mkv.size() * 4), p_getj = add(M->getj.data(), M->getj.size()),
+                 p_fwd = add(fwd.data(), fwd.size() * 8), p_bwd = add(bwd.data(), bwd.size() * 8),
+                 p_ubisl = add(M->ubisl.data(), M->ubisl.size() * 8), p_psisl = add(M->psisl.data(), M->psisl.size() * 8),
+                 p_erisl = add(M->erisl.data(), M->erisl.size() * 8), p_npi = add(M->npi.data(), (size_t)nis * 4),
+                 p_lp = add(M->lpisl.data(), M->lpisl.size() * 4), p_ip = add(M->ipisl.data(), M->ipisl.size() * 4),
+                 p_jp = add(M->jpisl.data(), M->jpisl.size() * 4);
+    hipSetDevice(e->device);
+    void* buf = nullptr;
+    if (hipMalloc(&buf, off) != hipSuccess) {
+        delete M;
+        return fail(e, CGO_E_NOMEM, "grid_set_momentum: hipMalloc(%zu)", off);
+    }
+    char* b = static_cast<char*>(buf);
+    auto at_ = [&](size_t p) { return b + pieces[p].off; };
+    MomGridDev h{};
+    h.maxi = mi;
+    h.maxj = mj;
+    h.maxk = mk;
+    h.n = n;
+    h.N = N;
+    h.isles = M->isles;
+    h.mpi = M->mpi;
+    h.dphi = g->d.dphi;
+    h.rdphi = g->d.rdphi;
+    h.rel = d->rel;
+    h.jt = (const double*)at_(p_jt);
+    h.dz = (const double*)at_(p_dz);
+    h.dza = (const double*)at_(p_dza);
+    h.rh = (const double*)at_(p_rh);
+    h.drag = (const double*)at_(p_drag);
+    h.rtv = (const double*)at_(p_rtv);
+    h.rtv3 = (const double*)at_(p_rtv3);
+    h.ku = (const int32_t*)at_(p_ku);
+    h.mk = (const int32_t*)at_(p_mk);
+    h.getj = (const uint8_t*)at_(p_getj);
+    h.fwd = (const double*)at_(p_fwd);
+    h.bwd = (const double*)at_(p_bwd);
+    h.ubisl = (const double*)at_(p_ubisl);
+    h.psisl = (const double*)at_(p_psisl);
+    h.erisl = (const double*)at_(p_erisl);
+    h.npi = (const int32_t*)at_(p_npi);
+    h.lpisl = (const int32_t*)at_(p_lp);
+    h.ipisl = (const int32_t*)at_(p_ip);
+    h.jpisl = (const int32_t*)at_(p_jp);
+    int rc = CGO_OK;
+    if (hipMemcpy(b + o_struct, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = CGO_E_HIP;
+    for (const Piece& pc : pieces)
+        if (rc == CGO_OK && pc.bytes && hipMemcpy(b + pc.off, pc.src, pc.bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = CGO_E_HIP;
+    if (rc != CGO_OK) {
+        hipFree(buf);
+        delete M;
+        return fail(e, CGO_E_HIP, "grid_set_momentum: upload failed");
+    }
+    // fingerprint of the set-up's inputs: a checkpoint of a member with the
+    // solve restores only onto a grid set up alike
+    uint64_t fp = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t nb) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        for (size_t x = 0; x < nb; x++) fp = (fp ^ q[x]) * 1099511628211ull;
+    };
+    mix(M->jt.data(), M->jt.size() * 8);
+    mix(d->labels, (size_t)mi * (mj + 1) * 4);
+    mix(&d->isles, 4);
+    mix(M->npi.data(), (size_t)nis * 4);
+    mix(M->lpisl.data(), M->lpisl.size() * 4);
+    mix(M->ipisl.data(), M->ipisl.size() * 4);
+    mix(M->jpisl.data(), M->jpisl.size() * 4);
+    mix(&d->adrag, 8);
+    mix(&d->rel, 8);
+    g->mom = M;
+    g->mbuf = buf;
+    g->d_mg = reinterpret_cast<const MomGridDev*>(b + o_struct);
+    g->mom_fp = fp;
+    return CGO_OK;
+}
+
+// the member's arrays of the solve, allocated and zeroed, and its u planes
+// zeroed: the solve leaves closed faces, and w at the top level, at zero
+static int mom_attach(cgo_engine* e, MemberH* m, int32_t id) {
+    const GridH* g = e->grids[m->grid];
+    const MomSetup& S = *g->mom;
+    const MomMemberLayout L = mom_member_layout(S.mi, S.mj, S.mk, m->prec != CGO_FP64);
+    void* buf = nullptr;
+    if (hipMalloc(&buf, L.total * 8) != hipSuccess) return fail(e, CGO_E_NOMEM, "set_wind: hipMalloc(%zu)", L.total * 8);
+    if (hipMemset(buf, 0, L.total * 8) != hipSuccess ||
+        hipMemset(const_cast<void*>(m->dev.u), 0, (size_t)m->dev.g.ncell * (m->prec == CGO_FP64 ? 3 * 8 : 7 * 4)) != hipSuccess) {
+        hipFree(buf);
+        return fail(e, CGO_E_HIP, "set_wind: hipMemset failed");
+    }
+    m->mbuf = buf;
+    m->mbytes = L.total * 8;
+    m->dev.mg = g->d_mg;
+    m->dev.mw = static_cast<double*>(buf);
+    m->dev.u64 = m->prec == CGO_FP64 ? static_cast<double*>(const_cast<void*>(m->dev.u)) : static_cast<double*>(buf) + L.u64;
+    e->table_dirty = true;
+    bump_gen(e, id);
+    return CGO_OK;
+}
+
+int cgo_set_wind(cgo_engine* e, int32_t id, const double* sxu, const double* syu, const double* sxv, const double* syv,
+                 double scf) {
+    MemberH* m = member(e, id);
+    if (!m || !sxu || !syu || !sxv || !syv) return fail(e, CGO_E_ARG, "set_wind: bad argument");
+    const GridH* g = e->grids[m->grid];
+    if (!g->mom) return fail(e, CGO_E_STATE, "set_wind: grid %d has no velocity solve (cgo_grid_set_momentum)", m->grid);
+    if (int rc = settle(e, m)) return rc;
+    hipSetDevice(e->device);
+    if (!m->mbuf)
+        if (int rc = mom_attach(e, m, id)) return rc;
+    const MomSetup& S = *g->mom;
+    const int mi = S.mi, mj = S.mj, mk = S.mk, n = S.n;
+    const size_t ncol = (size_t)mi * mj;
+    const MomMemberLayout L = mom_member_layout(mi, mj, mk, m->prec != CGO_FP64);
+    // dztau, dztav: the stresses scaled by scf and made non-dimensional per
+    // unit depth of the top layer's half-cell dzz; tau their x at u and y at
+    // v points (step_goldstein 102-107, dzz 1035)
+    const double dzz = S.dz[mk] * S.dza[mk - 1] / 2;
+    const double den = MOM_RH0SC * g->d.dsc * MOM_USC * MOM_FSC;
+    std::vector<double> tau(2 * ncol), dztau(2 * ncol), dztav(2 * ncol), gbw(S.N, 0.0);
+    for (size_t c = 0; c < ncol; c++) {
+        dztau[2 * c] = scf * sxu[c] / den / dzz;
+        dztau[2 * c + 1] = scf * syu[c] / den / dzz;
+        dztav[2 * c] = scf * sxv[c] / den / dzz;
+        dztav[2 * c + 1] = scf * syv[c] / den / dzz;
+        tau[2 * c] = dztau[2 * c] * dzz;
+        tau[2 * c + 1] = dztav[2 * c + 1] * dzz;
+    }
+    // the wind's curl at every wet psi point (wind, 3685-3714)
+    auto T = [&](int l, int i, int j) { return tau[(l - 1) + 2 * ((size_t)(i - 1) + (size_t)mi * (j - 1))]; };
+    for (int i = 1; i <= mi; i++)
+        for (int j = 0; j <= mj; j++) {
+            const int ip1 = i % mi + 1;
+            if (std::max(std::max(S.K1(i, j), S.K1(i + 1, j)), std::max(S.K1(i, j + 1), S.K1(i + 1, j + 1))) <= mk)
+                gbw[(i + j * n) - 1] = (T(2, ip1, j) * S.RH(2, i + 1, j) - T(2, i, j) * S.RH(2, i, j)) * S.rdphi * S.J(JT_RCV, j) -
+                                       (T(1, i, j + 1) * S.J(JT_C, j + 1) * S.RH(1, i, j + 1) -
+                                        T(1, i, j) * S.J(JT_C, j) * S.RH(1, i, j)) *
+                                           S.J(JT_RDSV, j);
+        }
+    double* w = m->dev.mw;
+    HIPCHK(e, hipMemcpy(w + L.gbw, gbw.data(), gbw.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(w + L.tau, tau.data(), tau.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(w + L.dztau, dztau.data(), dztau.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(w + L.dztav, dztav.data(), dztav.size() * 8, hipMemcpyHostToDevice));
+    return CGO_OK;
+}
+
+int cgo_download_velocity(cgo_engine* e, int32_t id, double* go_u) {
+    MemberH* m = member(e, id);
+    if (!m || !go_u) return fail(e, CGO_E_ARG, "download_velocity: bad argument");
+    if (m->prec != CGO_FP64 && !m->mbuf)
+        return fail(e, CGO_E_STATE, "download_velocity: member %d holds its prescribed velocities as FP32 flux coefficients", id);
+    if (int rc = settle(e, m)) return rc;
+    const GridDev& g = m->dev.g;
+    const size_t ncell = g.ncell;
+    std::vector<double> h(3 * ncell);
+    const void* src = m->prec == CGO_FP64 ? m->dev.u : m->dev.u64;
+    HIPCHK(e, hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
+    // [l][k][j][i] -> go_u(3, maxi, maxj, maxk)
+    for (size_t c = 0; c < ncell; c++)
+        for (int l = 0; l < 3; l++) go_u[(size_t)l + 3 * c] = h[(size_t)l * ncell + c];
+    return CGO_OK;
+}
+
+int cgo_download_psi(cgo_engine* e, int32_t id, double* go_psi) {
+    MemberH* m = member(e, id);
+    if (!m || !go_psi) return fail(e, CGO_E_ARG, "download_psi: bad argument");
+    if (!m->mbuf) return fail(e, CGO_E_STATE, "download_psi: member %d has no velocity solve (cgo_set_wind)", id);
+    if (int rc = settle(e, m)) return rc;
+    const GridH* g = e->grids[m->grid];
+    const MomMemberLayout L = mom_member_layout(g->mom->mi, g->mom->mj, g->mom->mk, m->prec != CGO_FP64);
+    HIPCHK(e, hipMemcpy(go_psi, m->dev.mw + L.psi, (size_t)(g->mom->mi + 1) * (g->mom->mj + 1) * 8, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
 int cgo_set_sflux(cgo_engine* e, int32_t id, const double* sflux) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "set_sflux: bad member");
@@ -785,6 +1055,23 @@ int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
             HIPCHK(e, hipMemcpy(m->dev.st[q], f.data(), n * 4, hipMemcpyHostToDevice));
             if (m->dev.lo[q]) HIPCHK(e, hipMemcpy(m->dev.lo[q], lo.data(), n * 2, hipMemcpyHostToDevice));
         }
+    }
+    // rho of the new state at every wet cell, as initialise_goldstein forms
+    // it from the initial or restart state (goldstein.f90:1450, 1754): the
+    // velocity solve's first step reads it
+    {
+        const GridH* gh = e->grids[m->grid];
+        const int mi = g.maxi, mj = g.maxj, mk = g.maxk;
+        std::vector<double> r(g.ncell, 0.0);
+        for (int k = 1; k <= mk; k++)
+            for (int J = 1; J <= mj; J++)
+                for (int I = 1; I <= mi; I++)
+                    if (k >= gh->k1[(size_t)J * (mi + 2) + I]) {
+                        const size_t c = (size_t)(k - 1) * g.ncol + (I - 1) + (size_t)mi * (J - 1);
+                        const double t = h[c], sa = h[g.ncell + c];
+                        r[c] = g.ec[0] * t + g.ec[1] * sa + g.ec[2] * (t * t) + g.ec[3] * (t * t * t);
+                    }
+        HIPCHK(e, hipMemcpy(m->dev.rho, r.data(), r.size() * 8, hipMemcpyHostToDevice));
     }
     const int32_t zero = 0;
     HIPCHK(e, hipMemcpy(m->dev.parity, &zero, 4, hipMemcpyHostToDevice));
@@ -914,14 +1201,15 @@ static void set_bgc_params(MemberH* m, const cgo_bgc_params& bp) {
 // ---------------------------------------------------------- checkpoints
 // blob = CkptHdr, then the member's whole device allocation as laid out by
 // layout() (state ping-pong and carry planes, velocities, surface flux,
-// rho, cost, control words, BIOGEM arrays and scalars): restoring it into a
-// member of the same grid, tracer count, precision and carry continues the
-// run bit for bit.
+// rho, cost, control words, BIOGEM arrays and scalars), then, for a member
+// with a velocity solve, its arrays of the solve (wind, relaxation memory,
+// streamfunction): restoring it into a member of the same grid, tracer
+// count, precision and carry continues the run bit for bit.
 struct CkptHdr {
     char magic[8];
     uint32_t version, hdr_bytes;
-    int32_t maxi, maxj, maxk, L, prec, carry, bgc, diso, wstream, coupling_due, has_sflux, pad;
-    uint64_t grid_fp, payload;
+    int32_t maxi, maxj, maxk, L, prec, carry, bgc, diso, wstream, coupling_due, has_sflux, mom;
+    uint64_t grid_fp, payload, mom_fp, mom_bytes;
     double diff[2];
     cgo_bgc_params bp;
 };
@@ -930,20 +1218,20 @@ static const char CKPT_MAGIC[8] = {'C', 'G', 'O', 'C', 'K', 'P', 'T', '1'};
 int64_t cgo_checkpoint_bytes(cgo_engine* e, int32_t id) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "checkpoint_bytes: bad member %d", id);
-    return (int64_t)(sizeof(CkptHdr) + m->bytes);
+    return (int64_t)(sizeof(CkptHdr) + m->bytes + m->mbytes);
 }
 
 int cgo_checkpoint(cgo_engine* e, int32_t id, void* buf, size_t n) {
     MemberH* m = member(e, id);
     if (!m || !buf) return fail(e, CGO_E_ARG, "checkpoint: bad argument");
-    if (n < sizeof(CkptHdr) + m->bytes)
-        return fail(e, CGO_E_ARG, "checkpoint: buffer of %zu bytes, %zu needed", n, sizeof(CkptHdr) + m->bytes);
+    if (n < sizeof(CkptHdr) + m->bytes + m->mbytes)
+        return fail(e, CGO_E_ARG, "checkpoint: buffer of %zu bytes, %zu needed", n, sizeof(CkptHdr) + m->bytes + m->mbytes);
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->s));
     CkptHdr h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, CKPT_MAGIC, 8);
-    h.version = 1;
+    h.version = 2;
     h.hdr_bytes = (uint32_t)sizeof(CkptHdr);
     const GridDev& g = m->dev.g;
     h.maxi = g.maxi;
@@ -962,8 +1250,13 @@ int cgo_checkpoint(cgo_engine* e, int32_t id, void* buf, size_t n) {
     h.diff[0] = m->p.diff[0];
     h.diff[1] = m->p.diff[1];
     h.bp = m->bp;
+    h.mom = m->mbuf != nullptr;
+    h.mom_fp = m->mbuf ? e->grids[m->grid]->mom_fp : 0;
+    h.mom_bytes = m->mbytes;
     memcpy(buf, &h, sizeof h);
     HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h, m->dbuf, m->bytes, hipMemcpyDeviceToHost));
+    if (m->mbuf)
+        HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h + m->bytes, m->mbuf, m->mbytes, hipMemcpyDeviceToHost));
     return CGO_OK;
 }
 
@@ -972,17 +1265,32 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
     if (!m || !buf || n < sizeof(CkptHdr)) return fail(e, CGO_E_ARG, "restore: bad argument");
     CkptHdr h;
     memcpy(&h, buf, sizeof h);
-    if (memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1 || h.hdr_bytes != sizeof(CkptHdr))
+    if (memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 2 || h.hdr_bytes != sizeof(CkptHdr))
         return fail(e, CGO_E_ARG, "restore: not a checkpoint of this library version");
     const GridH* gh = e->grids[m->grid];
     if (h.grid_fp != gh->fingerprint || h.L != m->L || h.prec != m->prec || h.carry != m->p.carry)
         return fail(e, CGO_E_ARG, "restore: checkpoint of another grid, tracer count, precision or carry "
                                   "(grid %dx%dx%d, %d tracers, precision %d)", h.maxi, h.maxj, h.maxk, h.L, h.prec);
-    if (n < sizeof h + h.payload) return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n,
-                                              (size_t)(sizeof h + h.payload));
+    if (n < sizeof h + h.payload + h.mom_bytes)
+        return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n, (size_t)(sizeof h + h.payload + h.mom_bytes));
+    if (h.mom && (!gh->mom || gh->mom_fp != h.mom_fp))
+        return fail(e, CGO_E_ARG, "restore: the checkpoint's velocity solve needs its grid set up as it was (cgo_grid_set_momentum)");
     HIPCHK(e, hipSetDevice(e->device));
     if (int rc = settle(e, m)) return rc;
     int32_t slot = id;
+    if (h.mom && !m->mbuf) {
+        if (int rc = mom_attach(e, m, id)) return rc;
+    } else if (!h.mom && m->mbuf) {
+        e->graveyard.push_back(m->mbuf);
+        m->mbuf = nullptr;
+        m->mbytes = 0;
+        m->dev.mg = nullptr;
+        m->dev.mw = nullptr;
+        m->dev.u64 = nullptr;
+    }
+    if (h.mom && h.mom_bytes != m->mbytes)
+        return fail(e, CGO_E_STATE, "restore: the velocity solve's arrays (%zu bytes) differ from the checkpoint's (%zu)",
+                    m->mbytes, (size_t)h.mom_bytes);
     if (h.bgc && !m->bgc) {
         m->bgc = 1;
         int rc = alloc_member(e, m);
@@ -992,6 +1300,8 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
         return fail(e, CGO_E_STATE, "restore: the member's layout (%zu bytes) differs from the checkpoint's (%zu)",
                     m->bytes, (size_t)h.payload);
     HIPCHK(e, hipMemcpy(m->dbuf, static_cast<const char*>(buf) + sizeof h, h.payload, hipMemcpyHostToDevice));
+    if (h.mom)
+        HIPCHK(e, hipMemcpy(m->mbuf, static_cast<const char*>(buf) + sizeof h + h.payload, h.mom_bytes, hipMemcpyHostToDevice));
     m->p.diff[0] = h.diff[0];
     m->p.diff[1] = h.diff[1];
     m->p.diso = h.diso;
@@ -1325,9 +1635,10 @@ int cgo_set_windspeed(cgo_engine* e, int32_t id, const double* wspeed) {
 
 // ---------------------------------------------------------------- stepping
 struct Lists {
-    // [p64 list][p32 list][bgc p64][bgc p32][all]
+    // [p64 list][p32 list][bgc p64][bgc p32][all][velocity solve]
     std::vector<int32_t> v;
-    int n[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
+    int n[6] = {0, 0, 0, 0, 0, 0}, off[6] = {0, 0, 0, 0, 0, 0};
+    int momN = 0;  // most psi points of a member with a velocity solve
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
     int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
     int ncol_atm = 0;  // widest grid of the members with air-sea exchange, 0 if none
@@ -1335,7 +1646,7 @@ struct Lists {
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
-    std::vector<int32_t> g[5];
+    std::vector<int32_t> g[6];
     for (int i = 0; i < n; i++) {
         MemberH* m = member(e, ids[i]);
         if (!m) return fail(e, CGO_E_ARG, "step: bad member %d", ids[i]);
@@ -1343,6 +1654,10 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         g[p].push_back(ids[i]);
         if (m->bgc) g[2 + p].push_back(ids[i]);
         g[4].push_back(ids[i]);
+        if (m->mbuf) {
+            g[5].push_back(ids[i]);
+            ls.momN = std::max(ls.momN, e->grids[m->grid]->mom->N);
+        }
         ls.maxcol[p] = std::max(ls.maxcol[p], m->dev.g.ncol);
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
         ls.maxi[p] = std::max(ls.maxi[p], m->dev.g.maxi);
@@ -1358,7 +1673,7 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             if (m->dev.bp.airsea) ls.ncol_atm = std::max(ls.ncol_atm, m->dev.g.ncol);
         }
     }
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < 6; q++) {
         ls.off[q] = (int)ls.v.size();
         ls.n[q] = (int)g[q].size();
         ls.v.insert(ls.v.end(), g[q].begin(), g[q].end());
@@ -1388,7 +1703,14 @@ static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lis
     launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
 }
 
+// the velocity solve of the members that have one (step_goldstein 198-233),
+// which precedes their tstepo
+static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
+    if (ls.n[5]) launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], ls.momN, s);
+}
+
 static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    enqueue_mom(e, ls, d_lists, e->s);
     enqueue_flux(e, ls, d_lists, 3);
     enqueue_co_flip(e, ls, d_lists);
 }
@@ -1428,6 +1750,7 @@ static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists
         // CUs first, and BIOGEM's blocks fill the slots both flux kernels
         // leave (enqueued first, they ran beside the pre-pass alone)
         if (bgc) hipEventRecord(e->ev_fork, e->s);
+        enqueue_mom(e, ls, d_lists, e->s);
         enqueue_flux(e, ls, d_lists, 1);
         if (bgc) {
             hipStreamWaitEvent(e->s2, e->ev_fork, 0);
@@ -1634,7 +1957,7 @@ int cgo_set_flux_iso(cgo_engine* e, int32_t precision, int32_t prepass) {
 }
 
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
-    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 4) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
+    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 5) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
@@ -1659,7 +1982,9 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
             if (bgc) hipStreamWaitEvent(e->s, e->ev_join, 0);
         }
     }
-    for (int r = 0; r < (which == 3 ? 0 : reps); r++)
+    if (which == 5)
+        for (int r = 0; r < reps; r++) enqueue_mom(e, ls, e->d_list, e->s);
+    for (int r = 0; r < (which == 3 || which == 5 ? 0 : reps); r++)
         for (int p = 0; p < 2; p++) {
             if (!ls.n[p]) continue;
             const int32_t* l = e->d_list + ls.off[p];

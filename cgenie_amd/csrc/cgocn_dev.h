@@ -95,6 +95,74 @@ struct BgcParams {
     double gas_a;           // par_gastransfer_a
 };
 
+// GOLDSTEIN's velocity solve of one grid (goldstein.f90:198-233): the
+// factors formed once on the host (the psi equation's band elimination, the
+// island basis, depths, drag) and the metric tables the solve reads.
+// Fortran indices throughout; layouts (l fastest where a leading l is given):
+//   rh    (3, 0:maxi+1, 0:maxj+1)     drag (2, 0:maxi+1, 0:maxj+1)
+//   rtv, rtv3 (maxi, maxj)            ku   (2, maxi, maxj)
+//   mk    (0:maxi+1, 0:maxj+1)        getj (maxi, maxj) bytes
+//   fwd   [N-1][n+1]: row i, entry q-1 = ratm(i+q, q), the multiplier that
+//         elimination step i applies to row i+q (0 past row N)
+//   bwd   [N][n+2]:   row i, entry k = gap(i, n+2+k) of the eliminated band
+//   ubisl isles x (2, 0:maxi+1, 0:maxj), psisl isles x (0:maxi, 0:maxj)
+//   erisl (isles, isles+1) column-major, inverted in place (matinv_gold)
+//   paths (mpi, isles): lpisl, ipisl, jpisl; npi (isles)
+enum { JT_S = 0, JT_SV, JT_C, JT_CV, JT_RC, JT_RCV, JT_RDS, JT_RDSV, JT_RDS2, JT_DS, JT_DSV, JT_N };
+struct MomGridDev {
+    int maxi, maxj, maxk, n, N, isles, mpi;
+    double dphi, rdphi, rel;
+    const double* jt;       // [JT_N][maxj+2]
+    const double* dz;       // [maxk+2]
+    const double* dza;      // [maxk+2]
+    const double* rh;
+    const double* drag;
+    const double* rtv;
+    const double* rtv3;
+    const int32_t* ku;
+    const int32_t* mk;
+    const uint8_t* getj;
+    const double* fwd;
+    const double* bwd;
+    const double* ubisl;
+    const double* psisl;
+    const double* erisl;
+    const int32_t* npi;
+    const int32_t* lpisl;
+    const int32_t* ipisl;
+    const int32_t* jpisl;
+};
+constexpr int MOM_MAXN = 8176;   // psi points (maxi x (maxj+1)): with the island values, 64 KiB of LDS
+constexpr int MOM_MAXISL = 16;   // islands
+
+// one member's arrays of the velocity solve, offsets in doubles from its base
+struct MomMemberLayout {
+    size_t gbw, tau, dztau, dztav, u1, ub, psi, bp, sbp, csum, u64, total;
+};
+__host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int mk, bool own_u64) {
+    MomMemberLayout L{};
+    const size_t ncol = (size_t)mi * mj, ncell = ncol * mk, N = (size_t)mi * (mj + 1);
+    size_t o = 0;
+    auto take = [&](size_t n) {
+        size_t r = o;
+        o += (n + 31) & ~(size_t)31;
+        return r;
+    };
+    L.gbw = take(N);                                       // the wind part of the psi forcing (wind, 3685-3714)
+    L.tau = take(2 * ncol);                                // tau (2, maxi, maxj)
+    L.dztau = take(2 * ncol);                              // dztau (2, maxi, maxj)
+    L.dztav = take(2 * ncol);                              // dztav (2, maxi, maxj)
+    L.u1 = take(2 * ncell);                                // u1 of u and v, [2][maxk][maxj][maxi]
+    L.ub = take((size_t)2 * (mi + 2) * (mj + 1));          // ub (2, 0:maxi+1, 0:maxj)
+    L.psi = take((size_t)(mi + 1) * (mj + 1));             // psi (0:maxi, 0:maxj)
+    L.bp = take((size_t)(mi + 2) * (mj + 2) * (mk + 1));   // bp (0:maxi+1, 0:maxj+1, 0:maxk)
+    L.sbp = take((size_t)(mi + 2) * (mj + 2));             // sbp (0:maxi+1, 0:maxj+1)
+    L.csum = take(2 * ncol);                               // velc's depth sums of the shear, [2][maxj][maxi]
+    L.u64 = own_u64 ? take(3 * ncell) : 0;                 // FP64 u, v, w of an FP32 member
+    L.total = o;
+    return L;
+}
+
 struct MemberDev {
     GridDev g;
     int32_t L, prec, diso, bgc;
@@ -132,6 +200,10 @@ struct MemberDev {
     int32_t* diag;          // [4] limps, carb_fail, carb_iters, co guard hits
     int32_t* sfresh;        // device flag: colsum holds the sums of the current state
     double* dmax;           // device scalar
+    // velocity solve: null when the velocities are prescribed
+    const MomGridDev* mg;
+    double* mw;             // the member's arrays of the solve (mom_member_layout)
+    double* u64;            // FP64 u, v, w [3][maxk][maxj][maxi] (the u planes of an FP64 member)
 };
 
 // remineralisation table rows: layer-thickness ratio dD(k+1)/dD(k) and the

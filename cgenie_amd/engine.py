@@ -44,6 +44,12 @@ class BgcParams(C.Structure):
         [("carb_maxit", _i32), ("airsea", _i32), ("gastransfer_a", _d), ("atm_pCO2", _d), ("atm_pO2", _d)]
 
 
+class MomentumDesc(C.Structure):
+    _fields_ = [(n, _dp) for n in ("s", "c", "rcv", "rds2")] + \
+        [("labels", C.POINTER(_i32)), ("isles", _i32), ("npi", C.POINTER(_i32)), ("paths", C.POINTER(_i32)),
+         ("adrag", _d), ("rel", _d)]
+
+
 class Diag(C.Structure):
     _fields_ = [("steps", _i64), ("limps", _i32), ("carb_fail", _i32), ("carb_iters", _i64), ("dmax", _d),
                 ("cost", _d), ("smean", _d)]
@@ -105,6 +111,10 @@ def load() -> C.CDLL:
         "cgo_set_flux_ring": ([P, _i32, _i32], C.c_int),
         "cgo_set_flux_iso": ([P, _i32, _i32], C.c_int),
         "cgo_time_kernel": ([P, _i32, ip, _i32, _i32, _dp], C.c_int),
+        "cgo_grid_set_momentum": ([P, _i32, C.POINTER(MomentumDesc)], C.c_int),
+        "cgo_set_wind": ([P, _i32, _dp, _dp, _dp, _dp, _d], C.c_int),
+        "cgo_download_velocity": ([P, _i32, _dp], C.c_int),
+        "cgo_download_psi": ([P, _i32, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -202,6 +212,44 @@ class Engine:
         self._chk(self.L.cgo_grid_create(self.h, C.byref(d), C.byref(gid)), "grid_create")
         self._keep.append(keep)
         return gid.value
+
+    def set_momentum(self, grid_id: int, g: G.Grid, world: "G.World", adrag: float = 2.5, rel: float = 0.9):
+        """The grid's velocity solve (cgo_grid_set_momentum): world holds the
+        .psiles labels and .paths island paths (grid.read_world); adrag in
+        days and rel as in ini_gold_nml."""
+        keep = [np.ascontiguousarray(getattr(g, n), dtype=np.float64) for n in ("s", "c", "rcv", "rds2")]
+        labels = np.ascontiguousarray(world.labels, dtype=np.int32)
+        npi = np.ascontiguousarray(world.npi if world.isles else np.zeros(1), dtype=np.int32)
+        paths = np.ascontiguousarray(world.paths if world.isles else np.zeros(3), dtype=np.int32)
+        keep += [labels, npi, paths]
+        d = MomentumDesc()
+        d.s, d.c, d.rcv, d.rds2 = (_ptr(a) for a in keep[:4])
+        d.labels = labels.ctypes.data_as(C.POINTER(_i32))
+        d.isles = int(world.isles)
+        d.npi = npi.ctypes.data_as(C.POINTER(_i32))
+        d.paths = paths.ctypes.data_as(C.POINTER(_i32))
+        d.adrag, d.rel = float(adrag), float(rel)
+        self._chk(self.L.cgo_grid_set_momentum(self.h, grid_id, C.byref(d)), "grid_set_momentum")
+
+    def set_wind(self, m: int, stress: np.ndarray, scf: float = 2.0):
+        """Wind stress (maxj, maxi, 4): x and y at u points, x and y at v
+        points in N m-2 (step_goldstein's stressxu/yu/xv/yv); the first call
+        switches the member to solved velocities."""
+        st = np.asarray(stress, dtype=np.float64)
+        parts = [np.ascontiguousarray(st[..., q]) for q in range(4)]
+        self._chk(self.L.cgo_set_wind(self.h, m, *(_ptr(a) for a in parts), float(scf)), "set_wind")
+
+    def velocity(self, m: int, g: G.Grid) -> np.ndarray:
+        """go_u(3, maxi, maxj, maxk) as (maxk, maxj, maxi, 3)."""
+        out = np.zeros((g.maxk, g.maxj, g.maxi, 3))
+        self._chk(self.L.cgo_download_velocity(self.h, m, _ptr(out)), "download_velocity")
+        return out
+
+    def psi(self, m: int, g: G.Grid) -> np.ndarray:
+        """go_psi(0:maxi, 0:maxj) as (maxj+1, maxi+1)."""
+        out = np.zeros((g.maxj + 1, g.maxi + 1))
+        self._chk(self.L.cgo_download_psi(self.h, m, _ptr(out)), "download_psi")
+        return out
 
     def add_member(self, grid_id: int, L: int, precision: int = FP64, diff=(None, None), diso: bool = True,
                    carry: bool = True) -> int:

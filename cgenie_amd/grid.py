@@ -74,6 +74,7 @@ class Grid:
     rdsv: np.ndarray = None
     rcv: np.ndarray = None
     cv2: np.ndarray = None
+    rds2: np.ndarray = None
     # per-k arrays, index 0..maxk+1
     dz: np.ndarray = None
     dza: np.ndarray = None
@@ -95,7 +96,7 @@ def make_grid(maxi=36, maxj=36, maxk=16, nyear=96, yearlen=365.25,
     J, K = maxj + 2, maxk + 2
     z = lambda n: np.zeros(n)  # noqa: E731
     g = Grid(maxi, maxj, maxk, nyear, yearlen)
-    for name in ("s", "sv", "c", "cv", "ds", "rds", "rc", "rc2", "dsv", "rdsv", "rcv", "cv2"):
+    for name in ("s", "sv", "c", "cv", "ds", "rds", "rc", "rc2", "dsv", "rdsv", "rcv", "cv2", "rds2"):
         setattr(g, name, z(J))
     for name in ("dz", "dza", "rdz", "rdza", "zw", "zro", "dt", "ssmax"):
         setattr(g, name, z(K))
@@ -122,6 +123,8 @@ def make_grid(maxi=36, maxj=36, maxk=16, nyear=96, yearlen=365.25,
             g.rdsv[j] = 1.0 / g.dsv[j]
             g.rcv[j] = 1.0 / g.cv[j]
             g.cv2[j] = g.cv[j] * g.cv[j] * g.rdsv[j]
+            if j > 1:
+                g.rds2[j] = 2.0 / (g.dsv[j] + g.dsv[j - 1])
     tv = 86400.0 * yearlen / (nyear * TSC)
     g.dt[:] = tv
     ez0 = 0.1
@@ -157,6 +160,54 @@ def make_grid(maxi=36, maxj=36, maxk=16, nyear=96, yearlen=365.25,
         for k in range(1, maxk):
             g.ssmax[k] = math.exp(mid + dif * math.tanh((g.zw[k] - dep0) / efold))
     return g
+
+
+@dataclass
+class World:
+    """A GOLDSTEIN world's files: the bathymetry k1(0:maxi+1, 0:maxj+1) as
+    (maxj+2, maxi+2), the landmass label of every psi point (maxi, 0:maxj) as
+    (maxj+1, maxi), the island count and the island paths."""
+    k1: np.ndarray
+    labels: np.ndarray
+    isles: int
+    npi: np.ndarray
+    paths: np.ndarray
+
+
+def read_world(prefix: str, maxi: int, maxj: int) -> World:
+    """Read <prefix>.k1, .psiles and .paths as initialise_goldstein does
+    (goldstein.f90:1113-1120, 1500-1574): k1 rows from j = maxj+1 down to 0
+    with the periodic columns copied from the interior, .psiles rows from
+    j = maxj down to 0, isles = the largest label - 1, then the path lengths
+    and the (lpisl, ipisl, jpisl) of each path point, island by island."""
+    def numbers(path, dtype):
+        with open(path) as f:
+            return np.array(f.read().split(), dtype=dtype)
+
+    v = numbers(prefix + ".k1", np.int64)
+    if v.size < (maxj + 2) * (maxi + 2):
+        raise ValueError(f"{prefix}.k1: {v.size} values, {(maxj + 2) * (maxi + 2)} needed")
+    k1 = np.zeros((maxj + 2, maxi + 2), dtype=np.int32)
+    for r in range(maxj + 2):
+        k1[maxj + 1 - r] = v[r * (maxi + 2):(r + 1) * (maxi + 2)]
+    k1[:, 0] = k1[:, maxi]
+    k1[:, maxi + 1] = k1[:, 1]
+    lab = numbers(prefix + ".psiles", np.float64)
+    if lab.size < (maxj + 1) * maxi:
+        raise ValueError(f"{prefix}.psiles: {lab.size} values, {(maxj + 1) * maxi} needed")
+    labels = np.zeros((maxj + 1, maxi), dtype=np.int32)
+    for r in range(maxj + 1):
+        labels[maxj - r] = lab[r * maxi:(r + 1) * maxi].astype(np.int32)
+    isles = max(int(labels.max()) - 1, 0)
+    npi = np.zeros(0, dtype=np.int32)
+    paths = np.zeros(0, dtype=np.int32)
+    if isles > 0:
+        tok = numbers(prefix + ".paths", np.int64)
+        npi = tok[:isles].astype(np.int32)
+        paths = tok[isles:isles + 3 * int(npi.sum())].astype(np.int32)
+        if paths.size != 3 * int(npi.sum()):
+            raise ValueError(f"{prefix}.paths: {paths.size // 3} path points, {int(npi.sum())} needed")
+    return World(k1, labels, isles, npi, paths)
 
 
 def diffusivities(kappa_h: float, kappa_v: float) -> tuple[float, float]:

@@ -164,6 +164,45 @@ int cgo_bgc_download(cgo_engine *e, int32_t m, int32_t what, double *out);
  * (biogem.f90:2228-2232); zero until set, kept across cgo_bgc_init */
 int cgo_set_windspeed(cgo_engine *e, int32_t m, const double *wspeed);
 
+/* GOLDSTEIN's velocity solve (step_goldstein, goldstein.f90:198-233): the
+ * frictional-geostrophic barotropic streamfunction with wind forcing, the
+ * bottom-pressure torque (JEBAR) and island circulations, then the
+ * baroclinic velocities and w from continuity (wind 3685-3714, jbar
+ * 3217-3315, ubarsolv 3500-3565, island goldstein_lib.f90:186-241, matmult
+ * 3470-3492, velc 3568-3679).  cgo_grid_set_momentum forms, once per grid,
+ * what initialise_goldstein does for it (goldstein.f90:1359-1575, 1788-1814:
+ * depths, drag, the psi equation's band elimination, the island basis);
+ * cgo_set_wind then switches a member from prescribed to solved velocities,
+ * and every cgo_step / cgo_tstepo / captured graph solves them before the
+ * member's tstepo.  cgo_set_velocity switches a member back to prescribed
+ * velocities.  maxi <= 63 and maxi x (maxj+1) <= 8176. */
+typedef struct cgo_momentum_desc {
+    /* per-j metrics the grid descriptor does not carry (maxj+2 entries,
+     * Fortran j; goldstein.f90:928-960): s and c, sine and cosine of
+     * latitude at T points; rcv = 1/cv; rds2 = 2/(dsv(j)+dsv(j-1)) */
+    const double *s, *c, *rcv, *rds2;
+    /* the .psiles file (goldstein.f90:1500-1517): the landmass number of
+     * every psi point, (maxi, 0:maxj), 0 over the ocean */
+    const int32_t *labels;
+    int32_t isles;            /* islands: landmasses other than the first (1518) */
+    const int32_t *npi;       /* (isles) path lengths, the .paths file (1538-1574) */
+    const int32_t *paths;     /* (3, sum npi): lpisl, ipisl, jpisl of each path point, island by island */
+    double adrag;             /* inverse minimum drag in days (ini_gold_nml, goin_GOLD.dat: 2.5) */
+    double rel;               /* velocity relaxation (ini_gold_nml rel: 0.9) */
+} cgo_momentum_desc;
+int cgo_grid_set_momentum(cgo_engine *e, int32_t grid_id, const cgo_momentum_desc *d);
+/* the wind stress of member m, step_goldstein's stressxu_ocn, stressyu_ocn,
+ * stressxv_ocn, stressyv_ocn (maxi, maxj) in N m-2 and scf (ini_gold_nml:
+ * 2.0), goldstein.f90:102-107; the first call allocates the member's arrays
+ * of the solve and switches it on */
+int cgo_set_wind(cgo_engine *e, int32_t m, const double *stressxu, const double *stressyu, const double *stressxv,
+                 const double *stressyv, double scf);
+/* go_u(3, maxi, maxj, maxk) and go_psi(0:maxi, 0:maxj) of step_goldstein
+ * (goldstein.f90:35-37): the velocities of the last step (FP64 members, or
+ * any member with a velocity solve) and the barotropic streamfunction */
+int cgo_download_velocity(cgo_engine *e, int32_t m, double *go_u);
+int cgo_download_psi(cgo_engine *e, int32_t m, double *go_psi);
+
 /* stepping: nsteps ocean steps of every listed member, batched into one
  * launch per kernel (members with BIOGEM run the full coupling order:
  * cgo_tstepo, cgo_bio_step, cgo_tracercoupling) */
@@ -174,7 +213,8 @@ int cgo_step(cgo_engine *e, const int32_t *members, int32_t n, int32_t nsteps);
  *
  * cgo_tstepo replaces tstepo (goldstein.f90:2280-2432) as called from
  * step_goldstein (goldstein.f90:17-38, 235): tstepo_flux + eos, co, and the
- * ts1 = ts hand-over, nsteps times.
+ * ts1 = ts hand-over, nsteps times, each preceded by the velocity solve for
+ * the members that have one (198-233).
  * cgo_bio_step replaces step_biogem (biogem.f90:528-547) for the biology,
  * remineralisation and surface chemistry of the carried species; dtyr is the
  * one given to cgo_bgc_init.
@@ -199,7 +239,7 @@ int cgo_set_flux_iso(cgo_engine *e, int32_t precision, int32_t prepass);    /* 1
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
  * 1 = co, 2 = bgc column + coupling, 3 = flux with the BIOGEM column step
  * running beside it on the second stream, as inside a whole step, 4 = the
- * BIOGEM column step alone) over the listed members, from HIP events on the
+ * BIOGEM column step alone, 5 = the velocity solve) over the listed members, from HIP events on the
  * engine stream; the state is advanced by the timed launches (mode 4
  * rewrites the BIOGEM arrays at every launch with no coupling between them:
  * a member timed with it is for timing only) */

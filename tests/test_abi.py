@@ -39,6 +39,8 @@ def test_library_is_gfx950_code_object():
 
 STRUCTS = {"cgo_grid_desc": engine.GridDesc, "cgo_member_params": engine.MemberParams,
            "cgo_bgc_params": engine.BgcParams, "cgo_diag": engine.Diag}
+# cgo_abi_info reports the first four; the layout check covers every struct
+LAYOUTS = dict(STRUCTS, cgo_momentum_desc=engine.MomentumDesc)
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
@@ -46,7 +48,7 @@ def test_struct_layouts_match_header(tmp_path):
     """Every field of the ctypes mirrors sits at the offset, and every struct
     has the size, that the C compiler gives include/cgocn.h."""
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cgocn.h"', "int main(void){"]
-    for cname, py in STRUCTS.items():
+    for cname, py in LAYOUTS.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f in py._fields_:
             lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
@@ -59,12 +61,12 @@ def test_struct_layouts_match_header(tmp_path):
     for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
         a, b, v = ln.split()
         got[(a, b)] = int(v)
-    for cname, py in STRUCTS.items():
+    for cname, py in LAYOUTS.items():
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
     # the sizes the Fortran bind(C) mirrors are also checked against (test_fortran_host)
-    assert [got[(n, "size")] for n in STRUCTS] == [208, 24, 256, 48]
+    assert [got[(n, "size")] for n in LAYOUTS] == [208, 24, 256, 48, 80]
 
 
 class _FakeLib:
