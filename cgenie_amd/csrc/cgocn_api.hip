@@ -30,7 +30,9 @@ void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, i
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
-void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, hipStream_t);
+void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, int maxcol, int maxn, hipStream_t);
+size_t mom_solve_lds(int N, int n);
+size_t mom_ring_terms(int n);
 }  // namespace cgo
 
 using namespace cgo;
@@ -72,6 +74,8 @@ struct MemberH {
     // a step, graph or kernel of this member may still be running on the
     // engine stream (set when one is enqueued, cleared when the stream drains)
     bool queued = false;
+    // the member has been stepped: rho is tstepo's from now on
+    bool stepped = false;
     MemberDev dev{};
     cgo_member_params p{};
     cgo_bgc_params bp{};
@@ -767,11 +771,14 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
     GridH* g = e->grids[grid_id];
     if (g->mom) return fail(e, CGO_E_STATE, "grid_set_momentum: grid %d already has its velocity solve", grid_id);
     const int mi = g->d.maxi, mj = g->d.maxj, mk = g->d.maxk, JS = mj + 2, KS = mk + 2;
-    // the band solve runs on one wave: a step of the elimination updates the
-    // maxi+1 rows below the pivot, one per lane
-    if (mi + 1 > 64) return fail(e, CGO_E_ARG, "grid_set_momentum: maxi %d above 63", mi);
-    if ((long)mi * (mj + 1) > MOM_MAXN)
-        return fail(e, CGO_E_ARG, "grid_set_momentum: %d psi points above %d", mi * (mj + 1), MOM_MAXN);
+    // the band solve runs on one wave: its 64 lanes hold the pivot row and
+    // the maxi+1 rows below it that a step of the elimination updates
+    if (mi > 62) return fail(e, CGO_E_ARG, "grid_set_momentum: maxi %d above 62", mi);
+    // the solve block's LDS: the psi forcing, the staged products and the
+    // ring of factor rows
+    if (mom_solve_lds(mi * (mj + 1), mi) > 65536)
+        return fail(e, CGO_E_ARG, "grid_set_momentum: %d psi points with maxi %d need %zu bytes of LDS, above 65536",
+                    mi * (mj + 1), mi, mom_solve_lds(mi * (mj + 1), mi));
     if (!(d->adrag > 0.0) || !(d->rel >= 0.0 && d->rel <= 1.0))
         return fail(e, CGO_E_ARG, "grid_set_momentum: adrag %g days, rel %g", d->adrag, d->rel);
     const double* jrows[JT_N];
@@ -794,12 +801,24 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
         return fail(e, CGO_E_ARG, "grid_set_momentum: %s", err.c_str());
     }
     const int n = M->n, N = M->N, nis = std::max(M->isles, 1);
+    {
+        size_t np = 0;
+        for (int q = 0; q < M->isles; q++) np += (size_t)M->npi[q];
+        if (2 * np > mom_ring_terms(n)) {
+            delete M;
+            return fail(e, CGO_E_ARG, "grid_set_momentum: %zu island path points, at most %zu", np, mom_ring_terms(n) / 2);
+        }
+    }
     // the elimination's multipliers by pivot row and the eliminated band by row
     std::vector<double> fwd((size_t)std::max(N - 1, 1) * (n + 1), 0.0), bwd((size_t)N * (n + 2), 0.0);
     for (int i = 1; i <= N - 1; i++)
         for (int q = 1; q <= n + 1 && i + q <= N; q++) fwd[(size_t)(i - 1) * (n + 1) + q - 1] = M->RATM(i + q, q);
+    std::vector<uint64_t> bmask(N, 0);
     for (int i = 1; i <= N; i++)
-        for (int k = 0; k <= n + 1 && i + k <= N; k++) bwd[(size_t)(i - 1) * (n + 2) + k] = M->GAP(i, n + 2 + k);
+        for (int k = 0; k <= n + 1 && i + k <= N; k++) {
+            bwd[(size_t)(i - 1) * (n + 2) + k] = M->GAP(i, n + 2 + k);
+            if (k >= 1 && M->GAP(i, n + 2 + k) != 0.0) bmask[i - 1] |= (uint64_t)1 << (k - 1);
+        }
     // one device buffer: the MomGridDev struct, then its arrays
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -823,6 +842,7 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
                  p_rtv3 = add(M->rtv3.data(), M->rtv3.size() * 8), p_ku = add(M->ku.data(), M->ku.size() * 4),
                  p_mk = add(M->mkv.data(), M->mkv.size() * 4), p_getj = add(M->getj.data(), M->getj.size()),
                  p_fwd = add(fwd.data(), fwd.size() * 8), p_bwd = add(bwd.data(), bwd.size() * 8),
+                 p_bm = add(bmask.data(), bmask.size() * 8),
                  p_ubisl = add(M->ubisl.data(), M->ubisl.size() * 8), p_psisl = add(M->psisl.data(), M->psisl.size() * 8),
                  p_erisl = add(M->erisl.data(), M->erisl.size() * 8), p_npi = add(M->npi.data(), (size_t)nis * 4),
                  p_lp = add(M->lpisl.data(), M->lpisl.size() * 4), p_ip = add(M->ipisl.data(), M->ipisl.size() * 4),
@@ -858,6 +878,7 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
     h.getj = (const uint8_t*)at_(p_getj);
     h.fwd = (const double*)at_(p_fwd);
     h.bwd = (const double*)at_(p_bwd);
+    h.bmask = (const uint64_t*)at_(p_bm);
     h.ubisl = (const double*)at_(p_ubisl);
     h.psisl = (const double*)at_(p_psisl);
     h.erisl = (const double*)at_(p_erisl);
@@ -1057,9 +1078,11 @@ int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
         }
     }
     // rho of the new state at every wet cell, as initialise_goldstein forms
-    // it from the initial or restart state (goldstein.f90:1450, 1754): the
-    // velocity solve's first step reads it
-    {
+    // it from the initial or restart state (goldstein.f90:1450, 1754), for
+    // the velocity solve's first step.  Once the member has stepped, rho is
+    // the last tstepo's, which jbar reads whatever the host does to go_ts in
+    // between (step_goldstein's ts = go_ts, 176-177, leaves rho alone).
+    if (!m->stepped) {
         const GridH* gh = e->grids[m->grid];
         const int mi = g.maxi, mj = g.maxj, mk = g.maxk;
         std::vector<double> r(g.ncell, 0.0);
@@ -1638,7 +1661,7 @@ struct Lists {
     // [p64 list][p32 list][bgc p64][bgc p32][all][velocity solve]
     std::vector<int32_t> v;
     int n[6] = {0, 0, 0, 0, 0, 0}, off[6] = {0, 0, 0, 0, 0, 0};
-    int momN = 0;  // most psi points of a member with a velocity solve
+    int momN = 0, momCol = 0, momBand = 0;  // most psi points, columns and maxi of a member with a velocity solve
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
     int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
     int ncol_atm = 0;  // widest grid of the members with air-sea exchange, 0 if none
@@ -1657,6 +1680,8 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         if (m->mbuf) {
             g[5].push_back(ids[i]);
             ls.momN = std::max(ls.momN, e->grids[m->grid]->mom->N);
+            ls.momCol = std::max(ls.momCol, m->dev.g.ncol);
+            ls.momBand = std::max(ls.momBand, m->dev.g.maxi);
         }
         ls.maxcol[p] = std::max(ls.maxcol[p], m->dev.g.ncol);
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
@@ -1706,7 +1731,7 @@ static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lis
 // the velocity solve of the members that have one (step_goldstein 198-233),
 // which precedes their tstepo
 static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
-    if (ls.n[5]) launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], ls.momN, s);
+    if (ls.n[5]) launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], ls.momN, ls.momCol, ls.momBand, s);
 }
 
 static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
@@ -1796,7 +1821,10 @@ static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
     if (rc) return rc;
     if ((rc = sync_table(e))) return rc;
     if ((rc = upload_lists(e, ls))) return rc;
-    for (int i = 0; i < n; i++) member(e, ids[i])->queued = true;
+    for (int i = 0; i < n; i++) {
+        member(e, ids[i])->queued = true;
+        member(e, ids[i])->stepped = true;
+    }
     return CGO_OK;
 }
 
@@ -1912,7 +1940,10 @@ int cgo_graph_launch(cgo_engine* e, int32_t gid, int32_t nreps) {
     // the graph reads the member table at run time: bring it up to date
     // (e.g. a surface-flux buffer attached since the capture)
     if ((rc = sync_table(e))) return rc;
-    for (int32_t id : gr.ids) member(e, id)->queued = true;
+    for (int32_t id : gr.ids) {
+        member(e, id)->queued = true;
+        member(e, id)->stepped = true;
+    }
     for (int r = 0; r < nreps; r++) HIPCHK(e, hipGraphLaunch(gr.exec, e->s));
     return CGO_OK;
 }

@@ -105,6 +105,7 @@ struct BgcParams {
 //   fwd   [N-1][n+1]: row i, entry q-1 = ratm(i+q, q), the multiplier that
 //         elimination step i applies to row i+q (0 past row N)
 //   bwd   [N][n+2]:   row i, entry k = gap(i, n+2+k) of the eliminated band
+//   bmask [N]: row i, bit k-1 set where gap(i, n+2+k), k >= 1, is not zero
 //   ubisl isles x (2, 0:maxi+1, 0:maxj), psisl isles x (0:maxi, 0:maxj)
 //   erisl (isles, isles+1) column-major, inverted in place (matinv_gold)
 //   paths (mpi, isles): lpisl, ipisl, jpisl; npi (isles)
@@ -124,6 +125,7 @@ struct MomGridDev {
     const uint8_t* getj;
     const double* fwd;
     const double* bwd;
+    const uint64_t* bmask;
     const double* ubisl;
     const double* psisl;
     const double* erisl;
@@ -132,12 +134,11 @@ struct MomGridDev {
     const int32_t* ipisl;
     const int32_t* jpisl;
 };
-constexpr int MOM_MAXN = 8176;   // psi points (maxi x (maxj+1)): with the island values, 64 KiB of LDS
 constexpr int MOM_MAXISL = 16;   // islands
 
 // one member's arrays of the velocity solve, offsets in doubles from its base
 struct MomMemberLayout {
-    size_t gbw, tau, dztau, dztav, u1, ub, psi, bp, sbp, csum, u64, total;
+    size_t gbw, tau, dztau, dztav, u1, ub, psi, bp, sbp, csum, psibc, u64, total;
 };
 __host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int mk, bool own_u64) {
     MomMemberLayout L{};
@@ -158,6 +159,7 @@ __host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int
     L.bp = take((size_t)(mi + 2) * (mj + 2) * (mk + 1));   // bp (0:maxi+1, 0:maxj+1, 0:maxk)
     L.sbp = take((size_t)(mi + 2) * (mj + 2));             // sbp (0:maxi+1, 0:maxj+1)
     L.csum = take(2 * ncol);                               // velc's depth sums of the shear, [2][maxj][maxi]
+    L.psibc = take(MOM_MAXISL);                            // the islands' streamfunction values of the step
     L.u64 = own_u64 ? take(3 * ncell) : 0;                 // FP64 u, v, w of an FP32 member
     L.total = o;
     return L;

@@ -4,342 +4,521 @@
 // ubarsolv 3500-3565, the island correction with island() of
 // goldstein_lib.f90:186-241 and matmult 3470-3492, velc 3568-3679).
 //
-// One workgroup of 256 threads per member.  The column work (bottom pressure,
-// the psi forcing, the shear integral, the barotropic correction, w) is spread
-// over the threads; the banded solve is a chain of dependent operations
-// (elimination step i needs row i final; back substitution subtracts the
-// terms of a row in the reference's order) and runs on one wave, with the
-// psi forcing in LDS and the factor rows streamed from L2 a few rows ahead,
-// while the other three waves integrate the baroclinic shear, which does not
-// depend on it.  Every operation rounds as written (-ffp-contract=off) in
-// the reference's order, so an FP64 member's velocities are the reference
-// arithmetic's.  For an FP32 member the kernel then forms the seven flux
-// coefficient planes from the FP64 velocities, as cgo_set_velocity does.
+// Four launches per step over the members with a solve:
+//   k_mom_cols_a  per column, many blocks per member: jbar's bottom pressure
+//                 and its integral, and velc's thermal-wind (and top-layer
+//                 Ekman) shear integrated up the column with its depth sum
+//   k_mom_solve   one block per member: the psi forcing (wind + JEBAR), the
+//                 banded substitutions, psi and the barotropic velocities,
+//                 the path integrals round the islands and psibc
+//   k_mom_cols_b  velc's barotropic part and relaxation at every open face
+//   k_mom_cols_c  w from continuity and, for an FP32 member, the seven flux
+//                 coefficient planes formed from the FP64 velocities; the
+//                 island correction of the stored ub and psi
+// The substitutions are chains of dependent operations (elimination step i
+// needs row i final; back substitution subtracts a row's terms in the
+// reference's order): one wave runs them while the block's other three waves
+// stream the factor rows from memory into a ring in LDS a chunk ahead.
+// Every operation rounds as written (-ffp-contract=off) in the reference's
+// order, so an FP64 member's velocities are the reference arithmetic's.
 #include "cgocn_dev.h"
 
 namespace cgo {
 
-constexpr int MOM_NT = 256;
-constexpr int MOM_PF = 4;  // factor rows loaded ahead of the elimination and the back substitution
+constexpr int MOM_NT = 256;    // threads of every block
+constexpr int MOM_CH = 32;     // factor rows per chunk of the LDS ring
+constexpr int MOM_RING = 3;    // chunks the ring holds: the one in use, the next, the one loading
+constexpr int MOM_PRE = 8;     // elimination steps whose multipliers are read from LDS ahead
 
 __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__global__ __launch_bounds__(MOM_NT) void k_mom(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list) {
-    extern __shared__ double lds[];
-    const MemberDev& M = mems[list[blockIdx.x]];
-    const MomGridDev& G = *M.mg;
-    const int mi = G.maxi, mj = G.maxj, mk = G.maxk, n = G.n, N = G.N, isles = G.isles;
-    const int ncol = mi * mj, ncell = ncol * mk;
-    const int tid = threadIdx.x;
-    double* gbs = lds;             // the psi forcing, then psi (N)
-    double* erhs = lds + N;        // island integrals, then psibc (isles)
-
-    const MomMemberLayout ML = mom_member_layout(mi, mj, mk, M.prec != 0);
-    double* __restrict__ W = M.mw;
-    double* __restrict__ U = M.u64;
-
-    auto K1 = [&](int i, int j) { return k1at(M.g, i, j); };
-    auto jt = [&](int t, int j) { return G.jt[t * (mj + 2) + j]; };
-    auto rh = [&](int l, int i, int j) { return G.rh[(l - 1) + 3 * (i + (mi + 2) * j)]; };
-    auto drag = [&](int l, int i, int j) { return G.drag[(l - 1) + 2 * (i + (mi + 2) * j)]; };
-    auto ku = [&](int l, int i, int j) { return G.ku[(l - 1) + 2 * ((i - 1) + mi * (j - 1))]; };
-    auto mkk = [&](int i, int j) { return G.mk[i + (mi + 2) * j]; };
-    auto col = [&](int i, int j) { return (i - 1) + mi * (j - 1); };
-    // rho(i, j, k) at a wet cell, i = 0..maxi+1 periodic
-    auto rho = [&](int i, int j, int k) {
+// the arrays of one member's solve, addressed with the reference's indices
+struct MomView {
+    const MemberDev& M;
+    const MomGridDev& G;
+    int mi, mj, mk, n, N, ncol, ncell;
+    MomMemberLayout ML;
+    double* W;
+    double* U;
+    __device__ explicit MomView(const MemberDev& m)
+        : M(m), G(*m.mg), mi(uni(G.maxi)), mj(uni(G.maxj)), mk(uni(G.maxk)), n(uni(G.n)), N(uni(G.N)),
+          ncol(mi * mj), ncell(mi * mj * mk), ML(mom_member_layout(mi, mj, mk, m.prec != 0)), W(m.mw), U(m.u64) {}
+    __device__ int K1(int i, int j) const { return k1at(M.g, i, j); }
+    __device__ double jt(int t, int j) const { return G.jt[t * (mj + 2) + j]; }
+    __device__ double rh(int l, int i, int j) const { return G.rh[(l - 1) + 3 * (i + (mi + 2) * j)]; }
+    __device__ double drag(int l, int i, int j) const { return G.drag[(l - 1) + 2 * (i + (mi + 2) * j)]; }
+    __device__ int ku(int l, int i, int j) const { return G.ku[(l - 1) + 2 * ((i - 1) + mi * (j - 1))]; }
+    __device__ int mkk(int i, int j) const { return G.mk[i + (mi + 2) * j]; }
+    __device__ int col(int i, int j) const { return (i - 1) + mi * (j - 1); }
+    // rho at a wet cell, i = 0..maxi+1 periodic
+    __device__ double rho(int i, int j, int k) const {
         const int ii = i == 0 ? mi : (i == mi + 1 ? 1 : i);
         return M.rho[(size_t)(k - 1) * ncol + col(ii, j)];
-    };
-    auto bp = [&](int i, int j, int k) -> double& { return W[ML.bp + i + (mi + 2) * (j + (mj + 2) * k)]; };
-    auto sbp = [&](int i, int j) -> double& { return W[ML.sbp + i + (mi + 2) * j]; };
-    auto ub = [&](int l, int i, int j) -> double& { return W[ML.ub + (l - 1) + 2 * (i + (mi + 2) * j)]; };
-    auto psi = [&](int i, int j) -> double& { return W[ML.psi + i + (mi + 1) * j]; };
-    auto tau = [&](int l, int i, int j) { return W[ML.tau + (l - 1) + 2 * col(i, j)]; };
-    auto dztau = [&](int l, int i, int j) { return W[ML.dztau + (l - 1) + 2 * col(i, j)]; };
-    auto dztav = [&](int l, int i, int j) { return W[ML.dztav + (l - 1) + 2 * col(i, j)]; };
+    }
+    __device__ double& bp(int i, int j, int k) const { return W[ML.bp + i + (mi + 2) * (j + (mj + 2) * k)]; }
+    __device__ double& sbp(int i, int j) const { return W[ML.sbp + i + (mi + 2) * j]; }
+    __device__ double& ub(int l, int i, int j) const { return W[ML.ub + (l - 1) + 2 * (i + (mi + 2) * j)]; }
+    __device__ double& psi(int i, int j) const { return W[ML.psi + i + (mi + 1) * j]; }
+    __device__ double tau(int l, int i, int j) const { return W[ML.tau + (l - 1) + 2 * col(i, j)]; }
+    __device__ double dztau(int l, int i, int j) const { return W[ML.dztau + (l - 1) + 2 * col(i, j)]; }
+    __device__ double dztav(int l, int i, int j) const { return W[ML.dztav + (l - 1) + 2 * col(i, j)]; }
     // u, v, w at interior points (i = 1..maxi, j = 1..maxj)
-    auto uu = [&](int l, int i, int j, int k) -> double& { return U[(size_t)(l - 1) * ncell + (size_t)(k - 1) * ncol + col(i, j)]; };
-    auto u1 = [&](int l, int i, int j, int k) -> double& { return W[ML.u1 + (size_t)(l - 1) * ncell + (size_t)(k - 1) * ncol + col(i, j)]; };
-    auto csum = [&](int l, int i, int j) -> double& { return W[ML.csum + (size_t)(l - 1) * ncol + col(i, j)]; };
-    const double dphi = G.dphi, rdphi = G.rdphi;
+    __device__ double& uu(int l, int i, int j, int k) const {
+        return U[(size_t)(l - 1) * ncell + (size_t)(k - 1) * ncol + col(i, j)];
+    }
+    __device__ double& u1(int l, int i, int j, int k) const {
+        return W[ML.u1 + (size_t)(l - 1) * ncell + (size_t)(k - 1) * ncol + col(i, j)];
+    }
+    __device__ double& csum(int l, int i, int j) const { return W[ML.csum + (size_t)(l - 1) * ncol + col(i, j)]; }
+    __device__ double psibc(int isl) const { return W[ML.psibc + isl - 1]; }
+};
 
-    // ---- jbar (3225-3248): bottom pressure from the bottom cell up, and its
-    // integral above the level mk of the deepest wet neighbour
-    for (int c = tid; c < ncol; c += MOM_NT) {
-        const int i = c % mi + 1, j = c / mi + 1, kb = K1(i, j);
-        if (kb > mk) continue;
-        double b = bp(i, j, kb);  // never written: zero
-        for (int k = kb + 1; k <= mk; k++) {
-            b = b - (rho(i, j, k) + rho(i, j, k - 1)) * G.dza[k - 1] * 0.5;
-            bp(i, j, k) = b;
-            if (i == 1 && kb < mk) bp(mi + 1, j, k) = b;  // periodic copy (3249-3255)
+// ------------------------------------------------------------ per column, before the solve
+__global__ __launch_bounds__(MOM_NT) void k_mom_cols_a(const MemberDev* __restrict__ mems,
+                                                       const int32_t* __restrict__ list) {
+    const MomView V(mems[list[blockIdx.y]]);
+    const int c = blockIdx.x * MOM_NT + threadIdx.x;
+    if (c >= V.ncol) return;
+    const int mi = V.mi, mk = V.mk, i = c % mi + 1, j = c / mi + 1, kb = V.K1(i, j);
+    if (kb > mk) return;
+    const MomGridDev& G = V.G;
+    // jbar (3225-3255): bottom pressure from the bottom cell up (zero there,
+    // never written), the periodic copy, and its integral above mk
+    double b = V.bp(i, j, kb);
+    for (int k = kb + 1; k <= mk; k++) {
+        b = b - (V.rho(i, j, k) + V.rho(i, j, k - 1)) * G.dza[k - 1] * 0.5;
+        V.bp(i, j, k) = b;
+        if (i == 1 && kb < mk) V.bp(mi + 1, j, k) = b;
+    }
+    if (V.mkk(i, j) > 0) {
+        double s = 0;
+        for (int k = V.mkk(i, j) + 1; k <= mk; k++) s = s + V.bp(i, j, k) * G.dz[k];
+        V.sbp(i, j) = s;
+    }
+    if (i == 1) V.sbp(mi + 1, j) = V.sbp(1, j);
+    // velc (3574-3640): the shear integrated up from zero at the bottom
+    double sum0 = 0, sum1 = 0, dp0 = 0, dp1 = 0, up0 = 0, up1 = 0;
+    const double sj = V.jt(JT_S, j), svj = V.jt(JT_SV, j), cj = V.jt(JT_C, j), cvj = V.jt(JT_CV, j);
+    const double rdphi = G.rdphi, rtvij = G.rtv[V.col(i, j)], rtv3ij = G.rtv3[V.col(i, j)];
+    const double d1 = V.drag(1, i, j), d2 = V.drag(2, i, j);
+    auto K1 = [&](int a, int b2) { return V.K1(a, b2); };
+    auto rho = [&](int a, int b2, int k) { return V.rho(a, b2, k); };
+    for (int k = kb; k <= mk; k++) {
+        double tv1, tv2, tv4, tv5;
+        if (K1(i + 1, j) > k) {
+            tv1 = 0;
+            tv2 = 0;
+        } else {
+            tv2 = -(rho(i + 1, j, k) - rho(i, j, k)) * rdphi * V.jt(JT_RC, j);
+            if (max(max(K1(i, j - 1), K1(i, j + 1)), max(K1(i + 1, j - 1), K1(i + 1, j + 1))) <= k)
+                tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j - 1, k) + rho(i, j + 1, k) - rho(i, j - 1, k)) *
+                      V.jt(JT_RDS2, j) * 0.25;
+            else if (max(K1(i, j - 1), K1(i + 1, j - 1)) <= k)
+                tv1 = -cj * (rho(i + 1, j, k) - rho(i + 1, j - 1, k) + rho(i, j, k) - rho(i, j - 1, k)) *
+                      V.jt(JT_RDSV, j - 1) * 0.5;
+            else if (max(K1(i, j + 1), K1(i + 1, j + 1)) <= k)
+                tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j, k) + rho(i, j + 1, k) - rho(i, j, k)) *
+                      V.jt(JT_RDSV, j) * 0.5;
+            else
+                tv1 = 0;
         }
-        if (mkk(i, j) > 0) {
-            double s = 0;
-            for (int k = mkk(i, j) + 1; k <= mk; k++) s = s + bp(i, j, k) * G.dz[k];
-            sbp(i, j) = s;
+        if (K1(i, j + 1) > k) {
+            tv4 = 0;
+            tv5 = 0;
+        } else {
+            tv4 = -cvj * (rho(i, j + 1, k) - rho(i, j, k)) * V.jt(JT_RDSV, j);
+            if (max(max(K1(i - 1, j), K1(i - 1, j + 1)), max(K1(i + 1, j), K1(i + 1, j + 1))) <= k)
+                tv5 = -(rho(i + 1, j + 1, k) - rho(i - 1, j + 1, k) + rho(i + 1, j, k) - rho(i - 1, j, k)) * rdphi *
+                      0.25 * V.jt(JT_RCV, j);
+            else if (max(K1(i - 1, j), K1(i - 1, j + 1)) <= k)
+                tv5 = -(rho(i, j + 1, k) - rho(i - 1, j + 1, k) + rho(i, j, k) - rho(i - 1, j, k)) * rdphi * 0.5 *
+                      V.jt(JT_RCV, j);
+            else if (max(K1(i + 1, j), K1(i + 1, j + 1)) <= k)
+                tv5 = -(rho(i + 1, j + 1, k) - rho(i, j + 1, k) + rho(i + 1, j, k) - rho(i, j, k)) * rdphi * 0.5 *
+                      V.jt(JT_RCV, j);
+            else
+                tv5 = 0;
         }
-        if (i == 1) sbp(mi + 1, j) = sbp(1, j);
+        if (k == mk) {
+            if (K1(i + 1, j) <= k) {
+                tv1 = tv1 - V.dztau(2, i, j);
+                tv2 = tv2 - V.dztau(1, i, j);
+            }
+            if (K1(i, j + 1) <= k) {
+                tv4 = tv4 - V.dztav(2, i, j);
+                tv5 = tv5 - V.dztav(1, i, j);
+            }
+        }
+        const double dz0 = -(sj * tv1 + d1 * tv2) * rtvij;
+        const double dz1 = -(d2 * tv4 - svj * tv5) * rtv3ij;
+        double x0, x1;
+        if (k == kb) {
+            x0 = 0;
+            x1 = 0;
+        } else {
+            x0 = up0 + G.dza[k - 1] * (dz0 + dp0) * 0.5;
+            sum0 = sum0 + G.dz[k] * x0;
+            x1 = up1 + G.dza[k - 1] * (dz1 + dp1) * 0.5;
+            sum1 = sum1 + G.dz[k] * x1;
+        }
+        V.uu(1, i, j, k) = x0;
+        V.uu(2, i, j, k) = x1;
+        up0 = x0;
+        up1 = x1;
+        dp0 = dz0;
+        dp1 = dz1;
+    }
+    V.csum(1, i, j) = sum0;
+    V.csum(2, i, j) = sum1;
+}
+
+// ------------------------------------------------------------ the band solve
+// ubarsolv's substitutions (goldstein.f90:3508-3522) on the psi forcing g in
+// LDS (row i at g[i-1]), which ends as the solution.  Called by the whole
+// block: wave 0 runs the chains, waves 1-3 fill the ring with factor rows
+// one chunk ahead, and the block synchronises once per chunk.
+//
+// Forward: elimination step i subtracts ratm(i+q, q) x_i from rows i+1..i+n+1.
+// Lane (j-1) mod 64 keeps row j in a register from the step it first changes
+// until it becomes the pivot (a row is live for at most n+2 < 64 steps, so
+// the 64 lanes hold every live row); step i reads the pivot from its lane and
+// every lane holding a row in the band updates it.  Ring row r of a chunk
+// holds the multipliers of one step.
+//
+// Back substitution: row i subtracts gap(i, n+2+k) x_{i+k} for k = 1, 2, ..
+// in that order, then divides by the diagonal.  A term with a zero
+// coefficient subtracts a signed zero and leaves the value as it is, so only
+// the others are taken (bit k-1 of the row's mask): the k = 1 term from the
+// row just solved, and the products for k >= 2, which need only rows solved
+// earlier, formed one row ahead by the lanes and packed in order into a
+// zero-padded LDS buffer that the chain reads.  Dividing by a diagonal of
+// exactly 1 (land points) is the identity and is skipped.  Ring row r of a
+// chunk holds a row of the eliminated band (n+2 values) and its mask.
+__device__ __forceinline__ void band_solve(double* g, double* pbuf, double* ring, const MomGridDev& G, int tid) {
+    const int n = uni(G.n), N = uni(G.N), W1 = n + 1, W2 = n + 2, RS = n + 3;
+    const int lane = tid & 63;
+    const double* __restrict__ fwd = G.fwd;
+    const double* __restrict__ bwd = G.bwd;
+    const uint64_t* __restrict__ bmask = G.bmask;
+    constexpr int PT = 12;  // ring values a producer thread moves per chunk (>= 32 x 64 / 192)
+    auto slot = [&](int c) { return ring + (size_t)(c % MOM_RING) * MOM_CH * RS; };
+    // producers: the multipliers of steps [c CH + 1, c CH + CH]
+    auto produce_f = [&](int c) {
+        double* dst = slot(c);
+        const int r0 = c * MOM_CH, nr = min(MOM_CH, N - 1 - r0), tot = nr * W1;
+        double t[PT];
+#pragma unroll
+        for (int e = 0; e < PT; e++) {
+            const int x = (tid - 64) + e * (MOM_NT - 64);
+            t[e] = x < tot ? fwd[(size_t)r0 * W1 + x] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < PT; e++) {
+            const int x = (tid - 64) + e * (MOM_NT - 64);
+            if (x < tot) dst[(x / W1) * RS + x % W1] = t[e];
+        }
+    };
+    // producers: rows i = N-1 - c CH - t of the eliminated band and their masks
+    auto produce_b = [&](int c) {
+        double* dst = slot(c);
+        const int ihi = N - 1 - c * MOM_CH, nr = min(MOM_CH, ihi), tot = nr * (W2 + 1);
+        double t[PT + 1];
+#pragma unroll
+        for (int e = 0; e < PT + 1; e++) {
+            const int x = (tid - 64) + e * (MOM_NT - 64);
+            const int r = x / (W2 + 1), k = x % (W2 + 1), i = ihi - r;
+            double val = 0.0;
+            if (x < tot) {
+                if (k < W2)
+                    val = bwd[(size_t)(i - 1) * W2 + k];
+                else
+                    val = __longlong_as_double((long long)bmask[i - 1]);
+            }
+            t[e] = val;
+        }
+#pragma unroll
+        for (int e = 0; e < PT + 1; e++) {
+            const int x = (tid - 64) + e * (MOM_NT - 64);
+            if (x < tot) dst[(x / (W2 + 1)) * RS + x % (W2 + 1)] = t[e];
+        }
+    };
+
+    // ---- forward elimination
+    const int nchf = uni((N - 1 + MOM_CH - 1) / MOM_CH);
+    if (tid >= 64) {
+        produce_f(0);
+        if (nchf > 1) produce_f(1);
+    }
+    double v = 0.0, vn = 0.0;
+    if (tid < 64) v = lane < N ? g[lane] : 0.0;
+    __syncthreads();
+    for (int c = 0; c < nchf; c++) {
+        c = uni(c);
+        if (tid < 64) {
+            const double* rr = slot(c);
+            const int sb = (c & ~1) * MOM_CH;             // the 64-step block's first step, minus one
+            const int pb = (c & 1) * MOM_CH;              // the pivot lane of the chunk's first step
+            if ((c & 1) == 0) vn = g[min(sb + lane + 64, N - 1)];   // row sb+L+65's forcing, untouched so far
+            double r[MOM_PRE];
+#pragma unroll
+            for (int d = 0; d < MOM_PRE; d++) {
+                const int q = (lane - (pb + d)) & 63;
+                r[d] = rr[d * RS + (q >= 1 && q <= n + 1 ? q - 1 : 0)];
+            }
+#pragma unroll
+            for (int d = 0; d < MOM_CH; d++) {
+                const int p = pb + d, s = c * MOM_CH + d + 1;
+                const double rd = r[d % MOM_PRE];
+                if (d + MOM_PRE < MOM_CH) {
+                    const int q2 = (lane - (p + MOM_PRE)) & 63;
+                    r[d % MOM_PRE] = rr[(d + MOM_PRE) * RS + (q2 >= 1 && q2 <= n + 1 ? q2 - 1 : 0)];
+                }
+                if (s <= N - 1) {
+                    const double x = readlane_d(v, p);
+                    const int q = (lane - p) & 63;
+                    const bool act = q >= 1 && q <= n + 1 && s + q <= N;
+                    const double upd = v - rd * x;
+                    v = act ? upd : v;
+                    if (lane == 0) g[s - 1] = x;
+                    if (lane == p) v = sb + lane + 65 <= N ? vn : 0.0;
+                }
+            }
+        } else if (c + 2 < nchf) {
+            produce_f(c + 2);
+        }
+        __syncthreads();
+    }
+    // ---- back substitution
+    const int nchb = uni((N - 1 + MOM_CH - 1) / MOM_CH);   // rows N-1 .. 1
+    if (tid >= 64) {
+        produce_b(0);
+        if (nchb > 1) produce_b(1);
+    }
+    double xp = 0.0;
+    if (tid < 64) {
+        // row N is final after the last step: it sits in lane (N-1) mod 64
+        xp = readlane_d(v, uni((N - 1) & 63));
+        const double dN = bwd[(size_t)(N - 1) * W2];
+        if (dN != 1.0) xp = xp / dN;
+        if (lane == 0) g[N - 1] = xp;
+        pbuf[lane] = 0.0;
+        pbuf[64 + lane] = 0.0;
     }
     __syncthreads();
+    const int ql = min(lane + 1, n + 1);
+    // ring row of band row i
+    auto brow = [&](int i) { return slot((N - 1 - i) / MOM_CH) + (size_t)((N - 1 - i) % MOM_CH) * RS; };
+    auto maskof = [&](const double* r) { return (uint64_t)__double_as_longlong(r[W2]); };
+    // carried from one row to the next: row i's mask, k = 1 coefficient,
+    // diagonal and forward value; row i-1's mask and this lane's coefficient
+    uint64_t m_i = 0, m_p = 0;
+    double a1_i = 0.0, d_i = 1.0, g_i = 0.0, c_p = 0.0;
+    if (tid < 64) {
+        const double* r0 = brow(N - 1);
+        m_i = maskof(r0);
+        a1_i = r0[1];
+        d_i = r0[0];
+        g_i = g[N - 2];
+        if (N - 2 >= 1) {
+            const double* r1 = brow(N - 2);
+            m_p = maskof(r1);
+            c_p = r1[ql];
+        }
+    }
+    for (int c = 0; c < nchb; c++) {
+        c = uni(c);
+        if (tid < 64) {
+            const int ihi = N - 1 - c * MOM_CH, ilo = max(1, ihi - MOM_CH + 1);
+            for (int i = ihi; i >= ilo; i--) {
+                i = uni(i);
+                const double* pc = pbuf + (i & 1) * 64;
+                const int pad = uni((__popcll(m_i & ~(uint64_t)1) + 7) & ~7);
+                double pa[8], pn[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) pa[e] = pc[e];
+                // the next rows' data, read now and used in the next iteration
+                uint64_t m_n = 0, m_pn = 0;
+                double a1_n = 0.0, d_n = 1.0, g_n = 0.0, c_pn = 0.0;
+                if (i >= 2) {
+                    const double* rn = brow(i - 1);
+                    m_n = maskof(rn);
+                    a1_n = rn[1];
+                    d_n = rn[0];
+                    g_n = g[i - 2];
+                }
+                if (i >= 3) {
+                    const double* rnn = brow(i - 2);
+                    m_pn = maskof(rnn);
+                    c_pn = rnn[ql];
+                }
+                const double xk = g[min(i + lane - 1, N - 1)];   // x_{i-1+q}, q = lane + 1 (rows i+1.. for q >= 2)
+                // the chain of row i
+                double t = g_i;
+                if (m_i & 1) t = t - a1_i * xp;
+                for (int q = 0; q < pad; q += 8) {
+                    if (q + 8 < pad) {
+#pragma unroll
+                        for (int e = 0; e < 8; e++) pn[e] = pc[q + 8 + e];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 8; e++) t = t - pa[e];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) pa[e] = pn[e];
+                }
+                if (d_i != 1.0) t = t / d_i;
+                if (lane == 0) g[i - 1] = t;
+                xp = t;
+                // products of row i-1 for k >= 2 into its buffer
+                if (i >= 2) {
+                    const uint64_t m = m_p & ~(uint64_t)1;
+                    const bool set = (m >> lane) & 1;
+                    const int rank = __popcll(m & (((uint64_t)1 << lane) - 1));
+                    const int cnt = __popcll(m), ppad = (cnt + 7) & ~7;
+                    double* pb = pbuf + ((i - 1) & 1) * 64;
+                    if (lane >= cnt && lane < ppad) pb[lane] = 0.0;   // the padding (rank < cnt: no overlap)
+                    if (set) pb[rank] = c_p * xk;
+                }
+                m_i = m_n;
+                a1_i = a1_n;
+                d_i = d_n;
+                g_i = g_n;
+                m_p = m_pn;
+                c_p = c_pn;
+            }
+        } else if (c + 2 < nchb) {
+            produce_b(c + 2);
+        }
+        __syncthreads();
+    }
+}
 
-    // ---- the psi forcing: the wind part (wind) plus the torque of the
-    // bottom pressure where the four columns round a psi point differ in depth
+// ------------------------------------------------------------ one block per member
+__global__ __launch_bounds__(MOM_NT) void k_mom_solve(const MemberDev* __restrict__ mems,
+                                                      const int32_t* __restrict__ list) {
+    extern __shared__ double lds[];
+    const MomView V(mems[list[blockIdx.x]]);
+    const MomGridDev& G = V.G;
+    const int mi = V.mi, mj = V.mj, n = V.n, N = V.N, isles = G.isles, tid = threadIdx.x;
+    double* gbs = lds;                          // the psi forcing, then psi (N)
+    double* erhs = lds + N;                     // island integrals, then psibc (isles)
+    double* pbuf = erhs + MOM_MAXISL;           // back substitution's staged products (2 x 64)
+    double* ring = pbuf + 128;                  // factor rows (MOM_RING x MOM_CH x (n+3)), then island terms
+    const double rdphi = G.rdphi, dphi = G.dphi;
+
+    // ---- the psi forcing: the wind part (wind) plus the torque of the bottom
+    // pressure where the four columns round a psi point differ in depth
     // (jbar 3256-3313)
     for (int l = tid + 1; l <= N; l += MOM_NT) {
         const int i = (l - 1) % n + 1, j = (l - 1) / n;
-        double g = W[ML.gbw + l - 1];
-        if (j >= 1 && j <= mj - 1 && G.getj[col(i, j)]) {
+        double g = V.W[V.ML.gbw + l - 1];
+        if (j >= 1 && j <= mj - 1 && G.getj[V.col(i, j)]) {
             const int ip1 = i % mi + 1;
             auto span = [&](int k0, int k1_, int ii, int jj) {
                 double t = 0;
-                for (int k = k0; k <= k1_; k++) t = t + bp(ii, jj, k) * G.dz[k];
+                for (int k = k0; k <= k1_; k++) t = t + V.bp(ii, jj, k) * G.dz[k];
                 return t;
             };
-            double tv1 = span(ku(2, ip1, j), mkk(ip1, j + 1), ip1, j + 1);
-            double tv2 = span(ku(2, ip1, j), mkk(ip1, j), ip1, j);
-            double tv3 = span(ku(2, i, j), mkk(i, j + 1), i, j + 1);
-            double tv4 = span(ku(2, i, j), mkk(i, j), i, j);
-            g = g + ((tv3 + sbp(i, j + 1) - tv4 - sbp(i, j)) * rh(2, i, j) -
-                     (tv1 + sbp(ip1, j + 1) - tv2 - sbp(ip1, j)) * rh(2, ip1, j)) *
-                        rdphi * jt(JT_RDSV, j);
-            tv1 = span(ku(1, i, j + 1), mkk(ip1, j + 1), ip1, j + 1);
-            tv2 = span(ku(1, i, j), mkk(ip1, j), ip1, j);
-            tv3 = span(ku(1, i, j + 1), mkk(i, j + 1), i, j + 1);
-            tv4 = span(ku(1, i, j), mkk(i, j), i, j);
-            g = g + ((tv1 + sbp(ip1, j + 1) - tv3 - sbp(i, j + 1)) * rh(1, i, j + 1) -
-                     (tv2 + sbp(ip1, j) - tv4 - sbp(i, j)) * rh(1, i, j)) *
-                        rdphi * jt(JT_RDSV, j);
+            double tv1 = span(V.ku(2, ip1, j), V.mkk(ip1, j + 1), ip1, j + 1);
+            double tv2 = span(V.ku(2, ip1, j), V.mkk(ip1, j), ip1, j);
+            double tv3 = span(V.ku(2, i, j), V.mkk(i, j + 1), i, j + 1);
+            double tv4 = span(V.ku(2, i, j), V.mkk(i, j), i, j);
+            g = g + ((tv3 + V.sbp(i, j + 1) - tv4 - V.sbp(i, j)) * V.rh(2, i, j) -
+                     (tv1 + V.sbp(ip1, j + 1) - tv2 - V.sbp(ip1, j)) * V.rh(2, ip1, j)) *
+                        rdphi * V.jt(JT_RDSV, j);
+            tv1 = span(V.ku(1, i, j + 1), V.mkk(ip1, j + 1), ip1, j + 1);
+            tv2 = span(V.ku(1, i, j), V.mkk(ip1, j), ip1, j);
+            tv3 = span(V.ku(1, i, j + 1), V.mkk(i, j + 1), i, j + 1);
+            tv4 = span(V.ku(1, i, j), V.mkk(i, j), i, j);
+            g = g + ((tv1 + V.sbp(ip1, j + 1) - tv3 - V.sbp(i, j + 1)) * V.rh(1, i, j + 1) -
+                     (tv2 + V.sbp(ip1, j) - tv4 - V.sbp(i, j)) * V.rh(1, i, j)) *
+                        rdphi * V.jt(JT_RDSV, j);
         }
         gbs[l - 1] = g;
     }
     __syncthreads();
 
-    if (tid < 64) {
-        // ---- ubarsolv (3508-3522) on wave 0: elimination step i takes row i
-        // (final) times the multiplier ratm(i+q, q) from row i+q, lane q-1
-        const int lane = tid, q = lane + 1;
-        const bool act = q <= n + 1;
-        double rc[MOM_PF], rn[MOM_PF];
-#pragma unroll
-        for (int s = 0; s < MOM_PF; s++) {
-            const int i = 1 + s;
-            rc[s] = (act && i <= N - 1) ? G.fwd[(size_t)(i - 1) * (n + 1) + q - 1] : 0.0;
-        }
-        for (int i0 = 1; i0 <= N - 1; i0 += MOM_PF) {
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) {
-                const int i = i0 + MOM_PF + s;
-                rn[s] = (act && i <= N - 1) ? G.fwd[(size_t)(i - 1) * (n + 1) + q - 1] : 0.0;
-            }
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) {
-                const int i = i0 + s;
-                if (i <= N - 1) {
-                    const double x = gbs[i - 1];
-                    if (act && i + q <= N) gbs[i + q - 1] = gbs[i + q - 1] - rc[s] * x;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) rc[s] = rn[s];
-        }
-        // back substitution: row i subtracts gap(i, n+2+k) G(i+k) for k = 1, 2, ..
-        // in that order, then divides by the diagonal; lane k-1 forms the
-        // product of term k >= 2 (rows solved earlier), the wave subtracts
-        // them in turn after the term of the row just solved
-        const int W2 = n + 2;
-        double tp = gbs[N - 1] / G.bwd[(size_t)(N - 1) * W2];
-        if (lane == 0) gbs[N - 1] = tp;
-        double bc[MOM_PF], b0c[MOM_PF], b1c[MOM_PF], bn[MOM_PF], b0n[MOM_PF], b1n[MOM_PF];
-        const int kl = lane + 1;
-#pragma unroll
-        for (int s = 0; s < MOM_PF; s++) {
-            const int i = N - 1 - s;
-            const bool ok = i >= 1;
-            bc[s] = (ok && kl >= 2 && kl <= n + 1) ? G.bwd[(size_t)(i - 1) * W2 + kl] : 0.0;
-            b0c[s] = ok ? G.bwd[(size_t)(i - 1) * W2] : 1.0;
-            b1c[s] = ok ? G.bwd[(size_t)(i - 1) * W2 + 1] : 0.0;
-        }
-        for (int i0 = N - 1; i0 >= 1; i0 -= MOM_PF) {
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) {
-                const int i = i0 - MOM_PF - s;
-                const bool ok = i >= 1;
-                bn[s] = (ok && kl >= 2 && kl <= n + 1) ? G.bwd[(size_t)(i - 1) * W2 + kl] : 0.0;
-                b0n[s] = ok ? G.bwd[(size_t)(i - 1) * W2] : 1.0;
-                b1n[s] = ok ? G.bwd[(size_t)(i - 1) * W2 + 1] : 0.0;
-            }
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) {
-                const int i = i0 - s;
-                if (i >= 1) {
-                    const int km = min(n + 1, N - i);
-                    const double gk = gbs[min(i + kl - 1, N - 1)];
-                    const double p = (kl >= 2 && kl <= km) ? bc[s] * gk : 0.0;
-                    double t = gbs[i - 1] - b1c[s] * tp;
-                    for (int k = 2; k <= km; k++) t = t - readlane_d(p, k - 1);
-                    t = t / b0c[s];
-                    if (lane == 0) gbs[i - 1] = t;
-                    tp = t;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < MOM_PF; s++) {
-                bc[s] = bn[s];
-                b0c[s] = b0n[s];
-                b1c[s] = b1n[s];
-            }
-        }
-    } else {
-        // ---- velc (3574-3640) on the other waves: the thermal-wind shear
-        // (and the wind's Ekman shear in the top layer) integrated up each
-        // column from zero at the bottom, and its depth sum
-        const double* dza = G.dza;
-        for (int c = tid - 64; c < ncol; c += MOM_NT - 64) {
-            const int i = c % mi + 1, j = c / mi + 1;
-            double sum0 = 0, sum1 = 0, dp0 = 0, dp1 = 0, up0 = 0, up1 = 0;
-            const double sj = jt(JT_S, j), svj = jt(JT_SV, j), cj = jt(JT_C, j), cvj = jt(JT_CV, j);
-            const double rtvij = G.rtv[col(i, j)], rtv3ij = G.rtv3[col(i, j)];
-            const double d1 = drag(1, i, j), d2 = drag(2, i, j);
-            for (int k = K1(i, j); k <= mk; k++) {
-                double tv1, tv2, tv4, tv5;
-                if (K1(i + 1, j) > k) {
-                    tv1 = 0;
-                    tv2 = 0;
-                } else {
-                    tv2 = -(rho(i + 1, j, k) - rho(i, j, k)) * rdphi * jt(JT_RC, j);
-                    if (max(max(K1(i, j - 1), K1(i, j + 1)), max(K1(i + 1, j - 1), K1(i + 1, j + 1))) <= k)
-                        tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j - 1, k) + rho(i, j + 1, k) - rho(i, j - 1, k)) *
-                              jt(JT_RDS2, j) * 0.25;
-                    else if (max(K1(i, j - 1), K1(i + 1, j - 1)) <= k)
-                        tv1 = -cj * (rho(i + 1, j, k) - rho(i + 1, j - 1, k) + rho(i, j, k) - rho(i, j - 1, k)) *
-                              jt(JT_RDSV, j - 1) * 0.5;
-                    else if (max(K1(i, j + 1), K1(i + 1, j + 1)) <= k)
-                        tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j, k) + rho(i, j + 1, k) - rho(i, j, k)) *
-                              jt(JT_RDSV, j) * 0.5;
-                    else
-                        tv1 = 0;
-                }
-                if (K1(i, j + 1) > k) {
-                    tv4 = 0;
-                    tv5 = 0;
-                } else {
-                    tv4 = -cvj * (rho(i, j + 1, k) - rho(i, j, k)) * jt(JT_RDSV, j);
-                    if (max(max(K1(i - 1, j), K1(i - 1, j + 1)), max(K1(i + 1, j), K1(i + 1, j + 1))) <= k)
-                        tv5 = -(rho(i + 1, j + 1, k) - rho(i - 1, j + 1, k) + rho(i + 1, j, k) - rho(i - 1, j, k)) * rdphi *
-                              0.25 * jt(JT_RCV, j);
-                    else if (max(K1(i - 1, j), K1(i - 1, j + 1)) <= k)
-                        tv5 = -(rho(i, j + 1, k) - rho(i - 1, j + 1, k) + rho(i, j, k) - rho(i - 1, j, k)) * rdphi * 0.5 *
-                              jt(JT_RCV, j);
-                    else if (max(K1(i + 1, j), K1(i + 1, j + 1)) <= k)
-                        tv5 = -(rho(i + 1, j + 1, k) - rho(i, j + 1, k) + rho(i + 1, j, k) - rho(i, j, k)) * rdphi * 0.5 *
-                              jt(JT_RCV, j);
-                    else
-                        tv5 = 0;
-                }
-                if (k == mk) {
-                    if (K1(i + 1, j) <= k) {
-                        tv1 = tv1 - dztau(2, i, j);
-                        tv2 = tv2 - dztau(1, i, j);
-                    }
-                    if (K1(i, j + 1) <= k) {
-                        tv4 = tv4 - dztav(2, i, j);
-                        tv5 = tv5 - dztav(1, i, j);
-                    }
-                }
-                const double dz0 = -(sj * tv1 + d1 * tv2) * rtvij;
-                const double dz1 = -(d2 * tv4 - svj * tv5) * rtv3ij;
-                double x0, x1;
-                if (k == K1(i, j)) {
-                    x0 = 0;
-                    x1 = 0;
-                } else {
-                    x0 = up0 + dza[k - 1] * (dz0 + dp0) * 0.5;
-                    sum0 = sum0 + G.dz[k] * x0;
-                    x1 = up1 + dza[k - 1] * (dz1 + dp1) * 0.5;
-                    sum1 = sum1 + G.dz[k] * x1;
-                }
-                uu(1, i, j, k) = x0;
-                uu(2, i, j, k) = x1;
-                up0 = x0;
-                up1 = x1;
-                dp0 = dz0;
-                dp1 = dz1;
-            }
-            csum(1, i, j) = sum0;
-            csum(2, i, j) = sum1;
-        }
-    }
-    __syncthreads();
+    band_solve(gbs, pbuf, ring, G, tid);
 
     // ---- psi and the barotropic velocities from the solution (3524-3565),
     // each boundary copy written by the thread that forms its source
     for (int l = tid + 1; l <= N; l += MOM_NT) {
         const int i = (l - 1) % n + 1, j = (l - 1) / n;
         const double p = gbs[l - 1];
-        psi(i, j) = p;
-        if (i == mi) psi(0, j) = p;
+        V.psi(i, j) = p;
+        if (i == mi) V.psi(0, j) = p;
         if (j >= 1) {
-            const double x = -rh(1, i, j) * jt(JT_C, j) * (p - gbs[l - 1 - n]) * jt(JT_RDS, j);
-            ub(1, i, j) = x;
-            if (i == mi) ub(1, 0, j) = x;
-            if (i == 1) ub(1, mi + 1, j) = x;
+            const double x = -V.rh(1, i, j) * V.jt(JT_C, j) * (p - gbs[l - 1 - n]) * V.jt(JT_RDS, j);
+            V.ub(1, i, j) = x;
+            if (i == mi) V.ub(1, 0, j) = x;
+            if (i == 1) V.ub(1, mi + 1, j) = x;
         }
         double y;
         if (j >= 1 && j <= mj - 1) {
             const double pw = gbs[(i == 1 ? mi : i - 1) + j * n - 1];
-            y = rh(2, i, j) * (p - pw) * jt(JT_RCV, j) * rdphi;
+            y = V.rh(2, i, j) * (p - pw) * V.jt(JT_RCV, j) * rdphi;
         } else {
             y = 0.0;
         }
-        ub(2, i, j) = y;
-        if (i == 1) ub(2, mi + 1, j) = y;
-        if (i == mi) ub(2, 0, j) = y;
+        V.ub(2, i, j) = y;
+        if (i == 1) V.ub(2, mi + 1, j) = y;
+        if (i == mi) V.ub(2, 0, j) = y;
     }
     __syncthreads();
+    if (isles == 0) return;
 
     // ---- the path integral round every island of the momentum balance with
-    // wind and bottom-pressure torque (island, indj = 1), then the island
-    // streamfunction values that cancel it (209-218)
-    if (tid < isles) {
-        const int isl = tid + 1;
-        double e = 0.0;
-        for (int p = 1; p <= G.npi[isl - 1]; p++) {
+    // wind and bottom-pressure torque (island, indj = 1): each point's two
+    // terms formed in parallel into LDS, then added in the path's order
+    int base = 0;
+    for (int isl = 1; isl <= isles; isl++) {
+        const int np = G.npi[isl - 1];
+        for (int p = tid + 1; p <= np; p += MOM_NT) {
             const int q = (p - 1) + G.mpi * (isl - 1);
             const int lpi = G.lpisl[q], ipi = G.ipisl[q], jpi = G.jpisl[q];
             const int al = abs(lpi);
             double cor;
             if (al == 1)
-                cor = -jt(JT_S, jpi) * 0.25 * (ub(2, ipi, jpi) + ub(2, ipi + 1, jpi) + ub(2, ipi, jpi - 1) + ub(2, ipi + 1, jpi - 1));
+                cor = -V.jt(JT_S, jpi) * 0.25 *
+                      (V.ub(2, ipi, jpi) + V.ub(2, ipi + 1, jpi) + V.ub(2, ipi, jpi - 1) + V.ub(2, ipi + 1, jpi - 1));
             else
-                cor = jt(JT_SV, jpi) * 0.25 * (ub(1, ipi - 1, jpi) + ub(1, ipi, jpi) + ub(1, ipi - 1, jpi + 1) + ub(1, ipi, jpi + 1));
+                cor = V.jt(JT_SV, jpi) * 0.25 *
+                      (V.ub(1, ipi - 1, jpi) + V.ub(1, ipi, jpi) + V.ub(1, ipi - 1, jpi + 1) + V.ub(1, ipi, jpi + 1));
             const double sgn = lpi >= 0 ? 1.0 : -1.0;
-            const double rhv = rh(al, ipi, jpi), tauv = tau(al, ipi, jpi);
-            e = e + sgn * (drag(al, ipi, jpi) * ub(al, ipi, jpi) + cor - 1.0 * tauv * rhv) *
-                        (jt(JT_C, jpi) * dphi * (2.0 - al) + jt(JT_RCV, jpi) * jt(JT_DSV, jpi) * (al - 1.0));
+            const double rhv = V.rh(al, ipi, jpi), tauv = V.tau(al, ipi, jpi);
+            ring[base + 2 * (p - 1)] = sgn * (V.drag(al, ipi, jpi) * V.ub(al, ipi, jpi) + cor - 1.0 * tauv * rhv) *
+                                       (V.jt(JT_C, jpi) * dphi * (2.0 - al) + V.jt(JT_RCV, jpi) * V.jt(JT_DSV, jpi) * (al - 1.0));
+            double t = 0.0, b2;
             if (al == 1) {
-                double t = 0.0;
-                for (int k = ku(1, ipi, jpi); k <= mkk(ipi + 1, jpi); k++) t = t + bp(ipi + 1, jpi, k) * G.dz[k];
-                for (int k = ku(1, ipi, jpi); k <= mkk(ipi, jpi); k++) t = t - bp(ipi, jpi, k) * G.dz[k];
-                e = e + (sbp(ipi + 1, jpi) - sbp(ipi, jpi) + t) * sgn * rh(1, ipi, jpi);
+                for (int k = V.ku(1, ipi, jpi); k <= V.mkk(ipi + 1, jpi); k++) t = t + V.bp(ipi + 1, jpi, k) * G.dz[k];
+                for (int k = V.ku(1, ipi, jpi); k <= V.mkk(ipi, jpi); k++) t = t - V.bp(ipi, jpi, k) * G.dz[k];
+                b2 = (V.sbp(ipi + 1, jpi) - V.sbp(ipi, jpi) + t) * sgn * V.rh(1, ipi, jpi);
             } else {
-                double t = 0.0;
-                for (int k = ku(2, ipi, jpi); k <= mkk(ipi, jpi + 1); k++) t = t + bp(ipi, jpi + 1, k) * G.dz[k];
-                for (int k = ku(2, ipi, jpi); k <= mkk(ipi, jpi); k++) t = t - bp(ipi, jpi, k) * G.dz[k];
-                e = e + (sbp(ipi, jpi + 1) - sbp(ipi, jpi) + t) * sgn * rh(2, ipi, jpi);
+                for (int k = V.ku(2, ipi, jpi); k <= V.mkk(ipi, jpi + 1); k++) t = t + V.bp(ipi, jpi + 1, k) * G.dz[k];
+                for (int k = V.ku(2, ipi, jpi); k <= V.mkk(ipi, jpi); k++) t = t - V.bp(ipi, jpi, k) * G.dz[k];
+                b2 = (V.sbp(ipi, jpi + 1) - V.sbp(ipi, jpi) + t) * sgn * V.rh(2, ipi, jpi);
             }
+            ring[base + 2 * (p - 1) + 1] = b2;
         }
-        erhs[isl - 1] = e;
+        base += 2 * np;
     }
     __syncthreads();
-    if (tid == 0 && isles > 0) {
-        auto A = [&](int a, int b) { return G.erisl[(a - 1) + isles * (b - 1)]; };
+    if (tid == 0) {
+        int b = 0;
+        for (int isl = 1; isl <= isles; isl++) {
+            double e = 0.0;
+            for (int x = 0; x < 2 * G.npi[isl - 1]; x++) e = e + ring[b + x];
+            erhs[isl - 1] = e;
+            b += 2 * G.npi[isl - 1];
+        }
+        // the island values that cancel them (209-218); matmult with the
+        // eliminated matrix of matinv_gold
+        auto A = [&](int a, int b2) { return G.erisl[(a - 1) + isles * (b2 - 1)]; };
         if (isles > 1) {
-            // matmult with the eliminated matrix of matinv_gold
             for (int i = 1; i <= isles - 1; i++)
                 for (int j = i + 1; j <= isles; j++) erhs[j - 1] = A(i, i) * erhs[j - 1] - A(j, i) * erhs[i - 1];
             erhs[isles - 1] = erhs[isles - 1] / A(isles, isles);
@@ -347,89 +526,103 @@ __global__ __launch_bounds__(MOM_NT) void k_mom(const MemberDev* __restrict__ me
                 for (int j = i + 1; j <= isles; j++) erhs[i - 1] = erhs[i - 1] - A(i, j) * erhs[j - 1];
                 erhs[i - 1] = erhs[i - 1] / A(i, i);
             }
-            for (int isl = 1; isl <= isles; isl++) erhs[isl - 1] = -erhs[isl - 1];
+            for (int isl = 1; isl <= isles; isl++) V.W[V.ML.psibc + isl - 1] = -erhs[isl - 1];
         } else {
-            erhs[0] = -erhs[0] / A(1, 1);
+            V.W[V.ML.psibc] = -erhs[0] / A(1, 1);
         }
     }
-    __syncthreads();
+}
 
-    // ---- add the island fields times their psi values (219-233)
+// ------------------------------------------------------------ per column, after the solve
+// velc (3641-3657): the barotropic velocity, with the island fields times
+// their psi values added (219-227), in place of the shear's depth mean at
+// every open face, relaxed towards the last step's
+__global__ __launch_bounds__(MOM_NT) void k_mom_cols_b(const MemberDev* __restrict__ mems,
+                                                       const int32_t* __restrict__ list) {
+    const MomView V(mems[list[blockIdx.y]]);
+    const MomGridDev& G = V.G;
+    const int c = blockIdx.x * MOM_NT + threadIdx.x;
+    if (c >= V.ncol) return;
+    const int mi = V.mi, mj = V.mj, mk = V.mk, isles = G.isles;
+    const int i = c % mi + 1, j = c / mi + 1;
+    const size_t nub = (size_t)2 * (mi + 2) * (mj + 1);
+    double ubc[2];
+    for (int l = 1; l <= 2; l++) {
+        const size_t o = (l - 1) + 2 * ((size_t)i + (size_t)(mi + 2) * j);
+        double sum = 0.0;
+        for (int isl = 1; isl <= isles; isl++) sum = sum + G.ubisl[nub * (isl - 1) + o] * V.psibc(isl);
+        ubc[l - 1] = isles > 0 ? V.W[V.ML.ub + o] + sum : V.W[V.ML.ub + o];
+    }
+    const double s0 = V.csum(1, i, j), s1 = V.csum(2, i, j), rel = G.rel;
+    for (int k = V.K1(i, j); k <= mk; k++) {
+        if (V.K1(i + 1, j) <= k) {
+            double y = V.uu(1, i, j, k) - s0 * V.rh(1, i, j) + ubc[0];
+            y = rel * V.u1(1, i, j, k) + (1.0 - rel) * y;
+            V.uu(1, i, j, k) = y;
+            V.u1(1, i, j, k) = y;
+        }
+        if (V.K1(i, j + 1) <= k) {
+            double y = V.uu(2, i, j, k) - s1 * V.rh(2, i, j) + ubc[1];
+            y = rel * V.u1(2, i, j, k) + (1.0 - rel) * y;
+            V.uu(2, i, j, k) = y;
+            V.u1(2, i, j, k) = y;
+        }
+    }
+}
+
+// w from continuity, integrated up from the bottom (3661-3678); for an FP32
+// member then the six face coefficients of the flux (with the
+// Peclet-weighted upwinding of goldstein.f90:2489-2523) and the discrete
+// divergence of the open faces, in FP64 from the FP64 velocities, rounded
+// once -- the arithmetic of cgo_set_velocity.  Then the island correction
+// of the stored ub and psi (219-233), which nothing in the step reads again.
+__global__ __launch_bounds__(MOM_NT) void k_mom_cols_c(const MemberDev* __restrict__ mems,
+                                                       const int32_t* __restrict__ list) {
+    const MomView V(mems[list[blockIdx.y]]);
+    const MomGridDev& G = V.G;
+    const int x = blockIdx.x * MOM_NT + threadIdx.x;
+    const int mi = V.mi, mj = V.mj, mk = V.mk, ncol = V.ncol, ncell = V.ncell, isles = G.isles;
     if (isles > 0) {
         const size_t nub = (size_t)2 * (mi + 2) * (mj + 1), nps = (size_t)(mi + 1) * (mj + 1);
-        for (int x = tid; x < 2 * (mi + 2) * mj; x += MOM_NT) {
-            const int l = x % 2 + 1, i = (x / 2) % (mi + 2), j = x / (2 * (mi + 2)) + 1;
-            const size_t o = (l - 1) + 2 * ((size_t)i + (size_t)(mi + 2) * j);
+        if (x < 2 * (mi + 2) * mj) {
+            const size_t o = (size_t)(x % 2) + 2 * ((size_t)((x / 2) % (mi + 2)) + (size_t)(mi + 2) * (x / (2 * (mi + 2)) + 1));
             double sum = 0.0;
-            for (int isl = 1; isl <= isles; isl++) sum = sum + G.ubisl[nub * (isl - 1) + o] * erhs[isl - 1];
-            W[ML.ub + o] = W[ML.ub + o] + sum;
+            for (int isl = 1; isl <= isles; isl++) sum = sum + G.ubisl[nub * (isl - 1) + o] * V.psibc(isl);
+            V.W[V.ML.ub + o] = V.W[V.ML.ub + o] + sum;
         }
-        for (int x = tid; x < (mi + 1) * (mj + 1); x += MOM_NT) {
+        if (x < (int)nps) {
             double sum = 0.0;
-            for (int isl = 1; isl <= isles; isl++) sum = sum + G.psisl[nps * (isl - 1) + x] * erhs[isl - 1];
-            W[ML.psi + x] = W[ML.psi + x] + sum;
+            for (int isl = 1; isl <= isles; isl++) sum = sum + G.psisl[nps * (isl - 1) + x] * V.psibc(isl);
+            V.W[V.ML.psi + x] = V.W[V.ML.psi + x] + sum;
         }
     }
-    __syncthreads();
-
-    // ---- velc (3641-3657): the barotropic velocity in place of the shear's
-    // depth mean at every open face, relaxed towards the last step's
-    for (int c = tid; c < ncol; c += MOM_NT) {
-        const int i = c % mi + 1, j = c / mi + 1;
-        const double s0 = csum(1, i, j), s1 = csum(2, i, j);
-        for (int k = K1(i, j); k <= mk; k++) {
-            if (K1(i + 1, j) <= k) {
-                double x = uu(1, i, j, k) - s0 * rh(1, i, j) + ub(1, i, j);
-                x = G.rel * u1(1, i, j, k) + (1.0 - G.rel) * x;
-                uu(1, i, j, k) = x;
-                u1(1, i, j, k) = x;
-            }
-            if (K1(i, j + 1) <= k) {
-                double x = uu(2, i, j, k) - s1 * rh(2, i, j) + ub(2, i, j);
-                x = G.rel * u1(2, i, j, k) + (1.0 - G.rel) * x;
-                uu(2, i, j, k) = x;
-                u1(2, i, j, k) = x;
-            }
-        }
+    if (x >= ncol) return;
+    const int cc = x, i = cc % mi + 1, j = cc / mi + 1, kb = V.K1(i, j);
+    double tv = 0;
+    for (int k = kb; k <= mk - 1; k++) {
+        const double tv1 = (V.uu(1, i, j, k) - V.uu(1, i == 1 ? mi : i - 1, j, k)) * G.rdphi * V.jt(JT_RC, j);
+        const double vs = j > 1 ? V.uu(2, i, j - 1, k) : 0.0;
+        const double tv2 = (V.uu(2, i, j, k) * V.jt(JT_CV, j) - vs * V.jt(JT_CV, j - 1)) * V.jt(JT_RDS, j);
+        const double w = tv - G.dz[k] * (tv1 + tv2);
+        V.uu(3, i, j, k) = w;
+        tv = w;
     }
-    __syncthreads();
-
-    // ---- w from continuity, integrated up from the bottom (3661-3678)
-    for (int c = tid; c < ncol; c += MOM_NT) {
-        const int i = c % mi + 1, j = c / mi + 1;
-        double tv = 0;
-        for (int k = K1(i, j); k <= mk - 1; k++) {
-            const double tv1 = (uu(1, i, j, k) - uu(1, i == 1 ? mi : i - 1, j, k)) * rdphi * jt(JT_RC, j);
-            const double vs = j > 1 ? uu(2, i, j - 1, k) : 0.0;
-            const double tv2 = (uu(2, i, j, k) * jt(JT_CV, j) - vs * jt(JT_CV, j - 1)) * jt(JT_RDS, j);
-            const double w = tv - G.dz[k] * (tv1 + tv2);
-            uu(3, i, j, k) = w;
-            tv = w;
-        }
-    }
-
+    const MemberDev& M = V.M;
     if (M.prec == 0) return;
-    __syncthreads();
-
-    // ---- FP32 form: the six face coefficients of the flux (with the
-    // Peclet-weighted upwinding of goldstein.f90:2489-2523) and the discrete
-    // divergence of the open faces, in FP64 from the FP64 velocities, rounded
-    // once -- the arithmetic of cgo_set_velocity
     float* __restrict__ F = (float*)M.u;
+    const double* __restrict__ U = V.U;
     const GridDev& g = M.g;
-    const int JS = mj + 2, KS = mk + 2;
+    const int JS = mj + 2, KS = mk + 2, I = i, J = j;
     const double diff1 = M.diff1, diffv = M.diff2;
     auto upwind = [](double pec) { return pec / (2.0 + fabs(pec)); };
-    for (int x = tid; x < ncell; x += MOM_NT) {
-        const int k = x / ncol + 1, cc = x % ncol, I = cc % mi + 1, J = cc / mi + 1;
-        const int k1c = K1(I, J);
-        if (k < k1c) continue;
+    for (int k = kb; k <= mk; k++) {
+        const int k1c = kb;
         const size_t o = (size_t)(k - 1) * ncol;
         const size_t c = o + cc;
         const size_t cw = o + (size_t)mi * (J - 1) + (I == 1 ? mi - 1 : I - 2);
         const size_t cs = o + (size_t)mi * (J > 1 ? J - 2 : 0) + (I - 1);
-        const bool oe = k >= max(k1c, K1(I + 1, J)), ow = k >= max(K1(I - 1, J), k1c);
-        const bool on = k >= max(k1c, K1(I, J + 1)), os = k >= max(K1(I, J - 1), k1c);
+        const bool oe = k >= max(k1c, V.K1(I + 1, J)), ow = k >= max(V.K1(I - 1, J), k1c);
+        const bool on = k >= max(k1c, V.K1(I, J + 1)), os = k >= max(V.K1(I, J - 1), k1c);
         const double rc = g.mj[MJ_RC * JS + J], rds = g.mj[MJ_RDS * JS + J];
         const double cvn = g.mj[MJ_CV * JS + J], cvs = g.mj[MJ_CV * JS + J - 1];
         const double rdz = g.mk[MK_RDZ * KS + k];
@@ -456,10 +649,23 @@ __global__ __launch_bounds__(MOM_NT) void k_mom(const MemberDev* __restrict__ me
     }
 }
 
-void launch_mom(const MemberDev* mems, const int32_t* list, int nlist, int maxN, hipStream_t s) {
+// LDS of the solve block of a member with N psi points and band n
+size_t mom_solve_lds(int N, int n) {
+    return (size_t)(N + MOM_MAXISL + 128 + (size_t)MOM_RING * MOM_CH * (n + 3)) * sizeof(double);
+}
+// island path terms the solve block's ring holds
+size_t mom_ring_terms(int n) { return (size_t)MOM_RING * MOM_CH * (n + 3); }
+
+void launch_mom(const MemberDev* mems, const int32_t* list, int nlist, int maxN, int maxcol, int maxn, hipStream_t s) {
     if (nlist <= 0) return;
-    const size_t lds = (size_t)(maxN + MOM_MAXISL) * sizeof(double);
-    hipLaunchKernelGGL(k_mom, dim3(nlist), dim3(MOM_NT), lds, s, mems, list);
+    const int cb = (maxcol + MOM_NT - 1) / MOM_NT;
+    // the stored ub and psi have at most (maxi+2) maxj x 2 and (maxi+1)(maxj+1)
+    // entries: fewer than 4 x the columns + 256
+    const int cb2 = (4 * maxcol + 256 + MOM_NT - 1) / MOM_NT;
+    hipLaunchKernelGGL(k_mom_cols_a, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
+    hipLaunchKernelGGL(k_mom_solve, dim3(nlist), dim3(MOM_NT), mom_solve_lds(maxN, maxn), s, mems, list);
+    hipLaunchKernelGGL(k_mom_cols_b, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
+    hipLaunchKernelGGL(k_mom_cols_c, dim3(cb2, nlist), dim3(MOM_NT), 0, s, mems, list);
 }
 
 }  // namespace cgo

@@ -11,9 +11,12 @@
 !   step_goldstein(istep, latent_ocn, ..., go_rho)   goldstein.f90:17-38: the
 !                                         surface-flux slot from the heat and
 !                                         freshwater fluxes (149-170), go_ts in,
-!                                         tstepo (235), go_ts/go_rho/go_cost and
-!                                         the surface and energy/water outputs
-!                                         back (427-478)
+!                                         with initialise_goldstein_momentum_hip
+!                                         the velocity solve from the wind
+!                                         stresses (102-107, 198-233), tstepo
+!                                         (235), go_ts/go_rho/go_cost (and
+!                                         go_u/go_tau/go_psi) and the surface
+!                                         and energy/water outputs back (427-478)
 !   step_biogem(dum_dts, dum_genie_clock, dum_sfcatm1, ..., dum_sfxsumrok1)
 !                                         biogem.f90:528-547 (biology,
 !                                         remineralisation and surface chemistry
@@ -38,7 +41,7 @@ module cgenie_hip
   public :: initialise_goldstein_hip, initialise_biogem_hip, end_cgenie_hip
   public :: tstepo, step_goldstein_tracers, step_biogem, biogem_tracercoupling
   public :: cgenie_hip_set_mirror, cgenie_hip_download, cgenie_hip_inventory, cgenie_hip_diag
-  public :: cgenie_hip_set_sflux, step_goldstein, cgenie_hip_atmosphere
+  public :: cgenie_hip_set_sflux, step_goldstein, cgenie_hip_atmosphere, initialise_goldstein_momentum_hip
 
   type(c_ptr), save :: eng = c_null_ptr
   integer(c_int32_t), save :: grid_id = -1, mem(1) = -1
@@ -53,6 +56,11 @@ module cgenie_hip
   real(c_double), allocatable, target, save :: u_last(:,:,:,:), sflux_c(:,:,:)
   real(c_double), save :: dphi_s = 0, dsc_s = 0, ini_energy = 0, ini_water = 0
   logical, save :: sflux_on = .false.
+  ! the velocity solve: on, scf (ini_gold_nml), dzz (goldstein.f90:1035) and
+  ! the wind stresses last handed to the engine
+  logical, save :: mom_on = .false.
+  real(c_double), save :: scf_s = 2.0_c_double, dzz_s = 0
+  real(c_double), allocatable, save :: stress_last(:,:,:)
   ! step_biogem: go_ts positions of DIC, ALK, PO4, O2, DOM_C, DOM_P and the step length
   integer, save :: bgc_slots(6) = 0
   real(c_double), save :: bgc_dtyr = 0
@@ -64,6 +72,7 @@ module cgenie_hip
   real(dp), parameter :: usc = 0.05_dp, rsc = 6.37E6_dp, dsc_p = 5.0E3_dp, rh0sc = 1.0E3_dp, cpsc = 3981.1_dp
   real(dp), parameter :: saln0 = 34.9_dp, m2mm = 1000.0_dp, mm2m = 1.0_dp / m2mm, albocn = 0.05_dp
   real(dp), parameter :: rfluxsc = rsc / (dsc_p * usc * rh0sc * cpsc)
+  real(dp), parameter :: fsc = 2 * 7.2921E-5_dp    ! goldstein_lib.f90:49
   ! BIOGEM constants (gem_cmn.f90:510-513, 688, 696, 719) and tracer indices (72-129, 190-255)
   real(dp), parameter :: conv_yr_s = 3600.0_dp * (24.0_dp * 365.25_dp), const_pi = 3.141592653589793_dp
   real(dp), parameter :: conv_s_yr = 1.0_dp / (3600.0_dp * (24.0_dp * 365.25_dp))
@@ -144,6 +153,45 @@ contains
     sflux_c = 0
     call energy_water(go_ts, ini_energy, ini_water)
   end subroutine initialise_goldstein_hip
+
+  ! GOLDSTEIN's velocity solve for the member (goldstein.f90:198-233), set up
+  ! as initialise_goldstein does (1359-1575, 1788-1814): the per-j metrics
+  ! s, c, rcv, rds2 (0:maxj+1), the world's .psiles labels as
+  ! labels(maxi,0:maxj) (gbold(i+j*maxi), 1505-1517), isles, the .paths
+  ! lengths npi(isles) and points paths(3,sum npi) (1538-1574), and adrag
+  ! (days), rel and scf of ini_gold_nml.  step_goldstein then solves the
+  ! velocities from its wind stresses and hands back go_u, go_tau and go_psi
+  ! as the reference does (102-107, 444-450).
+  subroutine initialise_goldstein_momentum_hip(s, c, rcv, rds2, labels, isles, npi, paths, adrag, rel, scf)
+    real(c_double), intent(in), dimension(0:ny+1) :: s, c, rcv, rds2
+    integer, intent(in) :: labels(nx, 0:ny), isles
+    integer, intent(in) :: npi(:), paths(:,:)
+    real(c_double), intent(in) :: adrag, rel, scf
+    type(cgo_momentum_desc) :: d
+    real(c_double), allocatable, target :: js(:,:)
+    integer(c_int32_t), allocatable, target :: lab(:,:), np(:), pth(:,:)
+
+    if (.not. c_associated(eng)) call check(CGO_E_STATE, 'initialise_goldstein_momentum_hip: initialise_goldstein_hip first')
+    allocate (js(0:ny+1, 4), lab(nx, 0:ny), np(max(isles, 1)), pth(3, max(size(paths, 2), 1)))
+    js(:, 1) = s; js(:, 2) = c; js(:, 3) = rcv; js(:, 4) = rds2
+    lab = int(labels, c_int32_t)
+    np = 0
+    if (isles > 0) np(1:isles) = int(npi(1:isles), c_int32_t)
+    pth = 0
+    if (size(paths, 2) > 0) pth(:, 1:size(paths, 2)) = int(paths, c_int32_t)
+    d%s = c_loc(js(0, 1)); d%c = c_loc(js(0, 2)); d%rcv = c_loc(js(0, 3)); d%rds2 = c_loc(js(0, 4))
+    d%labels = c_loc(lab)
+    d%isles = int(isles, c_int32_t)
+    d%npi = c_loc(np)
+    d%paths = c_loc(pth)
+    d%adrag = adrag
+    d%rel = rel
+    call check(cgo_grid_set_momentum(eng, grid_id, d), 'cgo_grid_set_momentum')
+    mom_on = .true.
+    scf_s = scf
+    dzz_s = ktab(nz, 6) * ktab(nz - 1, 4) / 2     ! dz(maxk) dza(maxk-1) / 2
+    if (allocated(stress_last)) deallocate (stress_last)
+  end subroutine initialise_goldstein_momentum_hip
 
   ! The total heat and water diagnostics of step_goldstein
   ! (goldstein.f90:79-96, 458-471), summed in the reference's order.
@@ -261,8 +309,10 @@ contains
   ! u(3,0:maxi,0:maxj,maxk) follow from it), handed to the engine when they
   ! change.  After tstepo: go_ts, go_ts1, go_rho and go_cost, the surface T,
   ! S and velocities, albedo and the energy and water diagnostics (427-478).
-  ! The momentum outputs of the reference's velocity solve (go_tau, go_psi,
-  ! go_mldta) are left as given.
+  ! With the velocity solve on (initialise_goldstein_momentum_hip) the
+  ! velocities come from the wind stresses instead (102-107, 198-233) and
+  ! go_u, go_tau and go_psi are handed back (444-450); go_mldta is left as
+  ! given (the mixed-layer scheme, imld, is off).
   subroutine step_goldstein(istep, latent_ocn, sensible_ocn, netsolar_ocn, netlong_ocn, fx0sic_ocn, evap_ocn, &
        pptn_ocn, runoff_ocn, fwsic_ocn, stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn, tsval_ocn, &
        ssval_ocn, usval_ocn, vsval_ocn, albedo_ocn, test_energy_ocean, test_water_ocean, go_ts, go_ts1, go_cost, &
@@ -276,7 +326,8 @@ contains
     real(c_double), dimension(:,:), intent(inout) :: go_cost, go_mldta
     real(c_double), intent(inout) :: go_u(:,:,:,:), go_tau(:,:,:), go_psi(:,:), go_rho(:,:,:)
     real(c_double) :: fx0neto(nx, ny), fwfxneto(nx, ny), tot_energy, tot_water
-    real(c_double), allocatable :: u(:,:,:,:)
+    real(c_double), allocatable :: u(:,:,:,:), psi(:,:)
+    real(c_double) :: den
     integer :: i, j
     logical :: wet
 
@@ -296,22 +347,47 @@ contains
        call check(cgo_set_sflux(eng, mem(1), c_null_ptr), 'cgo_set_sflux')
        sflux_on = .false.
     end if
-    ! velocities, when the caller's differ from the engine's
-    ! (the engine reads the interior faces only; the west face of i = 1 is the
-    ! east face of i = maxi)
-    allocate (u(3, 0:nx, 0:ny, nz))
-    u = 0.0_dp
-    u(:, 1:nx, 1:ny, :) = go_u(:, 1:nx, 1:ny, 1:nz)
-    u(:, 0, 1:ny, :) = u(:, nx, 1:ny, :)
-    if (any(u(:, 1:nx, 1:ny, :) /= u_last(:, 1:nx, 1:ny, :))) then
-       call check(cgo_set_velocity(eng, mem(1), u), 'cgo_set_velocity')
-       u_last = u
+    if (mom_on) then
+       ! the wind stresses, when they differ from the engine's (102-107)
+       if (.not. allocated(stress_last)) then
+          allocate (stress_last(nx, ny, 4))
+          stress_last = huge(1.0_dp)
+       end if
+       if (any(stressxu_ocn /= stress_last(:, :, 1)) .or. any(stressyu_ocn /= stress_last(:, :, 2)) .or. &
+           any(stressxv_ocn /= stress_last(:, :, 3)) .or. any(stressyv_ocn /= stress_last(:, :, 4))) then
+          call check(cgo_set_wind(eng, mem(1), stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn, scf_s), &
+               'cgo_set_wind')
+          stress_last(:, :, 1) = stressxu_ocn; stress_last(:, :, 2) = stressyu_ocn
+          stress_last(:, :, 3) = stressxv_ocn; stress_last(:, :, 4) = stressyv_ocn
+       end if
+    else
+       ! velocities, when the caller's differ from the engine's
+       ! (the engine reads the interior faces only; the west face of i = 1 is the
+       ! east face of i = maxi)
+       allocate (u(3, 0:nx, 0:ny, nz))
+       u = 0.0_dp
+       u(:, 1:nx, 1:ny, :) = go_u(:, 1:nx, 1:ny, 1:nz)
+       u(:, 0, 1:ny, :) = u(:, nx, 1:ny, :)
+       if (any(u(:, 1:nx, 1:ny, :) /= u_last(:, 1:nx, 1:ny, :))) then
+          call check(cgo_set_velocity(eng, mem(1), u), 'cgo_set_velocity')
+          u_last = u
+       end if
     end if
     if (mirror) then
        call check(cgo_upload_state(eng, mem(1), go_ts), 'cgo_upload_state')
        call check(cgo_set_cost(eng, mem(1), go_cost), 'cgo_set_cost')
     end if
     call tstepo()
+    if (mom_on) then
+       ! the solved velocities, tau and psi (444-450: go_psi = 1592.5 psi)
+       call check(cgo_download_velocity(eng, mem(1), go_u), 'cgo_download_velocity')
+       allocate (psi(0:nx, 0:ny))
+       call check(cgo_download_psi(eng, mem(1), psi), 'cgo_download_psi')
+       go_psi = 1592.5_dp * psi
+       den = rh0sc * dsc_s * usc * fsc
+       go_tau(1, :, :) = scf_s * stressxu_ocn / den / dzz_s * dzz_s
+       go_tau(2, :, :) = scf_s * stressyv_ocn / den / dzz_s * dzz_s
+    end if
     usval_ocn = go_u(1, 1:nx, 1:ny, nz)
     vsval_ocn = go_u(2, 1:nx, 1:ny, nz)
     if (.not. mirror) return

@@ -3,30 +3,39 @@
 ! tracer coupling every kbiogem ocean steps) through the cgenie_hip module,
 ! with the argument lists of the reference's wrappers
 ! (genie_loop_wrappers.f90:143-150, 313-315, 321): step_goldstein with zero
-! surface heat and freshwater fluxes and wind stress, step_biogem with the
-! BIOGEM step in seconds and the interface arrays of an ocean-only run,
-! biogem_tracercoupling.
+! surface heat and freshwater fluxes and the given (or zero) wind stress,
+! step_biogem with the BIOGEM step in seconds and the interface arrays of an
+! ocean-only run, biogem_tracercoupling.
 !
 !   cgenie_hip_driver INPUT OUTPUT
 !
 ! INPUT is a little-endian stream: int32 maxi, maxj, maxk, maxl, nsteps,
-! kbiogem, diso, precision, has_bgc, device, mirror; float64 dphi, rdphi, dsc,
+! kbiogem, diso, precision, has_bgc, device, mirror, has_mom; float64 dphi, rdphi, dsc,
 ! diff(2), ec(5); int32 k1(0:maxi+1,0:maxj+1); float64 rc, rc2, cv, cv2, rds,
 ! dsv, rdsv, ds, sv (each 0:maxj+1); float64 dt, rdz, rdza, dza, ssmax, dz
 ! (each 0:maxk+1); float64 go_u(3,0:maxi,0:maxj,maxk),
 ! go_ts(maxl,maxi,maxj,maxk); if has_bgc: int32 slots(6) (1-based, 0 =
-! absent), float64 dtyr, carb_tol, light(maxi,maxj), mld(maxi,maxj).
+! absent), float64 dtyr, carb_tol, light(maxi,maxj), mld(maxi,maxj); if
+! has_mom (the velocity solve): float64 s, c, rcv, rds2 (each 0:maxj+1),
+! int32 labels(maxi,0:maxj), isles, npi(isles), paths(3,sum npi), float64
+! adrag, rel, scf, stress(maxi,maxj,4) (stressxu, stressyu, stressxv, stressyv).
 ! OUTPUT receives float64 go_ts(maxl,maxi,maxj,maxk), inventory(maxl),
 ! go_rho(maxi,maxj,maxk), the energy and water diagnostics of the last
 ! step_goldstein, dum_sfcocn1(1:38,maxi,maxj) and dum_sfxsed1(1:34,maxi,maxj)
-! of the last step_biogem (both zero without mirroring).
+! of the last step_biogem (both zero without mirroring), then go_u(3,maxi,
+! maxj,maxk), go_psi(0:maxi,0:maxj) and go_tau(2,maxi,maxj) of the last
+! step_goldstein.
 program cgenie_hip_driver
   use, intrinsic :: iso_c_binding
   use cgenie_hip
   implicit none
-  integer(c_int32_t) :: hdr(11)
+  integer(c_int32_t) :: hdr(12)
   integer :: maxi, maxj, maxk, maxl, nsteps, kbiogem, device, koverall, u
-  logical :: diso, has_bgc
+  logical :: diso, has_bgc, has_mom
+  real(c_double), allocatable :: mt(:,:), stress(:,:,:)
+  integer(c_int32_t), allocatable :: labels(:,:), npi(:), paths(:,:)
+  integer(c_int32_t) :: isles
+  real(c_double) :: msc(3)
   real(c_double) :: sc(10), dtyr, carb_tol
   integer(c_int32_t), allocatable :: k1(:,:)
   real(c_double), allocatable :: jt(:,:), kt(:,:), go_u(:,:,:,:), go_ts(:,:,:,:), go_ts1(:,:,:,:)
@@ -48,7 +57,7 @@ program cgenie_hip_driver
   open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
   read (u) hdr
   maxi = hdr(1); maxj = hdr(2); maxk = hdr(3); maxl = hdr(4); nsteps = hdr(5); kbiogem = max(1, int(hdr(6)))
-  diso = hdr(7) /= 0; has_bgc = hdr(9) /= 0; device = hdr(10)
+  diso = hdr(7) /= 0; has_bgc = hdr(9) /= 0; device = hdr(10); has_mom = hdr(12) /= 0
   read (u) sc
   allocate (k1(0:maxi+1, 0:maxj+1), jt(0:maxj+1, 9), kt(0:maxk+1, 6))
   allocate (go_u(3, 0:maxi, 0:maxj, maxk), go_ts(maxl, maxi, maxj, maxk), go_ts1(maxl, maxi, maxj, maxk))
@@ -64,6 +73,20 @@ program cgenie_hip_driver
      read (u) light
      read (u) mld
   end if
+  if (has_mom) then
+     allocate (mt(0:maxj+1, 4), labels(maxi, 0:maxj), stress(maxi, maxj, 4))
+     read (u) mt
+     read (u) labels
+     read (u) isles
+     allocate (npi(max(isles, 1)))
+     npi = 0
+     if (isles > 0) read (u) npi(1:isles)
+     allocate (paths(3, max(sum(npi), 1)))
+     paths = 0
+     if (isles > 0) read (u) paths(:, 1:sum(npi))
+     read (u) msc
+     read (u) stress
+  end if
   close (u)
   go_ts1 = go_ts
 
@@ -71,6 +94,8 @@ program cgenie_hip_driver
        jt(:, 1), jt(:, 2), jt(:, 3), jt(:, 4), jt(:, 5), jt(:, 6), jt(:, 7), jt(:, 8), jt(:, 9), &
        kt(:, 1), kt(:, 2), kt(:, 3), kt(:, 4), kt(:, 5), kt(:, 6), sc(6:10), sc(3), sc(4:5), diso, &
        int(hdr(8)), go_ts, go_u)
+  if (has_mom) call initialise_goldstein_momentum_hip(mt(:, 1), mt(:, 2), mt(:, 3), mt(:, 4), int(labels), &
+       int(isles), int(npi), int(paths(:, 1:max(sum(npi), 0))), msc(1), msc(2), msc(3))
   if (has_bgc) call initialise_biogem_hip(int(slots), dtyr, light, mld, carb_tol)
   call cgenie_hip_set_mirror(hdr(11) /= 0)
 
@@ -81,6 +106,7 @@ program cgenie_hip_driver
   f2 = 0; go_cost = 0; go_tau = 0; go_psi = 0; go_mldta = 0; go_rho = 0
   sfcatm = 0; sfxatm = 0; sfcocn = 0; sfxocn = 0; sfcsed = 0; sfxsed = 0; sfxrok = 0
   go_uvw = go_u(:, 1:maxi, 1:maxj, :)
+  if (has_mom) f2(:, :, 10:13) = stress
   dts = dtyr * conv_yr_s    ! dtyr is the BIOGEM step (kbiogem ocean steps)
   clock = 0
   do koverall = 1, nsteps
@@ -104,6 +130,9 @@ program cgenie_hip_driver
   write (u) test_energy, test_water
   write (u) sfcocn(1:38, :, :)
   write (u) sfxsed(1:34, :, :)
+  write (u) go_uvw
+  write (u) go_psi
+  write (u) go_tau
   close (u)
   write (*, '(a,i0,a)') 'cgenie_hip_driver: ', nsteps, ' ocean steps done'
 end program cgenie_hip_driver

@@ -42,6 +42,17 @@ module cgo_api
      real(c_double) :: gastransfer_a, atm_pCO2, atm_pO2
   end type cgo_bgc_params
 
+  ! GOLDSTEIN's velocity solve of a grid: the per-j metrics s, c, rcv, rds2
+  ! (0:maxj+1), the .psiles labels (maxi,0:maxj), island count, path lengths
+  ! and (lpisl, ipisl, jpisl) triples, drag in days and relaxation
+  type, bind(C) :: cgo_momentum_desc
+     type(c_ptr) :: s, c, rcv, rds2
+     type(c_ptr) :: labels
+     integer(c_int32_t) :: isles
+     type(c_ptr) :: npi, paths
+     real(c_double) :: adrag, rel
+  end type cgo_momentum_desc
+
   type, bind(C) :: cgo_diag
      integer(c_int64_t) :: steps
      integer(c_int32_t) :: limps, carb_fail
@@ -107,6 +118,38 @@ module cgo_api
        integer(c_int32_t), value :: m
        real(c_double), intent(in) :: u(*)
      end function cgo_set_velocity
+
+     integer(c_int) function cgo_grid_set_momentum(e, grid_id, d) bind(C, name="cgo_grid_set_momentum")
+       import :: c_int, c_ptr, c_int32_t, cgo_momentum_desc
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: grid_id
+       type(cgo_momentum_desc), intent(in) :: d
+     end function cgo_grid_set_momentum
+
+     ! the four wind-stress arrays of step_goldstein, (maxi,maxj) each
+     integer(c_int) function cgo_set_wind(e, m, sxu, syu, sxv, syv, scf) bind(C, name="cgo_set_wind")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(in) :: sxu(*), syu(*), sxv(*), syv(*)
+       real(c_double), value :: scf
+     end function cgo_set_wind
+
+     ! go_u(3,maxi,maxj,maxk)
+     integer(c_int) function cgo_download_velocity(e, m, go_u) bind(C, name="cgo_download_velocity")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(out) :: go_u(*)
+     end function cgo_download_velocity
+
+     ! psi(0:maxi,0:maxj), non-dimensional
+     integer(c_int) function cgo_download_psi(e, m, psi) bind(C, name="cgo_download_psi")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       real(c_double), intent(out) :: psi(*)
+     end function cgo_download_psi
 
      ! sflux: c_loc of ts(:,1:maxi,1:maxj,maxk+1) values, or c_null_ptr
      integer(c_int) function cgo_set_sflux(e, m, sflux) bind(C, name="cgo_set_sflux")

@@ -8,8 +8,10 @@ program cgo_api_layout
   type(cgo_member_params) :: m
   type(cgo_bgc_params) :: b
   type(cgo_diag) :: d
+  type(cgo_momentum_desc) :: md
   write (*, '(a,i0)') 'cgo_grid_desc ', c_sizeof(g)
   write (*, '(a,i0)') 'cgo_member_params ', c_sizeof(m)
   write (*, '(a,i0)') 'cgo_bgc_params ', c_sizeof(b)
   write (*, '(a,i0)') 'cgo_diag ', c_sizeof(d)
+  write (*, '(a,i0)') 'cgo_momentum_desc ', c_sizeof(md)
 end program cgo_api_layout

@@ -113,7 +113,10 @@ int64_t cgo_member_bytes(cgo_engine *e, int32_t grid_id, int32_t maxl, int32_t p
 /* uploads / downloads in reference layouts */
 int cgo_set_velocity(cgo_engine *e, int32_t m, const double *u);        /* u(3,0:maxi,0:maxj,maxk) */
 int cgo_set_sflux(cgo_engine *e, int32_t m, const double *sflux);       /* ts(:,1:maxi,1:maxj,maxk+1) or NULL */
-int cgo_upload_state(cgo_engine *e, int32_t m, const double *go_ts);    /* go_ts(maxl,maxi,maxj,maxk) */
+/* go_ts(maxl,maxi,maxj,maxk); before the member's first step the upload also
+ * forms rho from it (initialise_goldstein, goldstein.f90:1450), afterwards
+ * rho stays the last tstepo's (step_goldstein's ts = go_ts, 176-177) */
+int cgo_upload_state(cgo_engine *e, int32_t m, const double *go_ts);
 int cgo_download_state(cgo_engine *e, int32_t m, double *go_ts);
 int cgo_download_rho(cgo_engine *e, int32_t m, double *rho);            /* rho(1:maxi,1:maxj,1:maxk) */
 int cgo_inventory(cgo_engine *e, int32_t m, double *inv);               /* maxl, diag2 weights */
@@ -175,7 +178,8 @@ int cgo_set_windspeed(cgo_engine *e, int32_t m, const double *wspeed);
  * cgo_set_wind then switches a member from prescribed to solved velocities,
  * and every cgo_step / cgo_tstepo / captured graph solves them before the
  * member's tstepo.  cgo_set_velocity switches a member back to prescribed
- * velocities.  maxi <= 63 and maxi x (maxj+1) <= 8176. */
+ * velocities.  maxi <= 62; the solve's LDS, 8 (maxi (maxj+1) + 96 maxi + 432)
+ * bytes, at most 64 KiB (36 x 36: 41 KiB). */
 typedef struct cgo_momentum_desc {
     /* per-j metrics the grid descriptor does not carry (maxj+2 entries,
      * Fortran j; goldstein.f90:928-960): s and c, sine and cosine of

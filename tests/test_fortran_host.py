@@ -39,12 +39,14 @@ def test_bind_c_types_match_c_structs(tmp_path):
     out = subprocess.run([LAYOUT], check=True, capture_output=True, text=True).stdout.split()
     fsz = {out[i]: int(out[i + 1]) for i in range(0, len(out), 2)}
     src = tmp_path / "sz.c"
-    src.write_text('#include <stdio.h>\n#include "cgocn.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
-                   'sizeof(cgo_grid_desc),sizeof(cgo_member_params),sizeof(cgo_bgc_params),sizeof(cgo_diag));}\n')
+    src.write_text('#include <stdio.h>\n#include "cgocn.h"\nint main(void){printf("%zu %zu %zu %zu %zu\\n",'
+                   'sizeof(cgo_grid_desc),sizeof(cgo_member_params),sizeof(cgo_bgc_params),sizeof(cgo_diag),'
+                   'sizeof(cgo_momentum_desc));}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     csz = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    assert [fsz["cgo_grid_desc"], fsz["cgo_member_params"], fsz["cgo_bgc_params"], fsz["cgo_diag"]] == csz
+    assert [fsz["cgo_grid_desc"], fsz["cgo_member_params"], fsz["cgo_bgc_params"], fsz["cgo_diag"],
+            fsz["cgo_momentum_desc"]] == csz
 
 
 REF_SRC = "/root/reference/src"
@@ -89,10 +91,12 @@ def test_reference_wrapper_calls_compile_against_module(tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
 
 
-def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, mirror=False, dtyr=1.0 / 96):
+def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, mirror=False, dtyr=1.0 / 96,
+                       mom=None):
+    """mom: None, or (world, stress (maxj, maxi, 4), adrag, rel, scf) for the velocity solve."""
     g = c.g
     hdr = np.array([g.maxi, g.maxj, g.maxk, c.tset.L, nsteps, kbiogem, 1, precision, 1 if bgc else 0, 0,
-                    1 if mirror else 0], dtype="<i4")
+                    1 if mirror else 0, 1 if mom is not None else 0], dtype="<i4")
     sc = np.array([g.dphi, g.rdphi, 5000.0, c.diff[0], c.diff[1], *list(g.ec)[:5]], dtype="<f8")
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
@@ -111,6 +115,18 @@ def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, m
             f.write(np.array([dtyr, 1e-12], dtype="<f8").tobytes())
             f.write(np.ascontiguousarray(c.light, dtype="<f8").tobytes())
             f.write(np.ascontiguousarray(c.mld, dtype="<f8").tobytes())
+        if mom is not None:
+            world, st, adrag, rel, scf = mom
+            for n in ("s", "c", "rcv", "rds2"):
+                f.write(np.ascontiguousarray(getattr(g, n), dtype="<f8")[: g.maxj + 2].tobytes())
+            f.write(np.ascontiguousarray(world.labels, dtype="<i4").tobytes())
+            f.write(np.array([world.isles], dtype="<i4").tobytes())
+            if world.isles:
+                f.write(np.ascontiguousarray(world.npi, dtype="<i4").tobytes())
+                f.write(np.ascontiguousarray(world.paths, dtype="<i4").tobytes())
+            f.write(np.array([adrag, rel, scf], dtype="<f8").tobytes())
+            # stress(maxi, maxj, 4) column-major = (4, maxj, maxi) C order
+            f.write(np.ascontiguousarray(np.moveaxis(st, 2, 0), dtype="<f8").tobytes())
 
 
 CONV_YR_S = 3600.0 * (24.0 * 365.25)              # gem_cmn.f90:511-513
@@ -146,6 +162,7 @@ def test_driver_genie_order_matches_engine_and_oracle(tmp_path, mirror):
     energy, water = raw[o], raw[o + 1]; o += 2
     sfcocn = raw[o:o + 38 * n_col].reshape(g.maxj, g.maxi, 38); o += 38 * n_col
     sfxsed = raw[o:o + 34 * n_col].reshape(g.maxj, g.maxi, 34); o += 34 * n_col
+    o += 3 * n_c + (g.maxi + 1) * (g.maxj + 1) + 2 * n_col    # go_u, go_psi, go_tau
     assert o == raw.size
     # the same library driven from Python
     eng = E.Engine(0)
@@ -219,3 +236,58 @@ def test_driver_genie_order_matches_engine_and_oracle(tmp_path, mirror):
             worst = max(worst, abs(sfxsed[j, i, isx - 1] - want) / max(abs(want), 1e-300))
     print("interface arrays vs oracle max rel err", worst, flush=True)
     assert worst <= 1e-12
+
+
+@pytest.mark.gpu
+def test_driver_velocity_solve_matches_engine(tmp_path):
+    """step_goldstein with the velocity solve (initialise_goldstein_momentum_hip
+    and the wind stresses in step_goldstein's own arguments) on the worbe2
+    world, mirroring on: the state, go_u and go_psi / 1592.5 equal the
+    Python path through the same library bit for bit (which equals the
+    oracle's coupled step, tests/test_gpu_velocity.py), and go_tau is the
+    stresses' tau (goldstein.f90:102-107, 444-450)."""
+    from types import SimpleNamespace
+
+    from cgenie_amd import grid as G
+    import test_gpu_velocity as V
+
+    assert os.path.exists(DRIVER), "build the host driver first (__graft_entry__.build())"
+    world = G.read_world(V.WORLD, V.MAXI, V.MAXJ)
+    g = G.make_grid(V.MAXI, V.MAXJ, V.MAXK)
+    L, nsteps = 4, 5
+    ts = V.state(world.k1, L)
+    st = V.wind()
+    case = SimpleNamespace(g=g, k1=world.k1, u=np.zeros((g.maxk, g.maxj + 1, g.maxi + 1, 3)), ts=ts,
+                           tset=SimpleNamespace(L=L), diff=V.DIFF)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_driver_input(fin, case, nsteps, bgc=False, mirror=True, mom=(world, st, 2.5, 0.9, 2.0))
+    print("driver with the velocity solve, worbe2, steps", nsteps, flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(fout, dtype="<f8")
+    n_c, n_col = g.maxi * g.maxj * g.maxk, g.maxi * g.maxj
+    o = 0
+    got = raw[o:o + ts.size].reshape(ts.shape); o += ts.size
+    o += L + n_c + 2 + 38 * n_col + 34 * n_col        # inventory, rho, diagnostics, BIOGEM outputs
+    go_u = raw[o:o + 3 * n_c].reshape(g.maxk, g.maxj, g.maxi, 3); o += 3 * n_c
+    go_psi = raw[o:o + (g.maxi + 1) * (g.maxj + 1)].reshape(g.maxj + 1, g.maxi + 1); o += (g.maxi + 1) * (g.maxj + 1)
+    go_tau = raw[o:o + 2 * n_col].reshape(g.maxj, g.maxi, 2); o += 2 * n_col
+    assert o == raw.size
+    eng = E.Engine(0)
+    gid = eng.add_grid(g, world.k1)
+    eng.set_momentum(gid, g, world)
+    m = eng.add_member(gid, L, E.FP64, V.DIFF, diso=True)
+    eng.upload(m, ts)
+    eng.set_wind(m, st)
+    eng.step([m], nsteps)
+    ref, u_ref, psi_ref = eng.download(m, ts.shape), eng.velocity(m, g), eng.psi(m, g)
+    eng.close()
+    assert np.abs(u_ref).max() > 0
+    assert np.array_equal(got, ref)
+    assert np.array_equal(go_u, u_ref)
+    assert np.array_equal(go_psi, 1592.5 * psi_ref)
+    dzz = g.dz[g.maxk] * g.dza[g.maxk - 1] / 2
+    den = 1.0e3 * 5000.0 * 0.05 * (2 * 7.2921e-5)
+    assert np.array_equal(go_tau[..., 0], 2.0 * st[..., 0] / den / dzz * dzz)
+    assert np.array_equal(go_tau[..., 1], 2.0 * st[..., 3] / den / dzz * dzz)
