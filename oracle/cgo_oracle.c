@@ -2048,3 +2048,319 @@ double og_mom_island_of(const og_mom *mo, const og_grid *g, const double *ubv, i
 {
     return mom_island(mo, g, ubv, isl, indj);
 }
+
+/* ------------------------------------------------------------------------ */
+/* EMBM surflux without ENTS (embm.f90:2548-3738)                            */
+/* ------------------------------------------------------------------------ */
+/* embm_lib.f90:36-157 (PARAMETERs, REAL*8) */
+static const double E_RHOAIR = 1.25, E_RHO0 = 1.0E3, E_RHOAO = 1.25 / 1.0E3;
+static const double E_CPA = 1004.0, E_HLV = 2.501E6, E_HLS = 2.501E6 + 3.34E5;  /* hls = hlv + hlf */
+static const double E_CO20 = 278.0E-6, E_CH40 = 700.0E-9, E_N2O0 = 275.0E-9;
+static const double E_ALPHACH4 = 0.036, E_ALPHAN2O = 0.12;
+static const double E_B00 = 2.43414E2, E_B10 = -3.47968E1, E_B20 = 1.02790E1;
+static const double E_B01 = 2.60065, E_B11 = -1.62064, E_B21 = 6.34856E-1;
+static const double E_B02 = 4.40272E-3, E_B12 = -2.26092E-2, E_B22 = 1.12265E-2;
+static const double E_B03 = -2.05237E-5, E_B13 = -9.67000E-5, E_B23 = 5.62925E-5;
+static const double E_EMO = 0.94 * 5.67E-8, E_EMA = 0.85 * 5.67E-8;  /* 0.94, 0.85 sigma */
+static const double E_TFREEZ = 0.0;
+static const double E_CONST1 = 3.80E-3, E_CONST2 = 21.87, E_CONST3 = 265.5, E_CONST4 = 17.67, E_CONST5 = 243.5;
+static const double E_CD = 0.0013;
+static const double E_CONSIC = 2.166;
+/* rhmin = 1/hmin, rhooi = rho0/rhoice, rrholf = 1/(rhoice hlf): hmin 0.01, rhoice 913, hlf 3.34E5 */
+static const double E_RHMIN = 1.0 / 0.01, E_RHOOI = 1.0E3 / 913.0, E_RRHOLF = 1.0 / (913.0 * 3.34E5);
+static const double E_M2MM = 1000.0, E_MM2M = 1.0 / 1000.0;
+static const double E_HATMBL2 = 1800.;   /* hatmbl(2), embm.f90:1081 */
+static const double E_SALN0 = 34.9, E_ZEROC = 273.15;
+static const double E_CPSC = 3981.1;
+
+#define F2(a, i, j) ((a)[(size_t)((i) - 1) + (size_t)p->maxi * (size_t)((j) - 1)])
+#define SK1(i, j) (k1[(i) + (p->maxi + 2) * (j)])
+
+/* ch4_func, embm.f90:3771-3777 */
+static double sfx_ch4_func(double ch4, double n2o)
+{
+    return 0.47 * log(1.0 + 2.01E-5 * pow(ch4 * n2o, 0.75) + 5.31E-15 * ch4 * pow(ch4 * n2o, 1.52));
+}
+
+static double fmaxf_(double a, double b) { return b > a ? b : a; }  /* MAX(a, b) */
+static double fminf_(double a, double b) { return b < a ? b : a; }  /* MIN(a, b) */
+
+void og_sfx_usurf(const og_sfx_par *p, const og_grid *g, const double *sxu, const double *syu,
+                  const double *sxv, const double *syv, double *usurf)
+{
+    const int maxi = p->maxi, maxj = p->maxj, maxk = p->maxk;
+    const double dzz = g->dz[maxk] * g->dza[maxk - 1] / 2;           /* embm.f90:808 */
+    const double den = RH0SC * DSC * USC * FSC;
+    double *tau1 = malloc(sizeof(double) * maxi * maxj), *tau2 = malloc(sizeof(double) * maxi * maxj);
+    /* 2756-2768 */
+    for (int j = 1; j <= maxj; j++)
+        for (int i = 1; i <= maxi; i++) {
+            const double dztau1 = p->scf * F2(sxu, i, j) / den / dzz;
+            const double dztav2 = p->scf * F2(syv, i, j) / den / dzz;
+            F2(tau1, i, j) = dztau1 * dzz;
+            F2(tau2, i, j) = dztav2 * dzz;
+        }
+    (void)syu;
+    (void)sxv;
+    /* 2789-2810 */
+    for (int j = 1; j <= maxj; j++) {
+        double tv3 = 0.0;
+        for (int i = 1; i <= maxi; i++) {
+            const double tv = i == 1 ? (F2(tau1, i, j) + F2(tau1, maxi, j)) / 2 : (F2(tau1, i, j) + F2(tau1, i - 1, j)) / 2;
+            const double tv2 = j == 1 ? F2(tau2, i, j) / 2 : (F2(tau2, i, j) + F2(tau2, i, j - 1)) / 2;
+            F2(usurf, i, j) = sqrt(sqrt(tv * tv + tv2 * tv2) * RH0SC * DSC * USC * FSC / (E_RHOAIR * E_CD * p->scf));
+            tv3 = tv3 + F2(usurf, i, j);
+        }
+        if (p->polar_avg != 2)
+            if (j <= 2 || j >= maxj - 1)
+                for (int i = 1; i <= maxi; i++) F2(usurf, i, j) = tv3 / maxi;
+    }
+    free(tau1);
+    free(tau2);
+}
+
+void og_surflux(const og_sfx_par *p, const og_grid *g, const int32_t *k1, og_sfx_state *st,
+                const double *usurf, const double *otemp, const double *osaln, const double *pco2,
+                double *out, int32_t *iters)
+{
+    const int maxi = p->maxi, maxj = p->maxj, maxk = p->maxk;
+    const size_t nc = (size_t)maxi * maxj;
+#define O(f, i, j) (out[(size_t)(f) * nc + (size_t)((i) - 1) + (size_t)maxi * (size_t)((j) - 1)])
+    const double tol = 1.0E-10;      /* 2673 */
+    const int itice = 21;            /* 2637 */
+    const int64_t istot = st->istep + 1;   /* istep_ocn of this call (genie.f90:276) */
+    const int imth = (int)((istot - 1) % p->nyear) + 1;
+    /* greenhouse gases (2853-2926): option 1 or 4; the fields are uniform,
+     * so one value stands for every cell */
+    double co2 = st->co2, ch4 = st->ch4, n2o = st->n2o;
+    if (!p->radfor_atchem) {
+        co2 = (1.0 + p->rate_co2) * co2;
+        ch4 = (1.0 + p->rate_ch4) * ch4;
+        n2o = (1.0 + p->rate_n2o) * n2o;
+        st->co2 = co2;
+        st->ch4 = ch4;
+        st->n2o = n2o;
+    }
+    double *runoff = calloc(nc, sizeof(double));
+    double *pptn = calloc(nc, sizeof(double)), *evap = calloc(nc, sizeof(double));
+    double *evapsic = calloc(nc, sizeof(double));
+    /* 2931-2950 */
+    memset(out, 0, sizeof(double) * OG_SF_N * nc);
+    for (int j = 1; j <= maxj; j++)
+        for (int i = 1; i <= maxi; i++) O(OG_SF_ALBEDO, i, j) = F2(p->albcl, i, j);
+    /* 2953 */
+    double meantemp = 0.0;
+    for (int j = 1; j <= maxj; j++)
+        for (int i = 1; i <= maxi; i++) meantemp = meantemp + F2(st->atemp, i, j);
+    meantemp = meantemp / (double)(maxj * maxi);
+    for (int i = 1; i <= maxi; i++)
+        for (int j = 1; j <= maxj; j++) {
+            double *ashum = &F2(st->ashum, i, j), *tice = &F2(st->tice, i, j);
+            const double atemp = F2(st->atemp, i, j), sica = F2(st->sica, i, j), sich = F2(st->sich, i, j);
+            const double us = F2(usurf, i, j), ca = F2(p->ca, i, j);
+            if (iters) F2(iters, i, j) = 0;
+            double cco2 = co2, cch4 = ch4, cn2o = n2o;
+            if (p->radfor_atchem) {
+                /* 2870-2887: dum_sfcatm(3) is the CO2; CH4 and N2O are not
+                 * carried, so they read zero and take their references */
+                const double pc = F2(pco2, i, j);
+                cco2 = pc < E_CO20 / 1000.0 ? E_CO20 : pc;
+                cch4 = E_CH40;
+                cn2o = E_N2O0;
+            }
+            /* 2973-2991 */
+            const double qsata = E_CONST1 * exp(E_CONST4 * atemp / (atemp + E_CONST5));
+            F2(pptn, i, j) = fmaxf_(0.0, (*ashum - p->rmax * qsata) * E_RHOAO * E_HATMBL2 * p->rdtdim);
+            /* 3000-3006 */
+            *ashum = fminf_(*ashum, p->rmax * qsata);
+            const double rq = *ashum / qsata;
+            /* 3020-3024 */
+            const double fxsw = p->solfor[(size_t)(j - 1) + (size_t)maxj * (size_t)(imth - 1)] * (1.0 - F2(p->albcl, i, j));
+            /* 3047-3077 */
+            const double tv0 = E_B00 + rq * (E_B10 + E_B20 * rq);
+            const double tv1 = E_B01 + rq * (E_B11 + E_B21 * rq);
+            const double tv2 = E_B02 + rq * (E_B12 + E_B22 * rq);
+            const double tv3 = E_B03 + rq * (E_B13 + E_B23 * rq);
+            const double ch4_term = E_ALPHACH4 * (sqrt(1.0E9 * cch4) - sqrt(1.0E9 * E_CH40)) -
+                                    sfx_ch4_func(1.0E9 * cch4, 1.0E9 * E_N2O0) + sfx_ch4_func(1.0E9 * E_CH40, 1.0E9 * E_N2O0);
+            const double n2o_term = E_ALPHAN2O * (sqrt(1.0E9 * cn2o) - sqrt(1.0E9 * E_N2O0)) -
+                                    sfx_ch4_func(1.0E9 * E_CH40, 1.0E9 * cn2o) + sfx_ch4_func(1.0E9 * E_CH40, 1.0E9 * E_N2O0);
+            const double fxplw = tv0 + atemp * (tv1 + atemp * (tv2 + atemp * tv3)) - p->delf2x * log(cco2 / E_CO20) -
+                                 ch4_term - n2o_term + p->olr_adj * (meantemp - p->t_eqm) - p->olr_adj0;
+            /* 3083 */
+            const double fxlata = E_RHO0 * F2(pptn, i, j) * E_HLV;
+            O(OG_SF_FXSW, i, j) = fxsw;
+            O(OG_SF_PPTN, i, j) = F2(pptn, i, j);
+            if (SK1(i, j) <= maxk) {
+                const double ot = F2(otemp, i, j);
+                /* 3088-3103 */
+                double alw = atemp + E_ZEROC;
+                alw = alw * alw;
+                alw = alw * alw;
+                alw = E_EMA * alw;
+                const double salt = E_SALN0 + F2(osaln, i, j);
+                const double tsfreez = salt * (-0.0575 + 0.0017 * sqrt(salt) - 0.0002 * salt);
+                const double qb = p->rsictscsf * (tsfreez - ot);
+                O(OG_SF_TSFREEZ, i, j) = tsfreez;
+                double qbsic = qb;
+                double albsic, fx0sica, dhsic, atm_latenti, atm_sensiblei, atm_netsoli, atm_netlongi;
+                if (sica > 0.0) {
+                    /* 3106-3227 */
+                    albsic = fmaxf_(p->albsic_min, fminf_(p->albsic_max, 0.40 - 0.04 * atemp));
+                    const double fxswsic = p->solfor[(size_t)(j - 1) + (size_t)maxj * (size_t)(imth - 1)] * (1.0 - albsic);
+                    double cesic, chsic, cfxsensic, qsatsic;
+                    for (int iter = 1; iter <= itice; iter++) {
+                        if (iters) F2(iters, i, j) = iter;
+                        const double ticold = *tice;
+                        cesic = 1.0E-3 * (1.0022 - 0.0822 * (atemp - ticold) + 0.0266 * us);
+                        cesic = fmaxf_(6.0E-5, fminf_(2.19E-3, cesic));
+                        chsic = 0.94 * cesic;
+                        cfxsensic = E_RHOAIR * chsic * E_CPA * us;
+                        qsatsic = E_CONST1 * exp(E_CONST2 * ticold / (ticold + E_CONST3));
+                        F2(evapsic, i, j) = fmaxf_(0.0, (qsatsic - *ashum) * E_RHOAO * cesic * us);
+                        const double tk = ticold + E_ZEROC, tc3 = ticold + E_CONST3;
+                        const double tieqn = sich * ((1 - ca) * fxswsic + alw - E_EMO * powi(tk, 4) -
+                                                     cfxsensic * (ticold - atemp) - E_RHO0 * E_HLS * F2(evapsic, i, j)) +
+                                             E_CONSIC * (tsfreez - ticold);
+                        const double dtieq = sich * (-4.0 * E_EMO * powi(tk, 3) - cfxsensic -
+                                                     E_HLS * E_RHOAIR * cesic * us * qsatsic * E_CONST2 * E_CONST3 /
+                                                         powi(tc3, 2) * 0.5 * (1.0 + copysign(1.0, qsatsic - *ashum))) -
+                                             E_CONSIC;
+                        *tice = ticold - tieqn / dtieq;
+                        if (fabs(*tice - ticold) < tol || (ticold > E_TFREEZ && tieqn > 0.0)) break;
+                    }
+                    /* label 10 */
+                    *tice = fminf_(E_TFREEZ, *tice);
+                    const double tk = *tice + E_ZEROC;
+                    const double fxlwsic = E_EMO * powi(tk, 4) - alw;
+                    cesic = 1.0E-3 * (1.0022 - 0.0822 * (atemp - *tice) + 0.0266 * us);
+                    cesic = fmaxf_(6.0E-5, fminf_(2.19E-3, cesic));
+                    chsic = 0.94 * cesic;
+                    cfxsensic = E_RHOAIR * chsic * E_CPA * us;
+                    const double fxsensic = cfxsensic * (*tice - atemp);
+                    qsatsic = E_CONST1 * exp(E_CONST2 * *tice / (*tice + E_CONST3));
+                    F2(evapsic, i, j) = fmaxf_(0.0, (qsatsic - *ashum) * E_RHOAO * cesic * us);
+                    const double fx0sic = (1 - ca) * fxswsic - fxsensic - fxlwsic - E_RHO0 * E_HLS * F2(evapsic, i, j);
+                    O(OG_SF_FX0SIC, i, j) = fx0sic;
+                    fx0sica = ca * fxswsic + fxlata + fxsensic + fxlwsic - fxplw;
+                    atm_latenti = +fxlata;
+                    atm_sensiblei = +fxsensic;
+                    atm_netsoli = +ca * fxswsic;
+                    atm_netlongi = +fxlwsic - fxplw;
+                    dhsic = E_RRHOLF * (qb - fx0sic) - E_RHOOI * F2(evapsic, i, j);
+                    if (sich >= p->sich_max)
+                        if (dhsic > 0.0) {
+                            qbsic = (0.0 + E_RHOOI * F2(evapsic, i, j)) / E_RRHOLF + fx0sic;
+                            dhsic = E_RRHOLF * (qbsic - fx0sic) - E_RHOOI * F2(evapsic, i, j);
+                        }
+                } else {
+                    /* 3228-3238 */
+                    albsic = 0.0;
+                    fx0sica = 0.0;
+                    dhsic = 0.0;
+                    F2(evapsic, i, j) = 0.0;
+                    *tice = 0.0;
+                    atm_latenti = atm_sensiblei = atm_netsoli = atm_netlongi = 0.0;
+                }
+                /* 3239-3254 */
+                const double tk = ot + E_ZEROC;
+                const double fxlw = E_EMO * powi(tk, 4) - alw;
+                double ce = 1.0E-3 * (1.0022 - 0.0822 * (atemp - ot) + 0.0266 * us);
+                ce = fmaxf_(6.0E-5, fminf_(2.19E-3, ce));
+                const double ch = 0.94 * ce;
+                const double fxsen = E_RHOAIR * ch * E_CPA * us * (ot - atemp);
+                const double qsato = E_CONST1 * exp(E_CONST4 * ot / (ot + E_CONST5));
+                F2(evap, i, j) = fmaxf_(0.0, (qsato - *ashum) * E_RHOAO * ce * us);
+                /* 3256-3277 */
+                const double fx0oa = ca * fxsw + fxlata + fxsen + fxlw - fxplw;
+                const double atm_latent = +fxlata, atm_sensible = +fxsen, atm_netsol = +ca * fxsw, atm_netlong = +fxlw - fxplw;
+                O(OG_SF_FX0A, i, j) = (1 - sica) * fx0oa + sica * fx0sica;
+                O(OG_SF_FXLHA, i, j) = (sica * atm_latenti) + ((1 - sica) * atm_latent);
+                O(OG_SF_FXSHA, i, j) = (sica * atm_sensiblei) + ((1 - sica) * atm_sensible);
+                O(OG_SF_FXSWA, i, j) = (sica * atm_netsoli) + ((1 - sica) * atm_netsol);
+                O(OG_SF_FXLWA, i, j) = (sica * atm_netlongi) + ((1 - sica) * atm_netlong);
+                /* 3279-3306 */
+                const double fx0o = (1 - ca) * fxsw - fxsen - fxlw - E_RHO0 * E_HLV * F2(evap, i, j);
+                O(OG_SF_FX0NETO, i, j) = sica * qbsic + (1 - sica) * fmaxf_(qb, fx0o);
+                O(OG_SF_FXLHO, i, j) = (1 - sica) * (-(E_RHO0 * E_HLV * F2(evap, i, j)) + fmaxf_(0.0, qb - fx0o)) + sica * qbsic;
+                O(OG_SF_FXSHO, i, j) = -((1 - sica) * fxsen);
+                O(OG_SF_FXSWO, i, j) = (1 - sica) * (1 - ca) * fxsw;
+                O(OG_SF_FXLWO, i, j) = -((1 - sica) * fxlw);
+                const double dho = fmaxf_(0.0, E_RRHOLF * (qb - fx0o));
+                O(OG_SF_DTHSIC, i, j) = sica * dhsic + (1 - sica) * dho;
+                O(OG_SF_DTAREASIC, i, j) = fmaxf_(0.0, E_RHMIN * dho * (1 - sica));
+                if (sich > 1.0E-12)
+                    O(OG_SF_DTAREASIC, i, j) = O(OG_SF_DTAREASIC, i, j) + fminf_(0.0, 0.5 * sica * sica * dhsic / sich);
+                /* 3324-3331 */
+                O(OG_SF_ALBEDO, i, j) = sica * albsic + (1 - sica) * F2(p->albcl, i, j);
+                O(OG_SF_ALBICE, i, j) = albsic;
+                O(OG_SF_FX0O, i, j) = fx0o;
+                O(OG_SF_FXSEN, i, j) = fxsen;
+                O(OG_SF_FXLW, i, j) = fxlw;
+            } else {
+                /* 3332-3360 */
+                O(OG_SF_FX0A, i, j) = fxsw + fxlata - fxplw;
+                O(OG_SF_FXLHA, i, j) = +fxlata;
+                O(OG_SF_FXSHA, i, j) = +0.0;
+                O(OG_SF_FXSWA, i, j) = +fxsw;
+                O(OG_SF_FXLWA, i, j) = -fxplw;
+                const int ir = F2(p->iroff, i, j), jr = F2(p->jroff, i, j);
+                if (p->igrid != 0)
+                    F2(runoff, ir, jr) = F2(runoff, ir, jr) + F2(pptn, i, j) * g->ds[j] * g->rds[jr];
+                else
+                    F2(runoff, ir, jr) = F2(runoff, ir, jr) + F2(pptn, i, j);
+                O(OG_SF_RUNOFF_LAND, i, j) = F2(pptn, i, j);
+            }
+        }
+    /* 3640-3679 */
+    for (int j = 1; j <= maxj; j++)
+        for (int i = 1; i <= maxi; i++) {
+            const double sica = F2(st->sica, i, j);
+            if (SK1(i, j) <= maxk) {
+                O(OG_SF_PPTN_OCN, i, j) = F2(pptn, i, j);
+                O(OG_SF_RUNOFF_OCN, i, j) = F2(runoff, i, j) + 0.0;
+                O(OG_SF_EVAP_ATM, i, j) = F2(evap, i, j) * (1 - sica) + F2(evapsic, i, j) * sica;
+                O(OG_SF_PPTN_OCN, i, j) = O(OG_SF_PPTN_OCN, i, j) + F2(p->pmeadj, i, j);
+            } else {
+                O(OG_SF_PPTN_OCN, i, j) = 0.0;
+                O(OG_SF_RUNOFF_OCN, i, j) = 0.0;
+                O(OG_SF_EVAP_ATM, i, j) = 0.0;
+            }
+            O(OG_SF_PPTN_ATM, i, j) = F2(pptn, i, j);
+            O(OG_SF_EVAP_OCN, i, j) = -O(OG_SF_EVAP_ATM, i, j);
+            O(OG_SF_PPTN_OCN, i, j) = O(OG_SF_PPTN_OCN, i, j) * E_M2MM;
+            O(OG_SF_EVAP_OCN, i, j) = O(OG_SF_EVAP_OCN, i, j) * E_M2MM;
+            O(OG_SF_RUNOFF_OCN, i, j) = O(OG_SF_RUNOFF_OCN, i, j) * E_M2MM;
+            O(OG_SF_RUNOFF_LAND, i, j) = O(OG_SF_RUNOFF_LAND, i, j) * E_M2MM;
+            O(OG_SF_PPTN_ATM, i, j) = O(OG_SF_PPTN_ATM, i, j) * E_M2MM;
+            O(OG_SF_EVAP_ATM, i, j) = O(OG_SF_EVAP_ATM, i, j) * E_M2MM;
+            O(OG_SF_EVAP, i, j) = F2(evap, i, j);
+        }
+    st->istep = istot;
+    free(runoff);
+    free(pptn);
+    free(evap);
+    free(evapsic);
+#undef O
+}
+
+void og_surface_slot(const og_sfx_par *p, const double *out, const double *fx0sic, const double *fwsic,
+                     int maxl, double *sflux)
+{
+    const size_t nc = (size_t)p->maxi * p->maxj;
+    /* goldstein_lib.f90:63 and goldstein.f90:904 */
+    const double rfluxsc = RSC / (DSC * USC * RH0SC * E_CPSC);
+    const double rpmesco = RSC * E_SALN0 / (DSC * USC);
+    for (size_t c = 0; c < nc; c++) {
+        /* 149-170; fw_hosing and fw_anom are zero without hosing or an anomaly file */
+        const double fx0neto = out[OG_SF_FXSWO * nc + c] + out[OG_SF_FXSHO * nc + c] + out[OG_SF_FXLWO * nc + c] +
+                               out[OG_SF_FXLHO * nc + c] + (fx0sic ? fx0sic[c] : 0.0);
+        double fwfxneto = out[OG_SF_PPTN_OCN * nc + c] + out[OG_SF_EVAP_OCN * nc + c] + out[OG_SF_RUNOFF_OCN * nc + c] +
+                          (fwsic ? fwsic[c] : 0.0) + 0.0 + 0.0;
+        fwfxneto = fwfxneto * E_MM2M;
+        sflux[0 + (size_t)maxl * c] = -fx0neto * rfluxsc;
+        sflux[1 + (size_t)maxl * c] = fwfxneto * rpmesco;
+    }
+}
+#undef F2
+#undef SK1

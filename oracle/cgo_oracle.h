@@ -230,6 +230,62 @@ double og_mom_island_of(const og_mom *mo, const og_grid *g, const double *ubv, i
  * initialisation, goldstein.f90:1430-1444, 1760-1765) */
 void og_rho_from_ts(const og_grid *g, int maxl, const int32_t *k1, const double *go_ts, double *rho);
 
+/* ---------------------------------------------------------------------
+ * EMBM surface fluxes, surflux (embm.f90:2548-3738) without ENTS
+ * (flag_ents = .false.): no orography, ice-sheet or time-series forcing, the
+ * greenhouse gases compounding at their namelist rates (option 4) or CO2 read
+ * from the atmosphere composition (option 1, atchem_radfor).  Set-up tables
+ * as initialise_embm leaves them; fields (maxi, maxj), column-major.
+ * --------------------------------------------------------------------- */
+typedef struct og_sfx_par {
+    int maxi, maxj, maxk, nyear;
+    int igrid;          /* 0: runoff added as is; else weighted by ds(j)/ds(jroff) */
+    int polar_avg;      /* par_wind_polar_avg: 2 leaves the polar rows' usurf alone */
+    int radfor_atchem;  /* atchem_radfor = 'y': CO2 from the atmosphere composition */
+    double scf, rmax, delf2x, rdtdim, rsictscsf, olr_adj, olr_adj0, t_eqm;
+    double albsic_min, albsic_max, sich_max, rate_co2, rate_ch4, rate_n2o;
+    const double *solfor;            /* solfor(maxj, nyear) */
+    const double *albcl, *ca, *pmeadj;
+    const int32_t *iroff, *jroff;    /* runoff destination of every cell (1-based) */
+} og_sfx_par;
+
+/* outputs of one call, each a (maxi, maxj) field, in this order */
+enum {
+    OG_SF_FXLHO = 0, OG_SF_FXSHO, OG_SF_FXSWO, OG_SF_FXLWO, OG_SF_EVAP_OCN, OG_SF_PPTN_OCN,
+    OG_SF_RUNOFF_OCN, OG_SF_RUNOFF_LAND, OG_SF_FXLHA, OG_SF_FXSHA, OG_SF_FXSWA, OG_SF_FXLWA,
+    OG_SF_EVAP_ATM, OG_SF_PPTN_ATM, OG_SF_DTHSIC, OG_SF_DTAREASIC, OG_SF_ALBEDO, OG_SF_ALBICE,
+    OG_SF_FXSW, OG_SF_FX0A, OG_SF_FX0O, OG_SF_FXSEN, OG_SF_FXLW, OG_SF_EVAP, OG_SF_PPTN,
+    /* EMBM's own arrays of the call over the ocean: fx0sic (0 without ice), fx0neto, tsfreez */
+    OG_SF_FX0SIC, OG_SF_FX0NETO, OG_SF_TSFREEZ, OG_SF_N
+};
+
+/* The state surflux carries between calls: air temperature and specific
+ * humidity (tq, clamped by each call), the sea-ice height, cover and
+ * surface temperature (tice is the Newton iteration's start), the
+ * greenhouse concentrations and the call count istep_ocn. */
+typedef struct og_sfx_state {
+    double *atemp, *ashum, *sich, *sica, *tice;   /* (maxi, maxj) */
+    double co2, ch4, n2o;
+    int64_t istep;
+} og_sfx_state;
+
+/* usurf (maxi, maxj) from the four stress fields (embm.f90:2756-2810) */
+void og_sfx_usurf(const og_sfx_par *p, const og_grid *g, const double *sxu, const double *syu,
+                  const double *sxv, const double *syv, double *usurf);
+/* One surflux call.  otemp, osaln: the ocean's top-level T and S
+ * (tstar_ocn, sstar_ocn); pco2: the atmosphere's pCO2 per cell (atm) when
+ * radfor_atchem, else ignored; out: OG_SF_N fields; iters: the sea-ice
+ * Newton iterations of every cell (or NULL). */
+void og_surflux(const og_sfx_par *p, const og_grid *g, const int32_t *k1, og_sfx_state *st,
+                const double *usurf, const double *otemp, const double *osaln, const double *pco2,
+                double *out, int32_t *iters);
+/* The surface slot ts(1:2, 1:maxi, 1:maxj, maxk+1) that step_goldstein
+ * forms from the flux components (goldstein.f90:149-170); fx0sic, fwsic the
+ * sea-ice model's heat and freshwater into the ocean (NULL = zero);
+ * sflux (maxl, maxi, maxj), only its first two tracers written. */
+void og_surface_slot(const og_sfx_par *p, const double *out, const double *fx0sic, const double *fwsic,
+                     int maxl, double *sflux);
+
 /* step_biogem restricted to the selected species (biogem.f90:596-611,
  * 873-969, 985-1105, 1693, 1811-1844).  Writes vdocn (maxl,maxi,maxj,maxk). */
 void og_biogem_step(og_bgc_state *st, const og_grid *g, const og_bgc_params *p,

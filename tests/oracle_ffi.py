@@ -65,6 +65,26 @@ class OgMom(C.Structure):
         [(n, C.POINTER(_d)) for n in ("tau", "dztau", "dztav", "u1")]
 
 
+class OgSfxPar(C.Structure):
+    _fields_ = [(n, _i) for n in ("maxi", "maxj", "maxk", "nyear", "igrid", "polar_avg", "radfor_atchem")] + \
+        [(n, _d) for n in ("scf", "rmax", "delf2x", "rdtdim", "rsictscsf", "olr_adj", "olr_adj0", "t_eqm",
+                           "albsic_min", "albsic_max", "sich_max", "rate_co2", "rate_ch4", "rate_n2o")] + \
+        [(n, C.POINTER(_d)) for n in ("solfor", "albcl", "ca", "pmeadj")] + \
+        [(n, C.POINTER(C.c_int32)) for n in ("iroff", "jroff")]
+
+
+class OgSfxState(C.Structure):
+    _fields_ = [(n, C.POINTER(_d)) for n in ("atemp", "ashum", "sich", "sica", "tice")] + \
+        [(n, _d) for n in ("co2", "ch4", "n2o")] + [("istep", C.c_int64)]
+
+
+# surflux outputs, in the oracle's order (cgo_oracle.h)
+SF_NAMES = ("fxlho", "fxsho", "fxswo", "fxlwo", "evap_ocn", "pptn_ocn", "runoff_ocn", "runoff_land", "fxlha",
+            "fxsha", "fxswa", "fxlwa", "evap_atm", "pptn_atm", "dthsic", "dtareasic", "albedo", "albice", "fxsw",
+            "fx0a", "fx0o", "fxsen", "fxlw", "evap", "pptn", "fx0sic", "fx0neto", "tsfreez")
+SF = {n: q for q, n in enumerate(SF_NAMES)}
+
+
 def build(force: bool = False) -> None:
     if force or not os.path.exists(LIB):
         subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True, capture_output=True)
@@ -109,6 +129,10 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.og_mom_island_of.argtypes = [C.POINTER(OgMom), C.POINTER(OgGrid), _dp, _i, _i]
     L.og_mom_island_of.restype = _d
     L.og_rho_from_ts.argtypes = [C.POINTER(OgGrid), _i, _i32p, _dp, _dp]
+    L.og_sfx_usurf.argtypes = [C.POINTER(OgSfxPar), C.POINTER(OgGrid), _dp, _dp, _dp, _dp, _dp]
+    L.og_surflux.argtypes = [C.POINTER(OgSfxPar), C.POINTER(OgGrid), _i32p, C.POINTER(OgSfxState), _dp, _dp, _dp,
+                             C.c_void_p, _dp, C.c_void_p]
+    L.og_surface_slot.argtypes = [C.POINTER(OgSfxPar), _dp, C.c_void_p, C.c_void_p, _i, _dp]
     return L
 
 
@@ -409,3 +433,60 @@ def rho_from_ts(g: OgGrid, k1, go_ts) -> np.ndarray:
     lib().og_rho_from_ts(C.byref(g), go_ts.shape[-1], np.ascontiguousarray(k1, dtype=np.int32),
                          np.ascontiguousarray(go_ts, dtype=np.float64), rho)
     return rho
+
+
+class SurfluxOracle:
+    """EMBM's surflux for one member (og_surflux): the set-up tables of
+    cgenie_amd.surface.SurfluxSetup, the atmosphere and sea-ice state
+    ((maxj, maxi) arrays, copied) and the wind speed usurf from the four
+    stress fields."""
+
+    def __init__(self, g: OgGrid, k1, su, atemp, ashum, sich=None, sica=None, tice=None):
+        self.g, self.su = g, su
+        self.k1 = np.ascontiguousarray(k1, dtype=np.int32)
+        mj, mi = g.maxj, g.maxi
+        z = np.zeros((mj, mi))
+        f = lambda a: np.ascontiguousarray(z if a is None else a, dtype=np.float64).copy()  # noqa: E731
+        self.atemp, self.ashum, self.sich, self.sica, self.tice = f(atemp), f(ashum), f(sich), f(sica), f(tice)
+        self.tabs = [np.ascontiguousarray(a, dtype=np.float64) for a in (su.solfor, su.albcl, su.ca, su.pmeadj)]
+        self.rout = [np.ascontiguousarray(a, dtype=np.int32) for a in (su.iroff, su.jroff)]
+        P = OgSfxPar()
+        P.maxi, P.maxj, P.maxk, P.nyear = mi, mj, g.maxk, su.solfor.shape[0]
+        for n in ("igrid", "polar_avg", "radfor_atchem", "scf", "rmax", "delf2x", "rdtdim", "rsictscsf", "olr_adj",
+                  "olr_adj0", "t_eqm", "albsic_min", "albsic_max", "sich_max", "rate_co2", "rate_ch4", "rate_n2o"):
+            setattr(P, n, getattr(su, n))
+        P.solfor, P.albcl, P.ca, P.pmeadj = [a.ctypes.data_as(C.POINTER(_d)) for a in self.tabs]
+        P.iroff, P.jroff = [a.ctypes.data_as(C.POINTER(C.c_int32)) for a in self.rout]
+        self.P = P
+        S = OgSfxState()
+        S.atemp, S.ashum, S.sich, S.sica, S.tice = [a.ctypes.data_as(C.POINTER(_d)) for a in
+                                                    (self.atemp, self.ashum, self.sich, self.sica, self.tice)]
+        S.co2, S.ch4, S.n2o, S.istep = su.co2, su.ch4, su.n2o, 0
+        self.S = S
+        self.usurf = np.zeros((mj, mi))
+        self.out = np.zeros((len(SF_NAMES), mj, mi))
+        self.iters = np.zeros((mj, mi), dtype=np.int32)
+
+    def set_wind(self, stress: np.ndarray):
+        """stress (maxj, maxi, 4) as MomOracle.set_wind takes it."""
+        st = np.ascontiguousarray(stress, dtype=np.float64)
+        parts = [np.ascontiguousarray(st[..., q]) for q in range(4)]
+        lib().og_sfx_usurf(C.byref(self.P), C.byref(self.g), *parts, self.usurf)
+
+    def step(self, otemp: np.ndarray, osaln: np.ndarray, pco2: np.ndarray | None = None) -> np.ndarray:
+        """One call; otemp, osaln (maxj, maxi) the ocean's top level; returns the outputs (SF_N, maxj, maxi)."""
+        pc = None if pco2 is None else np.ascontiguousarray(pco2, dtype=np.float64)
+        self._pc = pc
+        lib().og_surflux(C.byref(self.P), C.byref(self.g), self.k1, C.byref(self.S), self.usurf,
+                         np.ascontiguousarray(otemp, dtype=np.float64), np.ascontiguousarray(osaln, dtype=np.float64),
+                         None if pc is None else pc.ctypes.data_as(C.c_void_p), self.out,
+                         self.iters.ctypes.data_as(C.c_void_p))
+        return self.out
+
+    def slot(self, maxl: int, fx0sic=None, fwsic=None) -> np.ndarray:
+        """The surface slot sflux (maxj, maxi, maxl) from the last call's outputs."""
+        sf = np.zeros((self.g.maxj, self.g.maxi, maxl))
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (fx0sic, fwsic)]
+        lib().og_surface_slot(C.byref(self.P), self.out, *[None if a is None else a.ctypes.data_as(C.c_void_p)
+                                                           for a in keep], maxl, sf)
+        return sf

@@ -16,6 +16,7 @@ import pytest
 
 import oracle_ffi as O
 import test_oracle as T
+import test_surflux as S
 import test_velocity as V
 
 SRC = os.path.join(O.ORACLE_DIR, "cgo_oracle.c")
@@ -99,9 +100,50 @@ MUTANTS = [
      "(MRHO(i, j, k) + MRHO(i, j, k - 1)) * g->dza[k - 1] * 0.5;", "(MRHO(i, j, k) + MRHO(i, j, k - 1)) * 0.5;", 1),
     ("drag: equatorial band one row narrower",
      "if (kloc2 > kmxdrg || abs(j - MJ / 2) <= jeb)", "if (kloc2 > kmxdrg || abs(j - MJ / 2) < jeb)", 1),
+    # EMBM surflux, embm.f90:2756-3679, and the surface slot, goldstein.f90:149-170
+    ("ocean heat loss: sensible heat with the wrong sign",
+     "const double fx0o = (1 - ca) * fxsw - fxsen - fxlw", "const double fx0o = (1 - ca) * fxsw + fxsen - fxlw", 1),
+    ("atmosphere heat gain: longwave with the wrong sign",
+     "const double fx0oa = ca * fxsw + fxlata + fxsen + fxlw - fxplw;",
+     "const double fx0oa = ca * fxsw + fxlata + fxsen - fxlw - fxplw;", 1),
+    ("runoff destination indices swapped",
+     "F2(runoff, ir, jr) = F2(runoff, ir, jr) + F2(pptn, i, j);",
+     "F2(runoff, jr, ir) = F2(runoff, jr, ir) + F2(pptn, i, j);", 1),
+    ("runoff area weight from the source row",
+     "F2(pptn, i, j) * g->ds[j] * g->rds[jr];", "F2(pptn, i, j) * g->ds[j] * g->rds[j];", 1),
+    ("Dalton number bounds swapped",
+     "ce = fmaxf_(6.0E-5, fminf_(2.19E-3, ce));", "ce = fminf_(6.0E-5, fmaxf_(2.19E-3, ce));", 1),
+    ("ice surface: conduction with the wrong sign",
+     "E_CONSIC * (tsfreez - ticold);", "E_CONSIC * (ticold - tsfreez);", 1),
+    ("ice Newton step: derivative of the emission with the wrong sign",
+     "const double dtieq = sich * (-4.0 * E_EMO", "const double dtieq = sich * (4.0 * E_EMO", 1),
+    ("CO2 forcing with the wrong sign",
+     "- p->delf2x * log(cco2 / E_CO20)", "+ p->delf2x * log(cco2 / E_CO20)", 1),
+    ("insolation month one step late",
+     "const int imth = (int)((istot - 1) % p->nyear) + 1;", "const int imth = (int)(istot % p->nyear) + 1;", 1),
+    ("wind speed: one polar row zonally averaged",
+     "if (j <= 2 || j >= maxj - 1)", "if (j <= 1 || j >= maxj)", 1),
+    ("rain without the humidity ceiling rmax",
+     "(*ashum - p->rmax * qsata) * E_RHOAO", "(*ashum - qsata) * E_RHOAO", 1),
+    ("freshwater adjustment dropped",
+     "O(OG_SF_PPTN_OCN, i, j) + F2(p->pmeadj, i, j);", "O(OG_SF_PPTN_OCN, i, j);", 1),
+    ("freezing point: S instead of sqrt(S)",
+     "(-0.0575 + 0.0017 * sqrt(salt) - 0.0002 * salt)", "(-0.0575 + 0.0017 * salt - 0.0002 * salt)", 1),
+    ("ocean latent heat without the ice formed in open water",
+     "(-(E_RHO0 * E_HLV * F2(evap, i, j)) + fmaxf_(0.0, qb - fx0o))", "(-(E_RHO0 * E_HLV * F2(evap, i, j)))", 1),
+    ("heat slot with the wrong sign",
+     "sflux[0 + (size_t)maxl * c] = -fx0neto * rfluxsc;", "sflux[0 + (size_t)maxl * c] = fx0neto * rfluxsc;", 1),
+    ("freshwater slot left in mm",
+     "fwfxneto = fwfxneto * E_MM2M;", "fwfxneto = fwfxneto;", 1),
+    ("wind speed: row 1 without the half v stress",
+     "const double tv2 = j == 1 ? F2(tau2, i, j) / 2", "const double tv2 = j == 1 ? F2(tau2, i, j)", 1),
+    ("sea-ice albedo rising with air temperature",
+     "0.40 - 0.04 * atemp", "0.40 + 0.04 * atemp", 1),
+    ("ice growth cap: open-water growth kept",
+     "O(OG_SF_DTHSIC, i, j) = sica * dhsic + (1 - sica) * dho;", "O(OG_SF_DTHSIC, i, j) = sica * dhsic + dho;", 1),
 ]
 
-# the pins (tests/test_oracle.py, tests/test_velocity.py), cheapest first;
+# the pins (tests/test_oracle.py, tests/test_velocity.py, tests/test_surflux.py), cheapest first;
 # the carbonate chemistry's compiled-reference comparison is represented by
 # its golden vectors (the first case of test_carbchem_matches_reference_vectors)
 PINS = [
@@ -122,6 +164,14 @@ PINS = [
     V.test_worbe2_island_circulation_vanishes, V.test_worbe2_solve_is_linear_in_the_forcing,
     V.test_worbe2_velocity_leaves_a_uniform_tracer_uniform, V.test_worbe2_no_wind_and_level_density_give_no_flow,
     V.test_drag_values, V.test_bottom_pressure_is_the_hydrostatic_integral,
+    S.test_usurf_inverts_the_bulk_stress_formula, S.test_usurf_polar_rows_are_zonal_means,
+    S.test_ocean_and_atmosphere_heat_closures, S.test_air_sea_exchange_cancels_between_air_and_sea,
+    S.test_rain_is_the_humidity_the_clamp_removes, S.test_saturation_humidity_bolton,
+    S.test_freezing_point_millero, S.test_runoff_arrives_where_routed_and_totals_the_land_rain,
+    S.test_freshwater_adjustment_and_evaporation_signs, S.test_bulk_transfer_coefficient_is_observed_magnitude,
+    S.test_sea_ice_albedo_falls_with_air_temperature, S.test_sea_ice_temperature_is_the_surface_energy_balance_root,
+    S.test_sea_ice_growth_cap, S.test_co2_doubling_forcing, S.test_compound_co2_rise, S.test_seasonal_cycle_period,
+    S.test_surface_slot_heats_and_freshens_the_top_layer,
 ]
 # the reference's own island() where it is built (oracle/_ref): the one pin of
 # the island integral's Coriolis term, which no closed form fixes (the
