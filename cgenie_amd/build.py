@@ -3,7 +3,7 @@
 Each translation unit is compiled on its own (``hipcc -c``, parallel) and
 linked into ``cgenie_amd/libcgocn.so``:
 
-* k_tstep_f64.hip, k_bgc.hip, k_mom.hip  -ffp-contract=off: operations round
+* k_tstep_f64.hip, k_bgc.hip, k_mom.hip, k_sfx.hip  -ffp-contract=off: operations round
   as the reference writes them (the FP64 path is bit-comparable with the oracle)
 * k_tstep_f32.hip              -ffp-contract=fast: the FP32 form
 * cgocn_api.hip                host side of the C ABI
@@ -43,6 +43,7 @@ UNITS = {
     "k_tstep_f64.hip": ["-ffp-contract=off"],
     "k_bgc.hip": ["-ffp-contract=off"],
     "k_mom.hip": ["-ffp-contract=off"],
+    "k_sfx.hip": ["-ffp-contract=off"],
     "k_tstep_f32.hip": ["-ffp-contract=fast"],
     "cgocn_api.hip": ["-ffp-contract=off"],
 }

@@ -32,6 +32,8 @@ void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
 void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, int maxcol, int maxn, hipStream_t);
 size_t mom_solve_lds(int N, int n);
+void launch_sfx(const MemberDev*, const int32_t*, int nlist, int maxcol, hipStream_t);
+size_t sfx_lds(int ncol);
 size_t mom_ring_terms(int n);
 }  // namespace cgo
 
@@ -64,6 +66,13 @@ struct GridH {
     void* mbuf = nullptr;
     const MomGridDev* d_mg = nullptr;
     uint64_t mom_fp = 0;
+    // surface fluxes (cgo_grid_set_surflux): the device tables, the settings
+    // the host side needs (wind speed, initial gases) and a fingerprint
+    void* sbuf = nullptr;
+    const SfxGridDev* d_sg = nullptr;
+    double sfx_scf = 2.0, sfx_gas[3] = {0, 0, 0};
+    int sfx_polar = 0, sfx_atchem = 0;
+    uint64_t sfx_fp = 0;
 };
 
 struct MemberH {
@@ -84,6 +93,11 @@ struct MemberH {
     // velocity solve on (cgo_set_wind): the member's arrays of the solve
     void* mbuf = nullptr;
     size_t mbytes = 0;
+    // the last wind stress (cgo_set_wind), [4][ncol]: sxu, syu, sxv, syv
+    std::vector<double> stress;
+    // surface fluxes on (cgo_set_surface): the member's fields, outputs and scalars
+    void* sbuf = nullptr;
+    size_t sbytes = 0;
 };
 
 struct Graph {
@@ -618,6 +632,7 @@ static int alloc_member(cgo_engine* e, MemberH* m) {
         d.mink = reinterpret_cast<const int32_t*>(b + ly.mink);
     }
     m->dev.sflux = had_sflux ? reinterpret_cast<double*>(b + ly.sflux) : nullptr;
+    d.sfw = m->sbuf ? reinterpret_cast<double*>(b + ly.sflux) : nullptr;
     if (m->mbuf && m->prec == CGO_FP64) d.u64 = reinterpret_cast<double*>(b + ly.u);
     e->table_dirty = true;
     return CGO_OK;
@@ -668,8 +683,10 @@ int cgo_member_destroy(cgo_engine* e, int32_t id) {
     if (rc) return rc;
     if (m->dbuf) e->graveyard.push_back(m->dbuf);
     if (m->mbuf) e->graveyard.push_back(m->mbuf);
+    if (m->sbuf) e->graveyard.push_back(m->sbuf);
     m->dbuf = nullptr;
     m->mbuf = nullptr;
+    m->sbuf = nullptr;
     m->alive = false;
     e->table_dirty = true;
     bump_gen(e, id);
@@ -942,14 +959,264 @@ static int mom_attach(cgo_engine* e, MemberH* m, int32_t id) {
     return CGO_OK;
 }
 
+// ---------------------------------------------------------------- surface fluxes
+static int download_dev(cgo_engine* e, MemberH* m, std::vector<double>& h);
+// embm_lib.f90:37-85: the scales of the wind stress and the bulk drag
+static constexpr double SFX_DSC = 5.0E3, SFX_RHOAIR = 1.25, SFX_CD = 0.0013;
+
+// usurf, the wind speed of the bulk formulas, from the last wind stress
+// (surflux, embm.f90:2756-2810): the stress's magnitude at the T point from
+// its x at the u points either side and y at the v points either side,
+// inverted through |tau| = rho_air cd u^2; the two rows nearest each pole
+// take their zonal mean unless par_wind_polar_avg is 2
+static void sfx_usurf(const GridH* g, const std::vector<double>& st, std::vector<double>& us) {
+    const int mi = g->d.maxi, mj = g->d.maxj, mk = g->d.maxk, KS = mk + 2;
+    const size_t ncol = (size_t)mi * mj;
+    us.assign(ncol, 0.0);
+    if (st.size() != 4 * ncol) return;
+    const double dzz = g->mk[MK_DZ * KS + mk] * g->mk[MK_DZA * KS + mk - 1] / 2;
+    const double den = MOM_RH0SC * SFX_DSC * MOM_USC * MOM_FSC;
+    const double scf = g->sfx_scf;
+    std::vector<double> t1(ncol), t2(ncol);
+    for (size_t c = 0; c < ncol; c++) {
+        t1[c] = scf * st[c] / den / dzz * dzz;
+        t2[c] = scf * st[3 * ncol + c] / den / dzz * dzz;
+    }
+    for (int j = 1; j <= mj; j++) {
+        double tv3 = 0.0;
+        for (int i = 1; i <= mi; i++) {
+            const size_t c = (size_t)(i - 1) + (size_t)mi * (j - 1);
+            const size_t cw = i == 1 ? (size_t)(mi - 1) + (size_t)mi * (j - 1) : c - 1;
+            const double tv = (t1[c] + t1[cw]) / 2;
+            const double tv2 = j == 1 ? t2[c] / 2 : (t2[c] + t2[c - mi]) / 2;
+            us[c] = sqrt(sqrt(tv * tv + tv2 * tv2) * MOM_RH0SC * SFX_DSC * MOM_USC * MOM_FSC / (SFX_RHOAIR * SFX_CD * scf));
+            tv3 = tv3 + us[c];
+        }
+        if (g->sfx_polar != 2 && (j <= 2 || j >= mj - 1))
+            for (int i = 1; i <= mi; i++) us[(size_t)(i - 1) + (size_t)mi * (j - 1)] = tv3 / mi;
+    }
+}
+
+int cgo_grid_set_surflux(cgo_engine* e, int32_t grid_id, const cgo_surflux_desc* d) {
+    if (!e || !d || grid_id < 0 || grid_id >= (int)e->grids.size()) return fail(e, CGO_E_ARG, "grid_set_surflux: bad argument");
+    if (!d->solfor || !d->albcl || !d->ca || !d->iroff || !d->jroff || d->nyear < 1)
+        return fail(e, CGO_E_ARG, "grid_set_surflux: null table or nyear %d", d->nyear);
+    GridH* g = e->grids[grid_id];
+    if (g->sbuf) return fail(e, CGO_E_STATE, "grid_set_surflux: grid %d already has its surface fluxes", grid_id);
+    const int mi = g->d.maxi, mj = g->d.maxj, mk = g->d.maxk;
+    const int ncol = mi * mj;
+    if (sfx_lds(ncol) > 65536) return fail(e, CGO_E_ARG, "grid_set_surflux: %d columns above the 8192 one workgroup holds", ncol);
+    auto K1 = [&](int I, int J) { return g->k1[(size_t)J * (mi + 2) + I]; };
+    // runoff gather lists: each wet column's land sources in the order the
+    // reference's loop visits them (i outer, j inner, embm.f90:2955-2956)
+    std::vector<std::vector<int32_t>> src(ncol);
+    for (int i = 1; i <= mi; i++)
+        for (int j = 1; j <= mj; j++) {
+            if (K1(i, j) <= mk) continue;
+            const size_t c = (size_t)(i - 1) + (size_t)mi * (j - 1);
+            const int ir = d->iroff[c], jr = d->jroff[c];
+            if (ir < 1 || ir > mi || jr < 1 || jr > mj || K1(ir, jr) > mk)
+                return fail(e, CGO_E_ARG, "grid_set_surflux: runoff of (%d,%d) routed to (%d,%d), not an ocean cell", i, j, ir, jr);
+            src[(size_t)(ir - 1) + (size_t)mi * (jr - 1)].push_back((int32_t)c);
+        }
+    std::vector<int32_t> rstart(ncol + 1, 0), rsrc;
+    for (int c = 0; c < ncol; c++) {
+        rstart[c] = (int32_t)rsrc.size();
+        rsrc.insert(rsrc.end(), src[c].begin(), src[c].end());
+    }
+    rstart[ncol] = (int32_t)rsrc.size();
+    std::vector<double> pm(ncol, 0.0);
+    if (d->pmeadj) std::copy(d->pmeadj, d->pmeadj + ncol, pm.begin());
+    // one allocation: the struct, then the tables
+    size_t off = 0;
+    auto take = [&](size_t nb) {
+        size_t o = off;
+        off += (nb + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_s = take(sizeof(SfxGridDev)), o_sol = take((size_t)d->nyear * mj * 8), o_alb = take((size_t)ncol * 8),
+                 o_ca = take((size_t)ncol * 8), o_pm = take((size_t)ncol * 8), o_rs = take((size_t)(ncol + 1) * 4),
+                 o_src = take(std::max<size_t>(rsrc.size(), 1) * 4);
+    void* buf = nullptr;
+    hipSetDevice(e->device);
+    if (hipMalloc(&buf, off) != hipSuccess) return fail(e, CGO_E_NOMEM, "grid_set_surflux: hipMalloc(%zu)", off);
+    char* b = static_cast<char*>(buf);
+    SfxGridDev h{};
+    h.nyear = d->nyear;
+    h.igrid = d->igrid;
+    h.radfor_atchem = d->radfor_atchem ? 1 : 0;
+    h.nsrc = (int32_t)rsrc.size();
+    h.rmax = d->rmax;
+    h.delf2x = d->delf2x;
+    h.rdtdim = d->rdtdim;
+    h.rsictscsf = d->rsictscsf;
+    h.olr_adj = d->olr_adj;
+    h.olr_adj0 = d->olr_adj0;
+    h.t_eqm = d->t_eqm;
+    h.albsic_min = d->albsic_min;
+    h.albsic_max = d->albsic_max;
+    h.sich_max = d->sich_max;
+    h.rate_co2 = d->rate_co2;
+    h.rate_ch4 = d->rate_ch4;
+    h.rate_n2o = d->rate_n2o;
+    h.solfor = reinterpret_cast<const double*>(b + o_sol);
+    h.albcl = reinterpret_cast<const double*>(b + o_alb);
+    h.ca = reinterpret_cast<const double*>(b + o_ca);
+    h.pmeadj = reinterpret_cast<const double*>(b + o_pm);
+    h.rstart = reinterpret_cast<const int32_t*>(b + o_rs);
+    h.rsrc = reinterpret_cast<const int32_t*>(b + o_src);
+    bool ok = hipMemcpy(b + o_s, &h, sizeof h, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b + o_sol, d->solfor, (size_t)d->nyear * mj * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b + o_alb, d->albcl, (size_t)ncol * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b + o_ca, d->ca, (size_t)ncol * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b + o_pm, pm.data(), (size_t)ncol * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b + o_rs, rstart.data(), rstart.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              (rsrc.empty() || hipMemcpy(b + o_src, rsrc.data(), rsrc.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+    if (!ok) {
+        hipFree(buf);
+        return fail(e, CGO_E_HIP, "grid_set_surflux: upload failed");
+    }
+    uint64_t fp = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t nb) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        for (size_t x = 0; x < nb; x++) fp = (fp ^ q[x]) * 1099511628211ull;
+    };
+    mix(&h.nyear, 4 * sizeof(int32_t));
+    mix(&h.rmax, 13 * sizeof(double));
+    mix(d->solfor, (size_t)d->nyear * mj * 8);
+    mix(d->albcl, (size_t)ncol * 8);
+    mix(d->ca, (size_t)ncol * 8);
+    mix(pm.data(), (size_t)ncol * 8);
+    mix(rsrc.data(), rsrc.size() * 4);
+    mix(&d->scf, 8);
+    mix(&d->polar_avg, 4);
+    g->sbuf = buf;
+    g->d_sg = reinterpret_cast<const SfxGridDev*>(b + o_s);
+    g->sfx_scf = d->scf;
+    g->sfx_polar = d->polar_avg;
+    g->sfx_atchem = h.radfor_atchem;
+    g->sfx_gas[0] = d->co2;
+    g->sfx_gas[1] = d->ch4;
+    g->sfx_gas[2] = d->n2o;
+    g->sfx_fp = fp;
+    return CGO_OK;
+}
+
+// the top level's T and S of a state in device order [l][k][j][i]
+static void sfx_top(const GridDev& g, const std::vector<double>& st, double* t, double* s) {
+    const size_t top = (size_t)(g.maxk - 1) * g.ncol;
+    for (int c = 0; c < g.ncol; c++) {
+        t[c] = st[top + c];
+        s[c] = st[(size_t)g.ncell + top + c];
+    }
+}
+
+int cgo_set_surface(cgo_engine* e, int32_t id, const double* atemp, const double* ashum, const double* sich,
+                    const double* sica, const double* tice, const double* fx0sic, const double* fwsic) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "set_surface: bad member %d", id);
+    const GridH* g = e->grids[m->grid];
+    if (int rc = settle(e, m)) return rc;
+    hipSetDevice(e->device);
+    if (!atemp) {
+        if (m->sbuf) {
+            e->graveyard.push_back(m->sbuf);
+            m->sbuf = nullptr;
+            m->sbytes = 0;
+            m->dev.sg = nullptr;
+            m->dev.sw = nullptr;
+            m->dev.sfw = nullptr;
+            m->dev.sflux = nullptr;
+            e->table_dirty = true;
+            bump_gen(e, id);
+        }
+        return CGO_OK;
+    }
+    if (!g->sbuf) return fail(e, CGO_E_STATE, "set_surface: grid %d has no surface fluxes (cgo_grid_set_surflux)", m->grid);
+    if (!ashum) return fail(e, CGO_E_ARG, "set_surface: ashum is needed with atemp");
+    const size_t ncol = (size_t)m->dev.g.ncol;
+    const size_t nd = sfx_member_doubles((int)ncol);
+    std::vector<double> h(SW_SSTAR * ncol + ncol, 0.0);
+    const double* src[SW_USURF] = {atemp, ashum, sich, sica, tice};
+    for (int f = 0; f < SW_USURF; f++)
+        if (src[f]) std::copy(src[f], src[f] + ncol, h.begin() + f * ncol);
+    if (fx0sic) std::copy(fx0sic, fx0sic + ncol, h.begin() + SW_FX0SIC * ncol);
+    if (fwsic) std::copy(fwsic, fwsic + ncol, h.begin() + SW_FWSIC * ncol);
+    std::vector<double> us;
+    sfx_usurf(g, m->stress, us);
+    std::copy(us.begin(), us.end(), h.begin() + SW_USURF * ncol);
+    if (m->sbuf) {
+        // already on: new air and sea-ice fields; the top level, the outputs,
+        // the gases and the call count stay
+        HIPCHK(e, hipMemcpy(m->sbuf, h.data(), SW_TSTAR * ncol * 8, hipMemcpyHostToDevice));
+        return CGO_OK;
+    }
+    std::vector<double> st;
+    if (int rc = download_dev(e, m, st)) return rc;
+    sfx_top(m->dev.g, st, &h[SW_TSTAR * ncol], &h[SW_SSTAR * ncol]);
+    std::vector<double> tail(nd - h.size(), 0.0);
+    double* sc = tail.data() + SF_N * ncol;
+    sc[SS_CO2] = g->sfx_gas[0];
+    sc[SS_CH4] = g->sfx_gas[1];
+    sc[SS_N2O] = g->sfx_gas[2];
+    sc[SS_ISTEP] = 0.0;
+    void* buf = nullptr;
+    if (hipMalloc(&buf, nd * 8) != hipSuccess) return fail(e, CGO_E_NOMEM, "set_surface: hipMalloc(%zu)", nd * 8);
+    const MemLayout ly = layout(g, m->L, m->prec, m->p.carry, m->bgc);
+    double* slot = reinterpret_cast<double*>(static_cast<char*>(m->dbuf) + ly.sflux);
+    if (hipMemcpy(buf, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(static_cast<double*>(buf) + h.size(), tail.data(), tail.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(slot, 0, (size_t)m->L * ncol * 8) != hipSuccess) {
+        hipFree(buf);
+        return fail(e, CGO_E_HIP, "set_surface: upload failed");
+    }
+    m->sbuf = buf;
+    m->sbytes = nd * 8;
+    m->dev.sg = g->d_sg;
+    m->dev.sw = static_cast<double*>(buf);
+    m->dev.sfw = slot;
+    m->dev.sflux = slot;
+    e->table_dirty = true;
+    bump_gen(e, id);
+    return CGO_OK;
+}
+
+int cgo_download_surflux(cgo_engine* e, int32_t id, double* out, double* ashum, double* tice, double* gases) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "download_surflux: bad member %d", id);
+    if (!m->sbuf) return fail(e, CGO_E_STATE, "download_surflux: member %d has no surface fluxes (cgo_set_surface)", id);
+    if (int rc = settle(e, m)) return rc;
+    const size_t ncol = (size_t)m->dev.g.ncol;
+    const double* w = static_cast<const double*>(m->sbuf);
+    if (out) HIPCHK(e, hipMemcpy(out, w + SW_NF * ncol, SF_N * ncol * 8, hipMemcpyDeviceToHost));
+    if (ashum) HIPCHK(e, hipMemcpy(ashum, w + SW_ASHUM * ncol, ncol * 8, hipMemcpyDeviceToHost));
+    if (tice) HIPCHK(e, hipMemcpy(tice, w + SW_TICE * ncol, ncol * 8, hipMemcpyDeviceToHost));
+    if (gases) HIPCHK(e, hipMemcpy(gases, w + (SW_NF + SF_N) * ncol, SS_N * 8, hipMemcpyDeviceToHost));
+    return CGO_OK;
+}
+
 int cgo_set_wind(cgo_engine* e, int32_t id, const double* sxu, const double* syu, const double* sxv, const double* syv,
                  double scf) {
     MemberH* m = member(e, id);
     if (!m || !sxu || !syu || !sxv || !syv) return fail(e, CGO_E_ARG, "set_wind: bad argument");
     const GridH* g = e->grids[m->grid];
-    if (!g->mom) return fail(e, CGO_E_STATE, "set_wind: grid %d has no velocity solve (cgo_grid_set_momentum)", m->grid);
+    if (!g->mom && !g->sbuf)
+        return fail(e, CGO_E_STATE, "set_wind: grid %d has neither a velocity solve nor surface fluxes", m->grid);
     if (int rc = settle(e, m)) return rc;
     hipSetDevice(e->device);
+    {
+        // kept for the surface fluxes' wind speed (also when they are switched on later)
+        const size_t nc = (size_t)m->dev.g.ncol;
+        m->stress.resize(4 * nc);
+        const double* f[4] = {sxu, syu, sxv, syv};
+        for (int q = 0; q < 4; q++) std::copy(f[q], f[q] + nc, m->stress.begin() + q * nc);
+        if (m->sbuf) {
+            std::vector<double> us;
+            sfx_usurf(g, m->stress, us);
+            HIPCHK(e, hipMemcpy(static_cast<double*>(m->sbuf) + SW_USURF * nc, us.data(), nc * 8, hipMemcpyHostToDevice));
+        }
+    }
+    if (!g->mom) return CGO_OK;
     if (!m->mbuf)
         if (int rc = mom_attach(e, m, id)) return rc;
     const MomSetup& S = *g->mom;
@@ -1020,6 +1287,8 @@ int cgo_download_psi(cgo_engine* e, int32_t id, double* go_psi) {
 int cgo_set_sflux(cgo_engine* e, int32_t id, const double* sflux) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "set_sflux: bad member");
+    if (m->sbuf)
+        return fail(e, CGO_E_STATE, "set_sflux: member %d computes its surface slot (cgo_set_surface); switch that off first", id);
     if (int rc = settle(e, m)) return rc;
     const GridH* g = e->grids[m->grid];
     MemLayout ly = layout(g, m->L, m->prec, m->p.carry, m->bgc);
@@ -1095,6 +1364,15 @@ int cgo_upload_state(cgo_engine* e, int32_t id, const double* go_ts) {
                         r[c] = g.ec[0] * t + g.ec[1] * sa + g.ec[2] * (t * t) + g.ec[3] * (t * t * t);
                     }
         HIPCHK(e, hipMemcpy(m->dev.rho, r.data(), r.size() * 8, hipMemcpyHostToDevice));
+    }
+    // the surface fluxes read the top level of the state uploaded (as
+    // initialise_goldstein hands tstar_ocn over, goldstein.f90:2015-2017)
+    // until the next tstepo replaces it
+    if (m->sbuf) {
+        std::vector<double> top(2 * (size_t)g.ncol);
+        sfx_top(g, h, top.data(), top.data() + g.ncol);
+        HIPCHK(e, hipMemcpy(static_cast<double*>(m->sbuf) + SW_TSTAR * (size_t)g.ncol, top.data(), top.size() * 8,
+                            hipMemcpyHostToDevice));
     }
     const int32_t zero = 0;
     HIPCHK(e, hipMemcpy(m->dev.parity, &zero, 4, hipMemcpyHostToDevice));
@@ -1231,8 +1509,8 @@ static void set_bgc_params(MemberH* m, const cgo_bgc_params& bp) {
 struct CkptHdr {
     char magic[8];
     uint32_t version, hdr_bytes;
-    int32_t maxi, maxj, maxk, L, prec, carry, bgc, diso, wstream, coupling_due, has_sflux, mom;
-    uint64_t grid_fp, payload, mom_fp, mom_bytes;
+    int32_t maxi, maxj, maxk, L, prec, carry, bgc, diso, wstream, coupling_due, has_sflux, mom, sfx, pad;
+    uint64_t grid_fp, payload, mom_fp, mom_bytes, sfx_fp, sfx_bytes;
     double diff[2];
     cgo_bgc_params bp;
 };
@@ -1241,20 +1519,20 @@ static const char CKPT_MAGIC[8] = {'C', 'G', 'O', 'C', 'K', 'P', 'T', '1'};
 int64_t cgo_checkpoint_bytes(cgo_engine* e, int32_t id) {
     MemberH* m = member(e, id);
     if (!m) return fail(e, CGO_E_ARG, "checkpoint_bytes: bad member %d", id);
-    return (int64_t)(sizeof(CkptHdr) + m->bytes + m->mbytes);
+    return (int64_t)(sizeof(CkptHdr) + m->bytes + m->mbytes + m->sbytes);
 }
 
 int cgo_checkpoint(cgo_engine* e, int32_t id, void* buf, size_t n) {
     MemberH* m = member(e, id);
     if (!m || !buf) return fail(e, CGO_E_ARG, "checkpoint: bad argument");
-    if (n < sizeof(CkptHdr) + m->bytes + m->mbytes)
-        return fail(e, CGO_E_ARG, "checkpoint: buffer of %zu bytes, %zu needed", n, sizeof(CkptHdr) + m->bytes + m->mbytes);
+    const size_t need = sizeof(CkptHdr) + m->bytes + m->mbytes + m->sbytes;
+    if (n < need) return fail(e, CGO_E_ARG, "checkpoint: buffer of %zu bytes, %zu needed", n, need);
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->s));
     CkptHdr h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, CKPT_MAGIC, 8);
-    h.version = 2;
+    h.version = 3;
     h.hdr_bytes = (uint32_t)sizeof(CkptHdr);
     const GridDev& g = m->dev.g;
     h.maxi = g.maxi;
@@ -1276,10 +1554,16 @@ int cgo_checkpoint(cgo_engine* e, int32_t id, void* buf, size_t n) {
     h.mom = m->mbuf != nullptr;
     h.mom_fp = m->mbuf ? e->grids[m->grid]->mom_fp : 0;
     h.mom_bytes = m->mbytes;
+    h.sfx = m->sbuf != nullptr;
+    h.sfx_fp = m->sbuf ? e->grids[m->grid]->sfx_fp : 0;
+    h.sfx_bytes = m->sbytes;
     memcpy(buf, &h, sizeof h);
     HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h, m->dbuf, m->bytes, hipMemcpyDeviceToHost));
     if (m->mbuf)
         HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h + m->bytes, m->mbuf, m->mbytes, hipMemcpyDeviceToHost));
+    if (m->sbuf)
+        HIPCHK(e, hipMemcpy(static_cast<char*>(buf) + sizeof h + m->bytes + m->mbytes, m->sbuf, m->sbytes,
+                            hipMemcpyDeviceToHost));
     return CGO_OK;
 }
 
@@ -1288,16 +1572,19 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
     if (!m || !buf || n < sizeof(CkptHdr)) return fail(e, CGO_E_ARG, "restore: bad argument");
     CkptHdr h;
     memcpy(&h, buf, sizeof h);
-    if (memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 2 || h.hdr_bytes != sizeof(CkptHdr))
+    if (memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 3 || h.hdr_bytes != sizeof(CkptHdr))
         return fail(e, CGO_E_ARG, "restore: not a checkpoint of this library version");
     const GridH* gh = e->grids[m->grid];
     if (h.grid_fp != gh->fingerprint || h.L != m->L || h.prec != m->prec || h.carry != m->p.carry)
         return fail(e, CGO_E_ARG, "restore: checkpoint of another grid, tracer count, precision or carry "
                                   "(grid %dx%dx%d, %d tracers, precision %d)", h.maxi, h.maxj, h.maxk, h.L, h.prec);
-    if (n < sizeof h + h.payload + h.mom_bytes)
-        return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n, (size_t)(sizeof h + h.payload + h.mom_bytes));
+    if (n < sizeof h + h.payload + h.mom_bytes + h.sfx_bytes)
+        return fail(e, CGO_E_ARG, "restore: %zu bytes, the checkpoint holds %zu", n,
+                    (size_t)(sizeof h + h.payload + h.mom_bytes + h.sfx_bytes));
     if (h.mom && (!gh->mom || gh->mom_fp != h.mom_fp))
         return fail(e, CGO_E_ARG, "restore: the checkpoint's velocity solve needs its grid set up as it was (cgo_grid_set_momentum)");
+    if (h.sfx && (!gh->sbuf || gh->sfx_fp != h.sfx_fp || h.sfx_bytes != sfx_member_doubles(gh->dev.ncol) * 8))
+        return fail(e, CGO_E_ARG, "restore: the checkpoint's surface fluxes need its grid set up as it was (cgo_grid_set_surflux)");
     HIPCHK(e, hipSetDevice(e->device));
     if (int rc = settle(e, m)) return rc;
     int32_t slot = id;
@@ -1325,6 +1612,19 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
     HIPCHK(e, hipMemcpy(m->dbuf, static_cast<const char*>(buf) + sizeof h, h.payload, hipMemcpyHostToDevice));
     if (h.mom)
         HIPCHK(e, hipMemcpy(m->mbuf, static_cast<const char*>(buf) + sizeof h + h.payload, h.mom_bytes, hipMemcpyHostToDevice));
+    if (h.sfx && !m->sbuf) {
+        void* sb = nullptr;
+        if (hipMalloc(&sb, h.sfx_bytes) != hipSuccess) return fail(e, CGO_E_NOMEM, "restore: hipMalloc(%zu)", (size_t)h.sfx_bytes);
+        m->sbuf = sb;
+        m->sbytes = h.sfx_bytes;
+    } else if (!h.sfx && m->sbuf) {
+        e->graveyard.push_back(m->sbuf);
+        m->sbuf = nullptr;
+        m->sbytes = 0;
+    }
+    if (h.sfx)
+        HIPCHK(e, hipMemcpy(m->sbuf, static_cast<const char*>(buf) + sizeof h + h.payload + h.mom_bytes, h.sfx_bytes,
+                            hipMemcpyHostToDevice));
     m->p.diff[0] = h.diff[0];
     m->p.diff[1] = h.diff[1];
     m->p.diso = h.diso;
@@ -1340,6 +1640,9 @@ int cgo_restore(cgo_engine* e, int32_t id, const void* buf, size_t n) {
     m->coupling_due = h.coupling_due != 0;
     const MemLayout ly = layout(gh, m->L, m->prec, m->p.carry, m->bgc);
     m->dev.sflux = h.has_sflux ? reinterpret_cast<double*>(static_cast<char*>(m->dbuf) + ly.sflux) : nullptr;
+    m->dev.sg = m->sbuf ? gh->d_sg : nullptr;
+    m->dev.sw = static_cast<double*>(m->sbuf);
+    m->dev.sfw = m->sbuf ? reinterpret_cast<double*>(static_cast<char*>(m->dbuf) + ly.sflux) : nullptr;
     e->table_dirty = true;
     bump_gen(e, slot);
     return CGO_OK;
@@ -1658,10 +1961,11 @@ int cgo_set_windspeed(cgo_engine* e, int32_t id, const double* wspeed) {
 
 // ---------------------------------------------------------------- stepping
 struct Lists {
-    // [p64 list][p32 list][bgc p64][bgc p32][all][velocity solve]
+    // [p64 list][p32 list][bgc p64][bgc p32][all][velocity solve][surface fluxes]
     std::vector<int32_t> v;
-    int n[6] = {0, 0, 0, 0, 0, 0}, off[6] = {0, 0, 0, 0, 0, 0};
+    int n[7] = {0, 0, 0, 0, 0, 0, 0}, off[7] = {0, 0, 0, 0, 0, 0, 0};
     int momN = 0, momCol = 0, momBand = 0;  // most psi points, columns and maxi of a member with a velocity solve
+    int sfxCol = 0;                         // most columns of a member with surface fluxes
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
     int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
     int ncol_atm = 0;  // widest grid of the members with air-sea exchange, 0 if none
@@ -1669,7 +1973,7 @@ struct Lists {
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
-    std::vector<int32_t> g[6];
+    std::vector<int32_t> g[7];
     for (int i = 0; i < n; i++) {
         MemberH* m = member(e, ids[i]);
         if (!m) return fail(e, CGO_E_ARG, "step: bad member %d", ids[i]);
@@ -1682,6 +1986,13 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             ls.momN = std::max(ls.momN, e->grids[m->grid]->mom->N);
             ls.momCol = std::max(ls.momCol, m->dev.g.ncol);
             ls.momBand = std::max(ls.momBand, m->dev.g.maxi);
+        }
+        if (m->sbuf) {
+            if (e->grids[m->grid]->sfx_atchem && !(m->bgc && m->dev.bp.airsea))
+                return fail(e, CGO_E_STATE, "step: member %d's surface fluxes read CO2 from an atmosphere it does not have "
+                                            "(cgo_bgc_init with air-sea exchange)", ids[i]);
+            g[6].push_back(ids[i]);
+            ls.sfxCol = std::max(ls.sfxCol, m->dev.g.ncol);
         }
         ls.maxcol[p] = std::max(ls.maxcol[p], m->dev.g.ncol);
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
@@ -1698,7 +2009,7 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             if (m->dev.bp.airsea) ls.ncol_atm = std::max(ls.ncol_atm, m->dev.g.ncol);
         }
     }
-    for (int q = 0; q < 6; q++) {
+    for (int q = 0; q < 7; q++) {
         ls.off[q] = (int)ls.v.size();
         ls.n[q] = (int)g[q].size();
         ls.v.insert(ls.v.end(), g[q].begin(), g[q].end());
@@ -1732,6 +2043,12 @@ static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lis
 // which precedes their tstepo
 static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
     if (ls.n[5]) launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], ls.momN, ls.momCol, ls.momBand, s);
+}
+
+// the surface fluxes of the members that have them (surflux, called by
+// genie before the ocean step, genie.f90:271-282)
+static void enqueue_sfx(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
+    if (ls.n[6]) launch_sfx(e->d_table, d_lists + ls.off[6], ls.n[6], ls.sfxCol, e->s);
 }
 
 static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
@@ -1774,6 +2091,9 @@ static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists
         // captured graph keeps: the pre-pass (short, latency-bound) takes the
         // CUs first, and BIOGEM's blocks fill the slots both flux kernels
         // leave (enqueued first, they ran beside the pre-pass alone)
+        // the surface fluxes read the atmosphere that the last step's ATCHEM
+        // left: they come before the fork, so this step's ATCHEM waits for them
+        enqueue_sfx(e, ls, d_lists);
         if (bgc) hipEventRecord(e->ev_fork, e->s);
         enqueue_mom(e, ls, d_lists, e->s);
         enqueue_flux(e, ls, d_lists, 1);
@@ -1815,7 +2135,7 @@ static int upload_lists(cgo_engine* e, const Lists& ls) {
     return CGO_OK;
 }
 
-static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
+static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls, bool steps = true) {
     hipSetDevice(e->device);
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
@@ -1823,7 +2143,7 @@ static int prepare(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
     if ((rc = upload_lists(e, ls))) return rc;
     for (int i = 0; i < n; i++) {
         member(e, ids[i])->queued = true;
-        member(e, ids[i])->stepped = true;
+        if (steps) member(e, ids[i])->stepped = true;
     }
     return CGO_OK;
 }
@@ -1834,6 +2154,20 @@ int cgo_step(cgo_engine* e, const int32_t* ids, int32_t n, int32_t nsteps) {
     int rc = check_not_due(e, ids, n, "step");
     if (rc || (rc = prepare(e, ids, n, ls))) return rc;
     enqueue_steps(e, ls, e->d_list, nsteps);
+    HIPCHK(e, hipGetLastError());
+    return CGO_OK;
+}
+
+int cgo_surflux(cgo_engine* e, const int32_t* ids, int32_t n) {
+    if (!e || !ids || n <= 0) return fail(e, CGO_E_ARG, "surflux: bad argument");
+    for (int i = 0; i < n; i++) {
+        MemberH* m = member(e, ids[i]);
+        if (!m) return fail(e, CGO_E_ARG, "surflux: bad member %d", ids[i]);
+        if (!m->sbuf) return fail(e, CGO_E_STATE, "surflux: member %d has no surface fluxes (cgo_set_surface)", ids[i]);
+    }
+    Lists ls;
+    if (int rc = prepare(e, ids, n, ls, false)) return rc;
+    enqueue_sfx(e, ls, e->d_list);
     HIPCHK(e, hipGetLastError());
     return CGO_OK;
 }
@@ -1988,7 +2322,7 @@ int cgo_set_flux_iso(cgo_engine* e, int32_t precision, int32_t prepass) {
 }
 
 int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n, int32_t reps, double* avg_ms) {
-    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 5) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
+    if (!e || !ids || n <= 0 || reps <= 0 || !avg_ms || which < 0 || which > 6) return fail(e, CGO_E_ARG, "time_kernel: bad argument");
     Lists ls;
     int rc = make_lists(e, ids, n, ls);
     if (rc) return rc;
@@ -2015,7 +2349,9 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
     }
     if (which == 5)
         for (int r = 0; r < reps; r++) enqueue_mom(e, ls, e->d_list, e->s);
-    for (int r = 0; r < (which == 3 || which == 5 ? 0 : reps); r++)
+    if (which == 6)
+        for (int r = 0; r < reps; r++) enqueue_sfx(e, ls, e->d_list);
+    for (int r = 0; r < (which == 3 || which >= 5 ? 0 : reps); r++)
         for (int p = 0; p < 2; p++) {
             if (!ls.n[p]) continue;
             const int32_t* l = e->d_list + ls.off[p];

@@ -165,6 +165,36 @@ __host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int
     return L;
 }
 
+// EMBM's surface fluxes of one grid (surflux, embm.f90:2548-3738, without
+// ENTS): the set-up tables and the namelist constants, and the runoff
+// routing turned into gather lists, each wet column's land sources in the
+// order the reference's main loop (i outer, j inner) adds them.
+struct SfxGridDev {
+    int32_t nyear, igrid, radfor_atchem, nsrc;
+    double rmax, delf2x, rdtdim, rsictscsf, olr_adj, olr_adj0, t_eqm;
+    double albsic_min, albsic_max, sich_max, rate_co2, rate_ch4, rate_n2o;
+    const double* solfor;   // [nyear][maxj]
+    const double* albcl;    // [ncol]
+    const double* ca;       // [ncol]
+    const double* pmeadj;   // [ncol]
+    const int32_t* rstart;  // [ncol + 1] range of each column's sources in rsrc
+    const int32_t* rsrc;    // [nsrc] source columns
+};
+// per-member fields of the surface fluxes, each [ncol]: the air and sea-ice
+// state (ashum and tice in/out), the wind speed, the sea-ice model's heat
+// and freshwater into the ocean, and the top level's T and S as the last
+// tstepo left them (tsval_ocn, ssval_ocn of step_goldstein 427-430)
+enum { SW_ATEMP = 0, SW_ASHUM, SW_SICH, SW_SICA, SW_TICE, SW_USURF, SW_FX0SIC, SW_FWSIC, SW_TSTAR, SW_SSTAR, SW_NF };
+// outputs of one call, each [ncol], in the order of include/cgocn.h's CGO_SF_*
+enum {
+    SF_FXLHO = 0, SF_FXSHO, SF_FXSWO, SF_FXLWO, SF_EVAP_OCN, SF_PPTN_OCN, SF_RUNOFF_OCN, SF_RUNOFF_LAND, SF_FXLHA,
+    SF_FXSHA, SF_FXSWA, SF_FXLWA, SF_EVAP_ATM, SF_PPTN_ATM, SF_DTHSIC, SF_DTAREASIC, SF_ALBEDO, SF_ALBICE, SF_FXSW,
+    SF_FX0A, SF_FX0O, SF_FXSEN, SF_FXLW, SF_EVAP, SF_PPTN, SF_FX0SIC, SF_FX0NETO, SF_TSFREEZ, SF_N
+};
+// member scalars after the fields and outputs: the greenhouse gases and the call count
+enum { SS_CO2 = 0, SS_CH4, SS_N2O, SS_ISTEP, SS_N };
+__host__ __device__ inline size_t sfx_member_doubles(int ncol) { return (size_t)(SW_NF + SF_N) * ncol + SS_N; }
+
 struct MemberDev {
     GridDev g;
     int32_t L, prec, diso, bgc;
@@ -206,6 +236,10 @@ struct MemberDev {
     const MomGridDev* mg;
     double* mw;             // the member's arrays of the solve (mom_member_layout)
     double* u64;            // FP64 u, v, w [3][maxk][maxj][maxi] (the u planes of an FP64 member)
+    // surface fluxes: null when the surface slot is prescribed
+    const SfxGridDev* sg;
+    double* sw;             // [SW_NF][ncol] fields, [SF_N][ncol] outputs, [SS_N] scalars
+    double* sfw;            // the member's surface slot, sflux planes 0 and 1, which the fluxes write
 };
 
 // remineralisation table rows: layer-thickness ratio dD(k+1)/dD(k) and the

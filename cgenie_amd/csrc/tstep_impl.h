@@ -499,8 +499,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     const R Tc = SP(cb, q, p);
                     const R dE = D(cb, q, pe, p), dW = D(cb, q, pw, p), dN = D(cb, q, pn, p), dS = D(cb, q, ps, p);
                     R fa, fbn = 0;
+                    // the top cell's surface flux enters in FP64: rounded to
+                    // float it would bias every step's forcing alike
+                    double fsurf = 0.0;
                     if (top) {
-                        fa = M.sflux ? (R)at(gp(M.sflux), (uint32_t)((l0 + q) * plane + col)) : (R)0;
+                        fa = (R)0;
+                        if (M.sflux) fsurf = at(gp(M.sflux), (uint32_t)((l0 + q) * plane + col));
                     } else {
                         const R dU = DK(ub, cb, q, p);
                         R tv = 0;
@@ -518,14 +522,15 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux(const MemberDev* __restri
                     const R div = ((dE * ce - dW * cw) + (dN * cn - dS * cs)) + (fa - FB(q)) * rdz;
                     const int idx = (l0 + q) * tstride + o + col;
                     const R inc = -(dt * (div + Tc * divu));
+                    const double incs = fsurf != 0.0 ? -(gp(g.mk)[MK_DT * KS + k] * (fsurf * gp(g.mk)[MK_RDZ * KS + k])) : 0.0;
                     double nv;
                     if (lo_src) {
                         // compensated accumulation: the rounding residual of every
                         // update is carried to the next step in the int16 plane
-                        nv = ((double)Tc + (double)SR(cb, q, p)) + (double)inc;
+                        nv = ((double)Tc + (double)SR(cb, q, p)) + (double)inc + incs;
                         StateIO<R>::st((R*)dst, lo_dst, idx, nv);
                     } else {
-                        const R v = Tc + inc;
+                        const R v = Tc + (inc + (R)incs);
                         at(dst, (uint32_t)idx) = v;
                         nv = (double)v;
                     }
@@ -713,6 +718,16 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     int16_t* lo = M.lo[par ^ 1];
     const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
     AS_GLOBAL double* rho = gp(M.rho);
+    // members with surface fluxes keep the top level's T and S as tstepo
+    // leaves them (tsval_ocn, ssval_ocn of step_goldstein 427-430): the
+    // next call of the fluxes reads them, whatever the coupling does after
+    auto snap = [&]() {
+        if (M.sw) {
+            const size_t top = (size_t)(maxk - 1) * plane + col;
+            gp(M.sw)[(size_t)SW_TSTAR * plane + col] = StateIO<R>::ld(st, lo, top);
+            gp(M.sw)[(size_t)SW_SSTAR * plane + col] = StateIO<R>::ld(st, lo, tstride + top);
+        }
+    };
     // A column whose density falls strictly from k1 up is left as it is: the
     // loop below walks m down to k1 without mixing and the fill-in finds every
     // level in place.  Most columns are; they skip the per-level arrays.
@@ -724,7 +739,10 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
             stable = rb < ra;
             rb = ra;
         }
-        if (stable) return;
+        if (stable) {
+            snap();
+            return;
+        }
     }
     const AS_GLOBAL double* dz = gp(g.mk) + MK_DZ * (maxk + 2);
     const double e0 = g.ec[0], e1 = g.ec[1], e2 = g.ec[2], e3 = g.ec[3];
@@ -819,6 +837,7 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
         for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
         if (cost > 0.0) gp(M.cost)[col] += cost;
     }
+    snap();
 }
 
 #ifdef CGO_DEFINE_FLIP

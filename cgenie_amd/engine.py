@@ -50,6 +50,21 @@ class MomentumDesc(C.Structure):
          ("adrag", _d), ("rel", _d)]
 
 
+class SurfluxDesc(C.Structure):
+    _fields_ = [("solfor", _dp), ("nyear", _i32), ("albcl", _dp), ("ca", _dp), ("pmeadj", _dp),
+                ("iroff", C.POINTER(_i32)), ("jroff", C.POINTER(_i32)), ("igrid", _i32), ("polar_avg", _i32),
+                ("radfor_atchem", _i32)] + \
+        [(n, _d) for n in ("scf", "rmax", "delf2x", "rdtdim", "rsictscsf", "olr_adj", "olr_adj0", "t_eqm",
+                           "albsic_min", "albsic_max", "sich_max", "rate_co2", "rate_ch4", "rate_n2o", "co2", "ch4",
+                           "n2o")]
+
+
+# the surface fluxes' outputs (cgo_download_surflux), in the order of CGO_SF_*
+SF_NAMES = ("fxlho", "fxsho", "fxswo", "fxlwo", "evap_ocn", "pptn_ocn", "runoff_ocn", "runoff_land", "fxlha",
+            "fxsha", "fxswa", "fxlwa", "evap_atm", "pptn_atm", "dthsic", "dtareasic", "albedo", "albice", "fxsw",
+            "fx0a", "fx0o", "fxsen", "fxlw", "evap", "pptn", "fx0sic", "fx0neto", "tsfreez")
+
+
 class Diag(C.Structure):
     _fields_ = [("steps", _i64), ("limps", _i32), ("carb_fail", _i32), ("carb_iters", _i64), ("dmax", _d),
                 ("cost", _d), ("smean", _d)]
@@ -115,6 +130,10 @@ def load() -> C.CDLL:
         "cgo_set_wind": ([P, _i32, _dp, _dp, _dp, _dp, _d], C.c_int),
         "cgo_download_velocity": ([P, _i32, _dp], C.c_int),
         "cgo_download_psi": ([P, _i32, _dp], C.c_int),
+        "cgo_grid_set_surflux": ([P, _i32, C.POINTER(SurfluxDesc)], C.c_int),
+        "cgo_set_surface": ([P, _i32] + [_dp] * 7, C.c_int),
+        "cgo_surflux": ([P, ip, _i32], C.c_int),
+        "cgo_download_surflux": ([P, _i32, _dp, _dp, _dp, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -238,6 +257,46 @@ class Engine:
         st = np.asarray(stress, dtype=np.float64)
         parts = [np.ascontiguousarray(st[..., q]) for q in range(4)]
         self._chk(self.L.cgo_set_wind(self.h, m, *(_ptr(a) for a in parts), float(scf)), "set_wind")
+
+    def set_surflux(self, grid_id: int, su) -> None:
+        """The grid's surface fluxes (cgo_grid_set_surflux) from a
+        cgenie_amd.surface.SurfluxSetup."""
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (su.solfor, su.albcl, su.ca, su.pmeadj)]
+        rout = [np.ascontiguousarray(a, dtype=np.int32) for a in (su.iroff, su.jroff)]
+        d = SurfluxDesc()
+        d.solfor, d.albcl, d.ca, d.pmeadj = (_ptr(a) for a in keep)
+        d.nyear = int(su.solfor.shape[0])
+        d.iroff, d.jroff = (a.ctypes.data_as(C.POINTER(_i32)) for a in rout)
+        d.igrid, d.polar_avg, d.radfor_atchem = int(su.igrid), int(su.polar_avg), int(su.radfor_atchem)
+        for n in ("scf", "rmax", "delf2x", "rdtdim", "rsictscsf", "olr_adj", "olr_adj0", "t_eqm", "albsic_min",
+                  "albsic_max", "sich_max", "rate_co2", "rate_ch4", "rate_n2o", "co2", "ch4", "n2o"):
+            setattr(d, n, float(getattr(su, n)))
+        self._chk(self.L.cgo_grid_set_surflux(self.h, grid_id, C.byref(d)), "grid_set_surflux")
+
+    def set_surface(self, m: int, atemp, ashum=None, sich=None, sica=None, tice=None, fx0sic=None, fwsic=None):
+        """Switch member m's surface fluxes on with this air and sea-ice
+        state ((maxj, maxi) arrays; None = zero), or off with atemp None."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (atemp, ashum, sich, sica, tice, fx0sic, fwsic)]
+        self._chk(self.L.cgo_set_surface(self.h, m, *(None if a is None else _ptr(a) for a in keep)), "set_surface")
+
+    def surflux(self, members) -> None:
+        """One surflux call of the listed members."""
+        ids = np.ascontiguousarray(list(members), dtype=np.int32)
+        self._chk(self.L.cgo_surflux(self.h, ids.ctypes.data_as(C.POINTER(_i32)), len(ids)), "surflux")
+
+    def surface_outputs(self, m: int, g: G.Grid) -> dict:
+        """The last call's outputs by name ((maxj, maxi) each), with ashum,
+        tice and the scalars co2, ch4, n2o, istep."""
+        out = np.zeros((len(SF_NAMES), g.maxj, g.maxi))
+        ashum = np.zeros((g.maxj, g.maxi))
+        tice = np.zeros((g.maxj, g.maxi))
+        gases = np.zeros(4)
+        self._chk(self.L.cgo_download_surflux(self.h, m, _ptr(out), _ptr(ashum), _ptr(tice), _ptr(gases)),
+                  "download_surflux")
+        d = {n: out[q] for q, n in enumerate(SF_NAMES)}
+        d.update(ashum=ashum, tice=tice, co2=gases[0], ch4=gases[1], n2o=gases[2], istep=int(gases[3]))
+        return d
 
     def velocity(self, m: int, g: G.Grid) -> np.ndarray:
         """go_u(3, maxi, maxj, maxk) as (maxk, maxj, maxi, 3)."""

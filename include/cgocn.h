@@ -14,6 +14,13 @@
  *   biogem_tracercoupling  biogem.f90:1885-2077 -> cgo_tracercoupling
  *   all three in order                         -> cgo_step, cgo_graph_capture
  *
+ * and, where a member has them, the steps either side in genie's order:
+ *
+ *   surflux           embm.f90:2548-3738       -> cgo_surflux (EMBM's surface
+ *                     fluxes into ts(:,:,:,maxk+1), goldstein.f90:149-170)
+ *   velocity solve    goldstein.f90:198-233    -> before each member's tstepo
+ *   air-sea exchange, ATCHEM  biogem_box.f90:27-299, atchem.f90:63-158
+ *
  * The reference exposes these as Fortran module subroutines on module state
  * (goldstein_lib.f90:1-180, biogem_lib.f90); there is no C interface
  * upstream.  Host arrays cross this boundary only in the reference's own
@@ -137,7 +144,8 @@ int cgo_download_cost(cgo_engine *e, int32_t m, double *cost);
  * restarts are the netCDF files of goldstein_data.f90:153-354 and
  * biogem.f90:2280-2360; cgenie_amd/restart.py writes and reads those): a
  * header (grid fingerprint, tracer count, precision, carry, parameters,
- * BIOGEM state flags) and every device array of the member.  cgo_restore
+ * BIOGEM state flags) and every device array of the member (with those of
+ * its velocity solve and surface fluxes, where it has them).  cgo_restore
  * into a member of the same grid, tracer count, precision and carry (BIOGEM
  * arrays are added if the checkpoint has them) continues the run bit for
  * bit; anything else is refused with CGO_E_ARG. */
@@ -207,6 +215,65 @@ int cgo_set_wind(cgo_engine *e, int32_t m, const double *stressxu, const double 
 int cgo_download_velocity(cgo_engine *e, int32_t m, double *go_u);
 int cgo_download_psi(cgo_engine *e, int32_t m, double *go_psi);
 
+/* EMBM's surface fluxes (surflux, embm.f90:2548-3738) in the configuration
+ * without ENTS, and the surface slot ts(1:2, 1:maxi, 1:maxj, maxk+1) that
+ * step_goldstein forms from them (goldstein.f90:149-170).  Per grid, the
+ * tables and constants initialise_embm leaves (embm.f90:198-1530);
+ * per member, the air temperature and humidity (EMBM's tq), the sea-ice
+ * height, cover and surface temperature, and the sea-ice model's heat and
+ * freshwater into the ocean.  The ocean side is the member's own top level
+ * as its last tstepo left it (tsval_ocn, ssval_ocn, step_goldstein 427-430).
+ * Greenhouse gases: compounding at the namelist rates (embm.f90:2916-2926)
+ * or, with radfor_atchem, CO2 from the member's ATCHEM atmosphere
+ * (2870-2887; CH4 and N2O are not carried and read as their references).
+ * Not covered: ENTS, orography, ice sheets, time-series and orbital
+ * forcing, freshwater hosing and anomaly files. */
+typedef struct cgo_surflux_desc {
+    const double *solfor;     /* solfor(maxj, nyear), W m-2 (radfor, embm.f90:2383-2522) */
+    int32_t nyear;            /* ocean steps per year: the call count picks solfor(:, mod(istep-1, nyear)+1) */
+    const double *albcl;      /* (maxi, maxj) climatological albedo (911-920) */
+    const double *ca;         /* (maxi, maxj) shortwave absorbed in the atmosphere (924-929) */
+    const double *pmeadj;     /* (maxi, maxj) freshwater adjustment, m s-1 (1292-1345), or NULL for zero */
+    const int32_t *iroff, *jroff; /* (maxi, maxj) runoff destination of every cell, 1-based (readroff 3779-3836) */
+    int32_t igrid;            /* 0: runoff added as is; else weighted ds(j)/ds(jroff) (3350-3357) */
+    int32_t polar_avg;        /* par_wind_polar_avg: 2 leaves the polar rows' wind speed unaveraged */
+    int32_t radfor_atchem;    /* atchem_radfor = 'y' */
+    double scf, rmax, delf2x; /* embm-defaults.nml: 2.0, 0.85, 5.77 */
+    double rdtdim;            /* 1 / (ocean step in s) (770) */
+    double rsictscsf;         /* ocean-ice heat transfer (1133) */
+    double olr_adj, olr_adj0, t_eqm;      /* 0, 0, 12.371 */
+    double albsic_min, albsic_max, sich_max; /* 0.2, 0.7, 9999.9 */
+    double rate_co2, rate_ch4, rate_n2o;  /* compound rise per call (761-763) */
+    double co2, ch4, n2o;     /* initial mixing ratios (radfor_scl_* x 278e-6, 700e-9, 275e-9) */
+} cgo_surflux_desc;
+int cgo_grid_set_surflux(cgo_engine *e, int32_t grid_id, const cgo_surflux_desc *d);
+/* switches member m's surface fluxes on (atemp non-NULL) with this state,
+ * each (maxi, maxj): air temperature (deg C) and specific humidity (kg/kg),
+ * sea-ice height (m), cover (0-1) and surface temperature (deg C), the
+ * sea-ice model's heat (W m-2) and freshwater (mm s-1) into the ocean; NULL
+ * for any after atemp is zero.  From then on every cgo_step and captured
+ * graph computes the member's surface slot before its ocean step, and
+ * cgo_set_sflux is refused.  atemp NULL switches them off. */
+int cgo_set_surface(cgo_engine *e, int32_t m, const double *atemp, const double *ashum, const double *sich,
+                    const double *sica, const double *tice, const double *fx0sic, const double *fwsic);
+/* one surflux call for the listed members (a host that drives the genie.f90
+ * order itself calls it where genie calls surflux_wrapper, genie.f90:271-282,
+ * before cgo_tstepo) */
+int cgo_surflux(cgo_engine *e, const int32_t *members, int32_t n);
+/* The outputs of member m's last call, CGO_SF_N fields (maxi, maxj) in the
+ * order below: surflux's ocean and atmosphere fluxes (W m-2, mm s-1), the
+ * sea-ice changes dthsic, dtareasic, the albedos, and EMBM's own fxsw, fx0a,
+ * fx0o, fxsen, fxlw, evap, pptn, fx0sic (zero without ice), fx0neto and
+ * tsfreez; ashum and tice (maxi, maxj) as the call left them; gases the
+ * CO2, CH4, N2O mixing ratios and the call count.  Any pointer may be NULL. */
+#define CGO_SF_N 28
+enum { CGO_SF_FXLHO = 0, CGO_SF_FXSHO, CGO_SF_FXSWO, CGO_SF_FXLWO, CGO_SF_EVAP_OCN, CGO_SF_PPTN_OCN,
+       CGO_SF_RUNOFF_OCN, CGO_SF_RUNOFF_LAND, CGO_SF_FXLHA, CGO_SF_FXSHA, CGO_SF_FXSWA, CGO_SF_FXLWA,
+       CGO_SF_EVAP_ATM, CGO_SF_PPTN_ATM, CGO_SF_DTHSIC, CGO_SF_DTAREASIC, CGO_SF_ALBEDO, CGO_SF_ALBICE,
+       CGO_SF_FXSW, CGO_SF_FX0A, CGO_SF_FX0O, CGO_SF_FXSEN, CGO_SF_FXLW, CGO_SF_EVAP, CGO_SF_PPTN,
+       CGO_SF_FX0SIC, CGO_SF_FX0NETO, CGO_SF_TSFREEZ };
+int cgo_download_surflux(cgo_engine *e, int32_t m, double *out, double *ashum, double *tice, double *gases);
+
 /* stepping: nsteps ocean steps of every listed member, batched into one
  * launch per kernel (members with BIOGEM run the full coupling order:
  * cgo_tstepo, cgo_bio_step, cgo_tracercoupling) */
@@ -243,7 +310,7 @@ int cgo_set_flux_iso(cgo_engine *e, int32_t precision, int32_t prepass);    /* 1
 /* average device time (ms) of one launch of kernel `which` (0 = flux,
  * 1 = co, 2 = bgc column + coupling, 3 = flux with the BIOGEM column step
  * running beside it on the second stream, as inside a whole step, 4 = the
- * BIOGEM column step alone, 5 = the velocity solve) over the listed members, from HIP events on the
+ * BIOGEM column step alone, 5 = the velocity solve, 6 = the surface fluxes) over the listed members, from HIP events on the
  * engine stream; the state is advanced by the timed launches (mode 4
  * rewrites the BIOGEM arrays at every launch with no coupling between them:
  * a member timed with it is for timing only) */

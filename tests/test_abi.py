@@ -40,7 +40,7 @@ def test_library_is_gfx950_code_object():
 STRUCTS = {"cgo_grid_desc": engine.GridDesc, "cgo_member_params": engine.MemberParams,
            "cgo_bgc_params": engine.BgcParams, "cgo_diag": engine.Diag}
 # cgo_abi_info reports the first four; the layout check covers every struct
-LAYOUTS = dict(STRUCTS, cgo_momentum_desc=engine.MomentumDesc)
+LAYOUTS = dict(STRUCTS, cgo_momentum_desc=engine.MomentumDesc, cgo_surflux_desc=engine.SurfluxDesc)
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
@@ -66,7 +66,7 @@ def test_struct_layouts_match_header(tmp_path):
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
     # the sizes the Fortran bind(C) mirrors are also checked against (test_fortran_host)
-    assert [got[(n, "size")] for n in LAYOUTS] == [208, 24, 256, 48, 80]
+    assert [got[(n, "size")] for n in LAYOUTS] == [208, 24, 256, 48, 80, 208]
 
 
 class _FakeLib:
