@@ -32,7 +32,7 @@ void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
 void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, int maxcol, int maxn, hipStream_t);
 size_t mom_solve_lds(int N, int n);
-void launch_sfx(const MemberDev*, const int32_t*, int nlist, int maxcol, hipStream_t);
+void launch_sfx(const MemberDev*, const int32_t*, int nlist, int maxcol, int slot_only, hipStream_t);
 size_t sfx_lds(int ncol);
 size_t mom_ring_terms(int n);
 }  // namespace cgo
@@ -2048,7 +2048,7 @@ static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, 
 // the surface fluxes of the members that have them (surflux, called by
 // genie before the ocean step, genie.f90:271-282)
 static void enqueue_sfx(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
-    if (ls.n[6]) launch_sfx(e->d_table, d_lists + ls.off[6], ls.n[6], ls.sfxCol, e->s);
+    if (ls.n[6]) launch_sfx(e->d_table, d_lists + ls.off[6], ls.n[6], ls.sfxCol, 0, e->s);
 }
 
 static void enqueue_tstepo(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
@@ -2168,6 +2168,23 @@ int cgo_surflux(cgo_engine* e, const int32_t* ids, int32_t n) {
     Lists ls;
     if (int rc = prepare(e, ids, n, ls, false)) return rc;
     enqueue_sfx(e, ls, e->d_list);
+    HIPCHK(e, hipGetLastError());
+    return CGO_OK;
+}
+
+int cgo_surface_slot(cgo_engine* e, int32_t id, const double* fx0sic, const double* fwsic) {
+    MemberH* m = member(e, id);
+    if (!m) return fail(e, CGO_E_ARG, "surface_slot: bad member %d", id);
+    if (!m->sbuf) return fail(e, CGO_E_STATE, "surface_slot: member %d has no surface fluxes (cgo_set_surface)", id);
+    if (int rc = settle(e, m)) return rc;
+    const size_t ncol = (size_t)m->dev.g.ncol;
+    std::vector<double> h(2 * ncol, 0.0);
+    if (fx0sic) std::copy(fx0sic, fx0sic + ncol, h.begin());
+    if (fwsic) std::copy(fwsic, fwsic + ncol, h.begin() + ncol);
+    HIPCHK(e, hipMemcpy(static_cast<double*>(m->sbuf) + SW_FX0SIC * ncol, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    Lists ls;
+    if (int rc = prepare(e, &id, 1, ls, false)) return rc;
+    launch_sfx(e->d_table, e->d_list + ls.off[6], ls.n[6], ls.sfxCol, 1, e->s);
     HIPCHK(e, hipGetLastError());
     return CGO_OK;
 }

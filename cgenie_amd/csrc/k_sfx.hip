@@ -51,7 +51,10 @@ __device__ double ch4_func(double ch4, double n2o) {
     return 0.47 * log(1.0 + 2.01E-5 * pow(ch4 * n2o, 0.75) + 5.31E-15 * ch4 * pow(ch4 * n2o, 1.52));
 }
 
-__global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list) {
+// slot_only: form the surface slot again from the last call's outputs and
+// the current sea-ice fluxes, leaving everything else as it is
+__global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list,
+                                                 int slot_only) {
     extern __shared__ double s_pptn[];
     __shared__ double s_mean;
     const MemberDev& M = mems[list[blockIdx.x]];
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ me
                             ch4_func(1.0E9 * CH40, 1.0E9 * N2O0);
     const AS_GLOBAL double* pco2 = S.radfor_atchem ? gp(M.atm) : nullptr;
 
-    for (int col = threadIdx.x; col < ncol; col += SFX_NT) {
+    for (int col = threadIdx.x; col < ncol && !slot_only; col += SFX_NT) {
         const int I = col % mi + 1, J = col / mi + 1;
         const double atemp = atemp_a[col], sica = sica_a[col], sich = sich_a[col];
         const double us = us_a[col], ca = gp(S.ca)[col], albcl = gp(S.albcl)[col];
@@ -288,7 +291,9 @@ __global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ me
     for (int col = threadIdx.x; col < ncol; col += SFX_NT) {
         const int J = col / mi + 1;
         double ro = 0.0;
-        if (k1at(g, col % mi + 1, J) <= mk) {
+        if (slot_only) {
+            ro = out[(size_t)SF_RUNOFF_OCN * nc + col];
+        } else if (k1at(g, col % mi + 1, J) <= mk) {
             double r = 0.0;
             const int s0 = gp(S.rstart)[col], s1 = gp(S.rstart)[col + 1];
             for (int q = s0; q < s1; q++) {
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ me
             }
             ro = (r + 0.0) * M2MM;
         }
-        out[(size_t)SF_RUNOFF_OCN * nc + col] = ro;
+        if (!slot_only) out[(size_t)SF_RUNOFF_OCN * nc + col] = ro;
         const double fx0neto = out[(size_t)SF_FXSWO * nc + col] + out[(size_t)SF_FXSHO * nc + col] +
                                out[(size_t)SF_FXLWO * nc + col] + out[(size_t)SF_FXLHO * nc + col] + sw[SW_FX0SIC * nc + col];
         double fw = out[(size_t)SF_PPTN_OCN * nc + col] + out[(size_t)SF_EVAP_OCN * nc + col] + ro +
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ me
         slot[nc + col] = fw * RPMESCO;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !slot_only) {
         sc[SS_ISTEP] = istep + 1.0;
         sc[SS_CO2] = co2;
         sc[SS_CH4] = S.radfor_atchem ? sc[SS_CH4] : ch4;
@@ -323,9 +328,9 @@ __global__ __launch_bounds__(SFX_NT) void k_sfx(const MemberDev* __restrict__ me
 // LDS of one member's workgroup: the precipitation of every column
 size_t sfx_lds(int ncol) { return (size_t)ncol * 8; }
 
-void launch_sfx(const MemberDev* t, const int32_t* l, int n, int maxcol, hipStream_t s) {
+void launch_sfx(const MemberDev* t, const int32_t* l, int n, int maxcol, int slot_only, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_sfx, dim3(n), dim3(SFX_NT), sfx_lds(maxcol), s, t, l);
+    hipLaunchKernelGGL(k_sfx, dim3(n), dim3(SFX_NT), sfx_lds(maxcol), s, t, l, slot_only);
 }
 
 }  // namespace cgo

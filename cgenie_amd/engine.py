@@ -133,6 +133,7 @@ def load() -> C.CDLL:
         "cgo_grid_set_surflux": ([P, _i32, C.POINTER(SurfluxDesc)], C.c_int),
         "cgo_set_surface": ([P, _i32] + [_dp] * 7, C.c_int),
         "cgo_surflux": ([P, ip, _i32], C.c_int),
+        "cgo_surface_slot": ([P, _i32, _dp, _dp], C.c_int),
         "cgo_download_surflux": ([P, _i32, _dp, _dp, _dp, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -284,6 +285,11 @@ class Engine:
         """One surflux call of the listed members."""
         ids = np.ascontiguousarray(list(members), dtype=np.int32)
         self._chk(self.L.cgo_surflux(self.h, ids.ctypes.data_as(C.POINTER(_i32)), len(ids)), "surflux")
+
+    def surface_slot(self, m: int, fx0sic=None, fwsic=None) -> None:
+        """Form member m's surface slot again with new sea-ice fluxes ((maxj, maxi); None = zero)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (fx0sic, fwsic)]
+        self._chk(self.L.cgo_surface_slot(self.h, m, *(None if a is None else _ptr(a) for a in keep)), "surface_slot")
 
     def surface_outputs(self, m: int, g: G.Grid) -> dict:
         """The last call's outputs by name ((maxj, maxi) each), with ashum,

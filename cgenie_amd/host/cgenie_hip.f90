@@ -22,6 +22,13 @@
 !                                         remineralisation and surface chemistry
 !                                         of the carried species; bottom-water
 !                                         composition and rain out, 1719-1754)
+!   surflux_hip(istep, otemp, osaln, atemp, ashum, ..., dthsic, dtareasic)
+!                                         surflux, embm.f90:2548-3738, without
+!                                         ENTS (initialise_surflux_hip first):
+!                                         the fluxes from the device's own top
+!                                         level, the surface slot formed on the
+!                                         device; step_goldstein then adds only
+!                                         the sea-ice model's fluxes
 !   biogem_tracercoupling(dum_ts, dum_ts1) biogem.f90:1885-1890
 !   tstepo()                              goldstein.f90:2280-2432, called at 235
 !   step_goldstein_tracers(go_ts, go_ts1) tstepo and the go_ts hand-back alone
@@ -42,6 +49,7 @@ module cgenie_hip
   public :: tstepo, step_goldstein_tracers, step_biogem, biogem_tracercoupling
   public :: cgenie_hip_set_mirror, cgenie_hip_download, cgenie_hip_inventory, cgenie_hip_diag
   public :: cgenie_hip_set_sflux, step_goldstein, cgenie_hip_atmosphere, initialise_goldstein_momentum_hip
+  public :: initialise_surflux_hip, surflux_hip, cgenie_hip_surflux_outputs
 
   type(c_ptr), save :: eng = c_null_ptr
   integer(c_int32_t), save :: grid_id = -1, mem(1) = -1
@@ -61,6 +69,9 @@ module cgenie_hip
   logical, save :: mom_on = .false.
   real(c_double), save :: scf_s = 2.0_c_double, dzz_s = 0
   real(c_double), allocatable, save :: stress_last(:,:,:)
+  ! the surface fluxes: on, and the host arrays their calls hand over
+  logical, save :: sfx_on = .false.
+  real(c_double), allocatable, target, save :: sfx_in(:,:,:), sfx_out(:,:,:), sic_c(:,:,:)
   ! step_biogem: go_ts positions of DIC, ALK, PO4, O2, DOM_C, DOM_P and the step length
   integer, save :: bgc_slots(6) = 0
   real(c_double), save :: bgc_dtyr = 0
@@ -192,6 +203,111 @@ contains
     dzz_s = ktab(nz, 6) * ktab(nz - 1, 4) / 2     ! dz(maxk) dza(maxk-1) / 2
     if (allocated(stress_last)) deallocate (stress_last)
   end subroutine initialise_goldstein_momentum_hip
+
+  ! EMBM's surface fluxes without ENTS (initialise_embm, embm.f90:198-1530):
+  ! its tables solfor(maxj,nyear), albcl, ca, pmeadj, iroff, jroff
+  ! (maxi,maxj), and the namelist constants in the order of
+  ! cgo_surflux_desc: scl(17) = scf, rmax, delf2x, rdtdim, rsictscsf,
+  ! olr_adj, olr_adj0, t_eqm, par_albsic_min, par_albsic_max, par_sich_max,
+  ! rate_co2, rate_ch4, rate_n2o and the initial co2, ch4, n2o.  surflux_hip
+  ! then computes the fluxes and the surface slot on the device.
+  subroutine initialise_surflux_hip(solfor, albcl, ca, pmeadj, iroff, jroff, igrid, polar_avg, radfor_atchem, scl)
+    real(c_double), intent(in), target :: solfor(:,:), albcl(nx, ny), ca(nx, ny), pmeadj(nx, ny)
+    integer, intent(in) :: iroff(nx, ny), jroff(nx, ny), igrid, polar_avg
+    logical, intent(in) :: radfor_atchem
+    real(c_double), intent(in) :: scl(17)
+    type(cgo_surflux_desc) :: d
+    real(c_double), allocatable, target :: tab(:,:,:), sol(:,:)
+    integer(c_int32_t), allocatable, target :: ro(:,:,:)
+
+    if (.not. c_associated(eng)) call check(CGO_E_STATE, 'initialise_surflux_hip: initialise_goldstein_hip first')
+    if (size(solfor, 1) /= ny) call check(CGO_E_ARG, 'initialise_surflux_hip: solfor(maxj, nyear) expected')
+    allocate (sol(ny, size(solfor, 2)), tab(nx, ny, 3), ro(nx, ny, 2))
+    sol = solfor
+    tab(:, :, 1) = albcl; tab(:, :, 2) = ca; tab(:, :, 3) = pmeadj
+    ro(:, :, 1) = int(iroff, c_int32_t); ro(:, :, 2) = int(jroff, c_int32_t)
+    d%solfor = c_loc(sol)
+    d%nyear = int(size(solfor, 2), c_int32_t)
+    d%albcl = c_loc(tab(1, 1, 1)); d%ca = c_loc(tab(1, 1, 2)); d%pmeadj = c_loc(tab(1, 1, 3))
+    d%iroff = c_loc(ro(1, 1, 1)); d%jroff = c_loc(ro(1, 1, 2))
+    d%igrid = int(igrid, c_int32_t)
+    d%polar_avg = int(polar_avg, c_int32_t)
+    d%radfor_atchem = merge(1_c_int32_t, 0_c_int32_t, radfor_atchem)
+    d%scf = scl(1); d%rmax = scl(2); d%delf2x = scl(3); d%rdtdim = scl(4); d%rsictscsf = scl(5)
+    d%olr_adj = scl(6); d%olr_adj0 = scl(7); d%t_eqm = scl(8); d%albsic_min = scl(9); d%albsic_max = scl(10)
+    d%sich_max = scl(11); d%rate_co2 = scl(12); d%rate_ch4 = scl(13); d%rate_n2o = scl(14)
+    d%co2 = scl(15); d%ch4 = scl(16); d%n2o = scl(17)
+    call check(cgo_grid_set_surflux(eng, grid_id, d), 'cgo_grid_set_surflux')
+    if (.not. mom_on) dzz_s = ktab(nz, 6) * ktab(nz - 1, 4) / 2
+    allocate (sfx_in(nx, ny, 5), sfx_out(nx, ny, CGO_SF_N), sic_c(nx, ny, 2))
+    sfx_on = .true.
+  end subroutine initialise_surflux_hip
+
+  ! the wind stresses, handed to the engine when they differ from the last
+  ! (the velocity solve and the surface fluxes' wind speed read them)
+  subroutine set_stress(stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn)
+    real(c_double), dimension(:,:), intent(in) :: stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn
+    if (.not. allocated(stress_last)) then
+       allocate (stress_last(nx, ny, 4))
+       stress_last = huge(1.0_dp)
+    end if
+    if (any(stressxu_ocn /= stress_last(:, :, 1)) .or. any(stressyu_ocn /= stress_last(:, :, 2)) .or. &
+        any(stressxv_ocn /= stress_last(:, :, 3)) .or. any(stressyv_ocn /= stress_last(:, :, 4))) then
+       call check(cgo_set_wind(eng, mem(1), stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn, scf_s), &
+            'cgo_set_wind')
+       stress_last(:, :, 1) = stressxu_ocn; stress_last(:, :, 2) = stressyu_ocn
+       stress_last(:, :, 3) = stressxv_ocn; stress_last(:, :, 4) = stressyv_ocn
+    end if
+  end subroutine set_stress
+
+  ! surflux (embm.f90:2548-2578) with the arguments of its configuration
+  ! without ENTS, in the reference's order.  otemp and osaln, the ocean's top
+  ! level, are read on the device (the top level its last tstepo left); the
+  ! air temperature and humidity and the sea-ice height, cover and surface
+  ! temperature are the host's (EMBM and the sea-ice model step them), ashum
+  ! and tice come back as the call leaves them.  The surface slot for the
+  ! next step_goldstein is formed on the device.
+  subroutine surflux_hip(istep, otemp, osaln, atemp, ashum, sich, sica, tice, albice, stressxu_ocn, stressyu_ocn, &
+       stressxv_ocn, stressyv_ocn, albedo, fxlho, fxsho, fxswo, fxlwo, evap_ocn, pptn_ocn, runoff_ocn, runoff_land, &
+       fxlha, fxsha, fxswa, fxlwa, evap_atm, pptn_atm, dthsic, dtareasic)
+    integer, intent(in) :: istep
+    real(c_double), dimension(:,:), intent(in) :: otemp, osaln, atemp, sich, sica
+    real(c_double), dimension(:,:), intent(in) :: stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn
+    real(c_double), dimension(:,:), intent(inout) :: ashum, tice
+    real(c_double), dimension(:,:), intent(out) :: albice, albedo, fxlho, fxsho, fxswo, fxlwo, evap_ocn, &
+         pptn_ocn, runoff_ocn, runoff_land, fxlha, fxsha, fxswa, fxlwa, evap_atm, pptn_atm, dthsic, dtareasic
+    integer(c_int32_t) :: ids(1)
+
+    if (.not. sfx_on) call check(CGO_E_STATE, 'surflux_hip: initialise_surflux_hip first')
+    if (istep < 0 .or. size(otemp) /= nx * ny .or. size(osaln) /= nx * ny) &
+         call check(CGO_E_ARG, 'surflux_hip: bad istep or field shape')
+    call set_stress(stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn)
+    sfx_in(:, :, 1) = atemp; sfx_in(:, :, 2) = ashum; sfx_in(:, :, 3) = sich
+    sfx_in(:, :, 4) = sica; sfx_in(:, :, 5) = tice
+    call check(cgo_set_surface(eng, mem(1), c_loc(sfx_in(1, 1, 1)), c_loc(sfx_in(1, 1, 2)), c_loc(sfx_in(1, 1, 3)), &
+         c_loc(sfx_in(1, 1, 4)), c_loc(sfx_in(1, 1, 5)), c_null_ptr, c_null_ptr), 'cgo_set_surface')
+    ids(1) = mem(1)
+    call check(cgo_surflux(eng, ids, 1_c_int32_t), 'cgo_surflux')
+    call check(cgo_download_surflux(eng, mem(1), c_loc(sfx_out), c_loc(sfx_in(1, 1, 2)), c_loc(sfx_in(1, 1, 5)), &
+         c_null_ptr), 'cgo_download_surflux')
+    ashum = sfx_in(:, :, 2)
+    tice = sfx_in(:, :, 5)
+    fxlho = sfx_out(:, :, 1 + CGO_SF_FXLHO); fxsho = sfx_out(:, :, 1 + CGO_SF_FXSHO)
+    fxswo = sfx_out(:, :, 1 + CGO_SF_FXSWO); fxlwo = sfx_out(:, :, 1 + CGO_SF_FXLWO)
+    evap_ocn = sfx_out(:, :, 1 + CGO_SF_EVAP_OCN); pptn_ocn = sfx_out(:, :, 1 + CGO_SF_PPTN_OCN)
+    runoff_ocn = sfx_out(:, :, 1 + CGO_SF_RUNOFF_OCN); runoff_land = sfx_out(:, :, 1 + CGO_SF_RUNOFF_LAND)
+    fxlha = sfx_out(:, :, 1 + CGO_SF_FXLHA); fxsha = sfx_out(:, :, 1 + CGO_SF_FXSHA)
+    fxswa = sfx_out(:, :, 1 + CGO_SF_FXSWA); fxlwa = sfx_out(:, :, 1 + CGO_SF_FXLWA)
+    evap_atm = sfx_out(:, :, 1 + CGO_SF_EVAP_ATM); pptn_atm = sfx_out(:, :, 1 + CGO_SF_PPTN_ATM)
+    dthsic = sfx_out(:, :, 1 + CGO_SF_DTHSIC); dtareasic = sfx_out(:, :, 1 + CGO_SF_DTAREASIC)
+    albedo = sfx_out(:, :, 1 + CGO_SF_ALBEDO); albice = sfx_out(:, :, 1 + CGO_SF_ALBICE)
+  end subroutine surflux_hip
+
+  ! the last surflux_hip call's outputs, (maxi, maxj, CGO_SF_N) in the order of CGO_SF_*
+  subroutine cgenie_hip_surflux_outputs(out)
+    real(c_double), intent(out) :: out(nx, ny, CGO_SF_N)
+    out = sfx_out
+  end subroutine cgenie_hip_surflux_outputs
 
   ! The total heat and water diagnostics of step_goldstein
   ! (goldstein.f90:79-96, 458-471), summed in the reference's order.
@@ -333,33 +449,33 @@ contains
 
     if (.not. c_associated(eng)) call check(CGO_E_STATE, 'step_goldstein: initialise_goldstein_hip first')
     if (istep < 0) call check(CGO_E_ARG, 'step_goldstein: negative istep')
-    ! net heat and freshwater fluxes into the surface slot (149-170)
-    fx0neto = netsolar_ocn + sensible_ocn + netlong_ocn + latent_ocn + fx0sic_ocn
-    fwfxneto = pptn_ocn + evap_ocn + runoff_ocn + fwsic_ocn
-    fwfxneto = fwfxneto * mm2m
-    sflux_c = 0.0_dp
-    sflux_c(1, :, :) = -fx0neto * rfluxsc
-    if (nl >= 2) sflux_c(2, :, :) = fwfxneto * rpmesco()
-    if (any(fx0neto /= 0.0_dp) .or. any(fwfxneto /= 0.0_dp)) then
-       call check(cgo_set_sflux(eng, mem(1), c_loc(sflux_c)), 'cgo_set_sflux')
-       sflux_on = .true.
-    else if (sflux_on) then
-       call check(cgo_set_sflux(eng, mem(1), c_null_ptr), 'cgo_set_sflux')
-       sflux_on = .false.
+    if (sfx_on) then
+       ! surflux_hip formed the slot on the device; the sea-ice model's heat
+       ! and freshwater, which it hands over after surflux, join it here
+       if (any(fx0sic_ocn /= 0.0_dp) .or. any(fwsic_ocn /= 0.0_dp)) then
+          sic_c(:, :, 1) = fx0sic_ocn
+          sic_c(:, :, 2) = fwsic_ocn
+          call check(cgo_surface_slot(eng, mem(1), c_loc(sic_c(1, 1, 1)), c_loc(sic_c(1, 1, 2))), 'cgo_surface_slot')
+       end if
+    else
+       ! net heat and freshwater fluxes into the surface slot (149-170)
+       fx0neto = netsolar_ocn + sensible_ocn + netlong_ocn + latent_ocn + fx0sic_ocn
+       fwfxneto = pptn_ocn + evap_ocn + runoff_ocn + fwsic_ocn
+       fwfxneto = fwfxneto * mm2m
+       sflux_c = 0.0_dp
+       sflux_c(1, :, :) = -fx0neto * rfluxsc
+       if (nl >= 2) sflux_c(2, :, :) = fwfxneto * rpmesco()
+       if (any(fx0neto /= 0.0_dp) .or. any(fwfxneto /= 0.0_dp)) then
+          call check(cgo_set_sflux(eng, mem(1), c_loc(sflux_c)), 'cgo_set_sflux')
+          sflux_on = .true.
+       else if (sflux_on) then
+          call check(cgo_set_sflux(eng, mem(1), c_null_ptr), 'cgo_set_sflux')
+          sflux_on = .false.
+       end if
     end if
     if (mom_on) then
        ! the wind stresses, when they differ from the engine's (102-107)
-       if (.not. allocated(stress_last)) then
-          allocate (stress_last(nx, ny, 4))
-          stress_last = huge(1.0_dp)
-       end if
-       if (any(stressxu_ocn /= stress_last(:, :, 1)) .or. any(stressyu_ocn /= stress_last(:, :, 2)) .or. &
-           any(stressxv_ocn /= stress_last(:, :, 3)) .or. any(stressyv_ocn /= stress_last(:, :, 4))) then
-          call check(cgo_set_wind(eng, mem(1), stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn, scf_s), &
-               'cgo_set_wind')
-          stress_last(:, :, 1) = stressxu_ocn; stress_last(:, :, 2) = stressyu_ocn
-          stress_last(:, :, 3) = stressxv_ocn; stress_last(:, :, 4) = stressyv_ocn
-       end if
+       call set_stress(stressxu_ocn, stressyu_ocn, stressxv_ocn, stressyv_ocn)
     else
        ! velocities, when the caller's differ from the engine's
        ! (the engine reads the interior faces only; the west face of i = 1 is the
@@ -502,6 +618,8 @@ contains
     eng = c_null_ptr
     if (allocated(k1_c)) deallocate (k1_c, jtab, ktab)
     if (allocated(u_last)) deallocate (u_last, sflux_c)
+    if (allocated(sfx_in)) deallocate (sfx_in, sfx_out, sic_c)
+    sfx_on = .false.
   end subroutine end_cgenie_hip
 
 end module cgenie_hip

@@ -53,6 +53,26 @@ module cgo_api
      real(c_double) :: adrag, rel
   end type cgo_momentum_desc
 
+  ! EMBM's surface fluxes of a grid (cgo_grid_set_surflux): initialise_embm's
+  ! tables, each c_loc of the reference's array, and its namelist constants
+  type, bind(C) :: cgo_surflux_desc
+     type(c_ptr) :: solfor            ! solfor(maxj, nyear)
+     integer(c_int32_t) :: nyear
+     type(c_ptr) :: albcl, ca, pmeadj ! (maxi, maxj); pmeadj may be c_null_ptr
+     type(c_ptr) :: iroff, jroff      ! (maxi, maxj) integer(c_int32_t)
+     integer(c_int32_t) :: igrid, polar_avg, radfor_atchem
+     real(c_double) :: scf, rmax, delf2x, rdtdim, rsictscsf, olr_adj, olr_adj0, t_eqm
+     real(c_double) :: albsic_min, albsic_max, sich_max, rate_co2, rate_ch4, rate_n2o
+     real(c_double) :: co2, ch4, n2o
+  end type cgo_surflux_desc
+
+  ! the surface fluxes' outputs (include/cgocn.h CGO_SF_*, 0-based planes)
+  integer, parameter :: CGO_SF_N = 28
+  integer, parameter :: CGO_SF_FXLHO = 0, CGO_SF_FXSHO = 1, CGO_SF_FXSWO = 2, CGO_SF_FXLWO = 3, CGO_SF_EVAP_OCN = 4, &
+       CGO_SF_PPTN_OCN = 5, CGO_SF_RUNOFF_OCN = 6, CGO_SF_RUNOFF_LAND = 7, CGO_SF_FXLHA = 8, CGO_SF_FXSHA = 9, &
+       CGO_SF_FXSWA = 10, CGO_SF_FXLWA = 11, CGO_SF_EVAP_ATM = 12, CGO_SF_PPTN_ATM = 13, CGO_SF_DTHSIC = 14, &
+       CGO_SF_DTAREASIC = 15, CGO_SF_ALBEDO = 16, CGO_SF_ALBICE = 17
+
   type, bind(C) :: cgo_diag
      integer(c_int64_t) :: steps
      integer(c_int32_t) :: limps, carb_fail
@@ -150,6 +170,46 @@ module cgo_api
        integer(c_int32_t), value :: m
        real(c_double), intent(out) :: psi(*)
      end function cgo_download_psi
+
+     integer(c_int) function cgo_grid_set_surflux(e, grid_id, d) bind(C, name="cgo_grid_set_surflux")
+       import :: c_int, c_ptr, c_int32_t, cgo_surflux_desc
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: grid_id
+       type(cgo_surflux_desc), intent(in) :: d
+     end function cgo_grid_set_surflux
+
+     ! the member's air and sea-ice state, each c_loc of a (maxi,maxj) array
+     ! or c_null_ptr (zero); atemp c_null_ptr switches the fluxes off
+     integer(c_int) function cgo_set_surface(e, m, atemp, ashum, sich, sica, tice, fx0sic, fwsic) &
+          bind(C, name="cgo_set_surface")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       type(c_ptr), value :: atemp, ashum, sich, sica, tice, fx0sic, fwsic
+     end function cgo_set_surface
+
+     integer(c_int) function cgo_surflux(e, members, n) bind(C, name="cgo_surflux")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), intent(in) :: members(*)
+       integer(c_int32_t), value :: n
+     end function cgo_surflux
+
+     ! the slot again with new sea-ice fluxes, each c_loc of (maxi,maxj) or c_null_ptr
+     integer(c_int) function cgo_surface_slot(e, m, fx0sic, fwsic) bind(C, name="cgo_surface_slot")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       type(c_ptr), value :: fx0sic, fwsic
+     end function cgo_surface_slot
+
+     ! out(maxi,maxj,CGO_SF_N); ashum, tice (maxi,maxj); gases(4)
+     integer(c_int) function cgo_download_surflux(e, m, out, ashum, tice, gases) bind(C, name="cgo_download_surflux")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e
+       integer(c_int32_t), value :: m
+       type(c_ptr), value :: out, ashum, tice, gases
+     end function cgo_download_surflux
 
      ! sflux: c_loc of ts(:,1:maxi,1:maxj,maxk+1) values, or c_null_ptr
      integer(c_int) function cgo_set_sflux(e, m, sflux) bind(C, name="cgo_set_sflux")

@@ -9,9 +9,11 @@ program cgo_api_layout
   type(cgo_bgc_params) :: b
   type(cgo_diag) :: d
   type(cgo_momentum_desc) :: md
+  type(cgo_surflux_desc) :: sd
   write (*, '(a,i0)') 'cgo_grid_desc ', c_sizeof(g)
   write (*, '(a,i0)') 'cgo_member_params ', c_sizeof(m)
   write (*, '(a,i0)') 'cgo_bgc_params ', c_sizeof(b)
   write (*, '(a,i0)') 'cgo_diag ', c_sizeof(d)
   write (*, '(a,i0)') 'cgo_momentum_desc ', c_sizeof(md)
+  write (*, '(a,i0)') 'cgo_surflux_desc ', c_sizeof(sd)
 end program cgo_api_layout

@@ -260,6 +260,11 @@ int cgo_set_surface(cgo_engine *e, int32_t m, const double *atemp, const double 
  * order itself calls it where genie calls surflux_wrapper, genie.f90:271-282,
  * before cgo_tstepo) */
 int cgo_surflux(cgo_engine *e, const int32_t *members, int32_t n);
+/* the surface slot of member m formed again from its last call's outputs
+ * with new sea-ice heat and freshwater fluxes (fx0sic_ocn, fwsic_ocn of
+ * step_goldstein, goldstein.f90:149-170; NULL = zero), for a host whose
+ * sea-ice model runs between surflux and step_goldstein (genie.f90:271-313) */
+int cgo_surface_slot(cgo_engine *e, int32_t m, const double *fx0sic, const double *fwsic);
 /* The outputs of member m's last call, CGO_SF_N fields (maxi, maxj) in the
  * order below: surflux's ocean and atmosphere fluxes (W m-2, mm s-1), the
  * sea-ice changes dthsic, dtareasic, the albedos, and EMBM's own fxsw, fx0a,

@@ -39,14 +39,14 @@ def test_bind_c_types_match_c_structs(tmp_path):
     out = subprocess.run([LAYOUT], check=True, capture_output=True, text=True).stdout.split()
     fsz = {out[i]: int(out[i + 1]) for i in range(0, len(out), 2)}
     src = tmp_path / "sz.c"
-    src.write_text('#include <stdio.h>\n#include "cgocn.h"\nint main(void){printf("%zu %zu %zu %zu %zu\\n",'
+    src.write_text('#include <stdio.h>\n#include "cgocn.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(cgo_grid_desc),sizeof(cgo_member_params),sizeof(cgo_bgc_params),sizeof(cgo_diag),'
-                   'sizeof(cgo_momentum_desc));}\n')
+                   'sizeof(cgo_momentum_desc),sizeof(cgo_surflux_desc));}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     csz = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert [fsz["cgo_grid_desc"], fsz["cgo_member_params"], fsz["cgo_bgc_params"], fsz["cgo_diag"],
-            fsz["cgo_momentum_desc"]] == csz
+            fsz["cgo_momentum_desc"], fsz["cgo_surflux_desc"]] == csz
 
 
 REF_SRC = "/root/reference/src"
@@ -92,11 +92,12 @@ def test_reference_wrapper_calls_compile_against_module(tmp_path):
 
 
 def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, mirror=False, dtyr=1.0 / 96,
-                       mom=None):
-    """mom: None, or (world, stress (maxj, maxi, 4), adrag, rel, scf) for the velocity solve."""
+                       mom=None, sfx=None):
+    """mom: None, or (world, stress (maxj, maxi, 4), adrag, rel, scf) for the velocity solve;
+    sfx: None, or (SurfluxSetup, (atemp, ashum, sich, sica, tice), stress) for the surface fluxes."""
     g = c.g
     hdr = np.array([g.maxi, g.maxj, g.maxk, c.tset.L, nsteps, kbiogem, 1, precision, 1 if bgc else 0, 0,
-                    1 if mirror else 0, 1 if mom is not None else 0], dtype="<i4")
+                    1 if mirror else 0, 1 if mom is not None else 0, 1 if sfx is not None else 0], dtype="<i4")
     sc = np.array([g.dphi, g.rdphi, 5000.0, c.diff[0], c.diff[1], *list(g.ec)[:5]], dtype="<f8")
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
@@ -127,6 +128,25 @@ def write_driver_input(path, c, nsteps, kbiogem=1, precision=E.FP64, bgc=True, m
             f.write(np.array([adrag, rel, scf], dtype="<f8").tobytes())
             # stress(maxi, maxj, 4) column-major = (4, maxj, maxi) C order
             f.write(np.ascontiguousarray(np.moveaxis(st, 2, 0), dtype="<f8").tobytes())
+        if sfx is not None:
+            su, atm, st = sfx
+            f.write(np.array([su.nyear], dtype="<i4").tobytes())
+            f.write(np.ascontiguousarray(su.solfor, dtype="<f8").tobytes())       # solfor(maxj, nyear)
+            for a in (su.albcl, su.ca, su.pmeadj):
+                f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+            for a in (su.iroff, su.jroff):
+                f.write(np.ascontiguousarray(a, dtype="<i4").tobytes())
+            f.write(np.array([su.igrid, su.polar_avg, su.radfor_atchem], dtype="<i4").tobytes())
+            f.write(np.array([getattr(su, n) for n in SFX_SCALARS], dtype="<f8").tobytes())
+            for a in atm:
+                f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+            if mom is None:
+                f.write(np.ascontiguousarray(np.moveaxis(st, 2, 0), dtype="<f8").tobytes())
+
+
+# initialise_surflux_hip's scl(17)
+SFX_SCALARS = ("scf", "rmax", "delf2x", "rdtdim", "rsictscsf", "olr_adj", "olr_adj0", "t_eqm", "albsic_min",
+               "albsic_max", "sich_max", "rate_co2", "rate_ch4", "rate_n2o", "co2", "ch4", "n2o")
 
 
 CONV_YR_S = 3600.0 * (24.0 * 365.25)              # gem_cmn.f90:511-513
@@ -291,3 +311,62 @@ def test_driver_velocity_solve_matches_engine(tmp_path):
     den = 1.0e3 * 5000.0 * 0.05 * (2 * 7.2921e-5)
     assert np.array_equal(go_tau[..., 0], 2.0 * st[..., 0] / den / dzz * dzz)
     assert np.array_equal(go_tau[..., 1], 2.0 * st[..., 3] / den / dzz * dzz)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mirror", [False, True])
+def test_driver_surface_fluxes_match_engine(tmp_path, mirror):
+    """surflux_hip before every step_goldstein, with the velocity solve, on the
+    worbe2 world with its own winds and runoff routing and sea ice: the state
+    and the last call's outputs, ashum and tice equal the Python path through
+    the same library bit for bit (which matches the oracle,
+    tests/test_gpu_surflux.py)."""
+    from types import SimpleNamespace
+
+    from cgenie_amd import grid as G
+    from cgenie_amd import surface as SF
+    import test_gpu_surflux as S
+
+    assert os.path.exists(DRIVER), "build the host driver first (__graft_entry__.build())"
+    world = G.read_world(os.path.join(S.GOLDEN, "worbe2"), S.MAXI, S.MAXJ)
+    g = G.make_grid(S.MAXI, S.MAXJ, S.MAXK)
+    L, nsteps = 4, 5
+    ts = S.state(world.k1, L)
+    st = S.stress()
+    su = SF.surflux_setup(g, world.k1)
+    atm = SF.atmosphere(g, world.k1, 13, ice=True)
+    case = SimpleNamespace(g=g, k1=world.k1, u=np.zeros((g.maxk, g.maxj + 1, g.maxi + 1, 3)), ts=ts,
+                           tset=SimpleNamespace(L=L), diff=S.DIFF)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_driver_input(fin, case, nsteps, bgc=False, mirror=mirror, mom=(world, st, 2.5, 0.9, 2.0),
+                       sfx=(su, atm, st))
+    print("driver with surface fluxes and the velocity solve, worbe2, mirror", mirror, flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(fout, dtype="<f8")
+    n_c, n_col = g.maxi * g.maxj * g.maxk, g.maxi * g.maxj
+    o = 0
+    got = raw[o:o + ts.size].reshape(ts.shape); o += ts.size
+    o += L + n_c + 2 + 38 * n_col + 34 * n_col + 3 * n_c + (g.maxi + 1) * (g.maxj + 1) + 2 * n_col
+    out = raw[o:o + 28 * n_col].reshape(28, g.maxj, g.maxi); o += 28 * n_col
+    ashum = raw[o:o + n_col].reshape(g.maxj, g.maxi); o += n_col
+    tice = raw[o:o + n_col].reshape(g.maxj, g.maxi); o += n_col
+    assert o == raw.size
+    eng = E.Engine(0)
+    gid = eng.add_grid(g, world.k1)
+    eng.set_surflux(gid, su)
+    eng.set_momentum(gid, g, world)
+    m = eng.add_member(gid, L, E.FP64, S.DIFF, diso=True)
+    eng.upload(m, ts)
+    eng.set_wind(m, st)
+    eng.set_surface(m, *atm)
+    eng.step([m], nsteps)
+    ref = eng.download(m, ts.shape)
+    dev = eng.surface_outputs(m, g)
+    eng.close()
+    assert np.array_equal(got, ref)
+    for q, n in enumerate(E.SF_NAMES):
+        assert np.array_equal(out[q], dev[n]), n
+    assert np.array_equal(ashum, dev["ashum"]) and np.array_equal(tice, dev["tice"])
+    assert dev["istep"] == nsteps

@@ -315,3 +315,32 @@ def test_refusals(eng, world):
         eng.set_surflux(gid3, bad)
     for x in (m, m2):
         eng.remove_member(x)
+
+
+def test_surface_slot_with_sea_ice_fluxes(eng, world):
+    """cgo_surface_slot: the sea-ice model's heat and freshwater, handed over
+    after surflux (as genie.f90 orders them), join the slot the call formed;
+    one tstepo with it against the oracle's slot and tracer step."""
+    say("surface slot with the sea-ice model's fluxes")
+    g, gid, m, ts, su, atm = new_member(eng, world, mom=False)
+    u = G.make_velocity(g, world.k1, 46)
+    eng.set_velocity(m, u)
+    rng = np.random.default_rng(9)
+    ice = atm[3] > 0
+    fx0sic = np.where(ice, rng.normal(-30.0, 10.0, ice.shape), 0.0)
+    fwsic = np.where(ice, rng.normal(0.0, 2e-5, ice.shape), 0.0)
+    eng.surflux([m])
+    eng.surface_slot(m, fx0sic, fwsic)
+    eng.tstepo([m], 1)
+    so = surf_oracle(world, su, atm)
+    so.step(ts[MAXK - 1, ..., 0], ts[MAXK - 1, ..., 1])
+    gold = O.GoldsteinOracle(O.grid(MAXI, MAXJ, MAXK), world.k1, ts, u, sflux=so.slot(ts.shape[-1], fx0sic, fwsic))
+    gold.step(1)
+    err = rel_cells(eng.download(m, ts.shape), gold.ts, wet3(world.k1))
+    say("  state max rel err", err)
+    assert err <= 1e-12
+    # without the sea-ice fluxes the state differs where there is ice
+    gold0 = O.GoldsteinOracle(O.grid(MAXI, MAXJ, MAXK), world.k1, ts, u, sflux=so.slot(ts.shape[-1]))
+    gold0.step(1)
+    assert np.abs(gold.ts[MAXK - 1, ..., 0] - gold0.ts[MAXK - 1, ..., 0])[ice].max() > 1e-6
+    eng.remove_member(m)
