@@ -219,7 +219,66 @@ def run_gpu(args):
     for m in ids:
         eng.remove_member(m)
     eng.close()
+    res["tuned_fp32c"] = tuned["fp32c"]
     return res
+
+
+def run_prognostic(args, tuned):
+    """The fully prognostic ocean: every member on the reference's worbe2
+    world (36x36x8) with its own wind stress, EMBM's surface fluxes (sea
+    ice, runoff, CO2 radiative forcing from the member's own ATCHEM
+    atmosphere), the velocity solve, transport, BIOGEM with air-sea exchange
+    and the tracer coupling; FP32C, replayed one-year graphs.  Each member
+    has its own diffusivities, remineralisation length and air temperature."""
+    from cgenie_amd import engine as E
+    from cgenie_amd import grid as G
+    from cgenie_amd import surface as SF
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    world = G.read_world(os.path.join(here, "tests", "golden", "worbe2"), 36, 36)
+    stress = np.load(os.path.join(here, "tests", "golden", "worbe2_wind.npz"))["stress"]
+    g = G.make_grid(36, 36, 8, nyear=STEPS_PER_YEAR)
+    tset = G.tracer_set("config2")
+    ts0 = G.make_tracers(g, world.k1, tset, 11)
+    light, mld = G.make_light(g, 12), G.make_mld(g, 13, 0.0)
+    su = SF.surflux_setup(g, world.k1, radfor_atchem=1)
+    atemp, ashum, sich, sica, tice = SF.atmosphere(g, world.k1, 14, ice=True)
+    eng = E.Engine(0)
+    eng.set_tile(E.FP32C, tuned["tracer_block"], tuned["cols"], tuned["ring"], tuned["iso_prepass"])
+    gid = eng.add_grid(g, world.k1)
+    eng.set_momentum(gid, g, world)
+    eng.set_surflux(gid, su)
+    rng = np.random.default_rng(15)
+    ids = []
+    for q in range(args.members):
+        kh = rng.uniform(500.0, 5000.0)
+        kv = math.exp(rng.uniform(math.log(1e-5), math.log(1e-4)))
+        m = eng.add_member(gid, tset.L, E.FP32C, G.diffusivities(kh, kv), diso=True)
+        eng.upload(m, ts0)
+        bp = E.default_bgc_params(1.0 / STEPS_PER_YEAR, tset, carb_tol=1e-12, carb_maxit=1000,
+                                  POC_eL1=rng.uniform(300.0, 900.0), airsea=1, atm_pCO2=278e-6)
+        eng.bgc_init(m, bp, light, mld)
+        eng.set_windspeed(m, np.full((36, 36), 8.0))
+        eng.set_wind(m, stress)
+        eng.set_surface(m, atemp + rng.normal(0.0, 1.0), ashum, sich, sica, tice)
+        ids.append(m)
+    gy = eng.capture(ids, STEPS_PER_YEAR)
+    eng.launch(gy, 1)
+    eng.sync()
+    years = max(1, args.steps // STEPS_PER_YEAR)
+    t0 = time.perf_counter()
+    eng.launch(gy, years)
+    eng.sync()
+    tot = time.perf_counter() - t0
+    out = dict(member_years_per_s=round(len(ids) * years / tot, 2),
+               ms_per_step=round(1e3 * tot / (years * STEPS_PER_YEAR), 4), members=len(ids), years=years,
+               grid="36x36x8 worbe2", tracers=tset.L,
+               kernels_ms=dict(surflux=round(eng.time_kernel(6, ids, 20), 4), velocity=round(eng.time_kernel(5, ids, 5), 4),
+                               flux=round(eng.time_kernel(0, ids, 10), 4), co=round(eng.time_kernel(1, ids, 10), 4)))
+    print(f"# prognostic: {out}", file=sys.stderr, flush=True)
+    eng.drop_graph(gy)
+    eng.close()
+    return out
 
 
 def run_serving(seconds: float, rate: float = 4.0, seed: int = 5, uniform: bool = False,
@@ -320,6 +379,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("CGO_CPU_CORES", "16")))
     ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-prognostic", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--probe", choices=("fp32c", "fp64"), default=None,
                     help="counter-run mode: eager steps of the ensemble and a copy calibration, no JSON line")
@@ -340,6 +400,7 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline(cores, args.cpu_steps)
     print(f"# cpu baseline done: {cpu}", file=sys.stderr, flush=True)
     r = run_gpu(args)
+    prognostic = run_prognostic(args, r["tuned_fp32c"]) if not args.no_prognostic else None
     serving = run_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     serving_u = run_serving(args.serve_seconds, uniform=True) if args.serve_seconds > 0 else None
     serving_s = (run_serving(args.serve_seconds, rate=32.0, uniform=True, max_active=256)
@@ -390,7 +451,8 @@ def main():
                                   bgc_coupling=r["fp64"]["bgc_coupling_ms"], tracer_block=r["fp64"]["tracer_block"],
                                   flux_cols=r["fp64"]["flux_cols"], flux_ring=r["fp64"]["flux_ring"],
                                   flux_iso_prepass=r["fp64"]["flux_iso_prepass"])),
-        roofline=roof, roofline_bgc=bgc_roof, cpu_baseline=cpu, serving=serving, serving_uniform=serving_u,
+        roofline=roof, roofline_bgc=bgc_roof, cpu_baseline=cpu, prognostic=prognostic, serving=serving,
+        serving_uniform=serving_u,
         serving_uniform_vs_fp64_single=(round(serving_u["member_years_per_s"] / single, 1) if serving_u else None),
         serving_saturated=serving_s,
         serving_saturated_vs_fp64_single=(round(serving_s["member_years_per_s"] / single, 1) if serving_s else None))
