@@ -22,7 +22,7 @@ void launch_flux64(const MemberDev*, const int32_t*, int nlist, const FluxShape&
                    int iso_split, int part, size_t lds_cap, hipStream_t);
 void launch_flux32(const MemberDev*, const int32_t*, int nlist, const FluxShape&, int maxL, int lb, int nt, int nbuf,
                    int iso_split, int part, size_t lds_cap, hipStream_t);
-void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int prec, hipStream_t);
+void launch_co(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int prec, hipStream_t);
 void launch_flip(const MemberDev*, const int32_t*, int nlist, hipStream_t);
 void launch_bio(const MemberDev*, const int32_t*, int nlist, int nwetmax, int W, int prec, hipStream_t);
 void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, int maxkmax, int maxL, int prec,
@@ -1967,7 +1967,7 @@ struct Lists {
     int momN = 0, momCol = 0, momBand = 0;  // most psi points, columns and maxi of a member with a velocity solve
     int sfxCol = 0;                         // most columns of a member with surface fluxes
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
-    int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0};
+    int maxk_bgc[2] = {0, 0}, wstream[2] = {0, 0}, maxk[2] = {0, 0};
     int ncol_atm = 0;  // widest grid of the members with air-sea exchange, 0 if none
     FluxShape fs[2] = {};
 };
@@ -1998,6 +1998,7 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
         ls.maxL[p] = std::max(ls.maxL[p], m->L);
         ls.maxi[p] = std::max(ls.maxi[p], m->dev.g.maxi);
         ls.maxwet[p] = std::max(ls.maxwet[p], m->dev.g.nwet);
+        ls.maxk[p] = std::max(ls.maxk[p], m->dev.g.maxk);
         for (int v = 0; v < 2; v++) {
             ls.fs[p].nblk[v] = std::max(ls.fs[p].nblk[v], m->dev.g.nfblk[v]);
             ls.fs[p].tw[v] = std::max(ls.fs[p].tw[v], e->grids[m->grid]->ftw[v]);
@@ -2035,7 +2036,7 @@ static void enqueue_flux(cgo_engine* e, const Lists& ls, const int32_t* d_lists,
 // co and the parity flip that end tstepo
 static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lists) {
     for (int p = 0; p < 2; p++)
-        if (ls.n[p]) launch_co(e->d_table, d_lists + ls.off[p], ls.n[p], ls.maxwet[p], p, e->s);
+        if (ls.n[p]) launch_co(e->d_table, d_lists + ls.off[p], ls.n[p], ls.maxwet[p], ls.maxk[p], p, e->s);
     launch_flip(e->d_table, d_lists + ls.off[4], ls.n[4], e->s);
 }
 
@@ -2380,7 +2381,7 @@ int cgo_time_kernel(cgo_engine* e, int32_t which, const int32_t* ids, int32_t n,
                     launch_flux32(e->d_table, l, ls.n[p], ls.fs[p], ls.maxL[p], e->lb[1], e->nt[1], e->nbuf[1], e->isosplit[1], 3, e->lds_cap,
                           e->s);
             } else if (which == 1) {
-                launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], p, e->s);
+                launch_co(e->d_table, l, ls.n[p], ls.maxwet[p], ls.maxk[p], p, e->s);
             } else if (which == 4) {
                 if (ls.n[2 + p])
                     launch_bio(e->d_table, e->d_list + ls.off[2 + p], ls.n[2 + p], ls.maxwet_bgc[p], ls.wstream[p], p, e->s);

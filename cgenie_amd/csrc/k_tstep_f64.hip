@@ -11,12 +11,18 @@ void launch_flux64(const MemberDev* t, const int32_t* l, int n, const FluxShape&
     launch_flux_t<double>(t, l, n, fs, maxL, lb, nt, nbuf, iso_split, part, lds_cap, s);
 }
 
-void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int prec, hipStream_t s) {
-    const dim3 grid((nwetmax + 63) / 64, n);
+void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int maxkmax, int prec, hipStream_t s) {
+    // 256 threads per workgroup, fewer where the deepest grid's level arrays
+    // would not fit in 64 KiB of LDS
+    const int K = maxkmax + 2;
+    int cols = 256 / CO_LANES;
+    while (cols > 8 && co_lds_bytes(cols, K) > 65536) cols /= 2;
+    const dim3 grid((nwetmax + cols - 1) / cols, n);
+    const size_t lds = co_lds_bytes(cols, K);
     if (prec == 0)
-        hipLaunchKernelGGL(k_co<double>, grid, dim3(64), 0, s, t, l);
+        hipLaunchKernelGGL(k_co<double>, grid, dim3(cols * CO_LANES), lds, s, t, l, K);
     else
-        hipLaunchKernelGGL(k_co<float>, grid, dim3(64), 0, s, t, l);
+        hipLaunchKernelGGL(k_co<float>, grid, dim3(cols * CO_LANES), lds, s, t, l, K);
 }
 
 void launch_flip(const MemberDev* t, const int32_t* l, int n, hipStream_t s) {

@@ -56,6 +56,8 @@ template <> struct StateIO<double> {
     static __device__ __forceinline__ void st(void* s, int16_t*, uint32_t i, double x) {
         at((AS_GLOBAL double*)s, i) = x;
     }
+    // the value a later ld returns after st(x)
+    static __device__ __forceinline__ double rt(double x, bool) { return x; }
 };
 template <> struct StateIO<float> {
     struct Raw {
@@ -91,6 +93,19 @@ template <> struct StateIO<float> {
             ri = ri < -32767 ? -32767 : (ri > 32767 ? 32767 : ri);
             at((AS_GLOBAL int16_t*)lo, i) = (int16_t)ri;
         }
+    }
+    // the value a later ld returns after st(x): the float, plus the carry
+    // split and clamped as st stores it
+    static __device__ __forceinline__ double rt(double x, bool carry) {
+        const float h = (float)x;
+        double v = (double)h;
+        if (carry) {
+            const double r = rint(ldexp(x - (double)h, 23 + 16 - carry_exp(h)));
+            int ri = (int)r;
+            ri = ri < -32767 ? -32767 : (ri > 32767 ? 32767 : ri);
+            v += (double)(int16_t)ri * carry_unit(h);
+        }
+        return v;
     }
 };
 
@@ -700,14 +715,27 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
 #undef CGO_FLUX
 }
 
-// co, goldstein.f90:2657-2777 (iconv = 0, ieos = 0): one thread per wet
-// column of a member, FP64 arithmetic on the new state in place.
+// co, goldstein.f90:2657-2777 (iconv = 0, ieos = 0): CO_LANES threads per wet
+// column of a member, FP64 arithmetic on the new state in place.  The
+// column's level arrays (segment map, thicknesses, densities and its T and S
+// as the state stores them) sit in LDS, one copy per column: its lanes walk
+// the reference's mixing loop in lockstep and write the same values, so they
+// reach the same decisions; each lane mixes and copies its own share of the
+// tracers in memory (l = lane, lane + CO_LANES, ...), each tracer's operations
+// in the reference's order.  K = the level arrays' length (the launch's
+// deepest maxk + 2); the workgroup holds blockDim.x / CO_LANES columns.
+constexpr int CO_LANES = 4;
+constexpr int CO_CH = 4;  // tracers whose loads one lane issues together
+__host__ __device__ constexpr size_t co_lds_bytes(int cols, int K) { return (size_t)cols * K * (4 * 8 + 4); }
 template <typename R>
-__global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list) {
+__global__ __launch_bounds__(256) void k_co(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list, int K) {
+    extern __shared__ double co_lds[];
     const BlockXYZ B = xcd_blk();
     const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int w = B.x * 64 + threadIdx.x;
+    const int cols = (int)blockDim.x / CO_LANES;
+    const int sub = threadIdx.x % CO_LANES, cl = threadIdx.x / CO_LANES;
+    const int w = B.x * cols + cl;
     if (w >= g.nwet) return;
     const int col = g.wetsorted[w];
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
@@ -716,16 +744,16 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
     const int par = *gp(M.parity);
     void* st = M.st[par ^ 1];
     int16_t* lo = M.lo[par ^ 1];
+    const bool carry = lo != nullptr;
     const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
     AS_GLOBAL double* rho = gp(M.rho);
     // members with surface fluxes keep the top level's T and S as tstepo
     // leaves them (tsval_ocn, ssval_ocn of step_goldstein 427-430): the
     // next call of the fluxes reads them, whatever the coupling does after
-    auto snap = [&]() {
+    auto snap = [&](double t, double s) {
         if (M.sw) {
-            const size_t top = (size_t)(maxk - 1) * plane + col;
-            gp(M.sw)[(size_t)SW_TSTAR * plane + col] = StateIO<R>::ld(st, lo, top);
-            gp(M.sw)[(size_t)SW_SSTAR * plane + col] = StateIO<R>::ld(st, lo, tstride + top);
+            gp(M.sw)[(size_t)SW_TSTAR * plane + col] = t;
+            gp(M.sw)[(size_t)SW_SSTAR * plane + col] = s;
         }
     };
     // A column whose density falls strictly from k1 up is left as it is: the
@@ -740,22 +768,33 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
             rb = ra;
         }
         if (stable) {
-            snap();
+            if (sub == 0) {
+                const size_t top = (size_t)(maxk - 1) * plane + col;
+                snap(StateIO<R>::ld(st, lo, top), StateIO<R>::ld(st, lo, tstride + top));
+            }
             return;
         }
     }
     const AS_GLOBAL double* dz = gp(g.mk) + MK_DZ * (maxk + 2);
     const double e0 = g.ec[0], e1 = g.ec[1], e2 = g.ec[2], e3 = g.ec[3];
-    int kk[MAXK_DEV + 2];
-    double dzm[MAXK_DEV + 2];
-    double r[MAXK_DEV + 2];
-    for (int m = 0; m < MAXK_DEV + 2; m++) kk[m] = 0;
-    for (int m = k1; m <= maxk; m++) r[m] = rho[(size_t)(m - 1) * plane + col];
+    double* r = co_lds + (size_t)cl * 4 * K;
+    double* dzm = r + K;
+    double* tp = dzm + K;
+    double* sp = tp + K;
+    int* kk = reinterpret_cast<int*>(co_lds + (size_t)cols * 4 * K) + (size_t)cl * K;
+    for (int m = 0; m <= maxk + 1; m++) kk[m] = 0;
+    for (int m = k1; m <= maxk; m++) {
+        r[m] = rho[(size_t)(m - 1) * plane + col];
+        tp[m] = StateIO<R>::ld(st, lo, (size_t)(m - 1) * plane + col);
+        sp[m] = StateIO<R>::ld(st, lo, tstride + (size_t)(m - 1) * plane + col);
+    }
     kk[k1 - 1] = 0;
     for (int m = k1; m <= maxk; m++) {
         kk[m] = m;
         dzm[m] = dz[m];
     }
+    // this lane's tracers: l = sub + CO_LANES * j, j < nmine
+    const int nmine = L > sub ? (L - sub + CO_LANES - 1) / CO_LANES : 0;
     int m = maxk, lastmix = 0;
     bool mixed = false;
     // every pass either moves m or merges two segments, so the reference loop
@@ -774,19 +813,45 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
             int n = m - 1;
             while (kk[n - 1] > 0 && r[kk[n]] >= r[kk[n - 1]]) n = n - 1;
             const int km = kk[m];
-            for (int l = 0; l < L; l++) {
-                const size_t base = (size_t)l * tstride + col;
-                double s = StateIO<R>::ld(st, lo, base + (size_t)(km - 1) * plane) * dzm[km];
-                for (int ni = 1; ni <= m - n; ni++)
-                    s = s + StateIO<R>::ld(st, lo, base + (size_t)(kk[m - ni] - 1) * plane) * dzm[kk[m - ni]];
-                double dsum = dzm[km];
-                for (int ni = 1; ni <= m - n; ni++) dsum = dsum + dzm[kk[m - ni]];
-                StateIO<R>::st(st, lo, base + (size_t)(km - 1) * plane, s / dsum);
+            double dsum = dzm[km];
+            for (int ni = 1; ni <= m - n; ni++) dsum = dsum + dzm[kk[m - ni]];
+            // the thickness-weighted mean of levels kk[n..m]: T and S privately
+            // on every lane, this lane's tracers in memory, CO_CH at a time
+            // (their loads issued together)
+            {
+                double at = tp[km] * dzm[km], as = sp[km] * dzm[km];
+                for (int ni = 1; ni <= m - n; ni++) {
+                    const int kq = kk[m - ni];
+                    at = at + tp[kq] * dzm[kq];
+                    as = as + sp[kq] * dzm[kq];
+                }
+                tp[km] = StateIO<R>::rt(at / dsum, carry);
+                sp[km] = StateIO<R>::rt(as / dsum, carry);
+            }
+            for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
+                double sm[CO_CH];
+#pragma unroll
+                for (int c = 0; c < CO_CH; c++)
+                    if (j0 + c < nmine)
+                        sm[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col +
+                                                           (size_t)(km - 1) * plane) * dzm[km];
+                for (int ni = 1; ni <= m - n; ni++) {
+                    const int kq = kk[m - ni];
+                    const double wq = dzm[kq];
+#pragma unroll
+                    for (int c = 0; c < CO_CH; c++)
+                        if (j0 + c < nmine)
+                            sm[c] = sm[c] + StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col +
+                                                                       (size_t)(kq - 1) * plane) * wq;
+                }
+#pragma unroll
+                for (int c = 0; c < CO_CH; c++)
+                    if (j0 + c < nmine)
+                        StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col + (size_t)(km - 1) * plane,
+                                       sm[c] / dsum);
             }
             for (int ni = 1; ni <= m - n; ni++) dzm[km] = dzm[km] + dzm[kk[m - ni]];
-            const double t = StateIO<R>::ld(st, lo, (size_t)(km - 1) * plane + col);
-            const double sl = StateIO<R>::ld(st, lo, tstride + (size_t)(km - 1) * plane + col);
-            r[km] = eos<double>(e0, e1, e2, e3, t, sl);
+            r[km] = eos<double>(e0, e1, e2, e3, tp[km], sp[km]);
             int ni = m - 1;
             while (kk[ni + 1] > 0) {
                 kk[ni] = kk[ni - m + n];
@@ -794,19 +859,25 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
             }
         }
     }
-    if (guard <= 0) atomicAdd(&M.diag[3], 1);
+    if (guard <= 0 && sub == 0) atomicAdd(&M.diag[3], 1);
     m = maxk - 1;
     double cost = 0.0;
     for (int n = maxk - 1; n >= k1; n--) {
         if (n > kk[m]) {
-            for (int l = 0; l < L; l++) {
-                const size_t base = (size_t)l * tstride + col;
-                StateIO<R>::st(st, lo, base + (size_t)(n - 1) * plane,
-                               StateIO<R>::ld(st, lo, base + (size_t)(kk[m + 1] - 1) * plane));
+            const int kf = kk[m + 1];
+            const size_t from = (size_t)(kf - 1) * plane + col, to = (size_t)(n - 1) * plane + col;
+            for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
+                double v[CO_CH];
+#pragma unroll
+                for (int c = 0; c < CO_CH; c++)
+                    if (j0 + c < nmine) v[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + from);
+#pragma unroll
+                for (int c = 0; c < CO_CH; c++)
+                    if (j0 + c < nmine) StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + to, v[c]);
             }
-            const double t = StateIO<R>::ld(st, lo, (size_t)(n - 1) * plane + col);
-            const double sl = StateIO<R>::ld(st, lo, tstride + (size_t)(n - 1) * plane + col);
-            r[n] = eos<double>(e0, e1, e2, e3, t, sl);
+            tp[n] = tp[kf];
+            sp[n] = sp[kf];
+            r[n] = eos<double>(e0, e1, e2, e3, tp[n], sp[n]);
             cost += 1.0;
         } else {
             m = m - 1;
@@ -817,27 +888,39 @@ __global__ __launch_bounds__(64) void k_co(const MemberDev* __restrict__ mems, c
         const double rSold = 1.0 / gp(M.scal)[SC_SMEAN];
         const size_t ncl = (size_t)g.ncell, slot = (size_t)gp(g.vpos)[w];
         AS_GLOBAL double* cs = gp(M.colsum);
-        double aS = 0.0;
-        for (int q = k1; q <= maxk; q++) {
-            const size_t c = (size_t)(q - 1) * plane + col;
-            aS += (StateIO<R>::ld(st, lo, tstride + c) + SALN0 + 0.0) * gp(g.V)[c];
+        if (sub == 0) {
+            double aS = 0.0;
+            for (int q = k1; q <= maxk; q++) aS += (sp[q] + SALN0 + 0.0) * gp(g.V)[(size_t)(q - 1) * plane + col];
+            cs[slot] = aS;
         }
-        cs[slot] = aS;
-        for (int l = 2; l < L; l++) {
-            double a = 0.0;
+        // this lane's tracers from the third on
+        const int jfirst = sub >= 2 ? 0 : 1;
+        for (int j0 = jfirst; j0 < nmine; j0 += CO_CH) {
+            double a[CO_CH];
+#pragma unroll
+            for (int t = 0; t < CO_CH; t++) a[t] = 0.0;
             for (int q = k1; q <= maxk; q++) {
                 const size_t c = (size_t)(q - 1) * plane + col;
-                const double lv = StateIO<R>::ld(st, lo, (size_t)l * tstride + c) * gp(M.ocn)[ncl + c] * rSold;
-                a += lv * gp(M.M)[c];
+                const double so = gp(M.ocn)[ncl + c], mm = gp(M.M)[c];
+#pragma unroll
+                for (int t = 0; t < CO_CH; t++)
+                    if (j0 + t < nmine) {
+                        const double lv = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + t)) * tstride + c) * so * rSold;
+                        a[t] += lv * mm;
+                    }
             }
-            cs[(size_t)l * g.nwet + slot] = a;
+#pragma unroll
+            for (int t = 0; t < CO_CH; t++)
+                if (j0 + t < nmine) cs[(size_t)(sub + CO_LANES * (j0 + t)) * g.nwet + slot] = a[t];
         }
     }
-    if (mixed || cost > 0.0) {
-        for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
-        if (cost > 0.0) gp(M.cost)[col] += cost;
+    if (sub == 0) {
+        if (mixed || cost > 0.0) {
+            for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
+            if (cost > 0.0) gp(M.cost)[col] += cost;
+        }
+        snap(tp[maxk], sp[maxk]);
     }
-    snap();
 }
 
 #ifdef CGO_DEFINE_FLIP
