@@ -30,7 +30,8 @@ void launch_coupling(const MemberDev*, const int32_t*, int nlist, int nwetmax, i
 void launch_inventory(const MemberDev*, int32_t member, int L, double* out, const double* w, int prec, hipStream_t);
 void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
-void launch_mom(const MemberDev*, const int32_t*, int nlist, int maxN, int maxcol, int maxn, hipStream_t);
+void launch_mom(const MemberDev*, const int32_t*, int nlist, const int32_t* band, int nband, const int32_t* dense,
+                const int* groups, int ngroups, int maxN, int maxcol, int maxn, hipStream_t);
 size_t mom_solve_lds(int N, int n);
 void launch_sfx(const MemberDev*, const int32_t*, int nlist, int maxcol, int slot_only, hipStream_t);
 size_t sfx_lds(int ncol);
@@ -66,6 +67,7 @@ struct GridH {
     void* mbuf = nullptr;
     const MomGridDev* d_mg = nullptr;
     uint64_t mom_fp = 0;
+    bool ainv_on = false;         // the dense inverse is kept (FP32 members solve by it)
     // surface fluxes (cgo_grid_set_surflux): the device tables, the settings
     // the host side needs (wind speed, initial gases) and a fingerprint
     void* sbuf = nullptr;
@@ -864,6 +866,14 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
                  p_erisl = add(M->erisl.data(), M->erisl.size() * 8), p_npi = add(M->npi.data(), (size_t)nis * 4),
                  p_lp = add(M->lpisl.data(), M->lpisl.size() * 4), p_ip = add(M->ipisl.data(), M->ipisl.size() * 4),
                  p_jp = add(M->jpisl.data(), M->jpisl.size() * 4);
+    // the psi equation's inverse for the FP32 form's dense solve, where it is
+    // small enough to keep (N <= MOM_DENSE_MAXN: at most 128 MiB)
+    std::vector<double> ainv;
+    size_t p_ainv = SIZE_MAX;
+    if (N <= MOM_DENSE_MAXN) {
+        mom_inverse(*M, ainv);
+        p_ainv = add(ainv.data(), ainv.size() * 8);
+    }
     hipSetDevice(e->device);
     void* buf = nullptr;
     if (hipMalloc(&buf, off) != hipSuccess) {
@@ -903,6 +913,7 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
     h.lpisl = (const int32_t*)at_(p_lp);
     h.ipisl = (const int32_t*)at_(p_ip);
     h.jpisl = (const int32_t*)at_(p_jp);
+    h.ainv = p_ainv == SIZE_MAX ? nullptr : (const double*)at_(p_ainv);
     int rc = CGO_OK;
     if (hipMemcpy(b + o_struct, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = CGO_E_HIP;
     for (const Piece& pc : pieces)
@@ -933,6 +944,7 @@ int cgo_grid_set_momentum(cgo_engine* e, int32_t grid_id, const cgo_momentum_des
     g->mbuf = buf;
     g->d_mg = reinterpret_cast<const MomGridDev*>(b + o_struct);
     g->mom_fp = fp;
+    g->ainv_on = p_ainv != SIZE_MAX;
     return CGO_OK;
 }
 
@@ -1962,8 +1974,10 @@ int cgo_set_windspeed(cgo_engine* e, int32_t id, const double* wspeed) {
 // ---------------------------------------------------------------- stepping
 struct Lists {
     // [p64 list][p32 list][bgc p64][bgc p32][all][velocity solve][surface fluxes]
+    // [velocity solve through the band][velocity solve by the dense inverse, grouped by grid]
     std::vector<int32_t> v;
-    int n[7] = {0, 0, 0, 0, 0, 0, 0}, off[7] = {0, 0, 0, 0, 0, 0, 0};
+    int n[9] = {}, off[9] = {};
+    std::vector<int> momGroups;             // per grid of the dense solve: offset in list 8, count, N
     int momN = 0, momCol = 0, momBand = 0;  // most psi points, columns and maxi of a member with a velocity solve
     int sfxCol = 0;                         // most columns of a member with surface fluxes
     int maxcol[2] = {0, 0}, maxL[2] = {0, 0}, maxwet[2] = {0, 0}, maxwet_bgc[2] = {0, 0}, maxi[2] = {0, 0};
@@ -1973,7 +1987,9 @@ struct Lists {
 };
 
 static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
-    std::vector<int32_t> g[7];
+    std::vector<int32_t> g[9];
+    std::vector<int> dgrid;                 // grids of the dense solve, in order of first appearance
+    std::vector<std::vector<int32_t>> dmem;
     for (int i = 0; i < n; i++) {
         MemberH* m = member(e, ids[i]);
         if (!m) return fail(e, CGO_E_ARG, "step: bad member %d", ids[i]);
@@ -1986,6 +2002,18 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             ls.momN = std::max(ls.momN, e->grids[m->grid]->mom->N);
             ls.momCol = std::max(ls.momCol, m->dev.g.ncol);
             ls.momBand = std::max(ls.momBand, m->dev.g.maxi);
+            // FP32 members solve by the dense inverse where their grid keeps one
+            if (m->prec != CGO_FP64 && e->grids[m->grid]->ainv_on) {
+                size_t q = 0;
+                while (q < dgrid.size() && dgrid[q] != m->grid) q++;
+                if (q == dgrid.size()) {
+                    dgrid.push_back(m->grid);
+                    dmem.emplace_back();
+                }
+                dmem[q].push_back(ids[i]);
+            } else {
+                g[7].push_back(ids[i]);
+            }
         }
         if (m->sbuf) {
             if (e->grids[m->grid]->sfx_atchem && !(m->bgc && m->dev.bp.airsea))
@@ -2010,7 +2038,13 @@ static int make_lists(cgo_engine* e, const int32_t* ids, int32_t n, Lists& ls) {
             if (m->dev.bp.airsea) ls.ncol_atm = std::max(ls.ncol_atm, m->dev.g.ncol);
         }
     }
-    for (int q = 0; q < 7; q++) {
+    for (size_t q = 0; q < dgrid.size(); q++) {
+        ls.momGroups.push_back((int)g[8].size());
+        ls.momGroups.push_back((int)dmem[q].size());
+        ls.momGroups.push_back(e->grids[dgrid[q]]->mom->N);
+        g[8].insert(g[8].end(), dmem[q].begin(), dmem[q].end());
+    }
+    for (int q = 0; q < 9; q++) {
         ls.off[q] = (int)ls.v.size();
         ls.n[q] = (int)g[q].size();
         ls.v.insert(ls.v.end(), g[q].begin(), g[q].end());
@@ -2043,7 +2077,9 @@ static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lis
 // the velocity solve of the members that have one (step_goldstein 198-233),
 // which precedes their tstepo
 static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
-    if (ls.n[5]) launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], ls.momN, ls.momCol, ls.momBand, s);
+    if (ls.n[5])
+        launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], d_lists + ls.off[7], ls.n[7], d_lists + ls.off[8],
+                   ls.momGroups.data(), (int)ls.momGroups.size() / 3, ls.momN, ls.momCol, ls.momBand, s);
 }
 
 // the surface fluxes of the members that have them (surflux, called by

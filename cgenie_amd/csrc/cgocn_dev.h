@@ -109,6 +109,8 @@ struct BgcParams {
 //   ubisl isles x (2, 0:maxi+1, 0:maxj), psisl isles x (0:maxi, 0:maxj)
 //   erisl (isles, isles+1) column-major, inverted in place (matinv_gold)
 //   paths (mpi, isles): lpisl, ipisl, jpisl; npi (isles)
+//   ainv  [N][N] row-major: the psi equation's inverse (the FP32 form's
+//         dense solve), or null
 enum { JT_S = 0, JT_SV, JT_C, JT_CV, JT_RC, JT_RCV, JT_RDS, JT_RDSV, JT_RDS2, JT_DS, JT_DSV, JT_N };
 struct MomGridDev {
     int maxi, maxj, maxk, n, N, isles, mpi;
@@ -133,12 +135,15 @@ struct MomGridDev {
     const int32_t* lpisl;
     const int32_t* ipisl;
     const int32_t* jpisl;
+    const double* ainv;
 };
 constexpr int MOM_MAXISL = 16;   // islands
+constexpr int MOM_DENSE_MAXN = 4096;  // psi points up to which a grid keeps the dense inverse
+constexpr int MOM_SPLITK = 8;         // slices of the dense solve's sums
 
 // one member's arrays of the velocity solve, offsets in doubles from its base
 struct MomMemberLayout {
-    size_t gbw, tau, dztau, dztav, u1, ub, psi, bp, sbp, csum, psibc, u64, total;
+    size_t gbw, tau, dztau, dztav, u1, ub, psi, bp, sbp, csum, psibc, u64, gst, total;
 };
 __host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int mk, bool own_u64) {
     MomMemberLayout L{};
@@ -161,6 +166,7 @@ __host__ __device__ inline MomMemberLayout mom_member_layout(int mi, int mj, int
     L.csum = take(2 * ncol);                               // velc's depth sums of the shear, [2][maxj][maxi]
     L.psibc = take(MOM_MAXISL);                            // the islands' streamfunction values of the step
     L.u64 = own_u64 ? take(3 * ncell) : 0;                 // FP64 u, v, w of an FP32 member
+    L.gst = own_u64 ? take((1 + MOM_SPLITK) * N) : 0;      // an FP32 member's psi forcing, then psi's partial sums (the dense solve)
     L.total = o;
     return L;
 }

@@ -88,68 +88,85 @@ __global__ __launch_bounds__(MOM_NT) void k_mom_cols_a(const MemberDev* __restri
     if (kb > mk) return;
     const MomGridDev& G = V.G;
     // jbar (3225-3255): bottom pressure from the bottom cell up (zero there,
-    // never written), the periodic copy, and its integral above mk
+    // never written), the periodic copy, and its integral above mk, summed as
+    // the levels are formed (mk(i,j) >= k1(i,j), so every term is one of them)
+    const int mkij = V.mkk(i, j);
     double b = V.bp(i, j, kb);
+    double sb = 0;
     for (int k = kb + 1; k <= mk; k++) {
         b = b - (V.rho(i, j, k) + V.rho(i, j, k - 1)) * G.dza[k - 1] * 0.5;
         V.bp(i, j, k) = b;
         if (i == 1 && kb < mk) V.bp(mi + 1, j, k) = b;
+        if (mkij > 0 && k >= mkij + 1) sb = sb + b * G.dz[k];
     }
-    if (V.mkk(i, j) > 0) {
-        double s = 0;
-        for (int k = V.mkk(i, j) + 1; k <= mk; k++) s = s + V.bp(i, j, k) * G.dz[k];
-        V.sbp(i, j) = s;
-    }
+    if (mkij > 0) V.sbp(i, j) = sb;
     if (i == 1) V.sbp(mi + 1, j) = V.sbp(1, j);
     // velc (3574-3640): the shear integrated up from zero at the bottom
     double sum0 = 0, sum1 = 0, dp0 = 0, dp1 = 0, up0 = 0, up1 = 0;
     const double sj = V.jt(JT_S, j), svj = V.jt(JT_SV, j), cj = V.jt(JT_C, j), cvj = V.jt(JT_CV, j);
     const double rdphi = G.rdphi, rtvij = G.rtv[V.col(i, j)], rtv3ij = G.rtv3[V.col(i, j)];
     const double d1 = V.drag(1, i, j), d2 = V.drag(2, i, j);
-    auto K1 = [&](int a, int b2) { return V.K1(a, b2); };
-    auto rho = [&](int a, int b2, int k) { return V.rho(a, b2, k); };
+    const double jrc = V.jt(JT_RC, j), jrds2 = V.jt(JT_RDS2, j), jrdsvm = V.jt(JT_RDSV, j - 1), jrdsv = V.jt(JT_RDSV, j),
+                 jrcv = V.jt(JT_RCV, j);
+    // the neighbours' bottom levels, once
+    const int kE = V.K1(i + 1, j), kN = V.K1(i, j + 1), kS = V.K1(i, j - 1), kNE = V.K1(i + 1, j + 1),
+              kSE = V.K1(i + 1, j - 1), kW = V.K1(i - 1, j), kNW = V.K1(i - 1, j + 1);
+    // the eight densities a level reads: (i,j) (i+1,j) (i+1,j+1) (i+1,j-1)
+    // (i,j+1) (i,j-1) (i-1,j+1) (i-1,j), the next level's fetched while one
+    // level is formed; rows are clamped to the grid for the fetch, and a
+    // value is used only where its row is wet, so the clamp never shows
+    const int jp = min(j + 1, V.mj), jm = max(j - 1, 1);
+    auto fetch = [&](int k, double* r) {
+        r[0] = V.rho(i, j, k);
+        r[1] = V.rho(i + 1, j, k);
+        r[2] = V.rho(i + 1, jp, k);
+        r[3] = V.rho(i + 1, jm, k);
+        r[4] = V.rho(i, jp, k);
+        r[5] = V.rho(i, jm, k);
+        r[6] = V.rho(i - 1, jp, k);
+        r[7] = V.rho(i - 1, j, k);
+    };
+    double cur[8], nxt[8];
+    fetch(kb, cur);
     for (int k = kb; k <= mk; k++) {
+        if (k < mk) fetch(k + 1, nxt);
+        const double r_c = cur[0], r_e = cur[1], r_ne = cur[2], r_se = cur[3], r_n = cur[4], r_s = cur[5],
+                     r_nw = cur[6], r_w = cur[7];
         double tv1, tv2, tv4, tv5;
-        if (K1(i + 1, j) > k) {
+        if (kE > k) {
             tv1 = 0;
             tv2 = 0;
         } else {
-            tv2 = -(rho(i + 1, j, k) - rho(i, j, k)) * rdphi * V.jt(JT_RC, j);
-            if (max(max(K1(i, j - 1), K1(i, j + 1)), max(K1(i + 1, j - 1), K1(i + 1, j + 1))) <= k)
-                tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j - 1, k) + rho(i, j + 1, k) - rho(i, j - 1, k)) *
-                      V.jt(JT_RDS2, j) * 0.25;
-            else if (max(K1(i, j - 1), K1(i + 1, j - 1)) <= k)
-                tv1 = -cj * (rho(i + 1, j, k) - rho(i + 1, j - 1, k) + rho(i, j, k) - rho(i, j - 1, k)) *
-                      V.jt(JT_RDSV, j - 1) * 0.5;
-            else if (max(K1(i, j + 1), K1(i + 1, j + 1)) <= k)
-                tv1 = -cj * (rho(i + 1, j + 1, k) - rho(i + 1, j, k) + rho(i, j + 1, k) - rho(i, j, k)) *
-                      V.jt(JT_RDSV, j) * 0.5;
+            tv2 = -(r_e - r_c) * rdphi * jrc;
+            if (max(max(kS, kN), max(kSE, kNE)) <= k)
+                tv1 = -cj * (r_ne - r_se + r_n - r_s) * jrds2 * 0.25;
+            else if (max(kS, kSE) <= k)
+                tv1 = -cj * (r_e - r_se + r_c - r_s) * jrdsvm * 0.5;
+            else if (max(kN, kNE) <= k)
+                tv1 = -cj * (r_ne - r_e + r_n - r_c) * jrdsv * 0.5;
             else
                 tv1 = 0;
         }
-        if (K1(i, j + 1) > k) {
+        if (kN > k) {
             tv4 = 0;
             tv5 = 0;
         } else {
-            tv4 = -cvj * (rho(i, j + 1, k) - rho(i, j, k)) * V.jt(JT_RDSV, j);
-            if (max(max(K1(i - 1, j), K1(i - 1, j + 1)), max(K1(i + 1, j), K1(i + 1, j + 1))) <= k)
-                tv5 = -(rho(i + 1, j + 1, k) - rho(i - 1, j + 1, k) + rho(i + 1, j, k) - rho(i - 1, j, k)) * rdphi *
-                      0.25 * V.jt(JT_RCV, j);
-            else if (max(K1(i - 1, j), K1(i - 1, j + 1)) <= k)
-                tv5 = -(rho(i, j + 1, k) - rho(i - 1, j + 1, k) + rho(i, j, k) - rho(i - 1, j, k)) * rdphi * 0.5 *
-                      V.jt(JT_RCV, j);
-            else if (max(K1(i + 1, j), K1(i + 1, j + 1)) <= k)
-                tv5 = -(rho(i + 1, j + 1, k) - rho(i, j + 1, k) + rho(i + 1, j, k) - rho(i, j, k)) * rdphi * 0.5 *
-                      V.jt(JT_RCV, j);
+            tv4 = -cvj * (r_n - r_c) * jrdsv;
+            if (max(max(kW, kNW), max(kE, kNE)) <= k)
+                tv5 = -(r_ne - r_nw + r_e - r_w) * rdphi * 0.25 * jrcv;
+            else if (max(kW, kNW) <= k)
+                tv5 = -(r_n - r_nw + r_c - r_w) * rdphi * 0.5 * jrcv;
+            else if (max(kE, kNE) <= k)
+                tv5 = -(r_ne - r_n + r_e - r_c) * rdphi * 0.5 * jrcv;
             else
                 tv5 = 0;
         }
         if (k == mk) {
-            if (K1(i + 1, j) <= k) {
+            if (kE <= k) {
                 tv1 = tv1 - V.dztau(2, i, j);
                 tv2 = tv2 - V.dztau(1, i, j);
             }
-            if (K1(i, j + 1) <= k) {
+            if (kN <= k) {
                 tv4 = tv4 - V.dztav(2, i, j);
                 tv5 = tv5 - V.dztav(1, i, j);
             }
@@ -172,6 +189,8 @@ __global__ __launch_bounds__(MOM_NT) void k_mom_cols_a(const MemberDev* __restri
         up1 = x1;
         dp0 = dz0;
         dp1 = dz1;
+#pragma unroll
+        for (int q = 0; q < 8; q++) cur[q] = nxt[q];
     }
     V.csum(1, i, j) = sum0;
     V.csum(2, i, j) = sum1;
@@ -398,8 +417,11 @@ __device__ __forceinline__ void band_solve(double* g, double* pbuf, double* ring
 }
 
 // ------------------------------------------------------------ one block per member
+// mode 0: the whole solve; the FP32 form's dense solve splits it round
+// k_mom_gemm: mode 1 forms the psi forcing and stages it, mode 2 takes psi
+// from the staging and goes on from there
 __global__ __launch_bounds__(MOM_NT) void k_mom_solve(const MemberDev* __restrict__ mems,
-                                                      const int32_t* __restrict__ list) {
+                                                      const int32_t* __restrict__ list, int mode) {
     extern __shared__ double lds[];
     const MomView V(mems[list[blockIdx.x]]);
     const MomGridDev& G = V.G;
@@ -413,7 +435,7 @@ __global__ __launch_bounds__(MOM_NT) void k_mom_solve(const MemberDev* __restric
     // ---- the psi forcing: the wind part (wind) plus the torque of the bottom
     // pressure where the four columns round a psi point differ in depth
     // (jbar 3256-3313)
-    for (int l = tid + 1; l <= N; l += MOM_NT) {
+    for (int l = tid + 1; l <= N && mode != 2; l += MOM_NT) {
         const int i = (l - 1) % n + 1, j = (l - 1) / n;
         double g = V.W[V.ML.gbw + l - 1];
         if (j >= 1 && j <= mj - 1 && G.getj[V.col(i, j)]) {
@@ -439,10 +461,21 @@ __global__ __launch_bounds__(MOM_NT) void k_mom_solve(const MemberDev* __restric
                         rdphi * V.jt(JT_RDSV, j);
         }
         gbs[l - 1] = g;
+        if (mode == 1) V.W[V.ML.gst + l - 1] = g;
     }
+    if (mode == 1) return;
     __syncthreads();
 
-    band_solve(gbs, pbuf, ring, G, tid);
+    if (mode == 0) {
+        band_solve(gbs, pbuf, ring, G, tid);
+    } else {
+        for (int l = tid; l < N; l += MOM_NT) {
+            double p = V.W[V.ML.gst + N + l];
+            for (int z = 1; z < MOM_SPLITK; z++) p = p + V.W[V.ML.gst + (size_t)N * (1 + z) + l];
+            gbs[l] = p;
+        }
+        __syncthreads();
+    }
 
     // ---- psi and the barotropic velocities from the solution (3524-3565),
     // each boundary copy written by the thread that forms its source
@@ -653,17 +686,111 @@ __global__ __launch_bounds__(MOM_NT) void k_mom_cols_c(const MemberDev* __restri
 size_t mom_solve_lds(int N, int n) {
     return (size_t)(N + MOM_MAXISL + 128 + (size_t)MOM_RING * MOM_CH * (n + 3)) * sizeof(double);
 }
+// ------------------------------------------------------------ the dense solve
+// psi = A^-1 g for the FP32 members of one grid at once: C(i, c) =
+// sum_k A(i, k) B(k, c), B's column c the staged forcing of the group's
+// member c and C's column c its psi.  64 x 64 tiles of C per workgroup, 4 x 4
+// per thread, k in steps of 16 through LDS with the next step's loads issued
+// before the current one's products; the k range is cut into MOM_SPLITK
+// slices (blockIdx.z), each writing its partial sums, which the solve block
+// adds in slice order.
+constexpr int GT = 64, GK = 16;
+__global__ __launch_bounds__(256) void k_mom_gemm(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list,
+                                                   int nm) {
+    __shared__ double As[GK][GT + 1];
+    __shared__ double Bs[GK][GT + 1];
+    const MomGridDev& G = *mems[list[0]].mg;
+    const int N = G.N;
+    const MomMemberLayout L = mom_member_layout(G.maxi, G.maxj, G.maxk, true);
+    const double* __restrict__ A = G.ainv;
+    const int r0 = blockIdx.x * GT, c0 = blockIdx.y * GT;
+    const int slice = (N + MOM_SPLITK - 1) / MOM_SPLITK;
+    const int kb = blockIdx.z * slice, ke = min(N, kb + slice);
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    // each thread stages 4 values of A and 4 of B per step: rows / members
+    // rr = ty + 16 q and k offset tx, so its members are fixed
+    const double* bcol[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = c0 + ty + 16 * q;
+        bcol[q] = c < nm ? mems[list[c]].mw + L.gst : nullptr;
+    }
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    double na[4], nb[4];
+    auto fetch = [&](int k0) {
+        const int k = k0 + tx;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = r0 + ty + 16 * q;
+            na[q] = (r < N && k < ke) ? A[(size_t)r * N + k] : 0.0;
+            nb[q] = (bcol[q] && k < ke) ? bcol[q][k] : 0.0;
+        }
+    };
+    fetch(kb);
+    for (int k0 = kb; k0 < ke; k0 += GK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            As[tx][ty + 16 * q] = na[q];
+            Bs[tx][ty + 16 * q] = nb[q];
+        }
+        __syncthreads();
+        if (k0 + GK < ke) fetch(k0 + GK);
+#pragma unroll 4
+        for (int kq = 0; kq < GK; kq++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                a[q] = As[kq][ty * 4 + q];
+                b[q] = Bs[kq][tx * 4 + q];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
+            if (r < N && c < nm) mems[list[c]].mw[L.gst + (size_t)N * (1 + blockIdx.z) + r] = acc[x][y];
+        }
+}
+
 // island path terms the solve block's ring holds
 size_t mom_ring_terms(int n) { return (size_t)MOM_RING * MOM_CH * (n + 3); }
 
-void launch_mom(const MemberDev* mems, const int32_t* list, int nlist, int maxN, int maxcol, int maxn, hipStream_t s) {
+// list: every member with a velocity solve (the column kernels); band: those
+// solved through the band (FP64 members, FP32 members of grids without the
+// inverse); dense: the FP32 members of grids with the inverse, grouped by
+// grid, groups[g] = {offset into dense, count, N}
+void launch_mom(const MemberDev* mems, const int32_t* list, int nlist, const int32_t* band, int nband,
+                const int32_t* dense, const int* groups, int ngroups, int maxN, int maxcol, int maxn, hipStream_t s) {
     if (nlist <= 0) return;
     const int cb = (maxcol + MOM_NT - 1) / MOM_NT;
     // the stored ub and psi have at most (maxi+2) maxj x 2 and (maxi+1)(maxj+1)
     // entries: fewer than 4 x the columns + 256
     const int cb2 = (4 * maxcol + 256 + MOM_NT - 1) / MOM_NT;
+    const size_t lds = mom_solve_lds(maxN, maxn);
     hipLaunchKernelGGL(k_mom_cols_a, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
-    hipLaunchKernelGGL(k_mom_solve, dim3(nlist), dim3(MOM_NT), mom_solve_lds(maxN, maxn), s, mems, list);
+    if (nband > 0) hipLaunchKernelGGL(k_mom_solve, dim3(nband), dim3(MOM_NT), lds, s, mems, band, 0);
+    int ndense = 0;
+    for (int q = 0; q < ngroups; q++) ndense += groups[3 * q + 1];
+    if (ndense > 0) {
+        hipLaunchKernelGGL(k_mom_solve, dim3(ndense), dim3(MOM_NT), lds, s, mems, dense, 1);
+        for (int q = 0; q < ngroups; q++) {
+            const int off = groups[3 * q], nm = groups[3 * q + 1], N = groups[3 * q + 2];
+            hipLaunchKernelGGL(k_mom_gemm, dim3((N + GT - 1) / GT, (nm + GT - 1) / GT, MOM_SPLITK), dim3(256), 0, s,
+                               mems, dense + off, nm);
+        }
+        hipLaunchKernelGGL(k_mom_solve, dim3(ndense), dim3(MOM_NT), lds, s, mems, dense, 2);
+    }
     hipLaunchKernelGGL(k_mom_cols_b, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
     hipLaunchKernelGGL(k_mom_cols_c, dim3(cb2, nlist), dim3(MOM_NT), 0, s, mems, list);
 }

@@ -129,6 +129,34 @@ inline void mom_band(MomSetup& M) {
 // ubarsolv on a right-hand side gb (1-based, N entries): the eliminated band's
 // forward and back substitution, psi, and the barotropic velocities with
 // their boundary copies
+// The psi equation's inverse, column c being the band solve of the unit
+// vector e_c through the same factors (the forward elimination leaves the
+// rows above c at zero), row-major: ainv[i N + c].  For the FP32 form, whose
+// members multiply their forcing by it instead of substituting through the
+// band one row after another.
+inline void mom_inverse(MomSetup& M, std::vector<double>& ainv) {
+    const int n = M.n, N = M.N;
+    ainv.assign((size_t)N * N, 0.0);
+    std::vector<double> g(N + 1);
+    for (int c = 1; c <= N; c++) {
+        std::fill(g.begin(), g.end(), 0.0);
+        g[c] = 1.0;
+        for (int i = c; i <= N - 1; i++) {
+            if (g[i] == 0.0) continue;
+            const int last = std::min(i + n + 1, N);
+            for (int j = i + 1; j <= last; j++) g[j] = g[j] - M.RATM(j, j - i) * g[i];
+        }
+        g[N] = g[N] / M.GAP(N, n + 2);
+        for (int i = N - 1; i >= 1; i--) {
+            const int km = std::min(n + 1, N - i);
+            double v = g[i];
+            for (int k = 1; k <= km; k++) v = v - M.GAP(i, n + 2 + k) * g[i + k];
+            g[i] = v / M.GAP(i, n + 2);
+        }
+        for (int i = 1; i <= N; i++) ainv[(size_t)(i - 1) * N + (c - 1)] = g[i];
+    }
+}
+
 inline void mom_solve(MomSetup& M, std::vector<double>& gb, double* ub, double* psi) {
     const int n = M.n, N = M.N, mi = M.mi, mj = M.mj;
     auto G = [&](int k) -> double& { return gb[k - 1]; };
