@@ -12,17 +12,17 @@ void launch_flux64(const MemberDev* t, const int32_t* l, int n, const FluxShape&
 }
 
 void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int maxkmax, int prec, hipStream_t s) {
-    // 256 threads per workgroup, fewer where the deepest grid's level arrays
-    // would not fit in 64 KiB of LDS
+    // 256 wet columns per workgroup of 256 threads (64 convecting at a
+    // time), fewer where the level arrays would not fit in 64 KiB of LDS
     const int K = maxkmax + 2;
-    int cols = 256 / CO_LANES;
-    while (cols > 8 && co_lds_bytes(cols, K) > 65536) cols /= 2;
-    const dim3 grid((nwetmax + cols - 1) / cols, n);
-    const size_t lds = co_lds_bytes(cols, K);
+    int nt = 256;
+    while (nt > 32 && co_lds_bytes(nt, nt, K) > 65536) nt /= 2;
+    const dim3 grid((nwetmax + nt - 1) / nt, n);
+    const size_t lds = co_lds_bytes(nt, nt, K);
     if (prec == 0)
-        hipLaunchKernelGGL(k_co<double>, grid, dim3(cols * CO_LANES), lds, s, t, l, K);
+        hipLaunchKernelGGL(k_co<double>, grid, dim3(nt), lds, s, t, l, K, nt);
     else
-        hipLaunchKernelGGL(k_co<float>, grid, dim3(cols * CO_LANES), lds, s, t, l, K);
+        hipLaunchKernelGGL(k_co<float>, grid, dim3(nt), lds, s, t, l, K, nt);
 }
 
 void launch_flip(const MemberDev* t, const int32_t* l, int n, hipStream_t s) {

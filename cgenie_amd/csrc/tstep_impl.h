@@ -715,211 +715,241 @@ void launch_flux_t(const MemberDev* t, const int32_t* l, int n, const FluxShape&
 #undef CGO_FLUX
 }
 
-// co, goldstein.f90:2657-2777 (iconv = 0, ieos = 0): CO_LANES threads per wet
-// column of a member, FP64 arithmetic on the new state in place.  The
+// co, goldstein.f90:2657-2777 (iconv = 0, ieos = 0): FP64 arithmetic on the
+// new state in place, in two passes over a workgroup of CB <= blockDim.x wet
+// columns.  Pass 1, a thread per column, all its density loads issued
+// together: a column whose density falls strictly from k1 up -- most
+// columns; the reference loop leaves them as they are -- only has its top T
+// and S noted; the others are listed in LDS.  Pass 2: each group of CO_LANES
+// lanes takes the next listed column whenever it has finished one.  The
 // column's level arrays (segment map, thicknesses, densities and its T and S
 // as the state stores them) sit in LDS, one copy per column: its lanes walk
 // the reference's mixing loop in lockstep and write the same values, so they
 // reach the same decisions; each lane mixes and copies its own share of the
-// tracers in memory (l = lane, lane + CO_LANES, ...), each tracer's operations
-// in the reference's order.  K = the level arrays' length (the launch's
-// deepest maxk + 2); the workgroup holds blockDim.x / CO_LANES columns.
+// tracers in memory (l = lane, lane + CO_LANES, ...), each tracer's
+// operations in the reference's order.  K = the level arrays' length (the
+// launch's deepest maxk + 2).  Fewer columns per workgroup (CB) make more
+// workgroups, which hold more convecting columns per CU at once where most
+// convect, and more that only run pass 1 where few do.
 constexpr int CO_LANES = 4;
 constexpr int CO_CH = 4;  // tracers whose loads one lane issues together
-__host__ __device__ constexpr size_t co_lds_bytes(int cols, int K) { return (size_t)cols * K * (4 * 8 + 4); }
+__host__ __device__ constexpr size_t co_lds_bytes(int nt, int cb, int K) {
+    return (size_t)(nt / CO_LANES) * K * (4 * 8 + 4) + (size_t)cb * 4;
+}
 template <typename R>
-__global__ __launch_bounds__(256) void k_co(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list, int K) {
+__global__ __launch_bounds__(256) void k_co(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list, int K, int CB) {
     extern __shared__ double co_lds[];
+    __shared__ int s_n, s_next;
     const BlockXYZ B = xcd_blk();
     const MemberDev& M = mems[list[B.y]];
     const GridDev& g = M.g;
-    const int cols = (int)blockDim.x / CO_LANES;
-    const int sub = threadIdx.x % CO_LANES, cl = threadIdx.x / CO_LANES;
-    const int w = B.x * cols + cl;
-    if (w >= g.nwet) return;
-    const int col = g.wetsorted[w];
+    const int nt = (int)blockDim.x, P = nt / CO_LANES;
     const int maxi = g.maxi, maxk = g.maxk, L = M.L;
-    const int I = col % maxi + 1, J = col / maxi + 1;
-    const int k1 = k1at(g, I, J);
     const int par = *gp(M.parity);
     void* st = M.st[par ^ 1];
     int16_t* lo = M.lo[par ^ 1];
     const bool carry = lo != nullptr;
     const size_t plane = (size_t)g.ncol, tstride = (size_t)maxk * plane;
     AS_GLOBAL double* rho = gp(M.rho);
+    int* wl = reinterpret_cast<int*>(co_lds + (size_t)P * 4 * K) + (size_t)P * K;  // the list of pass 1
     // members with surface fluxes keep the top level's T and S as tstepo
     // leaves them (tsval_ocn, ssval_ocn of step_goldstein 427-430): the
     // next call of the fluxes reads them, whatever the coupling does after
-    auto snap = [&](double t, double s) {
+    auto snap = [&](int col, double t, double s) {
         if (M.sw) {
             gp(M.sw)[(size_t)SW_TSTAR * plane + col] = t;
             gp(M.sw)[(size_t)SW_SSTAR * plane + col] = s;
         }
     };
-    // A column whose density falls strictly from k1 up is left as it is: the
-    // loop below walks m down to k1 without mixing and the fill-in finds every
-    // level in place.  Most columns are; they skip the per-level arrays.
+    if (threadIdx.x == 0) s_n = 0, s_next = 0;
+    __syncthreads();
+    // ---- pass 1: the loop below walks a stable column's m down to k1
+    // without mixing and the fill-in finds every level in place
     {
-        bool stable = true;
-        double rb = rho[(size_t)(maxk - 1) * plane + col];
-        for (int q = maxk - 1; q >= k1 && stable; q--) {
-            const double ra = rho[(size_t)(q - 1) * plane + col];
-            stable = rb < ra;
-            rb = ra;
-        }
-        if (stable) {
-            if (sub == 0) {
-                const size_t top = (size_t)(maxk - 1) * plane + col;
-                snap(StateIO<R>::ld(st, lo, top), StateIO<R>::ld(st, lo, tstride + top));
+        const int w = B.x * CB + (int)threadIdx.x;
+        if ((int)threadIdx.x < CB && w < g.nwet) {
+            const int col = g.wetsorted[w];
+            const int k1 = k1at(g, col % maxi + 1, col / maxi + 1);
+            bool stable = true;
+            double rb = rho[(size_t)(maxk - 1) * plane + col];
+            for (int q = maxk - 1; q >= k1; q--) {
+                const double ra = rho[(size_t)(q - 1) * plane + col];
+                stable = stable & (rb < ra);
+                rb = ra;
             }
-            return;
+            if (stable) {
+                const size_t top = (size_t)(maxk - 1) * plane + col;
+                snap(col, StateIO<R>::ld(st, lo, top), StateIO<R>::ld(st, lo, tstride + top));
+            } else {
+                wl[atomicAdd(&s_n, 1)] = w;
+            }
         }
     }
+    __syncthreads();
+    const int n = s_n;
+    if (n == 0) return;  // uniform for the block
+    // ---- pass 2
+    const int sub = (int)threadIdx.x % CO_LANES, cl = (int)threadIdx.x / CO_LANES;
     const AS_GLOBAL double* dz = gp(g.mk) + MK_DZ * (maxk + 2);
     const double e0 = g.ec[0], e1 = g.ec[1], e2 = g.ec[2], e3 = g.ec[3];
     double* r = co_lds + (size_t)cl * 4 * K;
     double* dzm = r + K;
     double* tp = dzm + K;
     double* sp = tp + K;
-    int* kk = reinterpret_cast<int*>(co_lds + (size_t)cols * 4 * K) + (size_t)cl * K;
-    for (int m = 0; m <= maxk + 1; m++) kk[m] = 0;
-    for (int m = k1; m <= maxk; m++) {
-        r[m] = rho[(size_t)(m - 1) * plane + col];
-        tp[m] = StateIO<R>::ld(st, lo, (size_t)(m - 1) * plane + col);
-        sp[m] = StateIO<R>::ld(st, lo, tstride + (size_t)(m - 1) * plane + col);
-    }
-    kk[k1 - 1] = 0;
-    for (int m = k1; m <= maxk; m++) {
-        kk[m] = m;
-        dzm[m] = dz[m];
-    }
+    int* kk = reinterpret_cast<int*>(co_lds + (size_t)P * 4 * K) + (size_t)cl * K;
     // this lane's tracers: l = sub + CO_LANES * j, j < nmine
     const int nmine = L > sub ? (L - sub + CO_LANES - 1) / CO_LANES : 0;
-    int m = maxk, lastmix = 0;
-    bool mixed = false;
-    // every pass either moves m or merges two segments, so the reference loop
-    // ends within O(maxk^2) passes; the cap only guards non-finite densities
-    int guard = 4 * maxk * maxk + 64;
-    while ((kk[m - 1] > 0 || (lastmix != 0 && kk[m] != maxk)) && --guard > 0) {
-        if (kk[m - 1] == 0 || r[kk[m]] < r[kk[m - 1]]) {
-            if (lastmix == 0 || kk[m] == maxk)
-                m = m - 1;
-            else
-                m = m + 1;
-            lastmix = 0;
-        } else {
-            lastmix = 1;
-            mixed = true;
-            int n = m - 1;
-            while (kk[n - 1] > 0 && r[kk[n]] >= r[kk[n - 1]]) n = n - 1;
-            const int km = kk[m];
-            double dsum = dzm[km];
-            for (int ni = 1; ni <= m - n; ni++) dsum = dsum + dzm[kk[m - ni]];
-            // the thickness-weighted mean of levels kk[n..m]: T and S privately
-            // on every lane, this lane's tracers in memory, CO_CH at a time
-            // (their loads issued together)
-            {
-                double at = tp[km] * dzm[km], as = sp[km] * dzm[km];
-                for (int ni = 1; ni <= m - n; ni++) {
-                    const int kq = kk[m - ni];
-                    at = at + tp[kq] * dzm[kq];
-                    as = as + sp[kq] * dzm[kq];
-                }
-                tp[km] = StateIO<R>::rt(at / dsum, carry);
-                sp[km] = StateIO<R>::rt(as / dsum, carry);
+    // the group's lanes sit side by side in one wave; its first lane takes
+    // the next list entry for the group
+    const int lead = ((int)threadIdx.x & 63) & ~(CO_LANES - 1);
+    for (;;) {
+        int idx = 0;
+        if (sub == 0) idx = atomicAdd(&s_next, 1);
+        idx = __shfl(idx, lead);
+        if (idx >= n) break;
+        {
+            const int w = wl[idx];
+            const int col = g.wetsorted[w];
+            const int k1 = k1at(g, col % maxi + 1, col / maxi + 1);
+            for (int m = 0; m <= maxk + 1; m++) kk[m] = 0;
+            for (int m = k1; m <= maxk; m++) {
+                r[m] = rho[(size_t)(m - 1) * plane + col];
+                tp[m] = StateIO<R>::ld(st, lo, (size_t)(m - 1) * plane + col);
+                sp[m] = StateIO<R>::ld(st, lo, tstride + (size_t)(m - 1) * plane + col);
+                kk[m] = m;
+                dzm[m] = dz[m];
             }
-            for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
-                double sm[CO_CH];
-#pragma unroll
-                for (int c = 0; c < CO_CH; c++)
-                    if (j0 + c < nmine)
-                        sm[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col +
-                                                           (size_t)(km - 1) * plane) * dzm[km];
-                for (int ni = 1; ni <= m - n; ni++) {
-                    const int kq = kk[m - ni];
-                    const double wq = dzm[kq];
-#pragma unroll
-                    for (int c = 0; c < CO_CH; c++)
-                        if (j0 + c < nmine)
-                            sm[c] = sm[c] + StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col +
-                                                                       (size_t)(kq - 1) * plane) * wq;
-                }
-#pragma unroll
-                for (int c = 0; c < CO_CH; c++)
-                    if (j0 + c < nmine)
-                        StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col + (size_t)(km - 1) * plane,
-                                       sm[c] / dsum);
-            }
-            for (int ni = 1; ni <= m - n; ni++) dzm[km] = dzm[km] + dzm[kk[m - ni]];
-            r[km] = eos<double>(e0, e1, e2, e3, tp[km], sp[km]);
-            int ni = m - 1;
-            while (kk[ni + 1] > 0) {
-                kk[ni] = kk[ni - m + n];
-                ni = ni - 1;
-            }
-        }
-    }
-    if (guard <= 0 && sub == 0) atomicAdd(&M.diag[3], 1);
-    m = maxk - 1;
-    double cost = 0.0;
-    for (int n = maxk - 1; n >= k1; n--) {
-        if (n > kk[m]) {
-            const int kf = kk[m + 1];
-            const size_t from = (size_t)(kf - 1) * plane + col, to = (size_t)(n - 1) * plane + col;
-            for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
-                double v[CO_CH];
-#pragma unroll
-                for (int c = 0; c < CO_CH; c++)
-                    if (j0 + c < nmine) v[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + from);
-#pragma unroll
-                for (int c = 0; c < CO_CH; c++)
-                    if (j0 + c < nmine) StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + to, v[c]);
-            }
-            tp[n] = tp[kf];
-            sp[n] = sp[kf];
-            r[n] = eos<double>(e0, e1, e2, e3, tp[n], sp[n]);
-            cost += 1.0;
-        } else {
-            m = m - 1;
-        }
-    }
-    if ((mixed || cost > 0.0) && M.bgc) {
-        // the coupling's column sums of this column after mixing and fill-in
-        const double rSold = 1.0 / gp(M.scal)[SC_SMEAN];
-        const size_t ncl = (size_t)g.ncell, slot = (size_t)gp(g.vpos)[w];
-        AS_GLOBAL double* cs = gp(M.colsum);
-        if (sub == 0) {
-            double aS = 0.0;
-            for (int q = k1; q <= maxk; q++) aS += (sp[q] + SALN0 + 0.0) * gp(g.V)[(size_t)(q - 1) * plane + col];
-            cs[slot] = aS;
-        }
-        // this lane's tracers from the third on
-        const int jfirst = sub >= 2 ? 0 : 1;
-        for (int j0 = jfirst; j0 < nmine; j0 += CO_CH) {
-            double a[CO_CH];
-#pragma unroll
-            for (int t = 0; t < CO_CH; t++) a[t] = 0.0;
-            for (int q = k1; q <= maxk; q++) {
-                const size_t c = (size_t)(q - 1) * plane + col;
-                const double so = gp(M.ocn)[ncl + c], mm = gp(M.M)[c];
-#pragma unroll
-                for (int t = 0; t < CO_CH; t++)
-                    if (j0 + t < nmine) {
-                        const double lv = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + t)) * tstride + c) * so * rSold;
-                        a[t] += lv * mm;
+            int m = maxk, lastmix = 0;
+            bool mixed = false;
+            // every pass either moves m or merges two segments, so the
+            // reference loop ends within O(maxk^2) passes; the cap only
+            // guards non-finite densities
+            int guard = 4 * maxk * maxk + 64;
+            while ((kk[m - 1] > 0 || (lastmix != 0 && kk[m] != maxk)) && --guard > 0) {
+                if (kk[m - 1] == 0 || r[kk[m]] < r[kk[m - 1]]) {
+                    if (lastmix == 0 || kk[m] == maxk)
+                        m = m - 1;
+                    else
+                        m = m + 1;
+                    lastmix = 0;
+                } else {
+                    lastmix = 1;
+                    mixed = true;
+                    int nn = m - 1;
+                    while (kk[nn - 1] > 0 && r[kk[nn]] >= r[kk[nn - 1]]) nn = nn - 1;
+                    const int km = kk[m];
+                    double dsum = dzm[km];
+                    for (int ni = 1; ni <= m - nn; ni++) dsum = dsum + dzm[kk[m - ni]];
+                    // the thickness-weighted mean of levels kk[nn..m]: T and S
+                    // privately on every lane, this lane's tracers in memory,
+                    // CO_CH at a time (their loads issued together)
+                    {
+                        double at = tp[km] * dzm[km], as = sp[km] * dzm[km];
+                        for (int ni = 1; ni <= m - nn; ni++) {
+                            const int kq = kk[m - ni];
+                            at = at + tp[kq] * dzm[kq];
+                            as = as + sp[kq] * dzm[kq];
+                        }
+                        tp[km] = StateIO<R>::rt(at / dsum, carry);
+                        sp[km] = StateIO<R>::rt(as / dsum, carry);
                     }
-            }
+                    for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
+                        double sm[CO_CH];
 #pragma unroll
-            for (int t = 0; t < CO_CH; t++)
-                if (j0 + t < nmine) cs[(size_t)(sub + CO_LANES * (j0 + t)) * g.nwet + slot] = a[t];
+                        for (int c = 0; c < CO_CH; c++)
+                            if (j0 + c < nmine)
+                                sm[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col +
+                                                                   (size_t)(km - 1) * plane) * dzm[km];
+                        for (int ni = 1; ni <= m - nn; ni++) {
+                            const int kq = kk[m - ni];
+                            const double wq = dzm[kq];
+#pragma unroll
+                            for (int c = 0; c < CO_CH; c++)
+                                if (j0 + c < nmine)
+                                    sm[c] = sm[c] + StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride +
+                                                                               col + (size_t)(kq - 1) * plane) * wq;
+                        }
+#pragma unroll
+                        for (int c = 0; c < CO_CH; c++)
+                            if (j0 + c < nmine)
+                                StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + col + (size_t)(km - 1) * plane,
+                                               sm[c] / dsum);
+                    }
+                    for (int ni = 1; ni <= m - nn; ni++) dzm[km] = dzm[km] + dzm[kk[m - ni]];
+                    r[km] = eos<double>(e0, e1, e2, e3, tp[km], sp[km]);
+                    int ni = m - 1;
+                    while (kk[ni + 1] > 0) {
+                        kk[ni] = kk[ni - m + nn];
+                        ni = ni - 1;
+                    }
+                }
+            }
+            if (guard <= 0 && sub == 0) atomicAdd(&M.diag[3], 1);
+            // fill-in: every level not left as a segment's own takes the
+            // values of the segment it lies in
+            m = maxk - 1;
+            double cost = 0.0;
+            for (int nn = maxk - 1; nn >= k1; nn--) {
+                if (nn > kk[m]) {
+                    const int kf = kk[m + 1];
+                    const size_t from = (size_t)(kf - 1) * plane + col, to = (size_t)(nn - 1) * plane + col;
+                    for (int j0 = 0; j0 < nmine; j0 += CO_CH) {
+                        double v[CO_CH];
+#pragma unroll
+                        for (int c = 0; c < CO_CH; c++)
+                            if (j0 + c < nmine) v[c] = StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + from);
+#pragma unroll
+                        for (int c = 0; c < CO_CH; c++)
+                            if (j0 + c < nmine) StateIO<R>::st(st, lo, (size_t)(sub + CO_LANES * (j0 + c)) * tstride + to, v[c]);
+                    }
+                    tp[nn] = tp[kf];
+                    sp[nn] = sp[kf];
+                    r[nn] = eos<double>(e0, e1, e2, e3, tp[nn], sp[nn]);
+                    cost += 1.0;
+                } else {
+                    m = m - 1;
+                }
+            }
+            if ((mixed || cost > 0.0) && M.bgc) {
+                // the coupling's column sums of this column after mixing and fill-in
+                const double rSold = 1.0 / gp(M.scal)[SC_SMEAN];
+                const size_t ncl = (size_t)g.ncell, slot = (size_t)gp(g.vpos)[w];
+                AS_GLOBAL double* cs = gp(M.colsum);
+                if (sub == 0) {
+                    double aS = 0.0;
+                    for (int q = k1; q <= maxk; q++) aS += (sp[q] + SALN0 + 0.0) * gp(g.V)[(size_t)(q - 1) * plane + col];
+                    cs[slot] = aS;
+                }
+                // this lane's tracers from the third on
+                const int jfirst = sub >= 2 ? 0 : 1;
+                for (int j0 = jfirst; j0 < nmine; j0 += CO_CH) {
+                    double a[CO_CH];
+#pragma unroll
+                    for (int t = 0; t < CO_CH; t++) a[t] = 0.0;
+                    for (int q = k1; q <= maxk; q++) {
+                        const size_t c = (size_t)(q - 1) * plane + col;
+                        const double so = gp(M.ocn)[ncl + c], mm = gp(M.M)[c];
+#pragma unroll
+                        for (int t = 0; t < CO_CH; t++)
+                            if (j0 + t < nmine) {
+                                const double lv =
+                                    StateIO<R>::ld(st, lo, (size_t)(sub + CO_LANES * (j0 + t)) * tstride + c) * so * rSold;
+                                a[t] += lv * mm;
+                            }
+                    }
+#pragma unroll
+                    for (int t = 0; t < CO_CH; t++)
+                        if (j0 + t < nmine) cs[(size_t)(sub + CO_LANES * (j0 + t)) * g.nwet + slot] = a[t];
+                }
+            }
+            if (sub == 0) {
+                if (mixed || cost > 0.0) {
+                    for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
+                    if (cost > 0.0) gp(M.cost)[col] += cost;
+                }
+                snap(col, tp[maxk], sp[maxk]);
+            }
         }
-    }
-    if (sub == 0) {
-        if (mixed || cost > 0.0) {
-            for (int q = k1; q <= maxk; q++) rho[(size_t)(q - 1) * plane + col] = r[q];
-            if (cost > 0.0) gp(M.cost)[col] += cost;
-        }
-        snap(tp[maxk], sp[maxk]);
     }
 }
 

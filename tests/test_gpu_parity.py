@@ -68,6 +68,26 @@ def test_tstep_fp64_matches_oracle(eng, dims, kind, L, random_u, sflux, diso, ns
     eng.remove_member(m)
 
 
+def test_convection_of_many_tracers(eng):
+    """100 tracers on 16 levels with surface fluxes and divergent flow:
+    each of a convecting column's four lanes mixes 25 of its tracers, seven
+    rounds of four loads issued together; the state stays the oracle's bit
+    for bit and the filled levels are counted alike."""
+    c = make_case(36, 36, 16, kind="ts", L=100, random_u=True, sflux=True)
+    m = new_member(eng, c)
+    gold = O.GoldsteinOracle(c.og, c.k1, c.ts, c.u, c.sflux)
+    eng.step([m], 3)
+    gold.step(3)
+    got = eng.download(m, c.ts.shape)
+    w = wet(c)
+    diff = np.count_nonzero(got[w] != gold.ts[w])
+    say("levels filled by convection", gold.cost.sum(), "cells differing", diff)
+    assert gold.cost.sum() > 0
+    assert diff == 0
+    assert abs(eng.diag(m).cost - gold.cost.sum()) < 0.5
+    eng.remove_member(m)
+
+
 def test_tstep_fp64_bit_identical_to_oracle(eng):
     c = make_case(kind="ts", L=4, seed=5)
     m = new_member(eng, c)
