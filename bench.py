@@ -380,6 +380,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-prognostic", action="store_true")
+    ap.add_argument("--prognostic-only", action="store_true",
+                    help="profile mode: the prognostic-ocean leg alone at the flux tile 4x256x2x1 (the autotuner's "
+                         "usual pick), its figures on one line, no headline line")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--probe", choices=("fp32c", "fp64"), default=None,
                     help="counter-run mode: eager steps of the ensemble and a copy calibration, no JSON line")
@@ -394,6 +397,10 @@ def main():
         sys.exit(2)
     if args.probe:
         run_probe(args)
+        return
+    if args.prognostic_only:
+        print(json.dumps(dict(prognostic=run_prognostic(args, dict(tracer_block=4, cols=256, ring=2,
+                                                                   iso_prepass=1)))), flush=True)
         return
     args.latency_years = max(10, args.latency_years)
     cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))

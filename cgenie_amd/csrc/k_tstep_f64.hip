@@ -12,17 +12,20 @@ void launch_flux64(const MemberDev* t, const int32_t* l, int n, const FluxShape&
 }
 
 void launch_co(const MemberDev* t, const int32_t* l, int n, int nwetmax, int maxkmax, int prec, hipStream_t s) {
-    // 256 wet columns per workgroup of 256 threads (64 convecting at a
-    // time), fewer where the level arrays would not fit in 64 KiB of LDS
+    // 64 wet columns per workgroup of 128 threads (32 convecting at a time,
+    // a group taking a second column when it has finished its first): small
+    // workgroups, many resident per CU whether few columns convect or most;
+    // fewer threads where the level arrays would not fit in 64 KiB of LDS
     const int K = maxkmax + 2;
-    int nt = 256;
-    while (nt > 32 && co_lds_bytes(nt, nt, K) > 65536) nt /= 2;
-    const dim3 grid((nwetmax + nt - 1) / nt, n);
-    const size_t lds = co_lds_bytes(nt, nt, K);
+    int nt = 128;
+    while (nt > 32 && co_lds_bytes(nt, nt / 2, K) > 65536) nt /= 2;
+    const int cb = nt / 2;
+    const dim3 grid((nwetmax + cb - 1) / cb, n);
+    const size_t lds = co_lds_bytes(nt, cb, K);
     if (prec == 0)
-        hipLaunchKernelGGL(k_co<double>, grid, dim3(nt), lds, s, t, l, K, nt);
+        hipLaunchKernelGGL(k_co<double>, grid, dim3(nt), lds, s, t, l, K, cb);
     else
-        hipLaunchKernelGGL(k_co<float>, grid, dim3(nt), lds, s, t, l, K, nt);
+        hipLaunchKernelGGL(k_co<float>, grid, dim3(nt), lds, s, t, l, K, cb);
 }
 
 void launch_flip(const MemberDev* t, const int32_t* l, int n, hipStream_t s) {
