@@ -896,7 +896,12 @@ __global__ __launch_bounds__(ATM_THREADS) void k_atchem(const MemberDev* __restr
         const int q = threadIdx.x;
         double acc = 0.0;
         if (q == 2) {
-            for (int c = 0; c < ncol; c++) acc += ATM_TH * cell_area(sv, maxi, c / maxi + 1);
+            // a cell's volume depends only on its row: formed once per row,
+            // added once per cell in the same order (i fastest)
+            for (int j = 0; j < ncol / maxi; j++) {
+                const double v = ATM_TH * cell_area(sv, maxi, j + 1);
+                for (int i = 0; i < maxi; i++) acc += v;
+            }
         } else if (lds) {
 #pragma unroll 8
             for (int c = 0; c < ncol; c++) acc += stage[q * ncol + c];
