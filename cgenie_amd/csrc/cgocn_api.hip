@@ -31,7 +31,7 @@ void launch_inventory(const MemberDev*, int32_t member, int L, double* out, cons
 void launch_carbconst_surface(const MemberDev*, int32_t member, int nwet, double* out, hipStream_t);
 void launch_atchem(const MemberDev*, const int32_t*, int nlist, int ncolmax, hipStream_t);
 void launch_mom(const MemberDev*, const int32_t*, int nlist, const int32_t* band, int nband, const int32_t* dense,
-                const int* groups, int ngroups, int maxN, int maxcol, int maxn, hipStream_t);
+                const int* groups, int ngroups, int maxN, int maxcol, int maxn, int part, hipStream_t);
 size_t mom_solve_lds(int N, int n);
 void launch_sfx(const MemberDev*, const int32_t*, int nlist, int maxcol, int slot_only, hipStream_t);
 size_t sfx_lds(int ncol);
@@ -2076,10 +2076,11 @@ static void enqueue_co_flip(cgo_engine* e, const Lists& ls, const int32_t* d_lis
 
 // the velocity solve of the members that have one (step_goldstein 198-233),
 // which precedes their tstepo
-static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s) {
+// part 1: the column kernel before the solve; 2: the rest; 3: both
+static void enqueue_mom(cgo_engine* e, const Lists& ls, const int32_t* d_lists, hipStream_t s, int part = 3) {
     if (ls.n[5])
         launch_mom(e->d_table, d_lists + ls.off[5], ls.n[5], d_lists + ls.off[7], ls.n[7], d_lists + ls.off[8],
-                   ls.momGroups.data(), (int)ls.momGroups.size() / 3, ls.momN, ls.momCol, ls.momBand, s);
+                   ls.momGroups.data(), (int)ls.momGroups.size() / 3, ls.momN, ls.momCol, ls.momBand, part, s);
 }
 
 // the surface fluxes of the members that have them (surflux, called by
@@ -2129,10 +2130,15 @@ static void enqueue_steps(cgo_engine* e, const Lists& ls, const int32_t* d_lists
         // CUs first, and BIOGEM's blocks fill the slots both flux kernels
         // leave (enqueued first, they ran beside the pre-pass alone)
         // the surface fluxes read the atmosphere that the last step's ATCHEM
-        // left: they come before the fork, so this step's ATCHEM waits for them
+        // left: they come before the fork, so this step's ATCHEM waits for them.
+        // The velocity solve neither reads nor writes BIOGEM's arrays; the
+        // fork follows its first column kernel, so the BIOGEM step runs beside
+        // the solve blocks (one per member) and the small column kernels
+        // after them rather than beside that column kernel
         enqueue_sfx(e, ls, d_lists);
+        enqueue_mom(e, ls, d_lists, e->s, 1);
         if (bgc) hipEventRecord(e->ev_fork, e->s);
-        enqueue_mom(e, ls, d_lists, e->s);
+        enqueue_mom(e, ls, d_lists, e->s, 2);
         enqueue_flux(e, ls, d_lists, 1);
         if (bgc) {
             hipStreamWaitEvent(e->s2, e->ev_fork, 0);

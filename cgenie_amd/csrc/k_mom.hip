@@ -766,19 +766,22 @@ __global__ __launch_bounds__(256) void k_mom_gemm(const MemberDev* __restrict__ 
 // island path terms the solve block's ring holds
 size_t mom_ring_terms(int n) { return (size_t)MOM_RING * MOM_CH * (n + 3); }
 
+// part 1: the column kernel before the solve; 2: the rest; 3: both.
 // list: every member with a velocity solve (the column kernels); band: those
 // solved through the band (FP64 members, FP32 members of grids without the
 // inverse); dense: the FP32 members of grids with the inverse, grouped by
 // grid, groups[g] = {offset into dense, count, N}
 void launch_mom(const MemberDev* mems, const int32_t* list, int nlist, const int32_t* band, int nband,
-                const int32_t* dense, const int* groups, int ngroups, int maxN, int maxcol, int maxn, hipStream_t s) {
+                const int32_t* dense, const int* groups, int ngroups, int maxN, int maxcol, int maxn, int part,
+                hipStream_t s) {
     if (nlist <= 0) return;
     const int cb = (maxcol + MOM_NT - 1) / MOM_NT;
     // the stored ub and psi have at most (maxi+2) maxj x 2 and (maxi+1)(maxj+1)
     // entries: fewer than 4 x the columns + 256
     const int cb2 = (4 * maxcol + 256 + MOM_NT - 1) / MOM_NT;
     const size_t lds = mom_solve_lds(maxN, maxn);
-    hipLaunchKernelGGL(k_mom_cols_a, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
+    if (part & 1) hipLaunchKernelGGL(k_mom_cols_a, dim3(cb, nlist), dim3(MOM_NT), 0, s, mems, list);
+    if (!(part & 2)) return;
     if (nband > 0) hipLaunchKernelGGL(k_mom_solve, dim3(nband), dim3(MOM_NT), lds, s, mems, band, 0);
     int ndense = 0;
     for (int q = 0; q < ngroups; q++) ndense += groups[3 * q + 1];
