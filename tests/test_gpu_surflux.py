@@ -216,6 +216,51 @@ def test_graph_replay_equals_eager(eng, world):
     eng.remove_member(b)
 
 
+def test_prognostic_graph_replay_equals_eager(eng, world):
+    """The whole prognostic step in a replayed graph: surface fluxes with the
+    members' own ATCHEM CO2, the velocity solve (FP32 members: one dense
+    product for the grid), transport, BIOGEM with air-sea exchange on the
+    second stream (forked after the solve's first column kernel) and the
+    coupling.  Two members captured for 2 steps and replayed 4 times, two
+    more stepped eagerly 8 times: state, atmosphere and surface outputs
+    equal bit for bit."""
+    say("prognostic step (surflux + velocity + BIOGEM/ATCHEM) replayed = eager")
+    g = G.make_grid(MAXI, MAXJ, MAXK)
+    tset = G.tracer_set("config2")
+    ts = G.make_tracers(g, world.k1, tset, 61)
+    light, mld = G.make_light(g, 62), G.make_mld(g, 63, 0.0)
+    su = SF.surflux_setup(g, world.k1, radfor_atchem=1)
+    atm = SF.atmosphere(g, world.k1, 64, ice=True)
+    gid = eng.add_grid(g, world.k1)
+    eng.set_momentum(gid, g, world)
+    eng.set_surflux(gid, su)
+    ms = []
+    for q in range(4):
+        m = eng.add_member(gid, tset.L, E.FP32C, DIFF, diso=True)
+        eng.upload(m, ts)
+        eng.bgc_init(m, E.default_bgc_params(1.0 / 96, tset, carb_tol=1e-12, carb_maxit=1000, airsea=1,
+                                             atm_pCO2=400e-6), light, mld)
+        eng.set_windspeed(m, np.full((MAXJ, MAXI), 8.0))
+        eng.set_wind(m, stress())
+        eng.set_surface(m, *atm)
+        ms.append(m)
+    gr = eng.capture(ms[:2], 2)
+    for _ in range(4):
+        eng.launch(gr, 1)
+        eng.sync()
+    eng.drop_graph(gr)
+    eng.step(ms[2:], 8)
+    for a, b in ((ms[0], ms[2]), (ms[1], ms[3])):
+        assert np.array_equal(eng.download(a, ts.shape), eng.download(b, ts.shape))
+        assert np.array_equal(eng.velocity(a, g), eng.velocity(b, g))
+        assert np.array_equal(eng.atmosphere(a, MAXI, MAXJ), eng.atmosphere(b, MAXI, MAXJ))
+        oa, ob = eng.surface_outputs(a, g), eng.surface_outputs(b, g)
+        for n in list(E.SF_NAMES) + ["ashum", "tice", "co2"]:
+            assert np.array_equal(oa[n], ob[n]), n
+    for m in ms:
+        eng.remove_member(m)
+
+
 def test_checkpoint_continues_bitwise(eng, world):
     say("checkpoint of a member with surface fluxes continues bit for bit")
     g, gid, a, ts, su, atm = new_member(eng, world)
