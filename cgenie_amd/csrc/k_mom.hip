@@ -689,11 +689,13 @@ size_t mom_solve_lds(int N, int n) {
 // ------------------------------------------------------------ the dense solve
 // psi = A^-1 g for the FP32 members of one grid at once: C(i, c) =
 // sum_k A(i, k) B(k, c), B's column c the staged forcing of the group's
-// member c and C's column c its psi.  64 x 64 tiles of C per workgroup, 4 x 4
-// per thread, k in steps of 16 through LDS with the next step's loads issued
-// before the current one's products; the k range is cut into MOM_SPLITK
-// slices (blockIdx.z), each writing its partial sums, which the solve block
-// adds in slice order.
+// member c and C's column c its psi.  64 x 64 tiles of C per workgroup, k in
+// steps of 16 through LDS with the next step's loads issued before the
+// current one's products; each wave forms 16 rows x 64 members as four 16 x
+// 16 tiles on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane i + 16k
+// holds A(i, k) and B(k, j = i); item r of lane l is C(l / 16 + 4 r, l % 16)).
+// The k range is cut into MOM_SPLITK slices (blockIdx.z), each writing its
+// partial sums, which the solve block adds in slice order.
 constexpr int GT = 64, GK = 16;
 __global__ __launch_bounds__(256) void k_mom_gemm(const MemberDev* __restrict__ mems, const int32_t* __restrict__ list,
                                                    int nm) {
@@ -715,11 +717,11 @@ __global__ __launch_bounds__(256) void k_mom_gemm(const MemberDev* __restrict__ 
         const int c = c0 + ty + 16 * q;
         bcol[q] = c < nm ? mems[list[c]].mw + L.gst : nullptr;
     }
-    double acc[4][4];
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64, li = lane % 16, lk = lane / 16;
+    d4 acc[4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    for (int t = 0; t < 4; t++) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
     double na[4], nb[4];
     auto fetch = [&](int k0) {
         const int k = k0 + tx;
@@ -739,28 +741,26 @@ __global__ __launch_bounds__(256) void k_mom_gemm(const MemberDev* __restrict__ 
         }
         __syncthreads();
         if (k0 + GK < ke) fetch(k0 + GK);
-#pragma unroll 4
-        for (int kq = 0; kq < GK; kq++) {
-            double a[4], b[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                a[q] = As[kq][ty * 4 + q];
-                b[q] = Bs[kq][tx * 4 + q];
-            }
+        for (int ks = 0; ks < GK / 4; ks++) {
+            const double a = As[ks * 4 + lk][wv * 16 + li];
 #pragma unroll
-            for (int x = 0; x < 4; x++)
-#pragma unroll
-                for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+            for (int t = 0; t < 4; t++)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[ks * 4 + lk][t * 16 + li], acc[t], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int x = 0; x < 4; x++)
+    for (int t = 0; t < 4; t++) {
+        const int c = c0 + t * 16 + li;
+        if (c >= nm) continue;
+        double* out = mems[list[c]].mw + L.gst + (size_t)N * (1 + blockIdx.z);
 #pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const int r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
-            if (r < N && c < nm) mems[list[c]].mw[L.gst + (size_t)N * (1 + blockIdx.z) + r] = acc[x][y];
+        for (int q = 0; q < 4; q++) {
+            const int r = r0 + wv * 16 + lk + 4 * q;
+            if (r < N) out[r] = acc[t][q];
         }
+    }
 }
 
 // island path terms the solve block's ring holds
